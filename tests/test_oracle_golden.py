@@ -1,0 +1,101 @@
+"""Pins the FP64 oracle (oracle/rhmc_oracle.c) against golden vectors that the
+reference's own CPU path wrote (tests/golden/make_golden.py)."""
+import os
+
+import numpy as np
+import pytest
+
+from oracle import Oracle
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def load(name):
+    path = os.path.join(GOLD, f"golden_{name}.npz")
+    if not os.path.exists(path):
+        pytest.skip(f"{path} missing")
+    return dict(np.load(path))
+
+
+def rel(a, b):
+    return np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-300)
+
+
+@pytest.fixture(scope="module", params=["t4", "t4o"])
+def g(request):
+    return load(request.param)
+
+
+def test_dslash(g):
+    o = Oracle(tuple(g["dims"]))
+    U = g["links"]
+    assert rel(o.doe(U, g["dslash_in"][0]), g["doe"]) < 1e-14
+    assert rel(o.deo(U, g["dslash_in"][1]), g["deo"]) < 1e-14
+
+
+def test_multishift(g):
+    o = Oracle(tuple(g["dims"]))
+    for ps in range(int(g["no_ps"])):
+        x, it = o.multishift(g["links"], float(g["mass"]), g["chi"][ps], g["ms_shifts"], float(g["ms_res"]))
+        for k in range(len(g["ms_shifts"])):
+            assert rel(x[k], g["ms_x"][ps, k]) < 1e-12, (ps, k)
+
+
+def _md_coeffs(g):
+    from rhmc_amd.remez import RationalApprox
+    r = RationalApprox(1e-5, float(g["md_a0"]), list(g["md_a"]), list(g["md_b"]))
+    e = -float(g["n_flavours"]) / 4.0 / float(g["no_ps"])
+    a0, a, b, _ = r.rescaled(float(g["eig_min"]), float(g["eig_max"]), e)
+    return a0, a, b
+
+
+def test_fermion_force(g):
+    o = Oracle(tuple(g["dims"]))
+    a0, a, b = _md_coeffs(g)
+    ip, _ = o.full_force(g["links"], float(g["beta"]), float(g["mass"]), g["chi"], a0, a, b,
+                         float(g["res_md"]), with_gauge=False)
+    assert rel(ip, g["force_fermion"]) < 1e-11
+    ip, _ = o.full_force(g["links"], float(g["beta"]), float(g["mass"]), g["chi"], a0, a, b,
+                         float(g["res_md"]), with_gauge=True)
+    assert rel(ip, g["force_full"]) < 1e-11
+
+
+def test_gauge_force(g):
+    o = Oracle(tuple(g["dims"]))
+    ip = o.ipdot_total(g["links"], float(g["beta"]), None, True)
+    assert rel(ip, g["force_gauge"]) < 1e-14
+
+
+def test_exp_and_momenta(g):
+    o = Oracle(tuple(g["dims"]))
+    U = o.exp_update(g["links"], g["mom"], float(g["exp_step"]))
+    assert rel(U, g["exp_links"]) < 1e-14
+    P = o.mom_update(g["mom"], g["ipdot"], float(g["mom_step"]))
+    assert rel(P, g["mom_out"]) < 1e-15
+
+
+def test_action(g):
+    o = Oracle(tuple(g["dims"]))
+    v = o.action(g["links"], g["mom"], float(g["beta"]), g["phi"], g["chi"], True, None, 0)
+    np.testing.assert_allclose(v, g["action0"], rtol=1e-13, atol=1e-13)
+    v = o.action(g["links"], g["mom"], float(g["beta"]), g["phi"], g["chi"], True, v, -1)
+    np.testing.assert_allclose(v, g["action1"], rtol=1e-12, atol=1e-12)
+
+
+def test_trajectory(g):
+    o = Oracle(tuple(g["dims"]))
+    integ = "multistep_2mn" if int(g["use_multistep"]) == 1 else "omelyan"
+    dh, U, P, chi, phin = o.trajectory(
+        g["links"], g["mom"], g["phi"], beta=float(g["beta"]), mass=float(g["mass"]),
+        n_flavours=int(g["n_flavours"]),
+        first=(float(g["first_a0"]), g["first_a"], g["first_b"]),
+        md=(float(g["md_a0"]), g["md_a"], g["md_b"]),
+        last=(float(g["last_a0"]), g["last_a"], g["last_b"]),
+        eig_min=float(g["eig_min"]), eig_max=float(g["eig_max"]), no_md=int(g["no_md"]),
+        integrator=integ, gauge_scale=int(g["gauge_scale"]), res_md=float(g["res_md"]),
+        res_metro=float(g["res_metro"]))
+    assert rel(chi, g["traj_chi"]) < 1e-12
+    assert rel(U, g["traj_links"]) < 1e-11
+    assert rel(P, g["traj_mom"]) < 1e-11
+    assert rel(phin, g["traj_phinew"]) < 1e-11
+    assert abs(dh - float(g["traj_dh"])) < 1e-9
