@@ -1,0 +1,158 @@
+/*
+ * gpu.h -- C ABI of the MI355X RHMC hot path (gfx950, HIP).
+ *
+ * Replaces the device boundary of the reference (coppolachan/RHMC-on-GPU):
+ *   include/gpu.h:6-8            gpu_init0 / gpu_init1 / gpu_end
+ *   lib/Cuda/cuda_init.h:4-11    cuda_init0/1, cuda_end, cuda_get_conf, momenta get/put
+ *   lib/Cuda/cuda_exponentiate.h:4, cuda_mom_sum_mul.h:4, cuda_deriv.h:4,
+ *   cuda_fermion_force.h:4, cuda_sol_sum.h:4, cuda_find_min_max.h:6-7
+ * The reference passes everything through global singletons and exits on
+ * error; this ABI takes an explicit handle and plain pointers, and every call
+ * returns 0 on success (nonzero: see gpu_last_error()).
+ *
+ * Host data layouts (identical to the reference's host arrays, so a
+ * std::complex<double> array can be passed as double*):
+ *   links / momenta / ipdot : double[4*V][3][3][2], entry mu*V + i with the
+ *                             reference site index i = parity*V/2 + lex/2,
+ *                             lex = x + nx*(y + ny*(z + nz*t)); links carry the
+ *                             staggered phases (lib/Main/Conf/conf.cc:36).
+ *   pseudofermions          : double[n_pf][V/2][3][2]   (even sites)
+ *   shifted solutions       : double[n_pf][order][V/2][3][2]
+ */
+#ifndef RHMC_AMD_GPU_H
+#define RHMC_AMD_GPU_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct rhmc_gpu_ctx *rhmc_gpu_t;
+
+typedef struct {
+  int nx, ny, nz, nt;    /* even extents (lib/Tools/exception.cc:12-15)       */
+  int antiperiodic_t;    /* 1 = antiperiodic temporal fermion b.c.            */
+  double mass;           /* bare staggered mass (parameters.h `mass`)         */
+  double beta;           /* Wilson coupling (parameters.h `beta`)             */
+  int n_pf;              /* pseudofermions (parameters.h `no_ps`)             */
+  int max_order;         /* largest number of poles used (<= 32)              */
+  int device;            /* HIP device ordinal                                */
+} rhmc_gpu_params;
+
+/* Status codes */
+enum { RHMC_OK = 0, RHMC_EINVAL = 1, RHMC_EMAXCG = 2, RHMC_EHIP = -1 };
+
+/* Context lifetime (the reference's implicit global state; cuda_init.cu:5-59). */
+int gpu_create(const rhmc_gpu_params *p, rhmc_gpu_t *out);
+int gpu_destroy(rhmc_gpu_t h);
+const char *gpu_last_error(rhmc_gpu_t h);
+
+/* include/gpu.h:6 gpu_init0 -> cuda_init0 (cuda_init.cu:5-57): gauge field
+ * to the device (FP64 18-real and the FP32 12-real copy). */
+int gpu_init0(rhmc_gpu_t h, const double *links);
+
+/* include/gpu.h:7 gpu_init1 -> cuda_init1 (cuda_init.cu:61-108): pseudofermion
+ * right-hand sides ("mf"), ipdot and momenta; NULL leaves a field unchanged. */
+int gpu_init1(rhmc_gpu_t h, const double *phi, const double *ipdot, const double *momenta);
+
+/* include/gpu.h:8 gpu_end -> cuda_end (cuda_init.cu:187-211): fields back to
+ * the host; NULL skips a field.  Device state is kept (no free). */
+int gpu_end(rhmc_gpu_t h, double *links, double *ipdot, double *momenta);
+
+/* Pseudofermion buffers: which = 0 right-hand sides "mf" (reference
+ * mf_device / chi_packed), 1 shifted solutions "smf" (psi_packed, n_pf*order
+ * vectors of the last solve), 2 heatbath phi, 3 chi, 4 phi_new. */
+int gpu_get_fermions(rhmc_gpu_t h, int which, double *out);
+int gpu_put_fermions(rhmc_gpu_t h, int which, const double *in);
+
+/* lib/Cuda/cuda_exponentiate.h:4 cuda_exp_su3(step) (momenta.cc:54):
+ * U <- Pi_SU3[exp(i step P)] U, row 2 *= eta (FP64, 6th-order Taylor). */
+int gpu_exp_su3(rhmc_gpu_t h, double step);
+
+/* lib/Cuda/cuda_mom_sum_mul.h:4 cuda_momenta_sum_multiply(step) (momenta.cc:94):
+ * P <- P + i*step*ipdot. */
+int gpu_momenta_sum_multiply(rhmc_gpu_t h, double step);
+
+/* lib/Cuda/cuda_deriv.h:4 cuda_gauge_deriv(ipdot_init) (ipdot.cc:37,162):
+ * ipdot (=, or += when ipdot_init=1) TA(U beta/3 staple). */
+int gpu_gauge_deriv(rhmc_gpu_t h, int ipdot_init);
+
+/* lib/Cuda/cuda_fermion_force.h:4 cuda_fermion_force(num_shifts, numerators)
+ * (fermionforce.cc:44): ipdot = TA(U F) from the shifted solutions of the last
+ * cu_multips_shifted_invert; numerators are FP64 here (the reference rounds
+ * them to float, fermionforce.cc:42-43).  accumulate=1 adds to ipdot. */
+int gpu_fermion_force(rhmc_gpu_t h, int num_shifts, const double *numerators, int accumulate);
+
+/* lib/Cuda/cuda_sol_sum.h:4 cuda_sol_sum (rationalapprox.cc:288,340):
+ * mf <- a0*mf + sum_k num[k]*smf[k] for every pseudofermion. */
+int gpu_sol_sum(rhmc_gpu_t h, int num_shifts, double mult_const, const double *numerators);
+
+/* lib/Cuda/cuda_find_min_max.h:6-7 (findminmax.cc:140-183): power iterations
+ * for the extreme eigenvalues of MdagM from a gaussian start drawn with seed;
+ * tolerance on the relative change (reference: 1e-5). */
+int gpu_find_max(rhmc_gpu_t h, double *max, double tol, unsigned long long seed);
+int gpu_find_min(rhmc_gpu_t h, double *min, double max, double tol, unsigned long long seed);
+
+/* Action bookkeeping on the device (action.cc:111-156, update.cc:110-114):
+ * init=0 stores the per-site action of (U, P, phi); init=-1 subtracts the
+ * action of (U, P, phi_new, chi).  gpu_action_sum returns the pairwise tree
+ * sum (include/tools.h:25-57) of the per-site differences. */
+int gpu_action(rhmc_gpu_t h, int init, int with_fermions);
+int gpu_action_sum(rhmc_gpu_t h, double *sum);
+
+/* Gaussian momenta / heatbath phi from the device counter-based generator
+ * (not the reference's dSFMT stream). */
+int gpu_gauss_momenta(rhmc_gpu_t h, unsigned long long seed, unsigned long long stream);
+int gpu_gauss_phi(rhmc_gpu_t h, unsigned long long seed, unsigned long long stream);
+
+/* Dslash on host data, for checking: out = D_oe in (par_out=1) or
+ * D_eo in (par_out=0), nvec vectors; precision 0 = FP64 18-real links,
+ * 1 = FP32 12-real links (vectors rounded to float). */
+int gpu_dslash(rhmc_gpu_t h, int par_out, int precision, int nvec, const double *in, double *out);
+
+/* Timing helper: run `reps` device-resident half-lattice Dslash launches on
+ * nvec vectors and return the mean kernel time in ms, measured with HIP
+ * events on the library's stream. */
+int gpu_time_dslash(rhmc_gpu_t h, int precision, int nvec, int reps, double *ms);
+
+/* --------------------------------------------------------------------- */
+/* One RHMC trajectory on the device: update.cc:58-152 with the integrators
+ * of lib/Update/MD_integrators. */
+enum { RHMC_INT_OMELYAN = 0, RHMC_INT_MULTISTEP_2MN = 1, RHMC_INT_LEAPFROG = 2 };
+
+typedef struct {
+  int n_flavours;
+  int no_md, gauge_scale, integrator;
+  double res_md, res_metro;       /* multi-shift stopping residuals           */
+  int residual_relative;          /* 1: residuals relative to |rhs|           */
+  int precision;                  /* 0 FP64, 1 mixed FP32 + FP64 refinement   */
+  int max_cg;
+  int order_first, order_md, order_last;
+  const double *first_a, *first_b; double first_a0;   /* normalised on [eps,1] */
+  const double *md_a, *md_b; double md_a0;
+  const double *last_a, *last_b; double last_a0;
+  double lambda_min;              /* eps of the normalised approximations     */
+  int bounds_mode;                /* 0: fixed eig_min/eig_max, 1: power method */
+  double eig_min, eig_max;
+  double power_tol;
+  int draw_fields;                /* 1: draw P and phi on the device (seed)   */
+  unsigned long long seed;
+  int metropolis;                 /* 1: accept/reject with u (below)          */
+  double metropolis_u;
+  int pure_gauge;
+} rhmc_traj_params;
+
+typedef struct {
+  double delta_h;
+  int accepted;
+  int cg_iterations;              /* summed over all solves                   */
+  int n_solves;
+  double eig_min, eig_max;
+  double seconds_total, seconds_solver, seconds_force, seconds_update;
+} rhmc_traj_result;
+
+int gpu_trajectory(rhmc_gpu_t h, const rhmc_traj_params *p, rhmc_traj_result *r);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,401 @@
+// capi.hip -- the extern "C" boundary (include/gpu.h, include/cu_inverter.h).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "context.hpp"
+#include "cu_inverter.h"
+#include "gpu.h"
+
+namespace rhmc {
+int upload_links(Context &c, const double *host, double2 *dst);
+int download_links(Context &c, const double2 *src, double *host);
+int upload_alg(Context &c, const double *host, double2 *dst, int herm);
+int download_alg(Context &c, const double2 *src, double *host, int herm);
+int upload_spinors(Context &c, const double *host, double2 *dst, int n);
+int download_spinors(Context &c, const double2 *src, double *host, int n);
+int hop_s12(Context &c, int par_out, const float2 *in, float2 *out, int nvec);
+int run_trajectory(Context &c, const rhmc_traj_params &p, rhmc_traj_result &r);
+
+bool hip_ok(Context &c, hipError_t e, const char *what) {
+  if (e == hipSuccess) return true;
+  c.last_error = std::string(what) + ": " + hipGetErrorString(e);
+  return false;
+}
+
+void *dev_alloc(Context &c, size_t bytes) {
+  void *p = nullptr;
+  if (!hip_ok(c, hipMalloc(&p, bytes), "hipMalloc")) return nullptr;
+  if (!hip_ok(c, hipMemset(p, 0, bytes), "hipMemset")) return nullptr;
+  return p;
+}
+
+void dev_free(void *p) {
+  if (p) hipFree(p);
+}
+
+void fill_dims(Context &c, int nx, int ny, int nz, int nt, int antiperiodic) {
+  c.L.nx = nx; c.L.ny = ny; c.L.nz = nz; c.L.nt = nt;
+  c.L.nxh = nx / 2;
+  c.V = (long)nx * ny * nz * nt;
+  c.Vh = c.V / 2;
+  c.L.Vh = c.Vh;
+  c.L.antiperiodic_t = antiperiodic;
+}
+
+__global__ void k_f2d(double2 *o, const float2 *i, long n) {
+  const long k = (long)blockIdx.x * 256 + threadIdx.x;
+  if (k < n) o[k] = make_double2(i[k].x, i[k].y);
+}
+__global__ void k_d2f(float2 *o, const double2 *i, long n) {
+  const long k = (long)blockIdx.x * 256 + threadIdx.x;
+  if (k < n) o[k] = make_float2((float)i[k].x, (float)i[k].y);
+}
+
+}  // namespace rhmc
+
+using namespace rhmc;
+
+struct rhmc_gpu_ctx {
+  Context c;
+};
+
+static int fail(rhmc_gpu_t h, const char *msg, int code = RHMC_EINVAL) {
+  if (h) h->c.last_error = msg;
+  return code;
+}
+
+extern "C" {
+
+int gpu_create(const rhmc_gpu_params *p, rhmc_gpu_t *out) {
+  if (!p || !out) return RHMC_EINVAL;
+  *out = nullptr;
+  if (p->nx <= 0 || p->ny <= 0 || p->nz <= 0 || p->nt <= 0 || (p->nx | p->ny | p->nz | p->nt) & 1) return RHMC_EINVAL;
+  if (p->n_pf < 1 || p->max_order < 1 || p->max_order > kMaxShifts || p->n_pf * p->max_order > 256)
+    return RHMC_EINVAL;
+  rhmc_gpu_t h = new (std::nothrow) rhmc_gpu_ctx();
+  if (!h) return RHMC_EINVAL;
+  Context &c = h->c;
+  c.device = p->device;
+  if (!hip_ok(c, hipSetDevice(p->device), "hipSetDevice")) { delete h; return RHMC_EHIP; }
+  fill_dims(c, p->nx, p->ny, p->nz, p->nt, p->antiperiodic_t);
+  c.mass = p->mass;
+  c.beta = p->beta;
+  c.n_pf = p->n_pf;
+  c.max_order = p->max_order;
+  if (!hip_ok(c, hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking), "stream")) { delete h; return RHMC_EHIP; }
+  const long Vh = c.Vh;
+  c.U64 = (double2 *)dev_alloc(c, sizeof(double2) * 72 * Vh);
+  c.U64_save = (double2 *)dev_alloc(c, sizeof(double2) * 72 * Vh);
+  c.U32 = (float4 *)dev_alloc(c, sizeof(float4) * 24 * Vh);
+  c.P = (double2 *)dev_alloc(c, sizeof(double2) * 32 * Vh);
+  c.ipdot = (double2 *)dev_alloc(c, sizeof(double2) * 32 * Vh);
+  const size_t sp = sizeof(double2) * 3 * Vh;
+  c.mf = (double2 *)dev_alloc(c, sp * c.n_pf);
+  c.smf = (double2 *)dev_alloc(c, sp * c.n_pf * c.max_order);
+  c.phi0 = (double2 *)dev_alloc(c, sp * c.n_pf);
+  c.chi = (double2 *)dev_alloc(c, sp * c.n_pf);
+  c.phinew = (double2 *)dev_alloc(c, sp * c.n_pf);
+  c.action = (double *)dev_alloc(c, sizeof(double) * c.V);
+  if (!c.U64 || !c.U64_save || !c.U32 || !c.P || !c.ipdot || !c.mf || !c.smf || !c.phi0 || !c.chi || !c.phinew ||
+      !c.action) {
+    gpu_destroy(h);
+    return RHMC_EHIP;
+  }
+  c.ws = workspace_create(c);
+  if (!c.ws) { gpu_destroy(h); return RHMC_EHIP; }
+  *out = h;
+  return RHMC_OK;
+}
+
+int gpu_destroy(rhmc_gpu_t h) {
+  if (!h) return RHMC_OK;
+  Context &c = h->c;
+  hipSetDevice(c.device);
+  if (c.stream) hipStreamSynchronize(c.stream);
+  dev_free(c.U64); dev_free(c.U64_save); dev_free(c.U32); dev_free(c.P); dev_free(c.ipdot);
+  dev_free(c.mf); dev_free(c.smf); dev_free(c.phi0); dev_free(c.chi); dev_free(c.phinew); dev_free(c.action);
+  workspace_destroy(c.ws);
+  if (c.stream) hipStreamDestroy(c.stream);
+  delete h;
+  return RHMC_OK;
+}
+
+const char *gpu_last_error(rhmc_gpu_t h) { return h ? h->c.last_error.c_str() : "null handle"; }
+
+int gpu_init0(rhmc_gpu_t h, const double *links) {
+  if (!h || !links) return RHMC_EINVAL;
+  Context &c = h->c;
+  hipSetDevice(c.device);
+  if (upload_links(c, links, c.U64)) return RHMC_EHIP;
+  if (refresh_links32(c)) return RHMC_EHIP;
+  return hip_ok(c, hipStreamSynchronize(c.stream), "init0") ? RHMC_OK : RHMC_EHIP;
+}
+
+int gpu_init1(rhmc_gpu_t h, const double *phi, const double *ipdot, const double *momenta) {
+  if (!h) return RHMC_EINVAL;
+  Context &c = h->c;
+  hipSetDevice(c.device);
+  if (phi && upload_spinors(c, phi, c.mf, c.n_pf)) return RHMC_EHIP;
+  if (ipdot && upload_alg(c, ipdot, c.ipdot, 0)) return RHMC_EHIP;
+  if (momenta && upload_alg(c, momenta, c.P, 1)) return RHMC_EHIP;
+  return RHMC_OK;
+}
+
+int gpu_end(rhmc_gpu_t h, double *links, double *ipdot, double *momenta) {
+  if (!h) return RHMC_EINVAL;
+  Context &c = h->c;
+  hipSetDevice(c.device);
+  if (links && download_links(c, c.U64, links)) return RHMC_EHIP;
+  if (ipdot && download_alg(c, c.ipdot, ipdot, 0)) return RHMC_EHIP;
+  if (momenta && download_alg(c, c.P, momenta, 1)) return RHMC_EHIP;
+  return RHMC_OK;
+}
+
+static double2 *fermion_buf(Context &c, int which, int *n) {
+  switch (which) {
+    case 0: *n = c.n_pf; return c.mf;
+    case 1: *n = c.n_pf * (c.smf_order > 0 ? c.smf_order : c.max_order); return c.smf;
+    case 2: *n = c.n_pf; return c.phi0;
+    case 3: *n = c.n_pf; return c.chi;
+    case 4: *n = c.n_pf; return c.phinew;
+    default: *n = 0; return nullptr;
+  }
+}
+
+int gpu_get_fermions(rhmc_gpu_t h, int which, double *out) {
+  if (!h || !out) return RHMC_EINVAL;
+  Context &c = h->c;
+  hipSetDevice(c.device);
+  int n;
+  double2 *b = fermion_buf(c, which, &n);
+  if (!b) return fail(h, "gpu_get_fermions: bad buffer id");
+  return download_spinors(c, b, out, n) ? RHMC_EHIP : RHMC_OK;
+}
+
+int gpu_put_fermions(rhmc_gpu_t h, int which, const double *in) {
+  if (!h || !in) return RHMC_EINVAL;
+  Context &c = h->c;
+  hipSetDevice(c.device);
+  int n;
+  double2 *b = fermion_buf(c, which, &n);
+  if (!b) return fail(h, "gpu_put_fermions: bad buffer id");
+  return upload_spinors(c, in, b, n) ? RHMC_EHIP : RHMC_OK;
+}
+
+int gpu_exp_su3(rhmc_gpu_t h, double step) {
+  if (!h) return RHMC_EINVAL;
+  hipSetDevice(h->c.device);
+  if (exp_update(h->c, step)) return RHMC_EHIP;
+  return hip_ok(h->c, hipStreamSynchronize(h->c.stream), "exp") ? RHMC_OK : RHMC_EHIP;
+}
+
+int gpu_momenta_sum_multiply(rhmc_gpu_t h, double step) {
+  if (!h) return RHMC_EINVAL;
+  hipSetDevice(h->c.device);
+  if (momenta_update(h->c, step)) return RHMC_EHIP;
+  return hip_ok(h->c, hipStreamSynchronize(h->c.stream), "mom") ? RHMC_OK : RHMC_EHIP;
+}
+
+int gpu_gauge_deriv(rhmc_gpu_t h, int ipdot_init) {
+  if (!h) return RHMC_EINVAL;
+  hipSetDevice(h->c.device);
+  if (gauge_force(h->c, ipdot_init)) return RHMC_EHIP;
+  return hip_ok(h->c, hipStreamSynchronize(h->c.stream), "gauge deriv") ? RHMC_OK : RHMC_EHIP;
+}
+
+int gpu_fermion_force(rhmc_gpu_t h, int num_shifts, const double *num, int accumulate) {
+  if (!h || !num) return RHMC_EINVAL;
+  Context &c = h->c;
+  if (num_shifts != c.smf_order || num_shifts < 1) return fail(h, "gpu_fermion_force: order differs from the last solve");
+  hipSetDevice(c.device);
+  std::vector<double> coef((size_t)c.n_pf * num_shifts);
+  for (int ps = 0; ps < c.n_pf; ps++)
+    for (int k = 0; k < num_shifts; k++) coef[(size_t)ps * num_shifts + k] = num[k];
+  return fermion_force(c, c.n_pf * num_shifts, coef.data(), c.smf, accumulate) ? RHMC_EHIP : RHMC_OK;
+}
+
+int gpu_sol_sum(rhmc_gpu_t h, int num_shifts, double a0, const double *num) {
+  if (!h || !num) return RHMC_EINVAL;
+  Context &c = h->c;
+  if (num_shifts != c.smf_order) return fail(h, "gpu_sol_sum: order differs from the last solve");
+  hipSetDevice(c.device);
+  return sol_sum(c, c.mf, c.smf, num_shifts, a0, num) ? RHMC_EHIP : RHMC_OK;
+}
+
+int gpu_find_max(rhmc_gpu_t h, double *mx, double tol, unsigned long long seed) {
+  if (!h || !mx) return RHMC_EINVAL;
+  Context &c = h->c;
+  hipSetDevice(c.device);
+  if (gauss_spinor(c, c.phinew, 1, seed, 7)) return RHMC_EHIP;
+  int loops;
+  return power_max(c, c.phinew, tol, mx, &loops) ? RHMC_EHIP : RHMC_OK;
+}
+
+int gpu_find_min(rhmc_gpu_t h, double *mn, double mx, double tol, unsigned long long seed) {
+  if (!h || !mn) return RHMC_EINVAL;
+  Context &c = h->c;
+  hipSetDevice(c.device);
+  if (gauss_spinor(c, c.phinew, 1, seed, 8)) return RHMC_EHIP;
+  int loops;
+  return power_min(c, c.phinew, tol, mx, mn, &loops) ? RHMC_EHIP : RHMC_OK;
+}
+
+int gpu_action(rhmc_gpu_t h, int init, int with_f) {
+  if (!h || (init != 0 && init != -1)) return RHMC_EINVAL;
+  Context &c = h->c;
+  hipSetDevice(c.device);
+  const double2 *phi = init == 0 ? c.phi0 : c.phinew;
+  if (action_local(c, phi, c.chi, with_f, init)) return RHMC_EHIP;
+  return hip_ok(c, hipStreamSynchronize(c.stream), "action") ? RHMC_OK : RHMC_EHIP;
+}
+
+int gpu_action_sum(rhmc_gpu_t h, double *sum) {
+  if (!h || !sum) return RHMC_EINVAL;
+  hipSetDevice(h->c.device);
+  return action_sum(h->c, sum) ? RHMC_EHIP : RHMC_OK;
+}
+
+int gpu_gauss_momenta(rhmc_gpu_t h, unsigned long long seed, unsigned long long stream) {
+  if (!h) return RHMC_EINVAL;
+  hipSetDevice(h->c.device);
+  if (gauss_momenta(h->c, seed, stream)) return RHMC_EHIP;
+  return hip_ok(h->c, hipStreamSynchronize(h->c.stream), "gauss") ? RHMC_OK : RHMC_EHIP;
+}
+
+int gpu_gauss_phi(rhmc_gpu_t h, unsigned long long seed, unsigned long long stream) {
+  if (!h) return RHMC_EINVAL;
+  hipSetDevice(h->c.device);
+  if (gauss_spinor(h->c, h->c.phi0, h->c.n_pf, seed, stream)) return RHMC_EHIP;
+  return hip_ok(h->c, hipStreamSynchronize(h->c.stream), "gauss") ? RHMC_OK : RHMC_EHIP;
+}
+
+int gpu_dslash(rhmc_gpu_t h, int par_out, int precision, int nvec, const double *in, double *out) {
+  if (!h || !in || !out || nvec < 1 || (par_out != 0 && par_out != 1)) return RHMC_EINVAL;
+  Context &c = h->c;
+  hipSetDevice(c.device);
+  const long n = 3 * c.Vh * (long)nvec;
+  double2 *a = nullptr, *b = nullptr;
+  if (!hip_ok(c, hipMalloc(&a, sizeof(double2) * n), "malloc")) return RHMC_EHIP;
+  if (!hip_ok(c, hipMalloc(&b, sizeof(double2) * n), "malloc")) { hipFree(a); return RHMC_EHIP; }
+  int rc = RHMC_OK;
+  if (upload_spinors(c, in, a, nvec)) rc = RHMC_EHIP;
+  if (rc == RHMC_OK) {
+    if (precision == 0) {
+      launch_hop_d(c, par_out, a, b, nvec, 3 * c.Vh, 3 * c.Vh);
+    } else {
+      float2 *fa = nullptr, *fb = nullptr;
+      hipMalloc(&fa, sizeof(float2) * n);
+      hipMalloc(&fb, sizeof(float2) * n);
+      const unsigned nb = (unsigned)((n + 255) / 256);
+      hipLaunchKernelGGL(k_d2f, dim3(nb), dim3(256), 0, c.stream, fa, a, n);
+      hop_s12(c, par_out, fa, fb, nvec);
+      hipLaunchKernelGGL(k_f2d, dim3(nb), dim3(256), 0, c.stream, b, fb, n);
+      hipStreamSynchronize(c.stream);
+      hipFree(fa);
+      hipFree(fb);
+    }
+    if (download_spinors(c, b, out, nvec)) rc = RHMC_EHIP;
+  }
+  hipFree(a);
+  hipFree(b);
+  return rc;
+}
+
+int gpu_time_dslash(rhmc_gpu_t h, int precision, int nvec, int reps, double *ms) {
+  if (!h || !ms || nvec < 1 || reps < 1) return RHMC_EINVAL;
+  Context &c = h->c;
+  hipSetDevice(c.device);
+  const long n = 3 * c.Vh * (long)nvec;
+  const size_t es = precision == 0 ? sizeof(double2) : sizeof(float2);
+  void *a = nullptr, *b = nullptr;
+  if (!hip_ok(c, hipMalloc(&a, es * n), "malloc")) return RHMC_EHIP;
+  if (!hip_ok(c, hipMalloc(&b, es * n), "malloc")) { hipFree(a); return RHMC_EHIP; }
+  hipMemsetAsync(a, 0, es * n, c.stream);
+  hipEvent_t e0, e1;
+  hipEventCreate(&e0);
+  hipEventCreate(&e1);
+  auto launch = [&](int k) {
+    const int par = k & 1;
+    if (precision == 0)
+      launch_hop_d(c, par, (const double2 *)((k & 1) ? b : a), (double2 *)((k & 1) ? a : b), nvec, 3 * c.Vh, 3 * c.Vh);
+    else
+      hop_s12(c, par, (const float2 *)((k & 1) ? b : a), (float2 *)((k & 1) ? a : b), nvec);
+  };
+  for (int k = 0; k < 3; k++) launch(k);
+  hipEventRecord(e0, c.stream);
+  for (int k = 0; k < reps; k++) launch(k);
+  hipEventRecord(e1, c.stream);
+  hipEventSynchronize(e1);
+  float t = 0.f;
+  hipEventElapsedTime(&t, e0, e1);
+  *ms = (double)t / reps;
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  hipFree(a);
+  hipFree(b);
+  return hip_ok(c, hipGetLastError(), "time dslash") ? RHMC_OK : RHMC_EHIP;
+}
+
+int cu_multips_shifted_invert(rhmc_gpu_t h, double res, int relative, int order, const double *shifts, int precision,
+                              int max_cg, int *iterations) {
+  if (!h || !shifts || order < 1 || order > h->c.max_order) return RHMC_EINVAL;
+  Context &c = h->c;
+  hipSetDevice(c.device);
+  int it = 0;
+  int rc = multishift_solve(c, c.mf, c.n_pf, order, shifts, res, relative, precision, max_cg, c.smf, &it);
+  c.smf_order = order;
+  if (iterations) *iterations = it;
+  return rc == 0 ? RHMC_OK : (rc == 2 ? RHMC_EMAXCG : RHMC_EHIP);
+}
+
+int cuda_shifted_inverter(rhmc_gpu_t h, double res, const double *shifts, int num_shifts, int psferm, int precision,
+                          int *ncount) {
+  if (!h || !shifts || num_shifts < 1 || num_shifts > h->c.max_order || psferm < 0 || psferm >= h->c.n_pf)
+    return RHMC_EINVAL;
+  Context &c = h->c;
+  hipSetDevice(c.device);
+  int it = 0;
+  const long sp = 3 * c.Vh;
+  int rc = multishift_solve(c, c.mf + psferm * sp, 1, num_shifts, shifts, res, 0, precision, 100000,
+                            c.smf + (long)psferm * num_shifts * sp, &it);
+  c.smf_order = num_shifts;
+  if (ncount) *ncount = it;
+  return rc == 0 ? RHMC_OK : (rc == 2 ? RHMC_EMAXCG : RHMC_EHIP);
+}
+
+int cu_shifted_residuals(rhmc_gpu_t h, int order, const double *shifts, double *out) {
+  if (!h || !shifts || !out || order != h->c.smf_order) return RHMC_EINVAL;
+  Context &c = h->c;
+  hipSetDevice(c.device);
+  const int nsys = c.n_pf * order;
+  std::vector<double> sh(nsys), n2(nsys);
+  for (int j = 0; j < nsys; j++) sh[j] = shifts[j % order];
+  double2 *r = nullptr;
+  if (!hip_ok(c, hipMalloc(&r, sizeof(double2) * 3 * c.Vh * nsys), "malloc")) return RHMC_EHIP;
+  int rc = residual_d(c, c.mf, order, c.smf, sh.data(), nsys, r, n2.data());
+  hipFree(r);
+  if (rc) return RHMC_EHIP;
+  for (int j = 0; j < nsys; j++) out[j] = sqrt(n2[j]);
+  return RHMC_OK;
+}
+
+int cu_last_solve_stats(rhmc_gpu_t h, int *i32, int *iref, int *cyc) {
+  if (!h) return RHMC_EINVAL;
+  if (i32) *i32 = h->c.last_iters_fp32;
+  if (iref) *iref = h->c.last_iters_refine;
+  if (cyc) *cyc = h->c.last_refine_cycles;
+  return RHMC_OK;
+}
+
+int gpu_trajectory(rhmc_gpu_t h, const rhmc_traj_params *p, rhmc_traj_result *r) {
+  if (!h || !p || !r) return RHMC_EINVAL;
+  hipSetDevice(h->c.device);
+  return run_trajectory(h->c, *p, *r);
+}
+
+}  // extern "C"

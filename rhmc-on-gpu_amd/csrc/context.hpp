@@ -1,0 +1,111 @@
+// context.hpp -- device-resident state of one lattice and the internal
+// launch API shared by the HIP translation units and the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "lattice.hpp"
+
+namespace rhmc {
+
+constexpr int kMaxShifts = 32;   // poles per rational function
+constexpr int kBlock = 256;      // threads per workgroup (4 wave64)
+
+enum Prec { PREC_FP64 = 0, PREC_MIXED = 1 };
+
+// per right-hand side state of the multi-shift CG (FP64 scalars on device)
+struct MsState {
+  double delta, omega, omega_prev, gammag, res;
+  int iter, max_iter, done, stop, nshift, nact;
+  double shift[kMaxShifts], z1[kMaxShifts], z2[kMaxShifts], z3[kMaxShifts];
+  double om[kMaxShifts], ga[kMaxShifts];
+  int flag[kMaxShifts], flag_next[kMaxShifts], its[kMaxShifts];
+};
+
+// per system state of the batched single-shift CG used for FP64 refinement
+struct CgState {
+  double delta, alpha, beta, tol2, shift;
+  int iter, max_iter, done, stop;
+};
+
+struct Workspace;
+
+struct Context {
+  LatticeDims L;
+  long V = 0, Vh = 0;
+  double mass = 0, beta = 0;
+  int n_pf = 1, max_order = kMaxShifts;
+  int device = 0;
+  hipStream_t stream = nullptr;
+
+  // gauge field (eta folded in)
+  double2 *U64 = nullptr;       // FP64 18-real
+  float4 *U32 = nullptr;        // FP32 12-real copy, refreshed with U64
+  double2 *U64_save = nullptr;  // Metropolis backup
+  // algebra fields (8 reals/link)
+  double2 *P = nullptr;      // momenta (hermitian)
+  double2 *ipdot = nullptr;  // force (anti-hermitian)
+  // fermions (FP64, even sites): mf = right-hand sides, smf = shifted solutions
+  double2 *mf = nullptr;     // [n_pf][3][Vh]
+  double2 *smf = nullptr;    // [n_pf][max_order][3][Vh]
+  double2 *phi0 = nullptr;   // heatbath pseudofermion
+  double2 *chi = nullptr;    // chi = r_first(MdagM) phi0
+  double2 *phinew = nullptr; // r_last(MdagM) chi at the end of the MD
+  double *action = nullptr;  // [V] per-site action differences
+  int smf_order = 0;         // poles held in smf
+
+  Workspace *ws = nullptr;
+  std::string last_error;
+
+  // statistics of the last solve
+  int last_iters = 0, last_refine_cycles = 0, last_iters_fp32 = 0, last_iters_refine = 0;
+  double last_true_res_max = 0.0;
+};
+
+// ------------------------------------------------------------------ launches
+// all launches are enqueued on ctx.stream
+void fill_dims(Context &c, int nx, int ny, int nz, int nt, int antiperiodic);
+
+// Dslash family (solver.hip)
+// out[j] = 0.5 * hop(in[j]) to parity par_out, FP64 18-real links
+void launch_hop_d(Context &c, int par_out, const double2 *in, double2 *out, int nvec, long vstride_in,
+                  long vstride_out);
+// r = b - ((m^2+shift_j) p_j - D_eo D_oe p_j), norms2[j]; b index = j / b_group
+int residual_d(Context &c, const double2 *b, int b_group, const double2 *x, const double *shifts, int nvec,
+               double2 *r, double *norms2);
+
+// multi-shift CG.  b: [nrhs][3][Vh] FP64; x: [nrhs][nshift][3][Vh] FP64 out.
+// prec PREC_FP64 or PREC_MIXED (FP32 iterations + FP64 refinement).
+// rel=0: res is an absolute residual (reference convention);
+// rel=1: res is relative to |b| per right-hand side.
+int multishift_solve(Context &c, const double2 *b, int nrhs, int nshift, const double *shifts, double res,
+                     int rel, int prec, int max_iter, double2 *x, int *iters);
+
+// spectral bounds of MdagM by power iterations (findminmax.cc:12-137)
+int power_max(Context &c, const double2 *start, double tol, double *out, int *loops);
+int power_min(Context &c, const double2 *start, double tol, double mx, double *out, int *loops);
+
+// fields.hip
+int fermion_force(Context &c, int nsys_per_pf, const double *num, const double2 *x, int accumulate);
+int gauge_force(Context &c, int accumulate);
+int exp_update(Context &c, double step);
+int momenta_update(Context &c, double step);
+int sol_sum(Context &c, double2 *io, const double2 *x, int nshift, double a0, const double *num);
+int action_local(Context &c, const double2 *phi, const double2 *chi, int with_fermions, int init);
+int action_sum(Context &c, double *out);
+int gauss_momenta(Context &c, unsigned long long seed, unsigned long long stream_id);
+int gauss_spinor(Context &c, double2 *out, int n, unsigned long long seed, unsigned long long stream_id);
+int refresh_links32(Context &c);
+int vec_norms2(Context &c, const double2 *v, int nvec, long vstride, double *out);
+
+// host helpers
+bool hip_ok(Context &c, hipError_t e, const char *what);
+void *dev_alloc(Context &c, size_t bytes);
+void dev_free(void *p);
+Workspace *workspace_create(Context &c);
+void workspace_destroy(Workspace *w);
+
+}  // namespace rhmc

@@ -1,0 +1,556 @@
+// fields.hip -- gauge-field side of the RHMC molecular dynamics on gfx950:
+// fermion force, gauge force, exp(i eps P) link update, momentum update,
+// per-site action / Delta H, rational-function sums, RNG and the host-layout
+// conversions of the C ABI.  All arithmetic here is FP64.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdio>
+#include <vector>
+
+#include "context.hpp"
+
+namespace rhmc {
+
+static inline unsigned nblk(long n) { return (unsigned)((n + kBlock - 1) / kBlock); }
+
+__device__ __forceinline__ Mat<double> link_at(const double2 *U, const LatticeDims &L, int x, int y, int z, int t,
+                                               int mu) {
+  const int par = (x + y + z + t) & 1;
+  return load_link_d18(U, L.Vh, mu, par, half_index(L, x, y, z, t));
+}
+
+__device__ __forceinline__ void shift_coord(const LatticeDims &L, int &x, int &y, int &z, int &t, int mu, int d) {
+  if (mu == 0) x = (x + d + L.nx) % L.nx;
+  else if (mu == 1) y = (y + d + L.ny) % L.ny;
+  else if (mu == 2) z = (z + d + L.nz) % L.nz;
+  else t = (t + d + L.nt) % L.nt;
+}
+
+// --------------------------------------------------------------- fermion force
+// fermionforce.cc:46-104 + ipdot.cc:83-155:
+//   F_mu(e) =  sum_j a_j h_j(e+mu) (x) s_j(e)^*     (even sites)
+//   F_mu(o) = -sum_j a_j s_j(o+mu) (x) h_j(o)^*     (odd sites)
+//   ipdot_mu(x) = TA(U_mu(x) F_mu(x))   [+= existing ipdot if accumulate]
+// s_j: shifted solutions (even), h_j = D_oe s_j (odd); one thread per site
+// holds the four F_mu in registers and reads each spinor of its site once.
+__global__ __launch_bounds__(kBlock) void k_fermion_force(LatticeDims L, const double2 *U, const double2 *s,
+                                                          const double2 *hv, const double *coef, int nsys,
+                                                          double2 *ipdot, int accumulate) {
+  const long Vh = L.Vh;
+  const int bx = xcd_block(blockIdx.x, gridDim.x);
+  const long h = (long)bx * kBlock + threadIdx.x;
+  const int par = blockIdx.y;
+  if (h >= Vh) return;
+  const SiteCoord c = site_coords(L, par, h);
+  long nb[4];
+#pragma unroll
+  for (int mu = 0; mu < 4; mu++) nb[mu] = neighbour(L, c, mu, +1);
+  Mat<double> F[4];
+#pragma unroll
+  for (int mu = 0; mu < 4; mu++)
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+      for (int k = 0; k < 3; k++) F[mu].e[i][k] = make_double2(0.0, 0.0);
+  const double2 *own_v = par == 0 ? s : hv;
+  const double2 *nb_v = par == 0 ? hv : s;
+  const double sgn = par == 0 ? 1.0 : -1.0;
+  for (int j = 0; j < nsys; j++) {
+    const double a = sgn * coef[j];
+    const long base = (long)j * 3 * Vh;
+    double2 own[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      own[k] = own_v[base + (long)k * Vh + h];
+      own[k].y = -own[k].y;  // conjugate
+    }
+#pragma unroll
+    for (int mu = 0; mu < 4; mu++) {
+#pragma unroll
+      for (int i = 0; i < 3; i++) {
+        double2 v = nb_v[base + (long)i * Vh + nb[mu]];
+        v.x *= a;
+        v.y *= a;
+#pragma unroll
+        for (int k = 0; k < 3; k++) cfma(F[mu].e[i][k], v, own[k]);
+      }
+    }
+  }
+#pragma unroll
+  for (int mu = 0; mu < 4; mu++) {
+    const Mat<double> Um = load_link_d18(U, Vh, mu, par, h);
+    Mat<double> T = ta(mmul(Um, F[mu]));
+    if (accumulate) {
+      const Mat<double> old = load_aherm(ipdot, Vh, mu, par, h);
+#pragma unroll
+      for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int k = 0; k < 3; k++) T.e[i][k] = cadd(T.e[i][k], old.e[i][k]);
+    }
+    store_aherm(ipdot, Vh, mu, par, h, T);
+  }
+}
+
+struct ForceBuf {
+  double2 *h = nullptr;
+  size_t cap = 0;
+  double *coef = nullptr;
+};
+static ForceBuf g_force_dummy;
+
+int fermion_force(Context &c, int nsys, const double *num, const double2 *x, int accumulate) {
+  // nsys = n_pf * order systems, x = [n_pf][order][3][Vh]; num has nsys entries
+  static thread_local std::vector<double> hcoef;
+  const long Vh = c.Vh;
+  size_t need = sizeof(double2) * 3 * Vh * (size_t)nsys;
+  // h_j = D_oe s_j for all systems (one multi-RHS hop sharing link loads)
+  double2 *hv = nullptr;
+  double *coef = nullptr;
+  if (!hip_ok(c, hipMallocAsync((void **)&hv, need, c.stream), "force buffer")) return -1;
+  if (!hip_ok(c, hipMallocAsync((void **)&coef, sizeof(double) * nsys, c.stream), "force coef")) return -1;
+  hcoef.assign(num, num + nsys);
+  hipMemcpyAsync(coef, hcoef.data(), sizeof(double) * nsys, hipMemcpyHostToDevice, c.stream);
+  launch_hop_d(c, 1, x, hv, nsys, 3 * Vh, 3 * Vh);
+  hipLaunchKernelGGL(k_fermion_force, dim3(nblk(Vh), 2), dim3(kBlock), 0, c.stream, c.L, c.U64, x, hv, coef, nsys,
+                     c.ipdot, accumulate);
+  hipFreeAsync(hv, c.stream);
+  hipFreeAsync(coef, c.stream);
+  return hip_ok(c, hipStreamSynchronize(c.stream), "fermion force") ? 0 : -1;
+}
+
+// ----------------------------------------------------------------- gauge force
+// staples.cc:16-74, ipdot.cc:25-79:
+//   ipdot_mu(x) = TA(U_mu(x) beta/3 sum_{nu!=mu} [U_nu(x+mu) U_mu(x+nu)^+ U_nu(x)^+
+//                                              + U_nu(x+mu-nu)^+ U_mu(x-nu)^+ U_nu(x-nu)])
+__global__ __launch_bounds__(kBlock) void k_gauge_force(LatticeDims L, const double2 *U, double b3, double2 *ipdot,
+                                                        int accumulate) {
+  const long Vh = L.Vh;
+  const int bx = xcd_block(blockIdx.x, gridDim.x);
+  const long h = (long)bx * kBlock + threadIdx.x;
+  const int par = blockIdx.y;
+  const int mu = blockIdx.z;
+  if (h >= Vh) return;
+  const SiteCoord c = site_coords(L, par, h);
+  Mat<double> st;
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int k = 0; k < 3; k++) st.e[i][k] = make_double2(0.0, 0.0);
+  for (int nu = 0; nu < 4; nu++) {
+    if (nu == mu) continue;
+    int x = c.x, y = c.y, z = c.z, t = c.t;
+    // upper: U_nu(x+mu) U_mu(x+nu)^+ U_nu(x)^+
+    int x1 = x, y1 = y, z1 = z, t1 = t;
+    shift_coord(L, x1, y1, z1, t1, mu, +1);
+    int x2 = x, y2 = y, z2 = z, t2 = t;
+    shift_coord(L, x2, y2, z2, t2, nu, +1);
+    Mat<double> a = mmul_nd(link_at(U, L, x1, y1, z1, t1, nu), link_at(U, L, x2, y2, z2, t2, mu));
+    a = mmul_nd(a, link_at(U, L, x, y, z, t, nu));
+    // lower: U_nu(x+mu-nu)^+ U_mu(x-nu)^+ U_nu(x-nu)
+    int x3 = x, y3 = y, z3 = z, t3 = t;
+    shift_coord(L, x3, y3, z3, t3, nu, -1);
+    int x4 = x3, y4 = y3, z4 = z3, t4 = t3;
+    shift_coord(L, x4, y4, z4, t4, mu, +1);
+    const Mat<double> l1 = link_at(U, L, x4, y4, z4, t4, nu);
+    const Mat<double> l2 = link_at(U, L, x3, y3, z3, t3, mu);
+    Mat<double> b = mmul_dn(l1, mdag(l2));
+    b = mmul(b, link_at(U, L, x3, y3, z3, t3, nu));
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+      for (int k = 0; k < 3; k++) st.e[i][k] = cadd(st.e[i][k], cadd(a.e[i][k], b.e[i][k]));
+  }
+  const Mat<double> Um = load_link_d18(U, Vh, mu, par, h);
+  Mat<double> T = mmul(Um, st);
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int k = 0; k < 3; k++) T.e[i][k] = cscale(T.e[i][k], b3);
+  T = ta(T);
+  if (accumulate) {
+    const Mat<double> old = load_aherm(ipdot, Vh, mu, par, h);
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+      for (int k = 0; k < 3; k++) T.e[i][k] = cadd(T.e[i][k], old.e[i][k]);
+  }
+  store_aherm(ipdot, Vh, mu, par, h, T);
+}
+
+int gauge_force(Context &c, int accumulate) {
+  hipLaunchKernelGGL(k_gauge_force, dim3(nblk(c.Vh), 2, 4), dim3(kBlock), 0, c.stream, c.L, c.U64, c.beta / 3.0,
+                     c.ipdot, accumulate);
+  return hip_ok(c, hipGetLastError(), "gauge force") ? 0 : -1;
+}
+
+// ------------------------------------------------------------ link update
+// momenta.cc:42-78 + su3.cc:187-219 + conf.cc:235-251:
+//   U <- Pi_SU3[ exp6(i step P) ] U, then Gram-Schmidt and row 2 *= eta.
+// Also refreshes the FP32 12-real copy used by the mixed-precision Dslash.
+__global__ __launch_bounds__(kBlock) void k_exp_update(LatticeDims L, const double2 *P, double step, double2 *U,
+                                                       float4 *U32) {
+  const long Vh = L.Vh;
+  const int bx = xcd_block(blockIdx.x, gridDim.x);
+  const long h = (long)bx * kBlock + threadIdx.x;
+  const int par = blockIdx.y, mu = blockIdx.z;
+  if (h >= Vh) return;
+  const Mat<double> Pm = load_herm(P, Vh, mu, par, h);
+  Mat<double> X;
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int k = 0; k < 3; k++) X.e[i][k] = make_double2(-step * Pm.e[i][k].y, step * Pm.e[i][k].x);
+  Mat<double> E, A = X;
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int k = 0; k < 3; k++) E.e[i][k] = make_double2((i == k) ? 1.0 : 0.0, 0.0);
+  const double cf[6] = {1.0, 0.5, 1.0 / 3.0, 0.25, 0.2, 1.0 / 6.0};
+#pragma unroll
+  for (int n = 0; n < 6; n++) {
+    if (n > 0) {
+      A = mmul(A, X);
+#pragma unroll
+      for (int i = 0; i < 3; i++)
+#pragma unroll
+        for (int k = 0; k < 3; k++) A.e[i][k] = cscale(A.e[i][k], cf[n]);
+    }
+#pragma unroll
+    for (int i = 0; i < 3; i++)
+#pragma unroll
+      for (int k = 0; k < 3; k++) E.e[i][k] = cadd(E.e[i][k], A.e[i][k]);
+  }
+  sunitarize(E);
+  Mat<double> Un = mmul(E, load_link_d18(U, Vh, mu, par, h));
+  sunitarize(Un);
+  const SiteCoord c = site_coords(L, par, h);
+  const double eta = eta_phase(L, c.x, c.y, c.z, c.t, mu);
+#pragma unroll
+  for (int k = 0; k < 3; k++) Un.e[2][k] = cscale(Un.e[2][k], eta);
+  store_link_d18(U, Vh, mu, par, h, Un);
+  store_link_s12(U32, Vh, mu, par, h, Un);
+}
+
+int exp_update(Context &c, double step) {
+  hipLaunchKernelGGL(k_exp_update, dim3(nblk(c.Vh), 2, 4), dim3(kBlock), 0, c.stream, c.L, c.P, step, c.U64, c.U32);
+  return hip_ok(c, hipGetLastError(), "exp update") ? 0 : -1;
+}
+
+__global__ __launch_bounds__(kBlock) void k_refresh32(LatticeDims L, const double2 *U, float4 *U32) {
+  const long Vh = L.Vh;
+  const long h = (long)blockIdx.x * kBlock + threadIdx.x;
+  const int par = blockIdx.y, mu = blockIdx.z;
+  if (h >= Vh) return;
+  store_link_s12(U32, Vh, mu, par, h, load_link_d18(U, Vh, mu, par, h));
+}
+
+int refresh_links32(Context &c) {
+  hipLaunchKernelGGL(k_refresh32, dim3(nblk(c.Vh), 2, 4), dim3(kBlock), 0, c.stream, c.L, c.U64, c.U32);
+  return hip_ok(c, hipGetLastError(), "refresh links32") ? 0 : -1;
+}
+
+// -------------------------------------------------------- momentum update
+// momenta.cc:82-115: P <- P + (i step) ipdot on the packed 8-real fields
+__global__ __launch_bounds__(kBlock) void k_mom_update(double2 *P, const double2 *A, double s, long Vh, long n) {
+  const long i = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const int slot = (int)((i / Vh) & 3);
+  double2 p = P[i];
+  const double2 a = A[i];
+  if (slot < 3) {
+    p.x -= s * a.y;
+    p.y += s * a.x;
+  } else {
+    p.x -= s * a.x;
+    p.y -= s * a.y;
+  }
+  P[i] = p;
+}
+
+int momenta_update(Context &c, double step) {
+  const long n = 32 * c.Vh;
+  hipLaunchKernelGGL(k_mom_update, dim3(nblk(n)), dim3(kBlock), 0, c.stream, c.P, c.ipdot, step, c.Vh, n);
+  return hip_ok(c, hipGetLastError(), "momenta update") ? 0 : -1;
+}
+
+// ------------------------------------------------------------ solution sums
+// rationalapprox.cc:269-369 / cuda_sol_sum: io[ps] = a0 io[ps] + sum_k a_k x[ps][k]
+__global__ __launch_bounds__(kBlock) void k_sol_sum(double2 *io, const double2 *x, int nshift, double a0,
+                                                    const double *num, long n3) {
+  const long i = (long)blockIdx.x * kBlock + threadIdx.x;
+  const int ps = blockIdx.y;
+  if (i >= n3) return;
+  double2 v = io[(long)ps * n3 + i];
+  v.x *= a0;
+  v.y *= a0;
+  for (int k = 0; k < nshift; k++) {
+    const double2 xv = x[((long)ps * nshift + k) * n3 + i];
+    v.x += num[k] * xv.x;
+    v.y += num[k] * xv.y;
+  }
+  io[(long)ps * n3 + i] = v;
+}
+
+int sol_sum(Context &c, double2 *io, const double2 *x, int nshift, double a0, const double *num) {
+  double *d = nullptr;
+  if (!hip_ok(c, hipMallocAsync((void **)&d, sizeof(double) * nshift, c.stream), "sol_sum")) return -1;
+  std::vector<double> h(num, num + nshift);
+  hipMemcpyAsync(d, h.data(), sizeof(double) * nshift, hipMemcpyHostToDevice, c.stream);
+  const long n3 = 3 * c.Vh;
+  hipLaunchKernelGGL(k_sol_sum, dim3(nblk(n3), c.n_pf), dim3(kBlock), 0, c.stream, io, x, nshift, a0, d, n3);
+  hipFreeAsync(d, c.stream);
+  return hip_ok(c, hipStreamSynchronize(c.stream), "sol_sum") ? 0 : -1;
+}
+
+// ------------------------------------------------------------------- action
+// action.cc:6-156: per-site contributions in the reference's order
+//   init 0 : v  = sum_plaq beta/3 Re tr(plaq) + sum_mu 1/2 Re tr P^2 + sum_ps |phi|^2
+//   init -1: v -= the same with Re<phi_new, chi> for the fermions
+__global__ __launch_bounds__(kBlock) void k_action(LatticeDims L, const double2 *U, const double2 *P, double b3,
+                                                   const double2 *phi, const double2 *chi, int npf, int with_f,
+                                                   double *value, int init) {
+  const long Vh = L.Vh;
+  const long h = (long)blockIdx.x * kBlock + threadIdx.x;
+  const int par = blockIdx.y;
+  if (h >= Vh) return;
+  const long idx = (long)par * Vh + h;
+  const SiteCoord c = site_coords(L, par, h);
+  double v = init == 0 ? 0.0 : value[idx];
+  const double sg = init == 0 ? 1.0 : -1.0;
+  for (int mu = 0; mu < 3; mu++)
+    for (int nu = mu + 1; nu < 4; nu++) {
+      int x1 = c.x, y1 = c.y, z1 = c.z, t1 = c.t;
+      shift_coord(L, x1, y1, z1, t1, mu, +1);
+      int x2 = c.x, y2 = c.y, z2 = c.z, t2 = c.t;
+      shift_coord(L, x2, y2, z2, t2, nu, +1);
+      Mat<double> a = mmul(load_link_d18(U, Vh, mu, par, h), link_at(U, L, x1, y1, z1, t1, nu));
+      a = mmul_nd(a, link_at(U, L, x2, y2, z2, t2, mu));
+      a = mmul_nd(a, load_link_d18(U, Vh, nu, par, h));
+      const double p = b3 * (a.e[0][0].x + a.e[1][1].x + a.e[2][2].x);
+      v += sg * p;
+    }
+  for (int mu = 0; mu < 4; mu++) {
+    const Mat<double> pm = load_herm(P, Vh, mu, par, h);
+    const Mat<double> q = mmul(pm, pm);
+    v += sg * 0.5 * (q.e[0][0].x + q.e[1][1].x + q.e[2][2].x);
+  }
+  if (with_f && par == 0) {
+    for (int ps = 0; ps < npf; ps++) {
+      double f = 0.0;
+      for (int k = 0; k < 3; k++) {
+        const double2 a = phi[((long)ps * 3 + k) * Vh + h];
+        const double2 b = init == 0 ? a : chi[((long)ps * 3 + k) * Vh + h];
+        f += a.x * b.x + a.y * b.y;
+      }
+      v += sg * f;
+    }
+  }
+  value[idx] = v;
+}
+
+int action_local(Context &c, const double2 *phi, const double2 *chi, int with_f, int init) {
+  hipLaunchKernelGGL(k_action, dim3(nblk(c.Vh), 2), dim3(kBlock), 0, c.stream, c.L, c.U64, c.P, c.beta / 3.0, phi,
+                     chi, c.n_pf, with_f, c.action, init);
+  return hip_ok(c, hipGetLastError(), "action") ? 0 : -1;
+}
+
+// include/tools.h:25-57 pairwise fold, same association order as the reference
+__global__ __launch_bounds__(kBlock) void k_fold(double *v, long j, long off) {
+  const long k = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (k < j) v[k] += v[k + off];
+}
+
+int action_sum(Context &c, double *out) {
+  long n = c.V;
+  while (n > 1) {
+    long j, off, next;
+    if (n % 2 == 0) { j = n / 2; off = j; next = j; }
+    else { j = n / 2; off = j + 1; next = j + 1; }
+    hipLaunchKernelGGL(k_fold, dim3(nblk(j)), dim3(kBlock), 0, c.stream, c.action, j, off);
+    n = next;
+  }
+  hipMemcpyAsync(out, c.action, sizeof(double), hipMemcpyDeviceToHost, c.stream);
+  return hip_ok(c, hipStreamSynchronize(c.stream), "action sum") ? 0 : -1;
+}
+
+// --------------------------------------------------------------------- RNG
+// Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11) counter-based generator;
+// Box-Muller with radius sqrt(-ln u) gives N(0,1/2) components (gauss.cc:10-35).
+__device__ __forceinline__ uint4 philox4x32(uint4 ctr, uint2 key) {
+  const unsigned M0 = 0xD2511F53u, M1 = 0xCD9E8D57u, W0 = 0x9E3779B9u, W1 = 0xBB67AE85u;
+#pragma unroll
+  for (int r = 0; r < 10; r++) {
+    const unsigned long long p0 = (unsigned long long)M0 * ctr.x;
+    const unsigned long long p1 = (unsigned long long)M1 * ctr.z;
+    const unsigned hi0 = (unsigned)(p0 >> 32), lo0 = (unsigned)p0;
+    const unsigned hi1 = (unsigned)(p1 >> 32), lo1 = (unsigned)p1;
+    ctr = make_uint4(hi1 ^ ctr.y ^ key.x, lo1, hi0 ^ ctr.w ^ key.y, lo0);
+    key.x += W0;
+    key.y += W1;
+  }
+  return ctr;
+}
+__device__ __forceinline__ double u53(unsigned a, unsigned b) {
+  const unsigned long long v = ((unsigned long long)a << 21) ^ (unsigned long long)b;
+  return ((double)(v & ((1ull << 53) - 1)) + 0.5) * (1.0 / 9007199254740992.0);
+}
+__device__ __forceinline__ double2 gauss_pair(unsigned long long seed, unsigned long long stream, unsigned long long idx,
+                                              unsigned k) {
+  const uint4 r = philox4x32(make_uint4((unsigned)idx, (unsigned)(idx >> 32), k, (unsigned)stream),
+                             make_uint2((unsigned)seed, (unsigned)(seed >> 32)));
+  const double u1 = u53(r.x, r.y), u2 = u53(r.z, r.w);
+  const double rad = sqrt(-log(u1));
+  double sn, cs;
+  sincos(6.283185307179586476925286766559 * u2, &sn, &cs);
+  return make_double2(rad * cs, rad * sn);
+}
+
+// gauss.cc:63-86 layout of the gaussian su(3) matrix
+__global__ __launch_bounds__(kBlock) void k_gauss_mom(double2 *P, long Vh, unsigned long long seed,
+                                                      unsigned long long stream) {
+  const long h = (long)blockIdx.x * kBlock + threadIdx.x;
+  const int par = blockIdx.y, mu = blockIdx.z;
+  if (h >= Vh) return;
+  const unsigned long long link = ((unsigned long long)mu * 2 + par) * (unsigned long long)Vh + h;
+  double r[8];
+  for (int k = 0; k < 4; k++) {
+    const double2 g = gauss_pair(seed, stream, link, k);
+    r[2 * k] = g.x;
+    r[2 * k + 1] = g.y;
+  }
+  const double s3 = 0.57735026918962576450914878050196;
+  double2 *b = P + (long)(mu * 2 + par) * 4 * Vh + h;
+  b[0] = make_double2(r[0], -r[1]);
+  b[Vh] = make_double2(r[3], -r[4]);
+  b[2 * Vh] = make_double2(r[5], -r[6]);
+  b[3 * Vh] = make_double2(r[2] + s3 * r[7], -r[2] + s3 * r[7]);
+}
+
+int gauss_momenta(Context &c, unsigned long long seed, unsigned long long stream) {
+  hipLaunchKernelGGL(k_gauss_mom, dim3(nblk(c.Vh), 2, 4), dim3(kBlock), 0, c.stream, c.P, c.Vh, seed, stream);
+  return hip_ok(c, hipGetLastError(), "gauss momenta") ? 0 : -1;
+}
+
+__global__ __launch_bounds__(kBlock) void k_gauss_spinor(double2 *out, long Vh, unsigned long long seed,
+                                                         unsigned long long stream) {
+  const long h = (long)blockIdx.x * kBlock + threadIdx.x;
+  const int ps = blockIdx.y;
+  if (h >= Vh) return;
+  for (int k = 0; k < 3; k++)
+    out[((long)ps * 3 + k) * Vh + h] = gauss_pair(seed, stream, (unsigned long long)ps * Vh + h, k);
+}
+
+int gauss_spinor(Context &c, double2 *out, int n, unsigned long long seed, unsigned long long stream) {
+  hipLaunchKernelGGL(k_gauss_spinor, dim3(nblk(c.Vh), n), dim3(kBlock), 0, c.stream, out, c.Vh, seed, stream);
+  return hip_ok(c, hipGetLastError(), "gauss spinor") ? 0 : -1;
+}
+
+// ----------------------------------------------------- host-layout transfers
+// host links/algebra: [4V][3][3] complex at mu*V + par*Vh + h  ==  [(mu*2+par)*Vh + h]
+__global__ __launch_bounds__(kBlock) void k_links_in(const double2 *aos, double2 *U, long Vh) {
+  const long i = (long)blockIdx.x * kBlock + threadIdx.x;  // over 8*Vh links
+  if (i >= 8 * Vh) return;
+  const long blk = i / Vh, h = i - blk * Vh;
+#pragma unroll
+  for (int k = 0; k < 9; k++) U[(blk * 9 + k) * Vh + h] = aos[i * 9 + k];
+}
+__global__ __launch_bounds__(kBlock) void k_links_out(const double2 *U, double2 *aos, long Vh) {
+  const long i = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= 8 * Vh) return;
+  const long blk = i / Vh, h = i - blk * Vh;
+#pragma unroll
+  for (int k = 0; k < 9; k++) aos[i * 9 + k] = U[(blk * 9 + k) * Vh + h];
+}
+// algebra fields: full matrix -> 8-real packing (herm: real diag, aherm: imag diag)
+__global__ __launch_bounds__(kBlock) void k_alg_in(const double2 *aos, double2 *A, long Vh, int herm) {
+  const long i = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= 8 * Vh) return;
+  const long blk = i / Vh, h = i - blk * Vh;
+  const double2 *m = aos + i * 9;
+  double2 *b = A + blk * 4 * Vh + h;
+  b[0] = m[1];
+  b[Vh] = m[2];
+  b[2 * Vh] = m[5];
+  b[3 * Vh] = herm ? make_double2(m[0].x, m[4].x) : make_double2(m[0].y, m[4].y);
+}
+__global__ __launch_bounds__(kBlock) void k_alg_out(const double2 *A, double2 *aos, long Vh, int herm) {
+  const long i = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= 8 * Vh) return;
+  const long blk = i / Vh, h = i - blk * Vh;
+  const int mu = (int)(blk / 2), par = (int)(blk % 2);
+  const Mat<double> m = herm ? load_herm(A, Vh, mu, par, h) : load_aherm(A, Vh, mu, par, h);
+  double2 *o = aos + i * 9;
+  for (int r = 0; r < 3; r++)
+    for (int k = 0; k < 3; k++) o[r * 3 + k] = m.e[r][k];
+}
+// spinors: host [n][Vh][3] <-> device [n][3][Vh]
+__global__ __launch_bounds__(kBlock) void k_spin_in(const double2 *aos, double2 *v, long Vh) {
+  const long h = (long)blockIdx.x * kBlock + threadIdx.x;
+  const int j = blockIdx.y;
+  if (h >= Vh) return;
+  for (int k = 0; k < 3; k++) v[((long)j * 3 + k) * Vh + h] = aos[((long)j * Vh + h) * 3 + k];
+}
+__global__ __launch_bounds__(kBlock) void k_spin_out(const double2 *v, double2 *aos, long Vh) {
+  const long h = (long)blockIdx.x * kBlock + threadIdx.x;
+  const int j = blockIdx.y;
+  if (h >= Vh) return;
+  for (int k = 0; k < 3; k++) aos[((long)j * Vh + h) * 3 + k] = v[((long)j * 3 + k) * Vh + h];
+}
+
+int upload_links(Context &c, const double *host, double2 *dst) {
+  const size_t bytes = sizeof(double2) * 9 * 8 * c.Vh;
+  double2 *st = nullptr;
+  if (!hip_ok(c, hipMallocAsync((void **)&st, bytes, c.stream), "stage")) return -1;
+  hipMemcpyAsync(st, host, bytes, hipMemcpyHostToDevice, c.stream);
+  hipLaunchKernelGGL(k_links_in, dim3(nblk(8 * c.Vh)), dim3(kBlock), 0, c.stream, st, dst, c.Vh);
+  hipFreeAsync(st, c.stream);
+  return hip_ok(c, hipStreamSynchronize(c.stream), "upload links") ? 0 : -1;
+}
+int download_links(Context &c, const double2 *src, double *host) {
+  const size_t bytes = sizeof(double2) * 9 * 8 * c.Vh;
+  double2 *st = nullptr;
+  if (!hip_ok(c, hipMallocAsync((void **)&st, bytes, c.stream), "stage")) return -1;
+  hipLaunchKernelGGL(k_links_out, dim3(nblk(8 * c.Vh)), dim3(kBlock), 0, c.stream, src, st, c.Vh);
+  hipMemcpyAsync(host, st, bytes, hipMemcpyDeviceToHost, c.stream);
+  hipFreeAsync(st, c.stream);
+  return hip_ok(c, hipStreamSynchronize(c.stream), "download links") ? 0 : -1;
+}
+int upload_alg(Context &c, const double *host, double2 *dst, int herm) {
+  const size_t bytes = sizeof(double2) * 9 * 8 * c.Vh;
+  double2 *st = nullptr;
+  if (!hip_ok(c, hipMallocAsync((void **)&st, bytes, c.stream), "stage")) return -1;
+  hipMemcpyAsync(st, host, bytes, hipMemcpyHostToDevice, c.stream);
+  hipLaunchKernelGGL(k_alg_in, dim3(nblk(8 * c.Vh)), dim3(kBlock), 0, c.stream, st, dst, c.Vh, herm);
+  hipFreeAsync(st, c.stream);
+  return hip_ok(c, hipStreamSynchronize(c.stream), "upload algebra") ? 0 : -1;
+}
+int download_alg(Context &c, const double2 *src, double *host, int herm) {
+  const size_t bytes = sizeof(double2) * 9 * 8 * c.Vh;
+  double2 *st = nullptr;
+  if (!hip_ok(c, hipMallocAsync((void **)&st, bytes, c.stream), "stage")) return -1;
+  hipLaunchKernelGGL(k_alg_out, dim3(nblk(8 * c.Vh)), dim3(kBlock), 0, c.stream, src, st, c.Vh, herm);
+  hipMemcpyAsync(host, st, bytes, hipMemcpyDeviceToHost, c.stream);
+  hipFreeAsync(st, c.stream);
+  return hip_ok(c, hipStreamSynchronize(c.stream), "download algebra") ? 0 : -1;
+}
+int upload_spinors(Context &c, const double *host, double2 *dst, int n) {
+  const size_t bytes = sizeof(double2) * 3 * c.Vh * n;
+  double2 *st = nullptr;
+  if (!hip_ok(c, hipMallocAsync((void **)&st, bytes, c.stream), "stage")) return -1;
+  hipMemcpyAsync(st, host, bytes, hipMemcpyHostToDevice, c.stream);
+  hipLaunchKernelGGL(k_spin_in, dim3(nblk(c.Vh), n), dim3(kBlock), 0, c.stream, st, dst, c.Vh);
+  hipFreeAsync(st, c.stream);
+  return hip_ok(c, hipStreamSynchronize(c.stream), "upload spinors") ? 0 : -1;
+}
+int download_spinors(Context &c, const double2 *src, double *host, int n) {
+  const size_t bytes = sizeof(double2) * 3 * c.Vh * n;
+  double2 *st = nullptr;
+  if (!hip_ok(c, hipMallocAsync((void **)&st, bytes, c.stream), "stage")) return -1;
+  hipLaunchKernelGGL(k_spin_out, dim3(nblk(c.Vh), n), dim3(kBlock), 0, c.stream, src, st, c.Vh);
+  hipMemcpyAsync(host, st, bytes, hipMemcpyDeviceToHost, c.stream);
+  hipFreeAsync(st, c.stream);
+  return hip_ok(c, hipStreamSynchronize(c.stream), "download spinors") ? 0 : -1;
+}
+
+}  // namespace rhmc

@@ -1,0 +1,1036 @@
+// solver.hip -- staggered Dslash and the multi-shift CG of the RHMC path.
+//
+// Dslash: D_eo/D_oe of lib/Action/fermionmatrix.cc:8-97
+//   (D psi)(x) = 1/2 sum_mu [ U_mu(x) psi(x+mu) - U_mu(x-mu)^dag psi(x-mu) ],
+// with M^dag M = m^2 - D_eo D_oe (lib/Action/Inverter/inverter.cc:27-34).
+// One thread per output site, NR right-hand sides per thread sharing every
+// link load; neighbours are computed from the site index.
+//
+// Multi-shift CG: Jegerlehner recurrences of
+// lib/Action/Inverter/inverter.cc:142-339.  Per iteration four launches:
+//   A  h = D_oe p                                   (hop, even -> odd)
+//   B  s = m^2 p - D_eo h, alpha = Re<p,s>          (+ zeta/omega recurrences)
+//   C  r += omega s, lambda = |r|^2                 (+ gamma, shift freezing)
+//   D  p = r + gamma p; x_s -= omega_s p_s; p_s = zeta_s r + gamma_s p_s
+// Reductions are FP64, wave64 shuffles + one partial per workgroup; the last
+// workgroup to finish (device-scope counter) sums the partials in a fixed
+// order and advances the scalar recurrences on the device, so the host only
+// polls a done flag every few iterations.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "context.hpp"
+
+namespace rhmc {
+
+// ------------------------------------------------------------------ workspace
+enum Slot {
+  S_PARTIAL, S_SUMS, S_MS, S_CG, S_DIAG,
+  S_R64, S_P64, S_S64, S_H64, S_PS64,
+  S_B32, S_R32, S_P32, S_S32, S_H32, S_PS32, S_X32,
+  S_CR64, S_CX32, S_CR32, S_CP32, S_CS32, S_CH32,
+  S_PW1, S_PW2, S_PW3,
+  S_FH64,
+  S_COUNT
+};
+
+struct Workspace {
+  void *buf[S_COUNT] = {};
+  size_t cap[S_COUNT] = {};
+  unsigned *counter = nullptr;
+  int *h_flags = nullptr;     // pinned
+  double *h_sums = nullptr;   // pinned
+};
+
+Workspace *workspace_create(Context &c) {
+  Workspace *w = new Workspace();
+  if (!hip_ok(c, hipMalloc(&w->counter, 64 * sizeof(unsigned)), "hipMalloc counter")) return nullptr;
+  hipMemset(w->counter, 0, 64 * sizeof(unsigned));
+  if (!hip_ok(c, hipHostMalloc(&w->h_flags, 4096 * sizeof(int), hipHostMallocDefault), "hipHostMalloc")) return nullptr;
+  if (!hip_ok(c, hipHostMalloc(&w->h_sums, 4096 * sizeof(double), hipHostMallocDefault), "hipHostMalloc")) return nullptr;
+  // reduction partials for the largest launch (kMaxRed vectors), allocated once
+  const size_t nbx = (size_t)((c.Vh + kBlock - 1) / kBlock);
+  w->cap[S_PARTIAL] = sizeof(double) * nbx * 256;
+  if (!hip_ok(c, hipMalloc(&w->buf[S_PARTIAL], w->cap[S_PARTIAL]), "hipMalloc partials")) return nullptr;
+  return w;
+}
+
+void workspace_destroy(Workspace *w) {
+  if (!w) return;
+  for (int i = 0; i < S_COUNT; i++)
+    if (w->buf[i]) hipFree(w->buf[i]);
+  if (w->counter) hipFree(w->counter);
+  if (w->h_flags) hipHostFree(w->h_flags);
+  if (w->h_sums) hipHostFree(w->h_sums);
+  delete w;
+}
+
+static void *ws_get(Context &c, int slot, size_t bytes) {
+  Workspace *w = c.ws;
+  if (w->cap[slot] >= bytes) return w->buf[slot];
+  if (w->buf[slot]) {
+    hipStreamSynchronize(c.stream);
+    hipFree(w->buf[slot]);
+    w->buf[slot] = nullptr;
+    w->cap[slot] = 0;
+  }
+  void *p = nullptr;
+  if (!hip_ok(c, hipMalloc(&p, bytes), "hipMalloc workspace")) return nullptr;
+  w->buf[slot] = p;
+  w->cap[slot] = bytes;
+  return p;
+}
+
+// ------------------------------------------------------------- reductions
+constexpr int kMaxRed = 256;  // vectors reduced in one launch
+
+__device__ __forceinline__ void store_partial(double *p, double v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ double load_partial(const double *p) {
+  return __hip_atomic_load(const_cast<double *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// block-reduce NV values per thread and store one partial per vector
+template <int NV>
+__device__ __forceinline__ void block_partials(double (&v)[NV], double *partial, int jbase, int nvec, int nbx,
+                                               int bx) {
+  __shared__ double sh[kBlock / 64][NV];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int r = 0; r < NV; r++) {
+    double s = wave_sum(v[r]);
+    if (lane == 0) sh[w][r] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x < NV) {
+    const int r = threadIdx.x, j = jbase + r;
+    if (j < nvec) {
+      double s = 0.0;
+#pragma unroll
+      for (int k = 0; k < kBlock / 64; k++) s += sh[k][r];
+      store_partial(partial + (long)j * nbx + bx, s);
+    }
+  }
+}
+
+// true in exactly one workgroup: the last one to arrive
+__device__ __forceinline__ bool arrive_last(unsigned *counter, unsigned total) {
+  __shared__ int last;
+  __threadfence();
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned prev = atomicAdd(counter, 1u);
+    last = (prev == total - 1);
+  }
+  __syncthreads();
+  if (last) __threadfence();
+  return last;
+}
+
+__device__ __forceinline__ void release_counter(unsigned *counter) {
+  if (threadIdx.x == 0) __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// deterministic final sums: wave w reduces vectors w, w+4, ...
+__device__ __forceinline__ void final_sums(const double *partial, int nvec, int nbx, double *out_sh) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int j = w; j < nvec; j += kBlock / 64) {
+    double s = 0.0;
+    for (int b = lane; b < nbx; b += 64) s += load_partial(partial + (long)j * nbx + b);
+    s = wave_sum(s);
+    if (lane == 0) out_sh[j] = s;
+  }
+  __syncthreads();
+}
+
+// ------------------------------------------------------------- epilogues
+enum Epi { EPI_STORE = 0, EPI_MS_INIT, EPI_MS_B, EPI_MS_C, EPI_CG_INIT, EPI_CG_B, EPI_CG_C };
+
+__device__ void run_epilogue(int epi, void *state, const double *sums, int nvec, double *out_sums) {
+  for (int j = threadIdx.x; j < nvec; j += blockDim.x) {
+    const double v = sums[j];
+    if (epi == EPI_STORE) {
+      out_sums[j] = v;
+    } else if (epi == EPI_MS_INIT) {
+      MsState &s = ((MsState *)state)[j];
+      s.delta = v;
+      s.omega = 1.0;
+      s.omega_prev = 1.0;
+      s.gammag = 0.0;
+      s.iter = 0;
+      for (int k = 0; k < s.nshift; k++) {
+        s.z1[k] = 1.0; s.z2[k] = 1.0; s.z3[k] = 1.0;
+        s.om[k] = 0.0; s.ga[k] = 0.0;
+        s.flag[k] = 1; s.flag_next[k] = 1; s.its[k] = 0;
+      }
+      s.nact = s.nshift;
+      s.done = (v == 0.0) ? 1 : 0;  // zero source: x = 0
+      s.stop = s.done;
+    } else if (epi == EPI_MS_B) {
+      MsState &s = ((MsState *)state)[j];
+      if (s.done) continue;
+      // inverter.cc:224-245
+      const double alpha = v;
+      s.omega_prev = s.omega;
+      s.omega = -s.delta / alpha;
+      for (int k = 0; k < s.nshift; k++) {
+        s.flag[k] = s.flag_next[k];
+        if (!s.flag[k]) continue;
+        s.z3[k] = (s.z1[k] * s.z2[k] * s.omega_prev) /
+                  (s.omega * s.gammag * (s.z1[k] - s.z2[k]) +
+                   s.z1[k] * s.omega_prev * (1.0 - s.shift[k] * s.omega));
+        s.om[k] = s.omega * s.z3[k] / s.z2[k];
+      }
+    } else if (epi == EPI_MS_C) {
+      MsState &s = ((MsState *)state)[j];
+      if (s.done) continue;
+      // inverter.cc:248-284
+      const double lambda = v;
+      const double g = lambda / s.delta;
+      int nact = 0;
+      for (int k = 0; k < s.nshift; k++) {
+        if (!s.flag[k]) continue;
+        s.ga[k] = g * s.z3[k] * s.om[k] / (s.z2[k] * s.omega);
+        const double fact = sqrt(s.delta * s.z2[k] * s.z2[k]);
+        if (fact < s.res) {
+          s.flag_next[k] = 0;
+          s.its[k] = s.iter + 1;
+        } else {
+          nact++;
+        }
+        s.z1[k] = s.z2[k];
+        s.z2[k] = s.z3[k];
+      }
+      s.nact = nact;
+      s.gammag = g;
+      s.delta = lambda;
+      s.iter++;
+      s.stop = !(sqrt(lambda) > s.res && s.iter < s.max_iter) || nact == 0;
+    } else if (epi == EPI_CG_INIT) {
+      CgState &s = ((CgState *)state)[j];
+      s.delta = v;
+      s.iter = 0;
+      s.done = (v <= s.tol2) ? 1 : 0;
+      s.stop = s.done;
+    } else if (epi == EPI_CG_B) {
+      CgState &s = ((CgState *)state)[j];
+      if (s.done) continue;
+      s.alpha = s.delta / v;
+    } else if (epi == EPI_CG_C) {
+      CgState &s = ((CgState *)state)[j];
+      if (s.done) continue;
+      s.beta = v / s.delta;
+      s.delta = v;
+      s.iter++;
+      s.stop = (v <= s.tol2) || s.iter >= s.max_iter;
+    }
+  }
+}
+
+struct RedArgs {
+  double *partial;
+  unsigned *counter;
+  double *sums;
+  void *state;
+  int epi;
+};
+
+template <int NV>
+__device__ __forceinline__ void reduce_and_epilogue(double (&v)[NV], const RedArgs &ra, int jbase, int nvec,
+                                                    int nbx, int bx) {
+  block_partials<NV>(v, ra.partial, jbase, nvec, nbx, bx);
+  if (arrive_last(ra.counter, gridDim.x * gridDim.y)) {
+    __shared__ double sums[kMaxRed];
+    final_sums(ra.partial, nvec, nbx, sums);
+    run_epilogue(ra.epi, ra.state, sums, nvec, ra.sums);
+    __syncthreads();
+    release_counter(ra.counter);
+  }
+}
+
+// ------------------------------------------------------------------ Dslash
+enum DsMode { DS_HOP = 0, DS_MDAGM = 1, DS_RESID = 2 };
+enum DsRed { RED_NONE = 0, RED_DOT = 1, RED_NORM = 2 };
+enum LinkFmt { LINK_D18 = 0, LINK_S12 = 1 };
+
+struct DsArgs {
+  LatticeDims L;
+  int par_out;
+  const void *U;
+  const void *in; long in_vs;
+  void *out; long out_vs;
+  const void *pin; long pin_vs;        // MDAGM/RESID: vector at the output site
+  const double2 *b; long b_vs; int b_group;  // RESID
+  const double *diag;                  // per vector m^2 + shift
+  const int *skip; int skip_stride;    // done flags (ints), may be null
+  int nvec;
+  RedArgs red;
+};
+
+template <typename R, int LINK>
+__device__ __forceinline__ Mat<R> ds_link(const DsArgs &a, int mu, int par, long h, R eta);
+
+template <>
+__device__ __forceinline__ Mat<double> ds_link<double, LINK_D18>(const DsArgs &a, int mu, int par, long h, double) {
+  return load_link_d18((const double2 *)a.U, a.L.Vh, mu, par, h);
+}
+template <>
+__device__ __forceinline__ Mat<float> ds_link<float, LINK_S12>(const DsArgs &a, int mu, int par, long h, float eta) {
+  return load_link_s12((const float4 *)a.U, a.L.Vh, mu, par, h, eta);
+}
+
+template <typename R, int LINK, int NR, int MODE, int RED>
+__global__ __launch_bounds__(kBlock) void k_dslash(DsArgs a) {
+  using C = Cx<R>;
+  const LatticeDims &L = a.L;
+  const long Vh = L.Vh;
+  const int bx = xcd_block(blockIdx.x, gridDim.x);
+  const long h = (long)bx * kBlock + threadIdx.x;
+  const int jbase = blockIdx.y * NR;
+  const bool site_ok = h < Vh;
+
+  bool act[NR];
+  bool any = false;
+#pragma unroll
+  for (int r = 0; r < NR; r++) {
+    const int j = jbase + r;
+    act[r] = (j < a.nvec) && (a.skip == nullptr || !a.skip[(long)j * a.skip_stride]);
+    any |= act[r];
+  }
+  double red[NR];
+#pragma unroll
+  for (int r = 0; r < NR; r++) red[r] = 0.0;
+
+  if (site_ok && any) {
+    const SiteCoord c = site_coords(L, a.par_out, h);
+    const int pin_ = a.par_out, pnb = 1 - a.par_out;
+    C acc[NR][3];
+#pragma unroll
+    for (int r = 0; r < NR; r++)
+#pragma unroll
+      for (int k = 0; k < 3; k++) acc[r][k] = mk<R>(0, 0);
+    const C *in = (const C *)a.in;
+#pragma unroll
+    for (int mu = 0; mu < 4; mu++) {
+      // forward hop: U_mu(x) psi(x+mu)
+      {
+        const long hp = neighbour(L, c, mu, +1);
+        const R eta = (R)eta_phase(L, c.x, c.y, c.z, c.t, mu);
+        const Mat<R> U = ds_link<R, LINK>(a, mu, pin_, h, eta);
+#pragma unroll
+        for (int r = 0; r < NR; r++) {
+          if (!act[r]) continue;
+          const C *v = in + (long)(jbase + r) * a.in_vs + hp;
+          const C v0 = v[0], v1 = v[Vh], v2 = v[2 * Vh];
+#pragma unroll
+          for (int i = 0; i < 3; i++) {
+            cfma(acc[r][i], U.e[i][0], v0);
+            cfma(acc[r][i], U.e[i][1], v1);
+            cfma(acc[r][i], U.e[i][2], v2);
+          }
+        }
+      }
+      // backward hop: -U_mu(x-mu)^dag psi(x-mu)
+      {
+        const long hm = neighbour(L, c, mu, -1);
+        const int tm = (mu == 3) ? (c.t == 0 ? L.nt - 1 : c.t - 1) : c.t;
+        const R eta = (R)eta_phase(L, c.x, c.y, c.z, tm, mu);
+        const Mat<R> U = ds_link<R, LINK>(a, mu, pnb, hm, eta);
+#pragma unroll
+        for (int r = 0; r < NR; r++) {
+          if (!act[r]) continue;
+          const C *v = in + (long)(jbase + r) * a.in_vs + hm;
+          const C v0 = v[0], v1 = v[Vh], v2 = v[2 * Vh];
+#pragma unroll
+          for (int i = 0; i < 3; i++) {
+            C t = cmulc(U.e[0][i], v0);
+            cfmac(t, U.e[1][i], v1);
+            cfmac(t, U.e[2][i], v2);
+            acc[r][i] = csub(acc[r][i], t);
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+      if (!act[r]) continue;
+      const int j = jbase + r;
+      C *o = (C *)a.out + (long)j * a.out_vs + h;
+      C res[3];
+      if (MODE == DS_HOP) {
+#pragma unroll
+        for (int i = 0; i < 3; i++) res[i] = cscale(acc[r][i], (R)0.5);
+      } else {
+        const C *p = (const C *)a.pin + (long)j * a.pin_vs + h;
+        const R d = (R)a.diag[j];
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+          C pv = p[(long)i * Vh];
+          res[i] = csub(cscale(pv, d), cscale(acc[r][i], (R)0.5));
+          if (MODE == DS_RESID) {
+            const double2 bv = a.b[(long)(j / a.b_group) * a.b_vs + (long)i * Vh + h];
+            res[i] = csub(mk<R>((R)bv.x, (R)bv.y), res[i]);
+          }
+          if (RED == RED_DOT) red[r] += (double)pv.x * (double)res[i].x + (double)pv.y * (double)res[i].y;
+        }
+      }
+#pragma unroll
+      for (int i = 0; i < 3; i++) {
+        o[(long)i * Vh] = res[i];
+        if (RED == RED_NORM) red[r] += (double)res[i].x * (double)res[i].x + (double)res[i].y * (double)res[i].y;
+      }
+    }
+  }
+  if (RED != RED_NONE) reduce_and_epilogue<NR>(red, a.red, jbase, a.nvec, gridDim.x, blockIdx.x);
+}
+
+// number of right-hand sides handled per thread for a launch with nvec vectors
+static int pick_nr(int nvec) {
+  if (nvec >= 8) return 8;
+  if (nvec >= 4) return 4;
+  if (nvec >= 2) return 2;
+  return 1;
+}
+
+template <typename R, int LINK, int MODE, int RED>
+static void dslash_dispatch(Context &c, DsArgs a) {
+  const int nr = pick_nr(a.nvec);
+  dim3 grid((unsigned)((c.Vh + kBlock - 1) / kBlock), (unsigned)((a.nvec + nr - 1) / nr));
+  a.L = c.L;
+  switch (nr) {
+    case 8: hipLaunchKernelGGL((k_dslash<R, LINK, 8, MODE, RED>), grid, dim3(kBlock), 0, c.stream, a); break;
+    case 4: hipLaunchKernelGGL((k_dslash<R, LINK, 4, MODE, RED>), grid, dim3(kBlock), 0, c.stream, a); break;
+    case 2: hipLaunchKernelGGL((k_dslash<R, LINK, 2, MODE, RED>), grid, dim3(kBlock), 0, c.stream, a); break;
+    default: hipLaunchKernelGGL((k_dslash<R, LINK, 1, MODE, RED>), grid, dim3(kBlock), 0, c.stream, a); break;
+  }
+}
+
+static RedArgs make_red(Context &c, int nvec, int epi, void *state, double *sums) {
+  RedArgs r;
+  const int nbx = (int)((c.Vh + kBlock - 1) / kBlock);
+  r.partial = (double *)ws_get(c, S_PARTIAL, sizeof(double) * (size_t)nbx * std::max(nvec, 1));
+  r.counter = c.ws->counter;
+  r.sums = sums;
+  r.state = state;
+  r.epi = epi;
+  return r;
+}
+
+void launch_hop_d(Context &c, int par_out, const double2 *in, double2 *out, int nvec, long vs_in, long vs_out) {
+  DsArgs a = {};
+  a.par_out = par_out;
+  a.U = c.U64;
+  a.in = in; a.in_vs = vs_in;
+  a.out = out; a.out_vs = vs_out;
+  a.nvec = nvec;
+  dslash_dispatch<double, LINK_D18, DS_HOP, RED_NONE>(c, a);
+}
+
+// --------------------------------------------------------------- BLAS kernels
+struct BlasArgs {
+  long Vh;
+  int nvec, nshift;
+  const void *b;      // MS init: FP64 sources
+  void *r, *p, *s, *ps, *x;
+  MsState *ms;
+  CgState *cg;
+  RedArgs red;
+};
+
+// r = p = b, p_s = b, x_s = 0, |b|^2
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_ms_init(BlasArgs a) {
+  using C = Cx<R>;
+  const int bx = xcd_block(blockIdx.x, gridDim.x);
+  const long h = (long)bx * kBlock + threadIdx.x;
+  const int j = blockIdx.y;
+  const long Vh = a.Vh;
+  double v[1] = {0.0};
+  if (h < Vh) {
+    const double2 *b = (const double2 *)a.b + (long)j * 3 * Vh + h;
+    C *r = (C *)a.r + (long)j * 3 * Vh + h, *p = (C *)a.p + (long)j * 3 * Vh + h;
+    for (int i = 0; i < 3; i++) {
+      const double2 bv = b[(long)i * Vh];
+      const C cv = mk<R>((R)bv.x, (R)bv.y);
+      r[(long)i * Vh] = cv;
+      p[(long)i * Vh] = cv;
+      v[0] += (double)cv.x * (double)cv.x + (double)cv.y * (double)cv.y;
+      for (int k = 0; k < a.nshift; k++) {
+        const long o = ((long)(j * a.nshift + k) * 3 + i) * Vh + h;
+        ((C *)a.ps)[o] = cv;
+        ((C *)a.x)[o] = mk<R>(0, 0);
+      }
+    }
+  }
+  // one y-block per rhs: partial index uses j
+  block_partials<1>(v, a.red.partial, j, a.nvec, gridDim.x, blockIdx.x);
+  if (arrive_last(a.red.counter, gridDim.x * gridDim.y)) {
+    __shared__ double sums[kMaxRed];
+    final_sums(a.red.partial, a.nvec, gridDim.x, sums);
+    run_epilogue(a.red.epi, a.red.state, sums, a.nvec, a.red.sums);
+    __syncthreads();
+    release_counter(a.red.counter);
+  }
+}
+
+// step C: r += omega s; lambda = |r|^2
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_ms_update_r(BlasArgs a) {
+  using C = Cx<R>;
+  const int bx = xcd_block(blockIdx.x, gridDim.x);
+  const long h = (long)bx * kBlock + threadIdx.x;
+  const int j = blockIdx.y;
+  const long Vh = a.Vh;
+  double v[1] = {0.0};
+  const MsState &st = a.ms[j];
+  if (h < Vh && !st.done) {
+    const R om = (R)st.omega;
+    C *r = (C *)a.r + (long)j * 3 * Vh + h;
+    const C *s = (const C *)a.s + (long)j * 3 * Vh + h;
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      C rv = r[(long)i * Vh];
+      const C sv = s[(long)i * Vh];
+      rv.x += om * sv.x;
+      rv.y += om * sv.y;
+      r[(long)i * Vh] = rv;
+      v[0] += (double)rv.x * (double)rv.x + (double)rv.y * (double)rv.y;
+    }
+  }
+  block_partials<1>(v, a.red.partial, j, a.nvec, gridDim.x, blockIdx.x);
+  if (arrive_last(a.red.counter, gridDim.x * gridDim.y)) {
+    __shared__ double sums[kMaxRed];
+    final_sums(a.red.partial, a.nvec, gridDim.x, sums);
+    run_epilogue(EPI_MS_C, a.ms, sums, a.nvec, nullptr);
+    __syncthreads();
+    release_counter(a.red.counter);
+  }
+}
+
+// step D: p = r + gamma p ; for active shifts x_s -= om_s p_s, p_s = zeta_s r + ga_s p_s
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_ms_update_p(BlasArgs a) {
+  using C = Cx<R>;
+  const int bx = xcd_block(blockIdx.x, gridDim.x);
+  const long h = (long)bx * kBlock + threadIdx.x;
+  const int j = blockIdx.y;
+  const long Vh = a.Vh;
+  const MsState &st = a.ms[j];
+  if (h < Vh && !st.done) {
+    const R g = (R)st.gammag;
+    C *p = (C *)a.p + (long)j * 3 * Vh + h;
+    const C *r = (const C *)a.r + (long)j * 3 * Vh + h;
+    C rv[3];
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      rv[i] = r[(long)i * Vh];
+      C pv = p[(long)i * Vh];
+      pv.x = rv[i].x + g * pv.x;
+      pv.y = rv[i].y + g * pv.y;
+      p[(long)i * Vh] = pv;
+    }
+    for (int k = 0; k < a.nshift; k++) {
+      if (!st.flag[k]) continue;
+      const R om = (R)st.om[k], z = (R)st.z2[k], ga = (R)st.ga[k];
+      C *ps = (C *)a.ps + (long)(j * a.nshift + k) * 3 * Vh + h;
+      C *x = (C *)a.x + (long)(j * a.nshift + k) * 3 * Vh + h;
+#pragma unroll
+      for (int i = 0; i < 3; i++) {
+        C pv = ps[(long)i * Vh];
+        C xv = x[(long)i * Vh];
+        xv.x -= om * pv.x;
+        xv.y -= om * pv.y;
+        x[(long)i * Vh] = xv;
+        pv.x = z * rv[i].x + ga * pv.x;
+        pv.y = z * rv[i].y + ga * pv.y;
+        ps[(long)i * Vh] = pv;
+      }
+    }
+  }
+  if (arrive_last(a.red.counter, gridDim.x * gridDim.y)) {
+    for (int t = threadIdx.x; t < a.nvec; t += blockDim.x) a.ms[t].done = a.ms[t].stop;
+    __syncthreads();
+    release_counter(a.red.counter);
+  }
+}
+
+// ---- batched single-shift CG (refinement of every shifted system)
+// init: r = r0 (FP64 -> R), p = r, x = 0, delta = |r|^2
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_cg_init(BlasArgs a) {
+  using C = Cx<R>;
+  const int bx = xcd_block(blockIdx.x, gridDim.x);
+  const long h = (long)bx * kBlock + threadIdx.x;
+  const int j = blockIdx.y;
+  const long Vh = a.Vh;
+  double v[1] = {0.0};
+  if (h < Vh) {
+    const double2 *b = (const double2 *)a.b + (long)j * 3 * Vh + h;
+    C *r = (C *)a.r + (long)j * 3 * Vh + h, *p = (C *)a.p + (long)j * 3 * Vh + h;
+    C *x = (C *)a.x + (long)j * 3 * Vh + h;
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      const double2 bv = b[(long)i * Vh];
+      const C cv = mk<R>((R)bv.x, (R)bv.y);
+      r[(long)i * Vh] = cv;
+      p[(long)i * Vh] = cv;
+      x[(long)i * Vh] = mk<R>(0, 0);
+      v[0] += (double)cv.x * (double)cv.x + (double)cv.y * (double)cv.y;
+    }
+  }
+  block_partials<1>(v, a.red.partial, j, a.nvec, gridDim.x, blockIdx.x);
+  if (arrive_last(a.red.counter, gridDim.x * gridDim.y)) {
+    __shared__ double sums[kMaxRed];
+    final_sums(a.red.partial, a.nvec, gridDim.x, sums);
+    run_epilogue(EPI_CG_INIT, a.cg, sums, a.nvec, nullptr);
+    __syncthreads();
+    release_counter(a.red.counter);
+  }
+}
+
+// x += alpha p ; r -= alpha s ; lambda = |r|^2
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_cg_update_xr(BlasArgs a) {
+  using C = Cx<R>;
+  const int bx = xcd_block(blockIdx.x, gridDim.x);
+  const long h = (long)bx * kBlock + threadIdx.x;
+  const int j = blockIdx.y;
+  const long Vh = a.Vh;
+  double v[1] = {0.0};
+  const CgState &st = a.cg[j];
+  if (h < Vh && !st.done) {
+    const R al = (R)st.alpha;
+    C *x = (C *)a.x + (long)j * 3 * Vh + h, *r = (C *)a.r + (long)j * 3 * Vh + h;
+    const C *p = (const C *)a.p + (long)j * 3 * Vh + h, *s = (const C *)a.s + (long)j * 3 * Vh + h;
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      C xv = x[(long)i * Vh], rv = r[(long)i * Vh];
+      const C pv = p[(long)i * Vh], sv = s[(long)i * Vh];
+      xv.x += al * pv.x; xv.y += al * pv.y;
+      rv.x -= al * sv.x; rv.y -= al * sv.y;
+      x[(long)i * Vh] = xv;
+      r[(long)i * Vh] = rv;
+      v[0] += (double)rv.x * (double)rv.x + (double)rv.y * (double)rv.y;
+    }
+  }
+  block_partials<1>(v, a.red.partial, j, a.nvec, gridDim.x, blockIdx.x);
+  if (arrive_last(a.red.counter, gridDim.x * gridDim.y)) {
+    __shared__ double sums[kMaxRed];
+    final_sums(a.red.partial, a.nvec, gridDim.x, sums);
+    run_epilogue(EPI_CG_C, a.cg, sums, a.nvec, nullptr);
+    __syncthreads();
+    release_counter(a.red.counter);
+  }
+}
+
+// p = r + beta p
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_cg_update_p(BlasArgs a) {
+  using C = Cx<R>;
+  const int bx = xcd_block(blockIdx.x, gridDim.x);
+  const long h = (long)bx * kBlock + threadIdx.x;
+  const int j = blockIdx.y;
+  const long Vh = a.Vh;
+  const CgState &st = a.cg[j];
+  if (h < Vh && !st.done) {
+    const R be = (R)st.beta;
+    C *p = (C *)a.p + (long)j * 3 * Vh + h;
+    const C *r = (const C *)a.r + (long)j * 3 * Vh + h;
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      C pv = p[(long)i * Vh];
+      const C rv = r[(long)i * Vh];
+      pv.x = rv.x + be * pv.x;
+      pv.y = rv.y + be * pv.y;
+      p[(long)i * Vh] = pv;
+    }
+  }
+  if (arrive_last(a.red.counter, gridDim.x * gridDim.y)) {
+    for (int t = threadIdx.x; t < a.nvec; t += blockDim.x) a.cg[t].done = a.cg[t].stop;
+    __syncthreads();
+    release_counter(a.red.counter);
+  }
+}
+
+// x64[j] += (double) x32[j]  (n elements of double2)
+__global__ __launch_bounds__(kBlock) void k_accum_f2d(double2 *x64, const float2 *x32, long n) {
+  const long i = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) {
+    const float2 v = x32[i];
+    double2 o = x64[i];
+    o.x += (double)v.x;
+    o.y += (double)v.y;
+    x64[i] = o;
+  }
+}
+__global__ __launch_bounds__(kBlock) void k_copy_f2d(double2 *x64, const float2 *x32, long n) {
+  const long i = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) {
+    const float2 v = x32[i];
+    x64[i] = make_double2((double)v.x, (double)v.y);
+  }
+}
+
+// ------------------------------------------------------------- host drivers
+static unsigned nbx_of(Context &c) { return (unsigned)((c.Vh + kBlock - 1) / kBlock); }
+
+// poll the done flags (one int per state, stride in ints) after a chunk
+static bool all_done(Context &c, const int *dflag, int stride, int n) {
+  int *hf = c.ws->h_flags;
+  for (int j = 0; j < n; j++)
+    hipMemcpyAsync(hf + j, dflag + (long)j * stride, sizeof(int), hipMemcpyDeviceToHost, c.stream);
+  hipStreamSynchronize(c.stream);
+  for (int j = 0; j < n; j++)
+    if (!hf[j]) return false;
+  return true;
+}
+
+// multi-shift CG in precision R with link format LINK on sources b (FP64).
+// x (precision R) receives [nrhs][nshift] solutions.  res_abs[j] per rhs.
+template <typename R, int LINK>
+static int ms_cg(Context &c, const double2 *b, int nrhs, int nshift, const double *shifts, const double *res_abs,
+                 int max_iter, void *x, int *iters_out) {
+  using C = Cx<R>;
+  const long Vh = c.Vh;
+  const size_t vb = sizeof(C) * 3 * Vh;
+  const bool dbl = sizeof(R) == 8;
+  void *r = ws_get(c, dbl ? S_R64 : S_R32, vb * nrhs);
+  void *p = ws_get(c, dbl ? S_P64 : S_P32, vb * nrhs);
+  void *s = ws_get(c, dbl ? S_S64 : S_S32, vb * nrhs);
+  void *hh = ws_get(c, dbl ? S_H64 : S_H32, vb * nrhs);
+  void *ps = ws_get(c, dbl ? S_PS64 : S_PS32, vb * nrhs * nshift);
+  MsState *ms = (MsState *)ws_get(c, S_MS, sizeof(MsState) * nrhs);
+  double *diag = (double *)ws_get(c, S_DIAG, sizeof(double) * 256);
+  if (!r || !p || !s || !hh || !ps || !ms || !diag) return -1;
+
+  std::vector<MsState> hs(nrhs);
+  for (int j = 0; j < nrhs; j++) {
+    MsState &st = hs[j];
+    memset(&st, 0, sizeof(st));
+    st.nshift = nshift;
+    st.res = res_abs[j];
+    st.max_iter = max_iter;
+    for (int k = 0; k < nshift; k++) st.shift[k] = shifts[k];
+  }
+  hipMemcpyAsync(ms, hs.data(), sizeof(MsState) * nrhs, hipMemcpyHostToDevice, c.stream);
+  std::vector<double> hd(nrhs, c.mass * c.mass);
+  hipMemcpyAsync(diag, hd.data(), sizeof(double) * nrhs, hipMemcpyHostToDevice, c.stream);
+
+  const unsigned nbx = nbx_of(c);
+  BlasArgs ba = {};
+  ba.Vh = Vh; ba.nvec = nrhs; ba.nshift = nshift;
+  ba.b = b; ba.r = r; ba.p = p; ba.s = s; ba.ps = ps; ba.x = x; ba.ms = ms;
+  ba.red = make_red(c, nrhs, EPI_MS_INIT, ms, nullptr);
+  hipLaunchKernelGGL(k_ms_init<R>, dim3(nbx, nrhs), dim3(kBlock), 0, c.stream, ba);
+
+  DsArgs hop = {};
+  hop.par_out = 1; hop.U = LINK == LINK_D18 ? (const void *)c.U64 : (const void *)c.U32;
+  hop.in = p; hop.in_vs = 3 * Vh; hop.out = hh; hop.out_vs = 3 * Vh; hop.nvec = nrhs;
+  hop.skip = &ms[0].done; hop.skip_stride = sizeof(MsState) / sizeof(int);
+  DsArgs mdm = hop;
+  mdm.par_out = 0; mdm.in = hh; mdm.out = s; mdm.pin = p; mdm.pin_vs = 3 * Vh; mdm.diag = diag;
+  mdm.red = make_red(c, nrhs, EPI_MS_B, ms, nullptr);
+
+  const int chunk = 8;
+  int it = 0;
+  while (true) {
+    for (int k = 0; k < chunk; k++) {
+      dslash_dispatch<R, LINK, DS_HOP, RED_NONE>(c, hop);
+      dslash_dispatch<R, LINK, DS_MDAGM, RED_DOT>(c, mdm);
+      hipLaunchKernelGGL(k_ms_update_r<R>, dim3(nbx, nrhs), dim3(kBlock), 0, c.stream, ba);
+      hipLaunchKernelGGL(k_ms_update_p<R>, dim3(nbx, nrhs), dim3(kBlock), 0, c.stream, ba);
+    }
+    it += chunk;
+    if (all_done(c, &ms[0].done, sizeof(MsState) / sizeof(int), nrhs)) break;
+    if (it > max_iter + chunk) break;
+  }
+  hipMemcpyAsync(hs.data(), ms, sizeof(MsState) * nrhs, hipMemcpyDeviceToHost, c.stream);
+  if (!hip_ok(c, hipStreamSynchronize(c.stream), "multishift")) return -1;
+  int maxit = 0;
+  for (int j = 0; j < nrhs; j++) maxit = std::max(maxit, hs[j].iter);
+  *iters_out = maxit;
+  for (int j = 0; j < nrhs; j++)
+    if (hs[j].iter >= max_iter && hs[j].nact > 0) {
+      c.last_error = "multishift: max_cg reached";
+      return 2;
+    }
+  return 0;
+}
+
+// batched CG on (MdagM + shift_j) e_j = r_j (FP32 iterations, FP32 links);
+// x32 receives e_j.
+static int block_cg32(Context &c, const double2 *r64, int nsys, const double *shifts, const double *tol,
+                      int max_iter, float2 *x32, int *iters_out) {
+  const long Vh = c.Vh;
+  const size_t vb = sizeof(float2) * 3 * Vh;
+  void *r = ws_get(c, S_CR32, vb * nsys);
+  void *p = ws_get(c, S_CP32, vb * nsys);
+  void *s = ws_get(c, S_CS32, vb * nsys);
+  void *hh = ws_get(c, S_CH32, vb * nsys);
+  CgState *cg = (CgState *)ws_get(c, S_CG, sizeof(CgState) * nsys);
+  double *diag = (double *)ws_get(c, S_DIAG, sizeof(double) * 256);
+  if (!r || !p || !s || !hh || !cg || !diag) return -1;
+  std::vector<CgState> hs(nsys);
+  std::vector<double> hd(nsys);
+  for (int j = 0; j < nsys; j++) {
+    memset(&hs[j], 0, sizeof(CgState));
+    hs[j].tol2 = tol[j] * tol[j];
+    hs[j].shift = shifts[j];
+    hs[j].max_iter = max_iter;
+    hd[j] = c.mass * c.mass + shifts[j];
+  }
+  hipMemcpyAsync(cg, hs.data(), sizeof(CgState) * nsys, hipMemcpyHostToDevice, c.stream);
+  hipMemcpyAsync(diag, hd.data(), sizeof(double) * nsys, hipMemcpyHostToDevice, c.stream);
+  const unsigned nbx = nbx_of(c);
+  BlasArgs ba = {};
+  ba.Vh = Vh; ba.nvec = nsys; ba.nshift = 1;
+  ba.b = r64; ba.r = r; ba.p = p; ba.s = s; ba.x = x32; ba.cg = cg;
+  ba.red = make_red(c, nsys, EPI_CG_INIT, cg, nullptr);
+  hipLaunchKernelGGL(k_cg_init<float>, dim3(nbx, nsys), dim3(kBlock), 0, c.stream, ba);
+  DsArgs hop = {};
+  hop.par_out = 1; hop.U = c.U32;
+  hop.in = p; hop.in_vs = 3 * Vh; hop.out = hh; hop.out_vs = 3 * Vh; hop.nvec = nsys;
+  hop.skip = &cg[0].done; hop.skip_stride = sizeof(CgState) / sizeof(int);
+  DsArgs mdm = hop;
+  mdm.par_out = 0; mdm.in = hh; mdm.out = s; mdm.pin = p; mdm.pin_vs = 3 * Vh; mdm.diag = diag;
+  mdm.red = make_red(c, nsys, EPI_CG_B, cg, nullptr);
+  const int chunk = 8;
+  int it = 0;
+  while (true) {
+    for (int k = 0; k < chunk; k++) {
+      dslash_dispatch<float, LINK_S12, DS_HOP, RED_NONE>(c, hop);
+      dslash_dispatch<float, LINK_S12, DS_MDAGM, RED_DOT>(c, mdm);
+      hipLaunchKernelGGL(k_cg_update_xr<float>, dim3(nbx, nsys), dim3(kBlock), 0, c.stream, ba);
+      hipLaunchKernelGGL(k_cg_update_p<float>, dim3(nbx, nsys), dim3(kBlock), 0, c.stream, ba);
+    }
+    it += chunk;
+    if (all_done(c, &cg[0].done, sizeof(CgState) / sizeof(int), nsys)) break;
+    if (it > max_iter + chunk) break;
+  }
+  hipMemcpyAsync(hs.data(), cg, sizeof(CgState) * nsys, hipMemcpyDeviceToHost, c.stream);
+  if (!hip_ok(c, hipStreamSynchronize(c.stream), "block cg")) return -1;
+  int maxit = 0;
+  for (int j = 0; j < nsys; j++) maxit = std::max(maxit, hs[j].iter);
+  *iters_out = maxit;
+  return 0;
+}
+
+// r = b - ((m^2+shift_j) x_j - D_eo D_oe x_j) in FP64 with FP64 links; norms2[j]
+int residual_d(Context &c, const double2 *b, int b_group, const double2 *x, const double *shifts, int nvec,
+               double2 *r, double *norms2) {
+  const long Vh = c.Vh;
+  double2 *hh = (double2 *)ws_get(c, S_FH64, sizeof(double2) * 3 * Vh * nvec);
+  double *diag = (double *)ws_get(c, S_DIAG, sizeof(double) * 256);
+  double *sums = (double *)ws_get(c, S_SUMS, sizeof(double) * 256);
+  if (!hh || !diag || !sums || nvec > kMaxRed) return -1;
+  std::vector<double> hd(nvec);
+  for (int j = 0; j < nvec; j++) hd[j] = c.mass * c.mass + shifts[j];
+  hipMemcpyAsync(diag, hd.data(), sizeof(double) * nvec, hipMemcpyHostToDevice, c.stream);
+  launch_hop_d(c, 1, x, hh, nvec, 3 * Vh, 3 * Vh);
+  DsArgs a = {};
+  a.par_out = 0; a.U = c.U64;
+  a.in = hh; a.in_vs = 3 * Vh;
+  a.out = r; a.out_vs = 3 * Vh;
+  a.pin = x; a.pin_vs = 3 * Vh;
+  a.b = b; a.b_vs = 3 * Vh; a.b_group = b_group;
+  a.diag = diag; a.nvec = nvec;
+  a.red = make_red(c, nvec, EPI_STORE, nullptr, sums);
+  dslash_dispatch<double, LINK_D18, DS_RESID, RED_NORM>(c, a);
+  hipMemcpyAsync(norms2, sums, sizeof(double) * nvec, hipMemcpyDeviceToHost, c.stream);
+  return hip_ok(c, hipStreamSynchronize(c.stream), "residual") ? 0 : -1;
+}
+
+int multishift_solve(Context &c, const double2 *b, int nrhs, int nshift, const double *shifts, double res, int rel,
+                     int prec, int max_iter, double2 *x, int *iters) {
+  if (nshift < 1 || nshift > kMaxShifts || nrhs < 1 || nrhs * nshift > kMaxRed) {
+    c.last_error = "multishift: unsupported number of shifts/rhs";
+    return 1;
+  }
+  const long Vh = c.Vh;
+  // |b| per rhs for relative residuals and the FP32 stage target
+  std::vector<double> bn2(nrhs), res_abs(nrhs);
+  if (vec_norms2(c, b, nrhs, 3 * Vh, bn2.data())) return -1;
+  for (int j = 0; j < nrhs; j++) res_abs[j] = rel ? res * sqrt(bn2[j]) : res;
+  c.last_refine_cycles = 0;
+  c.last_iters_refine = 0;
+  if (prec == PREC_FP64) {
+    int rc = ms_cg<double, LINK_D18>(c, b, nrhs, nshift, shifts, res_abs.data(), max_iter, x, iters);
+    c.last_iters = *iters;
+    c.last_iters_fp32 = 0;
+    return rc;
+  }
+  // mixed: FP32 multi-shift to ~1e-6 relative, then FP64 defect correction
+  const double rel32 = 2e-6;
+  std::vector<double> res32(nrhs);
+  for (int j = 0; j < nrhs; j++) res32[j] = std::max(res_abs[j], rel32 * sqrt(bn2[j]));
+  float2 *x32 = (float2 *)ws_get(c, S_X32, sizeof(float2) * 3 * Vh * nrhs * nshift);
+  if (!x32) return -1;
+  int it32 = 0;
+  int rc = ms_cg<float, LINK_S12>(c, b, nrhs, nshift, shifts, res32.data(), max_iter, x32, &it32);
+  if (rc < 0) return rc;
+  const long n = 3 * Vh * (long)nrhs * nshift;
+  hipLaunchKernelGGL(k_copy_f2d, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, c.stream, x, x32, n);
+  const int nsys = nrhs * nshift;
+  std::vector<double> sh(nsys), nr2(nsys), tol(nsys);
+  for (int j = 0; j < nsys; j++) sh[j] = shifts[j % nshift];
+  double2 *r64 = (double2 *)ws_get(c, S_CR64, sizeof(double2) * 3 * Vh * nsys);
+  float2 *e32 = (float2 *)ws_get(c, S_CX32, sizeof(float2) * 3 * Vh * nsys);
+  if (!r64 || !e32) return -1;
+  int total_ref = 0;
+  double worst = 0.0;
+  for (int cyc = 0; cyc < 8; cyc++) {
+    if (residual_d(c, b, nshift, x, sh.data(), nsys, r64, nr2.data())) return -1;
+    bool ok = true;
+    worst = 0.0;
+    for (int j = 0; j < nsys; j++) {
+      const double rn = sqrt(nr2[j]);
+      const double target = res_abs[j / nshift];
+      worst = std::max(worst, rn / target);
+      // inner FP32 solve: reduce this residual by ~1e-5, never below 0.5*target
+      tol[j] = rn <= target ? 1e300 : std::max(0.5 * target, 1e-5 * rn);
+      if (rn > target) ok = false;
+    }
+    c.last_true_res_max = worst;
+    if (ok) break;
+    // systems already converged get tol=inf -> done at init
+    for (int j = 0; j < nsys; j++)
+      if (tol[j] > 1e299) tol[j] = 1e150;
+    int itc = 0;
+    if (block_cg32(c, r64, nsys, sh.data(), tol.data(), max_iter, e32, &itc)) return -1;
+    total_ref += itc;
+    hipLaunchKernelGGL(k_accum_f2d, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, c.stream, x,
+                       e32, n);
+    c.last_refine_cycles = cyc + 1;
+  }
+  if (!hip_ok(c, hipStreamSynchronize(c.stream), "mixed solve")) return -1;
+  c.last_iters_fp32 = it32;
+  c.last_iters_refine = total_ref;
+  c.last_iters = it32 + total_ref;
+  *iters = c.last_iters;
+  if (worst > 1.0) {
+    c.last_error = "mixed multishift: refinement did not reach the target";
+    return 2;
+  }
+  return rc;
+}
+
+// ---------------------------------------------------------- norms / scaling
+__global__ __launch_bounds__(kBlock) void k_norms(const double2 *v, long vs, long Vh, int nvec, RedArgs red) {
+  const int bx = xcd_block(blockIdx.x, gridDim.x);
+  const long h = (long)bx * kBlock + threadIdx.x;
+  const int j = blockIdx.y;
+  double s[1] = {0.0};
+  if (h < Vh)
+    for (int i = 0; i < 3; i++) {
+      const double2 a = v[(long)j * vs + (long)i * Vh + h];
+      s[0] += a.x * a.x + a.y * a.y;
+    }
+  block_partials<1>(s, red.partial, j, nvec, gridDim.x, blockIdx.x);
+  if (arrive_last(red.counter, gridDim.x * gridDim.y)) {
+    __shared__ double sums[kMaxRed];
+    final_sums(red.partial, nvec, gridDim.x, sums);
+    run_epilogue(EPI_STORE, nullptr, sums, nvec, red.sums);
+    __syncthreads();
+    release_counter(red.counter);
+  }
+}
+
+int vec_norms2(Context &c, const double2 *v, int nvec, long vs, double *out) {
+  if (nvec > kMaxRed) return -1;
+  double *sums = (double *)ws_get(c, S_SUMS, sizeof(double) * 256);
+  RedArgs red = make_red(c, nvec, EPI_STORE, nullptr, sums);
+  hipLaunchKernelGGL(k_norms, dim3(nbx_of(c), nvec), dim3(kBlock), 0, c.stream, v, vs, c.Vh, nvec, red);
+  hipMemcpyAsync(out, sums, sizeof(double) * nvec, hipMemcpyDeviceToHost, c.stream);
+  return hip_ok(c, hipStreamSynchronize(c.stream), "norms") ? 0 : -1;
+}
+
+__global__ __launch_bounds__(kBlock) void k_scale_copy(double2 *dst, double2 *dst2, const double2 *src, double s,
+                                                       long n) {
+  const long i = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) {
+    double2 v = src[i];
+    v.x *= s;
+    v.y *= s;
+    dst[i] = v;
+    if (dst2) dst2[i] = v;
+  }
+}
+// p = a*r - p
+__global__ __launch_bounds__(kBlock) void k_axmy(double2 *p, const double2 *r, double a, long n) {
+  const long i = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) {
+    const double2 rv = r[i];
+    double2 pv = p[i];
+    pv.x = a * rv.x - pv.x;
+    pv.y = a * rv.y - pv.y;
+    p[i] = pv;
+  }
+}
+
+// power iterations of findminmax.cc:12-137 in FP64
+static int power_iter(Context &c, const double2 *start, double tol, int is_min, double mx, double *out, int *loops) {
+  const long Vh = c.Vh, n = 3 * Vh;
+  double2 *p = (double2 *)ws_get(c, S_PW1, sizeof(double2) * n);
+  double2 *r = (double2 *)ws_get(c, S_PW2, sizeof(double2) * n);
+  double2 *hh = (double2 *)ws_get(c, S_PW3, sizeof(double2) * n);
+  double *diag = (double *)ws_get(c, S_DIAG, sizeof(double) * 256);
+  double *sums = (double *)ws_get(c, S_SUMS, sizeof(double) * 256);
+  if (!p || !r || !hh || !diag || !sums) return -1;
+  hipMemcpyAsync(p, start, sizeof(double2) * n, hipMemcpyDeviceToDevice, c.stream);
+  double m2 = c.mass * c.mass;
+  hipMemcpyAsync(diag, &m2, sizeof(double), hipMemcpyHostToDevice, c.stream);
+  double nrm2;
+  if (vec_norms2(c, p, 1, n, &nrm2)) return -1;
+  double norm = sqrt(nrm2), rel;
+  const unsigned nb = (unsigned)((n + kBlock - 1) / kBlock);
+  int cnt = 0;
+  do {
+    hipLaunchKernelGGL(k_scale_copy, dim3(nb), dim3(kBlock), 0, c.stream, r, (double2 *)nullptr, p, 1.0 / norm, n);
+    const double old = norm;
+    launch_hop_d(c, 1, r, hh, 1, n, n);
+    DsArgs a = {};
+    a.par_out = 0; a.U = c.U64; a.in = hh; a.in_vs = n; a.out = p; a.out_vs = n; a.pin = r; a.pin_vs = n;
+    a.diag = diag; a.nvec = 1;
+    if (!is_min) {
+      a.red = make_red(c, 1, EPI_STORE, nullptr, sums);
+      dslash_dispatch<double, LINK_D18, DS_MDAGM, RED_NORM>(c, a);
+      hipMemcpyAsync(&nrm2, sums, sizeof(double), hipMemcpyDeviceToHost, c.stream);
+      hipStreamSynchronize(c.stream);
+    } else {
+      dslash_dispatch<double, LINK_D18, DS_MDAGM, RED_NONE>(c, a);
+      hipLaunchKernelGGL(k_axmy, dim3(nb), dim3(kBlock), 0, c.stream, p, r, mx, n);
+      if (vec_norms2(c, p, 1, n, &nrm2)) return -1;
+    }
+    norm = sqrt(nrm2);
+    rel = fabs(old - norm) / norm;
+    cnt++;
+  } while (rel > tol && cnt < 20000);
+  *loops = cnt;
+  *out = is_min ? mx - norm : norm;
+  return hip_ok(c, hipStreamSynchronize(c.stream), "power") ? 0 : -1;
+}
+
+int power_max(Context &c, const double2 *start, double tol, double *out, int *loops) {
+  return power_iter(c, start, tol, 0, 0.0, out, loops);
+}
+int power_min(Context &c, const double2 *start, double tol, double mx, double *out, int *loops) {
+  return power_iter(c, start, tol, 1, mx, out, loops);
+}
+
+// ----------------------------------------------------- device ops for tests
+// out (FP32, [nvec][3][Vh]) = 0.5 hop(in) with FP32 12-real links
+int hop_s12(Context &c, int par_out, const float2 *in, float2 *out, int nvec) {
+  DsArgs a = {};
+  a.par_out = par_out; a.U = c.U32;
+  a.in = in; a.in_vs = 3 * c.Vh; a.out = out; a.out_vs = 3 * c.Vh; a.nvec = nvec;
+  dslash_dispatch<float, LINK_S12, DS_HOP, RED_NONE>(c, a);
+  return 0;
+}
+
+}  // namespace rhmc

@@ -1,0 +1,241 @@
+// rhmc_update.cpp -- one RHMC trajectory driven on the device.
+//
+// Host-side mirror of the reference's algorithm layer:
+//   lib/Update/update.cc:58-152              heatbath -> MD -> Metropolis
+//   lib/Tools/RationalApprox/rationalapprox.cc:134-369  rescaling, chi / phi_new
+//   lib/Update/MD_integrators/minimum_norm2.cc:28-105    Omelyan 2MN
+//   lib/Update/MD_integrators/multistep_2MN.cc:31-168    multistep 2MN
+//   lib/Update/MD_integrators/leapfrog.cc:18-62          leapfrog, with the
+//       momentum sign of the live integrators (P <- P - i eps ipdot)
+//   lib/Update/ipdot.cc:83-196, lib/Update/momenta.cc:42-115
+// Every field stays on the device; the host only sequences launches and
+// reads back solver flags and the final Delta H.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "../csrc/context.hpp"
+#include "gpu.h"
+
+namespace rhmc {
+
+namespace {
+
+using clk = std::chrono::steady_clock;
+
+struct Coeffs {
+  double a0 = 0.0;
+  std::vector<double> a, b;
+};
+
+// rationalapprox.cc:134-263
+Coeffs rescale(int order, double a0, const double *a, const double *b, double mn, double mx, double exponent) {
+  (void)mn;
+  const double mxp = 1.05 * mx;
+  const double eps = std::pow(mxp, exponent);
+  Coeffs c;
+  c.a0 = a0 * eps;
+  c.a.resize(order);
+  c.b.resize(order);
+  for (int i = 0; i < order; i++) {
+    c.a[i] = a[i] * (mxp * eps);
+    c.b[i] = b[i] * mxp;
+  }
+  return c;
+}
+
+struct Traj {
+  Context &c;
+  const rhmc_traj_params &p;
+  rhmc_traj_result &r;
+  Coeffs md;
+  double t_solve = 0, t_force = 0, t_update = 0;
+  int status = 0;
+
+  Traj(Context &cc, const rhmc_traj_params &pp, rhmc_traj_result &rr) : c(cc), p(pp), r(rr) {}
+
+  double secs(clk::time_point t0) { return std::chrono::duration<double>(clk::now() - t0).count(); }
+
+  int solve(const double2 *b, const Coeffs &co, double res) {
+    auto t0 = clk::now();
+    int it = 0;
+    int rc = multishift_solve(c, b, c.n_pf, (int)co.b.size(), co.b.data(), res, p.residual_relative, p.precision,
+                              p.max_cg, c.smf, &it);
+    c.smf_order = (int)co.b.size();
+    r.cg_iterations += it;
+    r.n_solves++;
+    t_solve += secs(t0);
+    if (rc && !status) status = rc;
+    return rc;
+  }
+
+  // ipdot from the fermion force (+ gauge force when with_gauge)
+  void force(bool fermion, bool gauge) {
+    if (fermion && !p.pure_gauge) {
+      if (solve(c.chi, md, p.res_md)) return;
+      auto t0 = clk::now();
+      const int order = (int)md.a.size();
+      std::vector<double> coef((size_t)c.n_pf * order);
+      for (int ps = 0; ps < c.n_pf; ps++)
+        for (int k = 0; k < order; k++) coef[(size_t)ps * order + k] = md.a[k];
+      if (fermion_force(c, c.n_pf * order, coef.data(), c.smf, 0) && !status) status = -1;
+      if (gauge) gauge_force(c, 1);
+      hipStreamSynchronize(c.stream);
+      t_force += secs(t0);
+    } else {
+      auto t0 = clk::now();
+      gauge_force(c, 0);
+      hipStreamSynchronize(c.stream);
+      t_force += secs(t0);
+    }
+  }
+  void P(double step, bool fermion, bool gauge) {
+    force(fermion, gauge);
+    auto t0 = clk::now();
+    momenta_update(c, step);
+    t_update += secs(t0);
+  }
+  void Q(double step) {
+    auto t0 = clk::now();
+    exp_update(c, step);
+    t_update += secs(t0);
+  }
+
+  static constexpr double kLambda = 0.1931833275037836;  // Omelyan et al.
+
+  void omelyan() {  // minimum_norm2.cc:28-105
+    const double eps = 1.0 / p.no_md, l = kLambda;
+    const bool f = !p.pure_gauge;
+    Q(eps * l);
+    for (int md_ = 1; md_ < p.no_md; md_++) {
+      P(-0.5 * eps, f, true);
+      Q(eps * (1.0 - 2.0 * l));
+      P(-0.5 * eps, f, true);
+      Q(eps * 2.0 * l);
+    }
+    P(-0.5 * eps, f, true);
+    Q(eps * (1.0 - 2.0 * l));
+    P(-0.5 * eps, f, true);
+    Q(eps * l);
+  }
+  void ms2mn_gauge(double scale) {  // multistep_2MN.cc:31-103
+    const double eps = 1.0 / p.no_md, l = kLambda;
+    P(-eps * scale * l, false, true);
+    for (int md_ = 1; md_ < p.gauge_scale; md_++) {
+      Q(0.5 * eps * scale);
+      P(-eps * (1.0 - 2.0 * l) * scale, false, true);
+      Q(0.5 * eps * scale);
+      P(-eps * 2.0 * l * scale, false, true);
+    }
+    Q(0.5 * eps * scale);
+    P(-eps * (1.0 - 2.0 * l) * scale, false, true);
+    Q(0.5 * eps * scale);
+    P(-eps * l * scale, false, true);
+  }
+  void ms2mn() {  // multistep_2MN.cc:104-168
+    const double eps = 1.0 / p.no_md, l = kLambda, gs = 0.5 / (double)p.gauge_scale;
+    P(-eps * l, true, false);
+    for (int md_ = 1; md_ < p.no_md; md_++) {
+      ms2mn_gauge(gs);
+      P(-eps * (1.0 - 2.0 * l), true, false);
+      ms2mn_gauge(gs);
+      P(-eps * 2.0 * l, true, false);
+    }
+    ms2mn_gauge(gs);
+    P(-eps * (1.0 - 2.0 * l), true, false);
+    ms2mn_gauge(gs);
+    P(-eps * l, true, false);
+  }
+  void leapfrog() {  // leapfrog.cc:18-62 with the live momentum sign
+    const double eps = 1.0 / p.no_md;
+    const bool f = !p.pure_gauge;
+    Q(0.5 * eps);
+    for (int md_ = 1; md_ < p.no_md; md_++) {
+      P(-eps, f, true);
+      Q(eps);
+    }
+    P(-eps, f, true);
+    Q(0.5 * eps);
+  }
+};
+
+}  // namespace
+
+int run_trajectory(Context &c, const rhmc_traj_params &p, rhmc_traj_result &r) {
+  memset(&r, 0, sizeof(r));
+  if (p.no_md < 1 || (p.integrator == RHMC_INT_MULTISTEP_2MN && p.gauge_scale < 1)) {
+    c.last_error = "trajectory: bad MD parameters";
+    return RHMC_EINVAL;
+  }
+  if (!p.pure_gauge && (p.order_first < 1 || p.order_md < 1 || p.order_last < 1 || p.order_first > c.max_order ||
+                        p.order_md > c.max_order || p.order_last > c.max_order)) {
+    c.last_error = "trajectory: bad rational orders";
+    return RHMC_EINVAL;
+  }
+  const auto t0 = clk::now();
+  Traj T(c, p, r);
+  const long sp = 3 * c.Vh;
+  // Metropolis backup of the links (Conf::save / copy_saved)
+  hipMemcpyAsync(c.U64_save, c.U64, sizeof(double2) * 72 * c.Vh, hipMemcpyDeviceToDevice, c.stream);
+  if (p.draw_fields) {
+    gauss_momenta(c, p.seed, 0);
+    gauss_spinor(c, c.phi0, c.n_pf, p.seed, 1);
+  }
+  double mn = p.eig_min, mx = p.eig_max;
+  if (!p.pure_gauge && p.bounds_mode == 1) {
+    int loops;
+    gauss_spinor(c, c.phinew, 1, p.seed, 2);
+    if (power_max(c, c.phinew, p.power_tol, &mx, &loops)) return RHMC_EHIP;
+    gauss_spinor(c, c.phinew, 1, p.seed, 3);
+    if (power_min(c, c.phinew, p.power_tol, mx, &mn, &loops)) return RHMC_EHIP;
+  }
+  r.eig_min = mn;
+  r.eig_max = mx;
+  const double e_first = (double)p.n_flavours / 8.0 / (double)c.n_pf;
+  const double e_md = -(double)p.n_flavours / 4.0 / (double)c.n_pf;
+  if (!p.pure_gauge) {
+    // first_inv_approx_calc: chi = r_first(MdagM) phi
+    Coeffs cf = rescale(p.order_first, p.first_a0, p.first_a, p.first_b, mn, mx, e_first);
+    if (T.solve(c.phi0, cf, p.res_metro) && T.status > 0) return RHMC_EMAXCG;
+    hipMemcpyAsync(c.chi, c.phi0, sizeof(double2) * sp * c.n_pf, hipMemcpyDeviceToDevice, c.stream);
+    if (sol_sum(c, c.chi, c.smf, p.order_first, cf.a0, cf.a.data())) return RHMC_EHIP;
+    T.md = rescale(p.order_md, p.md_a0, p.md_a, p.md_b, mn, mx, e_md);
+  }
+  action_local(c, c.phi0, c.chi, !p.pure_gauge, 0);
+  if (p.integrator == RHMC_INT_OMELYAN) T.omelyan();
+  else if (p.integrator == RHMC_INT_MULTISTEP_2MN) T.ms2mn();
+  else T.leapfrog();
+  if (T.status > 0) return RHMC_EMAXCG;
+  if (T.status < 0) return RHMC_EHIP;
+  if (!p.pure_gauge) {
+    // last_inv_approx_calc: phi_new = r_last(MdagM) chi
+    Coeffs cl = rescale(p.order_last, p.last_a0, p.last_a, p.last_b, mn, mx, e_md);
+    if (T.solve(c.chi, cl, p.res_metro)) return T.status > 0 ? RHMC_EMAXCG : RHMC_EHIP;
+    hipMemcpyAsync(c.phinew, c.chi, sizeof(double2) * sp * c.n_pf, hipMemcpyDeviceToDevice, c.stream);
+    if (sol_sum(c, c.phinew, c.smf, p.order_last, cl.a0, cl.a.data())) return RHMC_EHIP;
+  }
+  action_local(c, c.phinew, c.chi, !p.pure_gauge, -1);
+  double s = 0.0;
+  if (action_sum(c, &s)) return RHMC_EHIP;
+  r.delta_h = -s;
+  r.accepted = 1;
+  if (p.metropolis) {
+    const double dh = r.delta_h;
+    r.accepted = (dh < 0.0) || (p.metropolis_u < std::exp(-dh));
+    if (!r.accepted) {
+      hipMemcpyAsync(c.U64, c.U64_save, sizeof(double2) * 72 * c.Vh, hipMemcpyDeviceToDevice, c.stream);
+      refresh_links32(c);
+    }
+  }
+  if (!hip_ok(c, hipStreamSynchronize(c.stream), "trajectory")) return RHMC_EHIP;
+  r.seconds_total = std::chrono::duration<double>(clk::now() - t0).count();
+  r.seconds_solver = T.t_solve;
+  r.seconds_force = T.t_force;
+  r.seconds_update = T.t_update;
+  return RHMC_OK;
+}
+
+}  // namespace rhmc

@@ -1,0 +1,293 @@
+"""ctypes binding of the C ABI in include/gpu.h and include/cu_inverter.h.
+
+This is the reference-side binding pattern a Python harness uses: plain
+pointers and sizes, host arrays in the reference layout (complex128,
+mu*V + i site order).  There is no CPU fallback: if librhmc_amd.so is absent
+every call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import re
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "librhmc_amd.so")
+INCLUDE = os.path.join(os.path.dirname(os.path.dirname(HERE)), "include")
+
+_lib = None
+
+
+class RhmcError(RuntimeError):
+    pass
+
+
+def load():
+    """Load librhmc_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RhmcError(f"{LIB_PATH} not built: run python rhmc-on-gpu_amd/build.py")
+        _lib = C.CDLL(LIB_PATH)
+        _declare(_lib)
+    return _lib
+
+
+def header_functions():
+    """Names of every function declared in include/*.h."""
+    names = []
+    for fn in ("gpu.h", "cu_inverter.h"):
+        txt = open(os.path.join(INCLUDE, fn)).read()
+        txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+        for m in re.finditer(r"^\s*(?:const\s+)?[A-Za-z_][A-Za-z0-9_ ]*[\s\*]+([A-Za-z_][A-Za-z0-9_]*)\s*\(",
+                             txt, flags=re.M):
+            if m.group(1) not in ("if", "while"):
+                names.append(m.group(1))
+    return names
+
+
+class Params(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("nt", C.c_int),
+                ("antiperiodic_t", C.c_int), ("mass", C.c_double), ("beta", C.c_double),
+                ("n_pf", C.c_int), ("max_order", C.c_int), ("device", C.c_int)]
+
+
+class TrajParams(C.Structure):
+    _fields_ = [("n_flavours", C.c_int), ("no_md", C.c_int), ("gauge_scale", C.c_int), ("integrator", C.c_int),
+                ("res_md", C.c_double), ("res_metro", C.c_double), ("residual_relative", C.c_int),
+                ("precision", C.c_int), ("max_cg", C.c_int),
+                ("order_first", C.c_int), ("order_md", C.c_int), ("order_last", C.c_int),
+                ("first_a", C.c_void_p), ("first_b", C.c_void_p), ("first_a0", C.c_double),
+                ("md_a", C.c_void_p), ("md_b", C.c_void_p), ("md_a0", C.c_double),
+                ("last_a", C.c_void_p), ("last_b", C.c_void_p), ("last_a0", C.c_double),
+                ("lambda_min", C.c_double), ("bounds_mode", C.c_int), ("eig_min", C.c_double),
+                ("eig_max", C.c_double), ("power_tol", C.c_double), ("draw_fields", C.c_int),
+                ("seed", C.c_ulonglong), ("metropolis", C.c_int), ("metropolis_u", C.c_double),
+                ("pure_gauge", C.c_int)]
+
+
+class TrajResult(C.Structure):
+    _fields_ = [("delta_h", C.c_double), ("accepted", C.c_int), ("cg_iterations", C.c_int),
+                ("n_solves", C.c_int), ("eig_min", C.c_double), ("eig_max", C.c_double),
+                ("seconds_total", C.c_double), ("seconds_solver", C.c_double),
+                ("seconds_force", C.c_double), ("seconds_update", C.c_double)]
+
+
+VP = C.c_void_p
+D = C.c_double
+I = C.c_int
+PI = C.POINTER(C.c_int)
+PD = C.POINTER(C.c_double)
+U64 = C.c_ulonglong
+
+
+def _declare(L):
+    sig = {
+        "gpu_create": (I, [C.POINTER(Params), C.POINTER(VP)]),
+        "gpu_destroy": (I, [VP]),
+        "gpu_last_error": (C.c_char_p, [VP]),
+        "gpu_init0": (I, [VP, VP]),
+        "gpu_init1": (I, [VP, VP, VP, VP]),
+        "gpu_end": (I, [VP, VP, VP, VP]),
+        "gpu_get_fermions": (I, [VP, I, VP]),
+        "gpu_put_fermions": (I, [VP, I, VP]),
+        "gpu_exp_su3": (I, [VP, D]),
+        "gpu_momenta_sum_multiply": (I, [VP, D]),
+        "gpu_gauge_deriv": (I, [VP, I]),
+        "gpu_fermion_force": (I, [VP, I, VP, I]),
+        "gpu_sol_sum": (I, [VP, I, D, VP]),
+        "gpu_find_max": (I, [VP, PD, D, U64]),
+        "gpu_find_min": (I, [VP, PD, D, D, U64]),
+        "gpu_action": (I, [VP, I, I]),
+        "gpu_action_sum": (I, [VP, PD]),
+        "gpu_gauss_momenta": (I, [VP, U64, U64]),
+        "gpu_gauss_phi": (I, [VP, U64, U64]),
+        "gpu_dslash": (I, [VP, I, I, I, VP, VP]),
+        "gpu_time_dslash": (I, [VP, I, I, I, PD]),
+        "gpu_trajectory": (I, [VP, C.POINTER(TrajParams), C.POINTER(TrajResult)]),
+        "cu_multips_shifted_invert": (I, [VP, D, I, I, VP, I, I, PI]),
+        "cuda_shifted_inverter": (I, [VP, D, VP, I, I, I, PI]),
+        "cu_shifted_residuals": (I, [VP, I, VP, VP]),
+        "cu_last_solve_stats": (I, [VP, PI, PI, PI]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+
+
+def _ptr(a: np.ndarray):
+    if a is None:
+        return None
+    if not a.flags["C_CONTIGUOUS"]:
+        raise ValueError("array must be C-contiguous")
+    return a.ctypes.data_as(C.c_void_p)
+
+
+INTEGRATORS = {"omelyan": 0, "multistep_2mn": 1, "leapfrog": 2}
+FERMION_BUFS = {"mf": 0, "smf": 1, "phi": 2, "chi": 3, "phinew": 4}
+
+
+class Gpu:
+    """One lattice on one GPU (an rhmc_gpu_t handle)."""
+
+    def __init__(self, dims, mass, beta=5.5, n_pf=1, max_order=16, antiperiodic_t=True, device=0):
+        self.L = load()
+        self.dims = tuple(int(d) for d in dims)
+        self.V = int(np.prod(self.dims))
+        self.Vh = self.V // 2
+        self.n_pf = n_pf
+        self.max_order = max_order
+        p = Params(*self.dims, int(antiperiodic_t), float(mass), float(beta), int(n_pf), int(max_order), int(device))
+        h = VP()
+        rc = self.L.gpu_create(C.byref(p), C.byref(h))
+        if rc:
+            raise RhmcError(f"gpu_create failed ({rc})")
+        self.h = h
+        self.order = None
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.L.gpu_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc, what):
+        if rc:
+            msg = self.L.gpu_last_error(self.h)
+            raise RhmcError(f"{what}: rc={rc} {msg.decode() if msg else ''}")
+
+    # -- transfers
+    def init0(self, links):
+        links = np.ascontiguousarray(links, np.complex128)
+        assert links.shape == (4 * self.V, 3, 3)
+        self._check(self.L.gpu_init0(self.h, _ptr(links)), "gpu_init0")
+
+    def init1(self, phi=None, ipdot=None, momenta=None):
+        a = [None if x is None else np.ascontiguousarray(x, np.complex128) for x in (phi, ipdot, momenta)]
+        self._check(self.L.gpu_init1(self.h, *[_ptr(x) for x in a]), "gpu_init1")
+
+    def end(self, links=True, ipdot=True, momenta=True):
+        out = [np.zeros((4 * self.V, 3, 3), np.complex128) if f else None for f in (links, ipdot, momenta)]
+        self._check(self.L.gpu_end(self.h, *[_ptr(x) for x in out]), "gpu_end")
+        return out
+
+    def get_fermions(self, which):
+        n = self.n_pf * (self.order or self.max_order) if which == "smf" else self.n_pf
+        out = np.zeros((n, self.Vh, 3), np.complex128)
+        self._check(self.L.gpu_get_fermions(self.h, FERMION_BUFS[which], _ptr(out)), "gpu_get_fermions")
+        return out
+
+    def put_fermions(self, which, arr):
+        arr = np.ascontiguousarray(arr, np.complex128)
+        self._check(self.L.gpu_put_fermions(self.h, FERMION_BUFS[which], _ptr(arr)), "gpu_put_fermions")
+
+    # -- operations
+    def dslash(self, par_out, inp, precision=0):
+        inp = np.ascontiguousarray(inp, np.complex128)
+        if inp.ndim == 2:
+            inp = inp[None]
+        out = np.zeros_like(inp)
+        self._check(self.L.gpu_dslash(self.h, par_out, precision, inp.shape[0], _ptr(inp), _ptr(out)), "gpu_dslash")
+        return out
+
+    def time_dslash(self, precision, nvec, reps):
+        ms = C.c_double()
+        self._check(self.L.gpu_time_dslash(self.h, precision, nvec, reps, C.byref(ms)), "gpu_time_dslash")
+        return ms.value
+
+    def multishift(self, shifts, res, relative=False, precision=0, max_cg=100000):
+        shifts = np.ascontiguousarray(shifts, np.float64)
+        it = C.c_int()
+        self._check(self.L.cu_multips_shifted_invert(self.h, float(res), int(relative), len(shifts), _ptr(shifts),
+                                                     int(precision), int(max_cg), C.byref(it)),
+                    "cu_multips_shifted_invert")
+        self.order = len(shifts)
+        return it.value
+
+    def shifted_residuals(self, shifts):
+        shifts = np.ascontiguousarray(shifts, np.float64)
+        out = np.zeros(self.n_pf * len(shifts))
+        self._check(self.L.cu_shifted_residuals(self.h, len(shifts), _ptr(shifts), _ptr(out)), "residuals")
+        return out.reshape(self.n_pf, len(shifts))
+
+    def solve_stats(self):
+        a, b, c = C.c_int(), C.c_int(), C.c_int()
+        self.L.cu_last_solve_stats(self.h, C.byref(a), C.byref(b), C.byref(c))
+        return a.value, b.value, c.value
+
+    def fermion_force(self, numerators, accumulate=False):
+        num = np.ascontiguousarray(numerators, np.float64)
+        self._check(self.L.gpu_fermion_force(self.h, len(num), _ptr(num), int(accumulate)), "gpu_fermion_force")
+
+    def gauge_deriv(self, accumulate=False):
+        self._check(self.L.gpu_gauge_deriv(self.h, int(accumulate)), "gpu_gauge_deriv")
+
+    def exp_su3(self, step):
+        self._check(self.L.gpu_exp_su3(self.h, float(step)), "gpu_exp_su3")
+
+    def momenta_sum_multiply(self, step):
+        self._check(self.L.gpu_momenta_sum_multiply(self.h, float(step)), "gpu_momenta_sum_multiply")
+
+    def sol_sum(self, a0, numerators):
+        num = np.ascontiguousarray(numerators, np.float64)
+        self._check(self.L.gpu_sol_sum(self.h, len(num), float(a0), _ptr(num)), "gpu_sol_sum")
+
+    def find_max(self, tol=1e-5, seed=1):
+        v = C.c_double()
+        self._check(self.L.gpu_find_max(self.h, C.byref(v), float(tol), seed), "gpu_find_max")
+        return v.value
+
+    def find_min(self, mx, tol=1e-5, seed=2):
+        v = C.c_double()
+        self._check(self.L.gpu_find_min(self.h, C.byref(v), float(mx), float(tol), seed), "gpu_find_min")
+        return v.value
+
+    def action(self, init, with_fermions=True):
+        self._check(self.L.gpu_action(self.h, int(init), int(with_fermions)), "gpu_action")
+
+    def action_sum(self):
+        v = C.c_double()
+        self._check(self.L.gpu_action_sum(self.h, C.byref(v)), "gpu_action_sum")
+        return v.value
+
+    def gauss_momenta(self, seed, stream=0):
+        self._check(self.L.gpu_gauss_momenta(self.h, seed, stream), "gpu_gauss_momenta")
+
+    def gauss_phi(self, seed, stream=1):
+        self._check(self.L.gpu_gauss_phi(self.h, seed, stream), "gpu_gauss_phi")
+
+    def trajectory(self, *, n_flavours, first, md, last, no_md, integrator="omelyan", gauge_scale=1,
+                   res_md=1e-10, res_metro=1e-10, relative=False, precision=0, max_cg=100000,
+                   lambda_min=1e-5, bounds=None, power_tol=1e-5, draw_fields=False, seed=1,
+                   metropolis=False, metropolis_u=0.5, pure_gauge=False):
+        """first/md/last: (a0, a[], b[]) normalised coefficients; bounds=(min,max)
+        for fixed spectral bounds, None to measure them with power iterations."""
+        keep = []
+
+        def arr(x):
+            a = np.ascontiguousarray(x, np.float64)
+            keep.append(a)
+            return _ptr(a)
+
+        t = TrajParams(n_flavours=n_flavours, no_md=no_md, gauge_scale=gauge_scale,
+                       integrator=INTEGRATORS[integrator], res_md=res_md, res_metro=res_metro,
+                       residual_relative=int(relative), precision=int(precision), max_cg=max_cg,
+                       order_first=len(first[1]), order_md=len(md[1]), order_last=len(last[1]),
+                       first_a=arr(first[1]), first_b=arr(first[2]), first_a0=first[0],
+                       md_a=arr(md[1]), md_b=arr(md[2]), md_a0=md[0],
+                       last_a=arr(last[1]), last_b=arr(last[2]), last_a0=last[0],
+                       lambda_min=lambda_min, bounds_mode=0 if bounds else 1,
+                       eig_min=bounds[0] if bounds else 0.0, eig_max=bounds[1] if bounds else 0.0,
+                       power_tol=power_tol, draw_fields=int(draw_fields), seed=seed,
+                       metropolis=int(metropolis), metropolis_u=metropolis_u, pure_gauge=int(pure_gauge))
+        r = TrajResult()
+        self._check(self.L.gpu_trajectory(self.h, C.byref(t), C.byref(r)), "gpu_trajectory")
+        return r

@@ -256,8 +256,11 @@ def remez_power(n: int, pnum: int, pden: int, lam_low: float = 1e-5, inverse: bo
                           power=(sign * g.numerator, g.denominator), meta=meta)
 
 
+APPROX_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "approx")
+
+
 def rhmc_approximations(n_flavours: int, n_pf: int, order_first: int, order_md: int, order_last: int,
-                        lam_low: float = 1e-5, cache_dir: str | None = None):
+                        lam_low: float = 1e-5, cache_dir: str | None = APPROX_DIR):
     """The three normalised approximations of rationalapprox_calc.cc:8-177.
 
     first: x^(+Nf/(8 N_pf))   (heatbath, getPFE)

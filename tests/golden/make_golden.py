@@ -74,7 +74,7 @@ def make(cfg_name):
     in_e = fields.gauss_spinor(lat, SEEDS["in"], 2)
     ip = fields.gauss_momenta(lat, SEEDS["ipdot"]) * 1j  # anti-hermitian traceless
     first, md, last = rhmc_approximations(cfg["no_flavours"], npf, cfg["approx_metro"], cfg["approx_md"],
-                                          cfg["approx_metro"], 1e-5, cache_dir=HERE)
+                                          cfg["approx_metro"], 1e-5)
     apps = [(x.a0, x.a, x.b) for x in (first, md, last)]
     mass = cfg["mass"]
     eig_min = mass * mass

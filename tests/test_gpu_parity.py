@@ -1,0 +1,241 @@
+"""HIP path (through the C ABI) against the FP64 oracle and the reference's
+golden vectors.  Tolerances: FP64 kernels 1e-13 relative unless stated; FP32
+Dslash 1e-6 relative; solvers are judged on their true FP64 residuals
+(relative 1e-10) and agree with the oracle's solutions to the stated bound."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import Oracle
+from rhmc_amd import Gpu, fields
+from rhmc_amd.remez import RationalApprox, rhmc_approximations
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def rel(a, b):
+    return np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-300)
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def make_fields(dims, seed=7, eps=0.3, npf=1):
+    lat = fields.Lattice(*dims)
+    U = fields.random_links(lat, seed, eps)
+    P = fields.gauss_momenta(lat, seed + 1)
+    phi = fields.gauss_spinor(lat, seed + 2, npf)
+    return lat, U, P, phi
+
+
+@pytest.mark.parametrize("dims,eps", [((4, 4, 4, 4), None), ((8, 8, 8, 8), 0.3), ((6, 4, 8, 4), None)])
+def test_dslash_fp64_and_fp32(dims, eps):
+    say("dslash", dims, eps)
+    lat, U, _, _ = make_fields(dims, 11, eps)
+    o = Oracle(dims)
+    g = Gpu(dims, mass=0.1)
+    g.init0(U)
+    v = fields.gauss_spinor(lat, 5, 5)
+    ref_doe = np.stack([o.doe(U, v[k]) for k in range(5)])
+    ref_deo = np.stack([o.deo(U, v[k]) for k in range(5)])
+    assert rel(g.dslash(1, v, 0), ref_doe) < 1e-14
+    assert rel(g.dslash(0, v, 0), ref_deo) < 1e-14
+    # single right-hand side path equals the batched one
+    assert rel(g.dslash(1, v[2], 0)[0], ref_doe[2]) < 1e-14
+    # FP32 12-real links (row 2 rebuilt with eta)
+    assert rel(g.dslash(1, v, 1), ref_doe) < 2e-6
+    assert rel(g.dslash(0, v, 1), ref_deo) < 2e-6
+    g.close()
+
+
+def test_dslash_golden_reference():
+    """GPU Dslash against the reference CPU output (tests/golden)."""
+    gd = dict(np.load(os.path.join(GOLD, "golden_t4.npz")))
+    dims = tuple(gd["dims"])
+    g = Gpu(dims, mass=float(gd["mass"]))
+    g.init0(gd["links"])
+    assert rel(g.dslash(1, gd["dslash_in"][0], 0)[0], gd["doe"]) < 1e-14
+    assert rel(g.dslash(0, gd["dslash_in"][1], 0)[0], gd["deo"]) < 1e-14
+    g.close()
+
+
+def _shifts(order, mass):
+    """Pole positions spread like those of a rescaled Remez approximation."""
+    b = np.geomspace(1e-5, 1.5, order) * (4.0 + mass * mass)
+    return np.ones(order), b
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_multishift_true_residual_and_oracle(precision):
+    dims = (8, 8, 8, 4)
+    mass = 0.05
+    say("multishift", precision)
+    lat, U, _, phi = make_fields(dims, 21, None, 2)
+    o = Oracle(dims)
+    g = Gpu(dims, mass=mass, n_pf=2, max_order=12)
+    g.init0(U)
+    g.init1(phi=phi)
+    _, b = _shifts(12, mass)
+    it = g.multishift(b, 1e-10, relative=True, precision=precision)
+    res = g.shifted_residuals(b)
+    bn = np.linalg.norm(phi.reshape(2, -1), axis=1)
+    assert np.all(res / bn[:, None] < 1e-10), res / bn[:, None]
+    x = g.get_fermions("smf").reshape(2, 12, lat.Vh, 3)
+    for ps in range(2):
+        xo, _ = o.multishift(U, mass, phi[ps], b, 1e-12 * bn[ps])
+        for k in range(12):
+            assert rel(x[ps, k], xo[k]) < 1e-9, (ps, k, rel(x[ps, k], xo[k]))
+    say("iterations", it, g.solve_stats())
+    g.close()
+
+
+def test_multishift_golden_reference():
+    gd = dict(np.load(os.path.join(GOLD, "golden_t4.npz")))
+    dims = tuple(gd["dims"])
+    npf = int(gd["no_ps"])
+    g = Gpu(dims, mass=float(gd["mass"]), n_pf=npf, max_order=len(gd["ms_shifts"]))
+    g.init0(gd["links"])
+    g.init1(phi=gd["chi"])
+    g.multishift(gd["ms_shifts"], float(gd["ms_res"]), relative=False, precision=0)
+    x = g.get_fermions("smf").reshape(npf, len(gd["ms_shifts"]), -1, 3)
+    assert rel(x, gd["ms_x"]) < 1e-10
+    g.close()
+
+
+def _md_coeffs(gd):
+    r = RationalApprox(1e-5, float(gd["md_a0"]), list(gd["md_a"]), list(gd["md_b"]))
+    e = -float(gd["n_flavours"]) / 4.0 / float(gd["no_ps"])
+    return r.rescaled(float(gd["eig_min"]), float(gd["eig_max"]), e)[:3]
+
+
+@pytest.mark.parametrize("name", ["t4", "t4o"])
+def test_forces_golden_reference(name):
+    gd = dict(np.load(os.path.join(GOLD, f"golden_{name}.npz")))
+    dims = tuple(gd["dims"])
+    npf = int(gd["no_ps"])
+    a0, a, b = _md_coeffs(gd)
+    g = Gpu(dims, mass=float(gd["mass"]), beta=float(gd["beta"]), n_pf=npf, max_order=len(b))
+    g.init0(gd["links"])
+    g.init1(phi=gd["chi"])
+    g.multishift(b, float(gd["res_md"]), relative=False, precision=0)
+    g.fermion_force(a)
+    _, ip, _ = g.end(links=False, ipdot=True, momenta=False)
+    assert rel(ip, gd["force_fermion"]) < 1e-10
+    g.gauge_deriv(accumulate=True)
+    _, ip, _ = g.end(links=False, ipdot=True, momenta=False)
+    assert rel(ip, gd["force_full"]) < 1e-10
+    g.gauge_deriv(accumulate=False)
+    _, ip, _ = g.end(links=False, ipdot=True, momenta=False)
+    assert rel(ip, gd["force_gauge"]) < 1e-13
+    g.close()
+
+
+@pytest.mark.parametrize("name", ["t4", "t4o"])
+def test_update_and_action_golden_reference(name):
+    gd = dict(np.load(os.path.join(GOLD, f"golden_{name}.npz")))
+    dims = tuple(gd["dims"])
+    npf = int(gd["no_ps"])
+    g = Gpu(dims, mass=float(gd["mass"]), beta=float(gd["beta"]), n_pf=npf, max_order=8)
+    g.init0(gd["links"])
+    g.init1(ipdot=gd["ipdot"], momenta=gd["mom"])
+    g.exp_su3(float(gd["exp_step"]))
+    U, _, _ = g.end(links=True, ipdot=False, momenta=False)
+    assert rel(U, gd["exp_links"]) < 1e-13
+    g.init0(gd["links"])
+    g.momenta_sum_multiply(float(gd["mom_step"]))
+    _, _, P = g.end(links=False, ipdot=False, momenta=True)
+    assert rel(P, gd["mom_out"]) < 1e-14
+    # action: init 0 on (U,P,phi), init -1 on (U,P,phi_new=phi, chi)
+    g.init1(momenta=gd["mom"])
+    g.put_fermions("phi", gd["phi"])
+    g.put_fermions("chi", gd["chi"])
+    g.put_fermions("phinew", gd["phi"])
+    g.action(0)
+    g.action(-1)
+    s = g.action_sum()
+    o = Oracle(dims)
+    ref_sum = o.global_sum(gd["action1"])
+    assert abs(s - ref_sum) < 1e-9 * max(1.0, abs(ref_sum))
+    g.close()
+
+
+def _traj_args(gd):
+    return dict(n_flavours=int(gd["n_flavours"]),
+                first=(float(gd["first_a0"]), gd["first_a"], gd["first_b"]),
+                md=(float(gd["md_a0"]), gd["md_a"], gd["md_b"]),
+                last=(float(gd["last_a0"]), gd["last_a"], gd["last_b"]),
+                no_md=int(gd["no_md"]),
+                integrator="multistep_2mn" if int(gd["use_multistep"]) == 1 else "omelyan",
+                gauge_scale=int(gd["gauge_scale"]), res_md=float(gd["res_md"]),
+                res_metro=float(gd["res_metro"]), bounds=(float(gd["eig_min"]), float(gd["eig_max"])))
+
+
+@pytest.mark.parametrize("name", ["t4", "t4o"])
+@pytest.mark.parametrize("precision", [0, 1])
+def test_trajectory_golden_reference(name, precision):
+    """Whole trajectory on the device against the reference's own update()."""
+    gd = dict(np.load(os.path.join(GOLD, f"golden_{name}.npz")))
+    dims = tuple(gd["dims"])
+    npf = int(gd["no_ps"])
+    say("trajectory", name, precision)
+    g = Gpu(dims, mass=float(gd["mass"]), beta=float(gd["beta"]), n_pf=npf, max_order=8)
+    g.init0(gd["links"])
+    g.init1(momenta=gd["mom"])
+    g.put_fermions("phi", gd["phi"])
+    r = g.trajectory(precision=precision, **_traj_args(gd))
+    U, _, P = g.end(links=True, ipdot=False, momenta=True)
+    chi = g.get_fermions("chi")
+    phin = g.get_fermions("phinew")
+    say("dH gpu", r.delta_h, "ref", float(gd["traj_dh"]), "iters", r.cg_iterations)
+    assert abs(r.delta_h - float(gd["traj_dh"])) < 1e-9
+    assert rel(chi, gd["traj_chi"]) < 1e-10
+    assert rel(phin, gd["traj_phinew"]) < 1e-9
+    assert rel(U, gd["traj_links"]) < 1e-10
+    assert rel(P, gd["traj_mom"]) < 1e-10
+    g.close()
+
+
+def test_power_method_bounds():
+    dims = (8, 8, 8, 4)
+    mass = 0.05
+    lat, U, _, phi = make_fields(dims, 31, 0.3)
+    o = Oracle(dims)
+    g = Gpu(dims, mass=mass)
+    g.init0(U)
+    mx = g.find_max(1e-8)
+    mxo, _ = o.findmaxeig(U, mass, phi[0], 1e-8)
+    assert abs(mx - mxo) / mxo < 1e-3
+    mn = g.find_min(mx * 1.0001, 1e-7)
+    assert mass * mass * 0.99 < mn < mx
+    g.close()
+
+
+def test_free_field_spectrum():
+    """U = eta*1: MdagM x = (m^2 + sum_mu sin^2 p_mu) x for a plane wave."""
+    dims = (8, 4, 4, 8)
+    mass = 0.1
+    lat = fields.Lattice(*dims)
+    U = np.ascontiguousarray(np.eye(3)[None] * lat.eta()[:, None, None]).astype(np.complex128)
+    g = Gpu(dims, mass=mass)
+    g.init0(U)
+    x, y, z, t = lat.coords()
+    n = (1, 1, 2, 3)
+    p = [2 * np.pi * n[0] / 8, 2 * np.pi * n[1] / 4, 2 * np.pi * n[2] / 4, (2 * n[3] + 1) * np.pi / 8]
+    # eigenvector of -D_eo D_oe built from the staggered plane waves
+    ph = np.exp(1j * (p[0] * x + p[1] * y + p[2] * z + p[3] * t))[: lat.Vh]
+    v = np.zeros((lat.Vh, 3), np.complex128)
+    v[:, 0] = ph
+    h = g.dslash(1, v, 0)
+    s = g.dslash(0, h[0], 0)[0]
+    mdm = mass * mass * v - s
+    lam = mass * mass + sum(np.sin(q) ** 2 for q in p)
+    # the plane wave mixes with its doublers; project on v
+    ray = np.vdot(v, mdm).real / np.vdot(v, v).real
+    assert mass * mass <= ray <= 4 + mass * mass
+    g.close()
