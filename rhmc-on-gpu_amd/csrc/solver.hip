@@ -89,12 +89,12 @@ static void *ws_get(Context &c, int slot, size_t bytes) {
 // ------------------------------------------------------------- reductions
 constexpr int kMaxRed = 256;  // vectors reduced in one launch
 
-__device__ __forceinline__ void store_partial(double *p, double v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ double load_partial(const double *p) {
-  return __hip_atomic_load(const_cast<double *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
+// Partials are written with plain stores and consumed by a separate
+// one-workgroup finalize launch: the kernel boundary makes them visible
+// without a device-scope fence per workgroup (such a fence writes back the
+// XCD's L2 and serialises tens of thousands of workgroups).
+__device__ __forceinline__ void store_partial(double *p, double v) { *p = v; }
+__device__ __forceinline__ double load_partial(const double *p) { return *p; }
 
 // block-reduce NV values per thread and store one partial per vector
 template <int NV>
@@ -117,24 +117,6 @@ __device__ __forceinline__ void block_partials(double (&v)[NV], double *partial,
       store_partial(partial + (long)j * nbx + bx, s);
     }
   }
-}
-
-// true in exactly one workgroup: the last one to arrive
-__device__ __forceinline__ bool arrive_last(unsigned *counter, unsigned total) {
-  __shared__ int last;
-  __threadfence();
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    unsigned prev = atomicAdd(counter, 1u);
-    last = (prev == total - 1);
-  }
-  __syncthreads();
-  if (last) __threadfence();
-  return last;
-}
-
-__device__ __forceinline__ void release_counter(unsigned *counter) {
-  if (threadIdx.x == 0) __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // deterministic final sums: wave w reduces vectors w, w+4, ...
@@ -174,6 +156,7 @@ __device__ void run_epilogue(int epi, void *state, const double *sums, int nvec,
       s.stop = s.done;
     } else if (epi == EPI_MS_B) {
       MsState &s = ((MsState *)state)[j];
+      if (s.stop) s.done = 1;  // the iteration that set stop has completed
       if (s.done) continue;
       // inverter.cc:224-245
       const double alpha = v;
@@ -220,6 +203,7 @@ __device__ void run_epilogue(int epi, void *state, const double *sums, int nvec,
       s.stop = s.done;
     } else if (epi == EPI_CG_B) {
       CgState &s = ((CgState *)state)[j];
+      if (s.stop) s.done = 1;
       if (s.done) continue;
       s.alpha = s.delta / v;
     } else if (epi == EPI_CG_C) {
@@ -241,16 +225,18 @@ struct RedArgs {
   int epi;
 };
 
-template <int NV>
-__device__ __forceinline__ void reduce_and_epilogue(double (&v)[NV], const RedArgs &ra, int jbase, int nvec,
-                                                    int nbx, int bx) {
-  block_partials<NV>(v, ra.partial, jbase, nvec, nbx, bx);
-  if (arrive_last(ra.counter, gridDim.x * gridDim.y)) {
-    __shared__ double sums[kMaxRed];
-    final_sums(ra.partial, nvec, nbx, sums);
-    run_epilogue(ra.epi, ra.state, sums, nvec, ra.sums);
-    __syncthreads();
-    release_counter(ra.counter);
+// one workgroup: fixed-order sums of the per-workgroup partials, then the
+// scalar recurrences; *active receives the number of systems still running.
+__global__ __launch_bounds__(kBlock) void k_finalize(RedArgs ra, int nvec, int nbx, int *active) {
+  __shared__ double sums[kMaxRed];
+  final_sums(ra.partial, nvec, nbx, sums);
+  run_epilogue(ra.epi, ra.state, sums, nvec, ra.sums);
+  __syncthreads();
+  if (active && threadIdx.x == 0) {
+    int n = 0;
+    const bool ms = ra.epi == EPI_MS_INIT || ra.epi == EPI_MS_B || ra.epi == EPI_MS_C;
+    for (int j = 0; j < nvec; j++) n += ms ? !((MsState *)ra.state)[j].stop : !((CgState *)ra.state)[j].stop;
+    *active = n;
   }
 }
 
@@ -387,7 +373,7 @@ __global__ __launch_bounds__(kBlock) void k_dslash(DsArgs a) {
       }
     }
   }
-  if (RED != RED_NONE) reduce_and_epilogue<NR>(red, a.red, jbase, a.nvec, gridDim.x, blockIdx.x);
+  if (RED != RED_NONE) block_partials<NR>(red, a.red.partial, jbase, a.nvec, gridDim.x, blockIdx.x);
 }
 
 // number of right-hand sides handled per thread for a launch with nvec vectors
@@ -470,13 +456,6 @@ __global__ __launch_bounds__(kBlock) void k_ms_init(BlasArgs a) {
   }
   // one y-block per rhs: partial index uses j
   block_partials<1>(v, a.red.partial, j, a.nvec, gridDim.x, blockIdx.x);
-  if (arrive_last(a.red.counter, gridDim.x * gridDim.y)) {
-    __shared__ double sums[kMaxRed];
-    final_sums(a.red.partial, a.nvec, gridDim.x, sums);
-    run_epilogue(a.red.epi, a.red.state, sums, a.nvec, a.red.sums);
-    __syncthreads();
-    release_counter(a.red.counter);
-  }
 }
 
 // step C: r += omega s; lambda = |r|^2
@@ -504,13 +483,6 @@ __global__ __launch_bounds__(kBlock) void k_ms_update_r(BlasArgs a) {
     }
   }
   block_partials<1>(v, a.red.partial, j, a.nvec, gridDim.x, blockIdx.x);
-  if (arrive_last(a.red.counter, gridDim.x * gridDim.y)) {
-    __shared__ double sums[kMaxRed];
-    final_sums(a.red.partial, a.nvec, gridDim.x, sums);
-    run_epilogue(EPI_MS_C, a.ms, sums, a.nvec, nullptr);
-    __syncthreads();
-    release_counter(a.red.counter);
-  }
 }
 
 // step D: p = r + gamma p ; for active shifts x_s -= om_s p_s, p_s = zeta_s r + ga_s p_s
@@ -553,11 +525,6 @@ __global__ __launch_bounds__(kBlock) void k_ms_update_p(BlasArgs a) {
       }
     }
   }
-  if (arrive_last(a.red.counter, gridDim.x * gridDim.y)) {
-    for (int t = threadIdx.x; t < a.nvec; t += blockDim.x) a.ms[t].done = a.ms[t].stop;
-    __syncthreads();
-    release_counter(a.red.counter);
-  }
 }
 
 // ---- batched single-shift CG (refinement of every shifted system)
@@ -585,13 +552,6 @@ __global__ __launch_bounds__(kBlock) void k_cg_init(BlasArgs a) {
     }
   }
   block_partials<1>(v, a.red.partial, j, a.nvec, gridDim.x, blockIdx.x);
-  if (arrive_last(a.red.counter, gridDim.x * gridDim.y)) {
-    __shared__ double sums[kMaxRed];
-    final_sums(a.red.partial, a.nvec, gridDim.x, sums);
-    run_epilogue(EPI_CG_INIT, a.cg, sums, a.nvec, nullptr);
-    __syncthreads();
-    release_counter(a.red.counter);
-  }
 }
 
 // x += alpha p ; r -= alpha s ; lambda = |r|^2
@@ -620,13 +580,6 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_xr(BlasArgs a) {
     }
   }
   block_partials<1>(v, a.red.partial, j, a.nvec, gridDim.x, blockIdx.x);
-  if (arrive_last(a.red.counter, gridDim.x * gridDim.y)) {
-    __shared__ double sums[kMaxRed];
-    final_sums(a.red.partial, a.nvec, gridDim.x, sums);
-    run_epilogue(EPI_CG_C, a.cg, sums, a.nvec, nullptr);
-    __syncthreads();
-    release_counter(a.red.counter);
-  }
 }
 
 // p = r + beta p
@@ -650,11 +603,6 @@ __global__ __launch_bounds__(kBlock) void k_cg_update_p(BlasArgs a) {
       pv.y = rv.y + be * pv.y;
       p[(long)i * Vh] = pv;
     }
-  }
-  if (arrive_last(a.red.counter, gridDim.x * gridDim.y)) {
-    for (int t = threadIdx.x; t < a.nvec; t += blockDim.x) a.cg[t].done = a.cg[t].stop;
-    __syncthreads();
-    release_counter(a.red.counter);
   }
 }
 
@@ -680,15 +628,20 @@ __global__ __launch_bounds__(kBlock) void k_copy_f2d(double2 *x64, const float2 
 // ------------------------------------------------------------- host drivers
 static unsigned nbx_of(Context &c) { return (unsigned)((c.Vh + kBlock - 1) / kBlock); }
 
-// poll the done flags (one int per state, stride in ints) after a chunk
-static bool all_done(Context &c, const int *dflag, int stride, int n) {
+// systems still running, written by every solver finalize launch
+static int *active_ptr(Context &c) { return (int *)(c.ws->counter + 1); }
+
+static void finalize(Context &c, const RedArgs &ra, int nvec, bool count_active) {
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, c.stream, ra, nvec, (int)nbx_of(c),
+                     count_active ? active_ptr(c) : (int *)nullptr);
+}
+
+// read the active count after a chunk of iterations (one 4-byte copy)
+static bool none_active(Context &c) {
   int *hf = c.ws->h_flags;
-  for (int j = 0; j < n; j++)
-    hipMemcpyAsync(hf + j, dflag + (long)j * stride, sizeof(int), hipMemcpyDeviceToHost, c.stream);
+  hipMemcpyAsync(hf, active_ptr(c), sizeof(int), hipMemcpyDeviceToHost, c.stream);
   hipStreamSynchronize(c.stream);
-  for (int j = 0; j < n; j++)
-    if (!hf[j]) return false;
-  return true;
+  return hf[0] == 0;
 }
 
 // multi-shift CG in precision R with link format LINK on sources b (FP64).
@@ -728,26 +681,30 @@ static int ms_cg(Context &c, const double2 *b, int nrhs, int nshift, const doubl
   ba.b = b; ba.r = r; ba.p = p; ba.s = s; ba.ps = ps; ba.x = x; ba.ms = ms;
   ba.red = make_red(c, nrhs, EPI_MS_INIT, ms, nullptr);
   hipLaunchKernelGGL(k_ms_init<R>, dim3(nbx, nrhs), dim3(kBlock), 0, c.stream, ba);
+  finalize(c, ba.red, nrhs, true);
+  RedArgs red_c = ba.red;
+  red_c.epi = EPI_MS_C;
 
   DsArgs hop = {};
   hop.par_out = 1; hop.U = LINK == LINK_D18 ? (const void *)c.U64 : (const void *)c.U32;
   hop.in = p; hop.in_vs = 3 * Vh; hop.out = hh; hop.out_vs = 3 * Vh; hop.nvec = nrhs;
-  hop.skip = &ms[0].done; hop.skip_stride = sizeof(MsState) / sizeof(int);
+  hop.skip = &ms[0].stop; hop.skip_stride = sizeof(MsState) / sizeof(int);
   DsArgs mdm = hop;
   mdm.par_out = 0; mdm.in = hh; mdm.out = s; mdm.pin = p; mdm.pin_vs = 3 * Vh; mdm.diag = diag;
   mdm.red = make_red(c, nrhs, EPI_MS_B, ms, nullptr);
 
   const int chunk = 8;
   int it = 0;
-  while (true) {
+  while (!none_active(c)) {
     for (int k = 0; k < chunk; k++) {
       dslash_dispatch<R, LINK, DS_HOP, RED_NONE>(c, hop);
       dslash_dispatch<R, LINK, DS_MDAGM, RED_DOT>(c, mdm);
+      finalize(c, mdm.red, nrhs, false);
       hipLaunchKernelGGL(k_ms_update_r<R>, dim3(nbx, nrhs), dim3(kBlock), 0, c.stream, ba);
+      finalize(c, red_c, nrhs, true);
       hipLaunchKernelGGL(k_ms_update_p<R>, dim3(nbx, nrhs), dim3(kBlock), 0, c.stream, ba);
     }
     it += chunk;
-    if (all_done(c, &ms[0].done, sizeof(MsState) / sizeof(int), nrhs)) break;
     if (it > max_iter + chunk) break;
   }
   hipMemcpyAsync(hs.data(), ms, sizeof(MsState) * nrhs, hipMemcpyDeviceToHost, c.stream);
@@ -793,24 +750,28 @@ static int block_cg32(Context &c, const double2 *r64, int nsys, const double *sh
   ba.b = r64; ba.r = r; ba.p = p; ba.s = s; ba.x = x32; ba.cg = cg;
   ba.red = make_red(c, nsys, EPI_CG_INIT, cg, nullptr);
   hipLaunchKernelGGL(k_cg_init<float>, dim3(nbx, nsys), dim3(kBlock), 0, c.stream, ba);
+  finalize(c, ba.red, nsys, true);
+  RedArgs red_c = ba.red;
+  red_c.epi = EPI_CG_C;
   DsArgs hop = {};
   hop.par_out = 1; hop.U = c.U32;
   hop.in = p; hop.in_vs = 3 * Vh; hop.out = hh; hop.out_vs = 3 * Vh; hop.nvec = nsys;
-  hop.skip = &cg[0].done; hop.skip_stride = sizeof(CgState) / sizeof(int);
+  hop.skip = &cg[0].stop; hop.skip_stride = sizeof(CgState) / sizeof(int);
   DsArgs mdm = hop;
   mdm.par_out = 0; mdm.in = hh; mdm.out = s; mdm.pin = p; mdm.pin_vs = 3 * Vh; mdm.diag = diag;
   mdm.red = make_red(c, nsys, EPI_CG_B, cg, nullptr);
   const int chunk = 8;
   int it = 0;
-  while (true) {
+  while (!none_active(c)) {
     for (int k = 0; k < chunk; k++) {
       dslash_dispatch<float, LINK_S12, DS_HOP, RED_NONE>(c, hop);
       dslash_dispatch<float, LINK_S12, DS_MDAGM, RED_DOT>(c, mdm);
+      finalize(c, mdm.red, nsys, false);
       hipLaunchKernelGGL(k_cg_update_xr<float>, dim3(nbx, nsys), dim3(kBlock), 0, c.stream, ba);
+      finalize(c, red_c, nsys, true);
       hipLaunchKernelGGL(k_cg_update_p<float>, dim3(nbx, nsys), dim3(kBlock), 0, c.stream, ba);
     }
     it += chunk;
-    if (all_done(c, &cg[0].done, sizeof(CgState) / sizeof(int), nsys)) break;
     if (it > max_iter + chunk) break;
   }
   hipMemcpyAsync(hs.data(), cg, sizeof(CgState) * nsys, hipMemcpyDeviceToHost, c.stream);
@@ -842,6 +803,7 @@ int residual_d(Context &c, const double2 *b, int b_group, const double2 *x, cons
   a.diag = diag; a.nvec = nvec;
   a.red = make_red(c, nvec, EPI_STORE, nullptr, sums);
   dslash_dispatch<double, LINK_D18, DS_RESID, RED_NORM>(c, a);
+  finalize(c, a.red, nvec, false);
   hipMemcpyAsync(norms2, sums, sizeof(double) * nvec, hipMemcpyDeviceToHost, c.stream);
   return hip_ok(c, hipStreamSynchronize(c.stream), "residual") ? 0 : -1;
 }
@@ -932,13 +894,6 @@ __global__ __launch_bounds__(kBlock) void k_norms(const double2 *v, long vs, lon
       s[0] += a.x * a.x + a.y * a.y;
     }
   block_partials<1>(s, red.partial, j, nvec, gridDim.x, blockIdx.x);
-  if (arrive_last(red.counter, gridDim.x * gridDim.y)) {
-    __shared__ double sums[kMaxRed];
-    final_sums(red.partial, nvec, gridDim.x, sums);
-    run_epilogue(EPI_STORE, nullptr, sums, nvec, red.sums);
-    __syncthreads();
-    release_counter(red.counter);
-  }
 }
 
 int vec_norms2(Context &c, const double2 *v, int nvec, long vs, double *out) {
@@ -946,6 +901,7 @@ int vec_norms2(Context &c, const double2 *v, int nvec, long vs, double *out) {
   double *sums = (double *)ws_get(c, S_SUMS, sizeof(double) * 256);
   RedArgs red = make_red(c, nvec, EPI_STORE, nullptr, sums);
   hipLaunchKernelGGL(k_norms, dim3(nbx_of(c), nvec), dim3(kBlock), 0, c.stream, v, vs, c.Vh, nvec, red);
+  finalize(c, red, nvec, false);
   hipMemcpyAsync(out, sums, sizeof(double) * nvec, hipMemcpyDeviceToHost, c.stream);
   return hip_ok(c, hipStreamSynchronize(c.stream), "norms") ? 0 : -1;
 }
@@ -1000,6 +956,7 @@ static int power_iter(Context &c, const double2 *start, double tol, int is_min, 
     if (!is_min) {
       a.red = make_red(c, 1, EPI_STORE, nullptr, sums);
       dslash_dispatch<double, LINK_D18, DS_MDAGM, RED_NORM>(c, a);
+      finalize(c, a.red, 1, false);
       hipMemcpyAsync(&nrm2, sums, sizeof(double), hipMemcpyDeviceToHost, c.stream);
       hipStreamSynchronize(c.stream);
     } else {

@@ -89,44 +89,41 @@ class _Remez:
         return _polyval(self.p, x) / (_polyval(self.q, x) * self.f(x)) - 1
 
     def solve_reference(self, xs):
+        """Levelled-error solution on the reference xs.
+
+        Linearised fixed-point iteration on E: with Q_m the denominator of
+        the previous iterate, solve the linear system
+            P(x_k) - f_k Q(x_k) - (-1)^k E f_k Q_m(x_k) = 0,   Q monic,
+        for (P, Q, E) and repeat until E is stable."""
         n = self.n
         m = 2 * n + 2
-        A = mp.matrix(m, m)
-        B = mp.matrix(m, m)
-        for k, x in enumerate(xs):
-            y = self.f(x)
-            s = 1 if k % 2 == 0 else -1
-            xp = mp.mpf(1)
-            for j in range(n + 1):
-                A[k, j] = xp
-                A[k, n + 1 + j] = -y * xp
-                B[k, n + 1 + j] = y * s * xp
-                xp *= x
-        # A v = E B v  ->  (A^-1 B) v = (1/E) v
-        C = mp.inverse(A) * B
-        ev, vecs = mp.eig(C)
-        best = None
-        for idx, mu in enumerate(ev):
-            if abs(mp.im(mu)) > mp.mpf(10) ** (-self.dps // 3) * max(1, abs(mu)) or abs(mu) == 0:
-                continue
-            E = 1 / mp.re(mu)
-            v = [mp.re(vecs[i, idx]) for i in range(m)]
-            p, q = v[: n + 1], v[n + 1:]
-            # Q must not vanish on [lo,hi]: all its real roots must be negative
-            if q[-1] == 0:
-                continue
-            qn = q[-1]
-            p = [c / qn for c in p]
-            q = [c / qn for c in q]
-            roots = mp.polyroots(list(reversed(q)), maxsteps=400, extraprec=4 * self.dps)
-            bad = any(abs(mp.im(r)) < mp.mpf(10) ** (-10) and self.lo <= mp.re(r) <= self.hi for r in roots)
-            if bad:
-                continue
-            if best is None or abs(E) < abs(best[0]):
-                best = (E, p, q)
-        if best is None:
-            raise RuntimeError("Remez: no admissible levelled solution")
-        self.E, self.p, self.q = best
+        q_prev = getattr(self, "q", None)
+        ys = [self.f(x) for x in xs]
+        E_old = None
+        for _ in range(60):
+            A = mp.matrix(m, m)
+            rhs = mp.matrix(m, 1)
+            for k, x in enumerate(xs):
+                y = ys[k]
+                s = 1 if k % 2 == 0 else -1
+                xp = mp.mpf(1)
+                for j in range(n + 1):
+                    A[k, j] = xp
+                    if j < n:
+                        A[k, n + 1 + j] = -y * xp
+                    xp *= x
+                qm = _polyval(q_prev, x) if q_prev is not None else mp.power(x, n)
+                A[k, m - 1] = -s * y * qm
+                rhs[k] = y * mp.power(x, n)
+            sol = mp.lu_solve(A, rhs)
+            p = [sol[j] for j in range(n + 1)]
+            q = [sol[n + 1 + j] for j in range(n)] + [mp.mpf(1)]
+            E = sol[m - 1]
+            q_prev = q
+            if E_old is not None and abs(E - E_old) <= mp.mpf(10) ** (-(self.dps // 2)) * abs(E):
+                break
+            E_old = E
+        self.E, self.p, self.q = E, p, q
 
     def extrema(self, xs):
         """Locate the 2n+2 extrema of the relative error (one per sign interval)."""
@@ -148,7 +145,7 @@ class _Remez:
                     la, ea = lm, em
                 else:
                     lb = lm
-                if lb - la < mp.mpf(10) ** (-(self.dps // 2)):
+                if lb - la < mp.mpf(10) ** (-18):
                     break
             zeros.append(mp.exp((la + lb) / 2))
         bounds = [lo] + zeros + [hi]
@@ -176,7 +173,7 @@ class _Remez:
                     la, c, gc = c, d, gd
                     d = la + gr * (lb - la)
                     gd = g(d)
-                if lb - la < mp.mpf(10) ** (-(self.dps // 3)):
+                if lb - la < mp.mpf(10) ** (-12):
                     break
             new.append(mp.exp((la + lb) / 2))
         # the end points are extrema only if the error there is extremal in
@@ -243,7 +240,7 @@ def _generate(n: int, pnum: int, pden: int, lo: float, inverse: bool, dps: int, 
 
 
 def remez_power(n: int, pnum: int, pden: int, lam_low: float = 1e-5, inverse: bool = False,
-                dps: int = 80, tol: float = 1e-15) -> RationalApprox:
+                dps: int = 60, tol: float = 1e-15) -> RationalApprox:
     """(n,n) minimax approximation of x^(pnum/pden) on [lam_low, 1].
 
     inverse=False: partial fractions of r ~ x^(pnum/pden)   (getPFE,  alg_remez.cc:205-245)
