@@ -4,10 +4,14 @@ MI355X, 32^3x8 lattice.
 Workload (one "step" = one RHMC trajectory, all fields resident in HBM):
   32^3x8, Nf=2 staggered, am=0.01, beta=5.6, 8 pseudofermions, 16 poles for
   the heatbath, MD and Metropolis rational functions (Remez on [1e-5,1]),
-  Omelyan 2MN with 10 steps of tau=1, every multi-shift solve to relative
-  residual 1e-10, links exp(i 0.3 H) seeded, momenta/pseudofermions drawn on
-  the device from the seed.  The value is the mean mixed-precision trajectory
-  time; the FP64 path runs the same trajectories from the same start.
+  Omelyan 2MN with 10 steps of tau=1, links exp(i 0.3 H) seeded,
+  momenta/pseudofermions drawn on the device from the seed.  Heatbath and
+  Metropolis solves to relative residual 1e-10, MD force solves to 1e-6
+  relative.  Mixed precision: MD solves in FP32 (12-real links), heatbath and
+  Metropolis solves FP32 + FP64 refinement; Delta H of every mixed trajectory
+  is recomputed on the same endpoints with FP64 solves.  The FP64 path runs
+  the same trajectories from the same start with every solve in FP64.  The
+  value is the mean mixed-precision trajectory time.
 
 Prints ONE JSON line.  --gpus above 1 is refused (one-GPU build).
 """
@@ -75,6 +79,8 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--dims", type=str, default="32,32,32,8")
     ap.add_argument("--npf", type=int, default=8)
+    ap.add_argument("--res-md", type=float, default=1e-6)
+    ap.add_argument("--res-metro", type=float, default=1e-10)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported"}))
@@ -93,13 +99,14 @@ def main():
     mn = g.find_min(mx, 1e-5, seed=12)
     mn = max(mn, mass * mass)
     print(f"[bench] spectral bounds {mn:.4e} {mx:.4e}", file=sys.stderr, flush=True)
-    traj_kw = dict(n_flavours=2, no_md=args.no_md, integrator="omelyan", res_md=1e-10, res_metro=1e-10,
-                   relative=True, bounds=(mn, mx), draw_fields=True, metropolis=False, **co)
+    traj_kw = dict(n_flavours=2, no_md=args.no_md, integrator="omelyan", res_md=args.res_md,
+                   res_metro=args.res_metro, relative=True, bounds=(mn, mx), draw_fields=True,
+                   metropolis=False, fp32_threshold=1e-7, **co)
 
-    def run(precision, seed):
+    def run(precision, seed, verify=False):
         g.init0(U0)
         t0 = time.perf_counter()
-        r = g.trajectory(precision=precision, seed=seed, **traj_kw)
+        r = g.trajectory(precision=precision, seed=seed, verify_fp64=verify, **traj_kw)
         dt = time.perf_counter() - t0
         print(f"[bench] trajectory precision={'mixed' if precision else 'fp64'} seed={seed} {dt:.3f}s "
               f"dH={r.delta_h:.3e} iters={r.cg_iterations} solves={r.n_solves} "
@@ -109,12 +116,16 @@ def main():
     for w in range(args.warmup):
         run(1, 100 + w)
     t_mixed, t_fp64, dh_m, dh_d, iters_m, iters_d = [], [], [], [], [], []
+    dh_m64 = []
     r_d = None
     for s in range(args.steps):
         dt, r = run(1, 1000 + s)
         t_mixed.append(dt)
         dh_m.append(r.delta_h)
         iters_m.append(r.cg_iterations)
+        # same trajectory again (identical), now with the FP64 recomputation of Delta H
+        _, rv = run(1, 1000 + s, verify=True)
+        dh_m64.append(rv.delta_h_fp64)
         if not args.no_fp64:
             dt, r_d = run(0, 1000 + s)
             t_fp64.append(dt)
@@ -159,13 +170,15 @@ def main():
         "data": "synthetic (seeded exp(i 0.3 H) links, device-drawn momenta and pseudofermions)",
         "config": {"workload": f"RHMC trajectory {'x'.join(map(str, dims))}, Nf=2, am=0.01, beta=5.6, "
                                f"{n_pf} pseudofermions x {order} poles, Omelyan {args.no_md} steps, "
-                               f"relative residual 1e-10",
+                               f"residual 1e-10 (heatbath/Metropolis), {args.res_md:g} (MD), relative",
                    "lattice": list(dims), "n_pf": n_pf, "poles": order, "no_md": args.no_md},
         "fp64_s_per_traj": fp64,
         "mixed_vs_fp64_speedup": (fp64 / val) if fp64 else None,
         "delta_h_mixed": dh_m,
         "delta_h_fp64": dh_d,
-        "delta_h_abs_diff": [abs(a - b) for a, b in zip(dh_m, dh_d)],
+        "delta_h_mixed_fp64_recomputed": dh_m64,
+        "delta_h_error_vs_fp64_recomputation": [abs(a - b) for a, b in zip(dh_m, dh_m64)],
+        "delta_h_abs_diff_mixed_vs_fp64_trajectory": [abs(a - b) for a, b in zip(dh_m, dh_d)],
         "cg_iterations_mixed": iters_m,
         "cg_iterations_fp64": iters_d,
         "spectral_bounds": [mn, mx],

@@ -18,7 +18,7 @@
 extern "C" {
 #endif
 
-enum { RHMC_PREC_FP64 = 0, RHMC_PREC_MIXED = 1 };
+enum { RHMC_PREC_FP64 = 0, RHMC_PREC_MIXED = 1, RHMC_PREC_FP32 = 2 };
 
 /* include/cu_inverter.h:7 + lib/Action/Inverter/cu_inverter.cc:21-121.
  * res: stopping residual; relative=0 reproduces the reference's absolute
@@ -26,6 +26,9 @@ enum { RHMC_PREC_FP64 = 0, RHMC_PREC_MIXED = 1 };
  * precision RHMC_PREC_FP64: FP64 vectors, FP64 18-real links.
  * precision RHMC_PREC_MIXED: FP32 iterations on FP32 12-real links followed by
  * FP64 defect correction until every shifted system reaches res in FP64.
+ * precision RHMC_PREC_FP32: FP32 iterations only (the reference's single
+ * precision path, lib/Cuda/cuda_inverter.cu:647-858): the recursive residual
+ * reaches res, the true residual is limited by FP32 rounding (~1e-6 relative).
  * iterations (may be NULL) receives the CG iteration count.
  * Returns RHMC_EMAXCG when max_cg is exhausted (the reference exits). */
 int cu_multips_shifted_invert(rhmc_gpu_t h, double res, int relative, int order, const double *shifts,

@@ -125,6 +125,12 @@ typedef struct {
   double res_md, res_metro;       /* multi-shift stopping residuals           */
   int residual_relative;          /* 1: residuals relative to |rhs|           */
   int precision;                  /* 0 FP64, 1 mixed FP32 + FP64 refinement   */
+  double fp32_threshold;          /* mixed: solves with res >= this run FP32
+                                     only (the reference's
+                                     inv_single_double_prec dispatch,
+                                     cu_inverter.cc:45-60); 0 = never       */
+  int verify_fp64;                /* 1: recompute Delta H of the same
+                                     endpoints with FP64 solves             */
   int max_cg;
   int order_first, order_md, order_last;
   const double *first_a, *first_b; double first_a0;   /* normalised on [eps,1] */
@@ -143,6 +149,7 @@ typedef struct {
 
 typedef struct {
   double delta_h;
+  double delta_h_fp64;            /* FP64 recomputation (verify_fp64=1)       */
   int accepted;
   int cg_iterations;              /* summed over all solves                   */
   int n_solves;

@@ -14,7 +14,7 @@ namespace rhmc {
 constexpr int kMaxShifts = 32;   // poles per rational function
 constexpr int kBlock = 256;      // threads per workgroup (4 wave64)
 
-enum Prec { PREC_FP64 = 0, PREC_MIXED = 1 };
+enum Prec { PREC_FP64 = 0, PREC_MIXED = 1, PREC_FP32 = 2 };
 
 // per right-hand side state of the multi-shift CG (FP64 scalars on device)
 struct MsState {
@@ -78,7 +78,8 @@ int residual_d(Context &c, const double2 *b, int b_group, const double2 *x, cons
                double2 *r, double *norms2);
 
 // multi-shift CG.  b: [nrhs][3][Vh] FP64; x: [nrhs][nshift][3][Vh] FP64 out.
-// prec PREC_FP64 or PREC_MIXED (FP32 iterations + FP64 refinement).
+// prec PREC_FP64, PREC_MIXED (FP32 iterations + FP64 refinement) or PREC_FP32
+// (FP32 iterations only; the target is floored at the FP32 accuracy).
 // rel=0: res is an absolute residual (reference convention);
 // rel=1: res is relative to |b| per right-hand side.
 int multishift_solve(Context &c, const double2 *b, int nrhs, int nshift, const double *shifts, double res,
@@ -100,6 +101,8 @@ int gauss_momenta(Context &c, unsigned long long seed, unsigned long long stream
 int gauss_spinor(Context &c, double2 *out, int n, unsigned long long seed, unsigned long long stream_id);
 int refresh_links32(Context &c);
 int vec_norms2(Context &c, const double2 *v, int nvec, long vstride, double *out);
+// out[j] = Re<a_j, b_j> (fixed-order FP64 reduction)
+int vec_dots(Context &c, const double2 *a, const double2 *b, int nvec, long vstride, double *out);
 
 // host helpers
 bool hip_ok(Context &c, hipError_t e, const char *what);

@@ -96,6 +96,32 @@ constexpr int kMaxRed = 256;  // vectors reduced in one launch
 __device__ __forceinline__ void store_partial(double *p, double v) { *p = v; }
 __device__ __forceinline__ double load_partial(const double *p) { return *p; }
 
+// ----------------------------------------------------- work decomposition
+// Reducing launches use `nbr` workgroups per vector, each owning a contiguous
+// range of sites (a multiple of kBlock); fewer partials keep the finalize
+// short while the grid still holds >= 8 workgroups per CU.
+__device__ __forceinline__ void site_range(long Vh, unsigned bx, unsigned nbr, long &h0, long &h1) {
+  const long nb = (Vh + kBlock - 1) / kBlock;
+  const long per = (nb + nbr - 1) / nbr * kBlock;
+  h0 = (long)bx * per;
+  h1 = h0 + per < Vh ? h0 + per : Vh;
+}
+
+// 1-D grid of nsb site blocks x nch vector chunks, flattened so that the
+// workgroups dispatched consecutively to one XCD (hardware ids b, b+8, ...)
+// cover the same site block for every chunk: the shared links then stay in
+// that XCD's L2.  Each XCD owns a contiguous slab of site blocks.
+__device__ __forceinline__ void xcd_split(unsigned b, unsigned nsb, unsigned nch, unsigned &sb, unsigned &ch) {
+  if (nsb % 8 == 0) {
+    const unsigned xcd = b % 8, l = b / 8;
+    sb = xcd * (nsb / 8) + l / nch;
+    ch = l % nch;
+  } else {
+    sb = b % nsb;
+    ch = b / nsb;
+  }
+}
+
 // block-reduce NV values per thread and store one partial per vector
 template <int NV>
 __device__ __forceinline__ void block_partials(double (&v)[NV], double *partial, int jbase, int nvec, int nbx,
@@ -114,7 +140,7 @@ __device__ __forceinline__ void block_partials(double (&v)[NV], double *partial,
       double s = 0.0;
 #pragma unroll
       for (int k = 0; k < kBlock / 64; k++) s += sh[k][r];
-      store_partial(partial + (long)j * nbx + bx, s);
+      partial[(long)j * nbx + bx] = s;
     }
   }
 }
@@ -124,7 +150,7 @@ __device__ __forceinline__ void final_sums(const double *partial, int nvec, int 
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   for (int j = w; j < nvec; j += kBlock / 64) {
     double s = 0.0;
-    for (int b = lane; b < nbx; b += 64) s += load_partial(partial + (long)j * nbx + b);
+    for (int b = lane; b < nbx; b += 64) s += partial[(long)j * nbx + b];
     s = wave_sum(s);
     if (lane == 0) out_sh[j] = s;
   }
@@ -134,80 +160,106 @@ __device__ __forceinline__ void final_sums(const double *partial, int nvec, int 
 // ------------------------------------------------------------- epilogues
 enum Epi { EPI_STORE = 0, EPI_MS_INIT, EPI_MS_B, EPI_MS_C, EPI_CG_INIT, EPI_CG_B, EPI_CG_C };
 
-__device__ void run_epilogue(int epi, void *state, const double *sums, int nvec, double *out_sums) {
+struct RedArgs {
+  double *partial;
+  unsigned *counter;
+  double *sums;
+  void *state;
+  int epi;
+  int nbr;  // partials per vector
+};
+
+// Multi-shift recurrences of inverter.cc:186-284, one thread per right-hand
+// side for the shared scalars and one per (rhs, shift) for the shifted ones.
+__device__ void ms_epilogue(int epi, MsState *st, const double *sums, int nvec) {
+  __shared__ int live[kMaxRed];
+  __shared__ double g_new[kMaxRed], delta_old[kMaxRed];
+  const int ns = st[0].nshift;
   for (int j = threadIdx.x; j < nvec; j += blockDim.x) {
+    MsState &s = st[j];
     const double v = sums[j];
-    if (epi == EPI_STORE) {
-      out_sums[j] = v;
-    } else if (epi == EPI_MS_INIT) {
-      MsState &s = ((MsState *)state)[j];
+    if (epi == EPI_MS_INIT) {
       s.delta = v;
       s.omega = 1.0;
       s.omega_prev = 1.0;
       s.gammag = 0.0;
       s.iter = 0;
-      for (int k = 0; k < s.nshift; k++) {
-        s.z1[k] = 1.0; s.z2[k] = 1.0; s.z3[k] = 1.0;
-        s.om[k] = 0.0; s.ga[k] = 0.0;
-        s.flag[k] = 1; s.flag_next[k] = 1; s.its[k] = 0;
-      }
       s.nact = s.nshift;
       s.done = (v == 0.0) ? 1 : 0;  // zero source: x = 0
       s.stop = s.done;
+      live[j] = 1;
     } else if (epi == EPI_MS_B) {
-      MsState &s = ((MsState *)state)[j];
       if (s.stop) s.done = 1;  // the iteration that set stop has completed
-      if (s.done) continue;
-      // inverter.cc:224-245
-      const double alpha = v;
-      s.omega_prev = s.omega;
-      s.omega = -s.delta / alpha;
-      for (int k = 0; k < s.nshift; k++) {
-        s.flag[k] = s.flag_next[k];
-        if (!s.flag[k]) continue;
-        s.z3[k] = (s.z1[k] * s.z2[k] * s.omega_prev) /
-                  (s.omega * s.gammag * (s.z1[k] - s.z2[k]) +
-                   s.z1[k] * s.omega_prev * (1.0 - s.shift[k] * s.omega));
-        s.om[k] = s.omega * s.z3[k] / s.z2[k];
+      live[j] = !s.done;
+      if (live[j]) {
+        s.omega_prev = s.omega;
+        s.omega = -s.delta / v;  // omega = -(r,r)/(p,Ap)
       }
-    } else if (epi == EPI_MS_C) {
-      MsState &s = ((MsState *)state)[j];
-      if (s.done) continue;
-      // inverter.cc:248-284
-      const double lambda = v;
-      const double g = lambda / s.delta;
-      int nact = 0;
-      for (int k = 0; k < s.nshift; k++) {
-        if (!s.flag[k]) continue;
-        s.ga[k] = g * s.z3[k] * s.om[k] / (s.z2[k] * s.omega);
-        const double fact = sqrt(s.delta * s.z2[k] * s.z2[k]);
-        if (fact < s.res) {
-          s.flag_next[k] = 0;
-          s.its[k] = s.iter + 1;
-        } else {
-          nact++;
-        }
-        s.z1[k] = s.z2[k];
-        s.z2[k] = s.z3[k];
+    } else {  // EPI_MS_C
+      live[j] = !s.done;
+      delta_old[j] = s.delta;
+      g_new[j] = v / s.delta;  // gamma = (r_new,r_new)/(r,r)
+    }
+  }
+  __syncthreads();
+  for (int idx = threadIdx.x; idx < nvec * ns; idx += blockDim.x) {
+    const int j = idx / ns, k = idx - j * ns;
+    if (!live[j]) continue;
+    MsState &s = st[j];
+    if (epi == EPI_MS_INIT) {
+      s.z1[k] = 1.0; s.z2[k] = 1.0; s.z3[k] = 1.0;
+      s.om[k] = 0.0; s.ga[k] = 0.0;
+      s.flag[k] = 1; s.flag_next[k] = 1; s.its[k] = 0;
+    } else if (epi == EPI_MS_B) {
+      const int f = s.flag_next[k];
+      s.flag[k] = f;
+      if (!f) continue;
+      const double z1 = s.z1[k], z2 = s.z2[k], om = s.omega, omp = s.omega_prev;
+      const double z3 = (z1 * z2 * omp) / (om * s.gammag * (z1 - z2) + z1 * omp * (1.0 - s.shift[k] * om));
+      s.z3[k] = z3;
+      s.om[k] = om * z3 / z2;
+    } else {
+      if (!s.flag[k]) continue;
+      const double z2 = s.z2[k], z3 = s.z3[k];
+      s.ga[k] = g_new[j] * z3 * s.om[k] / (z2 * s.omega);
+      const double fact = sqrt(delta_old[j] * z2 * z2);
+      if (fact < s.res) {
+        s.flag_next[k] = 0;
+        s.its[k] = s.iter + 1;
       }
-      s.nact = nact;
-      s.gammag = g;
-      s.delta = lambda;
-      s.iter++;
-      s.stop = !(sqrt(lambda) > s.res && s.iter < s.max_iter) || nact == 0;
-    } else if (epi == EPI_CG_INIT) {
-      CgState &s = ((CgState *)state)[j];
+      s.z1[k] = z2;
+      s.z2[k] = z3;
+    }
+  }
+  if (epi != EPI_MS_C) return;
+  __syncthreads();
+  for (int j = threadIdx.x; j < nvec; j += blockDim.x) {
+    if (!live[j]) continue;
+    MsState &s = st[j];
+    int nact = 0;
+    for (int k = 0; k < ns; k++) nact += s.flag_next[k];
+    s.nact = nact;
+    s.gammag = g_new[j];
+    s.delta = sums[j];
+    s.iter++;
+    s.stop = !(sqrt(sums[j]) > s.res && s.iter < s.max_iter) || nact == 0;
+  }
+}
+
+__device__ void cg_epilogue(int epi, CgState *st, const double *sums, int nvec) {
+  for (int j = threadIdx.x; j < nvec; j += blockDim.x) {
+    CgState &s = st[j];
+    const double v = sums[j];
+    if (epi == EPI_CG_INIT) {
       s.delta = v;
       s.iter = 0;
       s.done = (v <= s.tol2) ? 1 : 0;
       s.stop = s.done;
     } else if (epi == EPI_CG_B) {
-      CgState &s = ((CgState *)state)[j];
       if (s.stop) s.done = 1;
       if (s.done) continue;
       s.alpha = s.delta / v;
-    } else if (epi == EPI_CG_C) {
-      CgState &s = ((CgState *)state)[j];
+    } else {
       if (s.done) continue;
       s.beta = v / s.delta;
       s.delta = v;
@@ -217,26 +269,23 @@ __device__ void run_epilogue(int epi, void *state, const double *sums, int nvec,
   }
 }
 
-struct RedArgs {
-  double *partial;
-  unsigned *counter;
-  double *sums;
-  void *state;
-  int epi;
-};
-
 // one workgroup: fixed-order sums of the per-workgroup partials, then the
 // scalar recurrences; *active receives the number of systems still running.
-__global__ __launch_bounds__(kBlock) void k_finalize(RedArgs ra, int nvec, int nbx, int *active) {
+__global__ __launch_bounds__(kBlock) void k_finalize(RedArgs ra, int nvec, int *active) {
   __shared__ double sums[kMaxRed];
-  final_sums(ra.partial, nvec, nbx, sums);
-  run_epilogue(ra.epi, ra.state, sums, nvec, ra.sums);
-  __syncthreads();
-  if (active && threadIdx.x == 0) {
-    int n = 0;
-    const bool ms = ra.epi == EPI_MS_INIT || ra.epi == EPI_MS_B || ra.epi == EPI_MS_C;
-    for (int j = 0; j < nvec; j++) n += ms ? !((MsState *)ra.state)[j].stop : !((CgState *)ra.state)[j].stop;
-    *active = n;
+  final_sums(ra.partial, nvec, ra.nbr, sums);
+  const bool ms = ra.epi == EPI_MS_INIT || ra.epi == EPI_MS_B || ra.epi == EPI_MS_C;
+  const bool cg = ra.epi == EPI_CG_INIT || ra.epi == EPI_CG_B || ra.epi == EPI_CG_C;
+  if (ms) ms_epilogue(ra.epi, (MsState *)ra.state, sums, nvec);
+  else if (cg) cg_epilogue(ra.epi, (CgState *)ra.state, sums, nvec);
+  else
+    for (int j = threadIdx.x; j < nvec; j += blockDim.x) ra.sums[j] = sums[j];
+  if (active) {
+    __syncthreads();
+    const int t = threadIdx.x;
+    const int run = t < nvec ? (ms ? !((MsState *)ra.state)[t].stop : !((CgState *)ra.state)[t].stop) : 0;
+    const int n = __syncthreads_count(run);
+    if (t == 0) *active = n;
   }
 }
 
@@ -254,8 +303,9 @@ struct DsArgs {
   const void *pin; long pin_vs;        // MDAGM/RESID: vector at the output site
   const double2 *b; long b_vs; int b_group;  // RESID
   const double *diag;                  // per vector m^2 + shift
-  const int *skip; int skip_stride;    // done flags (ints), may be null
+  const int *skip; int skip_stride;    // stop flags (ints), may be null
   int nvec;
+  unsigned nsb, nch;                   // site blocks x vector chunks
   RedArgs red;
 };
 
@@ -276,10 +326,11 @@ __global__ __launch_bounds__(kBlock) void k_dslash(DsArgs a) {
   using C = Cx<R>;
   const LatticeDims &L = a.L;
   const long Vh = L.Vh;
-  const int bx = xcd_block(blockIdx.x, gridDim.x);
-  const long h = (long)bx * kBlock + threadIdx.x;
-  const int jbase = blockIdx.y * NR;
-  const bool site_ok = h < Vh;
+  unsigned sb, ch;
+  xcd_split(blockIdx.x, a.nsb, a.nch, sb, ch);
+  const int jbase = ch * NR;
+  long h0, h1;
+  site_range(Vh, sb, a.nsb, h0, h1);
 
   bool act[NR];
   bool any = false;
@@ -293,7 +344,7 @@ __global__ __launch_bounds__(kBlock) void k_dslash(DsArgs a) {
 #pragma unroll
   for (int r = 0; r < NR; r++) red[r] = 0.0;
 
-  if (site_ok && any) {
+  for (long h = h0 + threadIdx.x; any && h < h1; h += kBlock) {
     const SiteCoord c = site_coords(L, a.par_out, h);
     const int pin_ = a.par_out, pnb = 1 - a.par_out;
     C acc[NR][3];
@@ -373,7 +424,7 @@ __global__ __launch_bounds__(kBlock) void k_dslash(DsArgs a) {
       }
     }
   }
-  if (RED != RED_NONE) block_partials<NR>(red, a.red.partial, jbase, a.nvec, gridDim.x, blockIdx.x);
+  if (RED != RED_NONE) block_partials<NR>(red, a.red.partial, jbase, a.nvec, a.nsb, sb);
 }
 
 // number of right-hand sides handled per thread for a launch with nvec vectors
@@ -384,11 +435,26 @@ static int pick_nr(int nvec) {
   return 1;
 }
 
+static unsigned nbx_of(Context &c) { return (unsigned)((c.Vh + kBlock - 1) / kBlock); }
+
+// workgroups per vector for a reducing launch with `ny` vectors/chunks:
+// at least 2048 workgroups overall, at most one per kBlock sites
+static unsigned reduce_blocks(Context &c, int ny) {
+  const unsigned nbx = nbx_of(c);
+  unsigned want = (2048u + ny - 1) / ny;
+  if (want < 64) want = 64;
+  want = (want + 7) / 8 * 8;
+  return want < nbx ? want : nbx;
+}
+
 template <typename R, int LINK, int MODE, int RED>
-static void dslash_dispatch(Context &c, DsArgs a) {
+static void dslash_dispatch(Context &c, DsArgs &a) {
   const int nr = pick_nr(a.nvec);
-  dim3 grid((unsigned)((c.Vh + kBlock - 1) / kBlock), (unsigned)((a.nvec + nr - 1) / nr));
   a.L = c.L;
+  a.nch = (unsigned)((a.nvec + nr - 1) / nr);
+  a.nsb = RED == RED_NONE ? nbx_of(c) : reduce_blocks(c, a.nch);
+  a.red.nbr = (int)a.nsb;
+  dim3 grid(a.nsb * a.nch);
   switch (nr) {
     case 8: hipLaunchKernelGGL((k_dslash<R, LINK, 8, MODE, RED>), grid, dim3(kBlock), 0, c.stream, a); break;
     case 4: hipLaunchKernelGGL((k_dslash<R, LINK, 4, MODE, RED>), grid, dim3(kBlock), 0, c.stream, a); break;
@@ -399,12 +465,13 @@ static void dslash_dispatch(Context &c, DsArgs a) {
 
 static RedArgs make_red(Context &c, int nvec, int epi, void *state, double *sums) {
   RedArgs r;
-  const int nbx = (int)((c.Vh + kBlock - 1) / kBlock);
+  const int nbx = (int)nbx_of(c);
   r.partial = (double *)ws_get(c, S_PARTIAL, sizeof(double) * (size_t)nbx * std::max(nvec, 1));
   r.counter = c.ws->counter;
   r.sums = sums;
   r.state = state;
   r.epi = epi;
+  r.nbr = nbx;
   return r;
 }
 
@@ -419,10 +486,12 @@ void launch_hop_d(Context &c, int par_out, const double2 *in, double2 *out, int 
 }
 
 // --------------------------------------------------------------- BLAS kernels
+// grid: (nbr site blocks, one y-block per vector); reducing kernels loop over
+// their contiguous site range
 struct BlasArgs {
   long Vh;
   int nvec, nshift;
-  const void *b;      // MS init: FP64 sources
+  const void *b;      // init: FP64 sources / residuals
   void *r, *p, *s, *ps, *x;
   MsState *ms;
   CgState *cg;
@@ -433,12 +502,13 @@ struct BlasArgs {
 template <typename R>
 __global__ __launch_bounds__(kBlock) void k_ms_init(BlasArgs a) {
   using C = Cx<R>;
-  const int bx = xcd_block(blockIdx.x, gridDim.x);
-  const long h = (long)bx * kBlock + threadIdx.x;
+  const unsigned bx = xcd_block(blockIdx.x, gridDim.x);
   const int j = blockIdx.y;
   const long Vh = a.Vh;
+  long h0, h1;
+  site_range(Vh, bx, gridDim.x, h0, h1);
   double v[1] = {0.0};
-  if (h < Vh) {
+  for (long h = h0 + threadIdx.x; h < h1; h += kBlock) {
     const double2 *b = (const double2 *)a.b + (long)j * 3 * Vh + h;
     C *r = (C *)a.r + (long)j * 3 * Vh + h, *p = (C *)a.p + (long)j * 3 * Vh + h;
     for (int i = 0; i < 3; i++) {
@@ -454,35 +524,37 @@ __global__ __launch_bounds__(kBlock) void k_ms_init(BlasArgs a) {
       }
     }
   }
-  // one y-block per rhs: partial index uses j
-  block_partials<1>(v, a.red.partial, j, a.nvec, gridDim.x, blockIdx.x);
+  block_partials<1>(v, a.red.partial, j, a.nvec, gridDim.x, bx);
 }
 
 // step C: r += omega s; lambda = |r|^2
 template <typename R>
 __global__ __launch_bounds__(kBlock) void k_ms_update_r(BlasArgs a) {
   using C = Cx<R>;
-  const int bx = xcd_block(blockIdx.x, gridDim.x);
-  const long h = (long)bx * kBlock + threadIdx.x;
+  const unsigned bx = xcd_block(blockIdx.x, gridDim.x);
   const int j = blockIdx.y;
   const long Vh = a.Vh;
+  long h0, h1;
+  site_range(Vh, bx, gridDim.x, h0, h1);
   double v[1] = {0.0};
   const MsState &st = a.ms[j];
-  if (h < Vh && !st.done) {
+  if (!st.done) {
     const R om = (R)st.omega;
-    C *r = (C *)a.r + (long)j * 3 * Vh + h;
-    const C *s = (const C *)a.s + (long)j * 3 * Vh + h;
+    for (long h = h0 + threadIdx.x; h < h1; h += kBlock) {
+      C *r = (C *)a.r + (long)j * 3 * Vh + h;
+      const C *s = (const C *)a.s + (long)j * 3 * Vh + h;
 #pragma unroll
-    for (int i = 0; i < 3; i++) {
-      C rv = r[(long)i * Vh];
-      const C sv = s[(long)i * Vh];
-      rv.x += om * sv.x;
-      rv.y += om * sv.y;
-      r[(long)i * Vh] = rv;
-      v[0] += (double)rv.x * (double)rv.x + (double)rv.y * (double)rv.y;
+      for (int i = 0; i < 3; i++) {
+        C rv = r[(long)i * Vh];
+        const C sv = s[(long)i * Vh];
+        rv.x += om * sv.x;
+        rv.y += om * sv.y;
+        r[(long)i * Vh] = rv;
+        v[0] += (double)rv.x * (double)rv.x + (double)rv.y * (double)rv.y;
+      }
     }
   }
-  block_partials<1>(v, a.red.partial, j, a.nvec, gridDim.x, blockIdx.x);
+  block_partials<1>(v, a.red.partial, j, a.nvec, gridDim.x, bx);
 }
 
 // step D: p = r + gamma p ; for active shifts x_s -= om_s p_s, p_s = zeta_s r + ga_s p_s
@@ -532,12 +604,13 @@ __global__ __launch_bounds__(kBlock) void k_ms_update_p(BlasArgs a) {
 template <typename R>
 __global__ __launch_bounds__(kBlock) void k_cg_init(BlasArgs a) {
   using C = Cx<R>;
-  const int bx = xcd_block(blockIdx.x, gridDim.x);
-  const long h = (long)bx * kBlock + threadIdx.x;
+  const unsigned bx = xcd_block(blockIdx.x, gridDim.x);
   const int j = blockIdx.y;
   const long Vh = a.Vh;
+  long h0, h1;
+  site_range(Vh, bx, gridDim.x, h0, h1);
   double v[1] = {0.0};
-  if (h < Vh) {
+  for (long h = h0 + threadIdx.x; h < h1; h += kBlock) {
     const double2 *b = (const double2 *)a.b + (long)j * 3 * Vh + h;
     C *r = (C *)a.r + (long)j * 3 * Vh + h, *p = (C *)a.p + (long)j * 3 * Vh + h;
     C *x = (C *)a.x + (long)j * 3 * Vh + h;
@@ -551,35 +624,38 @@ __global__ __launch_bounds__(kBlock) void k_cg_init(BlasArgs a) {
       v[0] += (double)cv.x * (double)cv.x + (double)cv.y * (double)cv.y;
     }
   }
-  block_partials<1>(v, a.red.partial, j, a.nvec, gridDim.x, blockIdx.x);
+  block_partials<1>(v, a.red.partial, j, a.nvec, gridDim.x, bx);
 }
 
 // x += alpha p ; r -= alpha s ; lambda = |r|^2
 template <typename R>
 __global__ __launch_bounds__(kBlock) void k_cg_update_xr(BlasArgs a) {
   using C = Cx<R>;
-  const int bx = xcd_block(blockIdx.x, gridDim.x);
-  const long h = (long)bx * kBlock + threadIdx.x;
+  const unsigned bx = xcd_block(blockIdx.x, gridDim.x);
   const int j = blockIdx.y;
   const long Vh = a.Vh;
+  long h0, h1;
+  site_range(Vh, bx, gridDim.x, h0, h1);
   double v[1] = {0.0};
   const CgState &st = a.cg[j];
-  if (h < Vh && !st.done) {
+  if (!st.done) {
     const R al = (R)st.alpha;
-    C *x = (C *)a.x + (long)j * 3 * Vh + h, *r = (C *)a.r + (long)j * 3 * Vh + h;
-    const C *p = (const C *)a.p + (long)j * 3 * Vh + h, *s = (const C *)a.s + (long)j * 3 * Vh + h;
+    for (long h = h0 + threadIdx.x; h < h1; h += kBlock) {
+      C *x = (C *)a.x + (long)j * 3 * Vh + h, *r = (C *)a.r + (long)j * 3 * Vh + h;
+      const C *p = (const C *)a.p + (long)j * 3 * Vh + h, *s = (const C *)a.s + (long)j * 3 * Vh + h;
 #pragma unroll
-    for (int i = 0; i < 3; i++) {
-      C xv = x[(long)i * Vh], rv = r[(long)i * Vh];
-      const C pv = p[(long)i * Vh], sv = s[(long)i * Vh];
-      xv.x += al * pv.x; xv.y += al * pv.y;
-      rv.x -= al * sv.x; rv.y -= al * sv.y;
-      x[(long)i * Vh] = xv;
-      r[(long)i * Vh] = rv;
-      v[0] += (double)rv.x * (double)rv.x + (double)rv.y * (double)rv.y;
+      for (int i = 0; i < 3; i++) {
+        C xv = x[(long)i * Vh], rv = r[(long)i * Vh];
+        const C pv = p[(long)i * Vh], sv = s[(long)i * Vh];
+        xv.x += al * pv.x; xv.y += al * pv.y;
+        rv.x -= al * sv.x; rv.y -= al * sv.y;
+        x[(long)i * Vh] = xv;
+        r[(long)i * Vh] = rv;
+        v[0] += (double)rv.x * (double)rv.x + (double)rv.y * (double)rv.y;
+      }
     }
   }
-  block_partials<1>(v, a.red.partial, j, a.nvec, gridDim.x, blockIdx.x);
+  block_partials<1>(v, a.red.partial, j, a.nvec, gridDim.x, bx);
 }
 
 // p = r + beta p
@@ -626,13 +702,11 @@ __global__ __launch_bounds__(kBlock) void k_copy_f2d(double2 *x64, const float2 
 }
 
 // ------------------------------------------------------------- host drivers
-static unsigned nbx_of(Context &c) { return (unsigned)((c.Vh + kBlock - 1) / kBlock); }
-
 // systems still running, written by every solver finalize launch
 static int *active_ptr(Context &c) { return (int *)(c.ws->counter + 1); }
 
 static void finalize(Context &c, const RedArgs &ra, int nvec, bool count_active) {
-  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, c.stream, ra, nvec, (int)nbx_of(c),
+  hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, c.stream, ra, nvec,
                      count_active ? active_ptr(c) : (int *)nullptr);
 }
 
@@ -680,7 +754,9 @@ static int ms_cg(Context &c, const double2 *b, int nrhs, int nshift, const doubl
   ba.Vh = Vh; ba.nvec = nrhs; ba.nshift = nshift;
   ba.b = b; ba.r = r; ba.p = p; ba.s = s; ba.ps = ps; ba.x = x; ba.ms = ms;
   ba.red = make_red(c, nrhs, EPI_MS_INIT, ms, nullptr);
-  hipLaunchKernelGGL(k_ms_init<R>, dim3(nbx, nrhs), dim3(kBlock), 0, c.stream, ba);
+  const unsigned nbr = reduce_blocks(c, nrhs);
+  ba.red.nbr = (int)nbr;
+  hipLaunchKernelGGL(k_ms_init<R>, dim3(nbr, nrhs), dim3(kBlock), 0, c.stream, ba);
   finalize(c, ba.red, nrhs, true);
   RedArgs red_c = ba.red;
   red_c.epi = EPI_MS_C;
@@ -700,7 +776,7 @@ static int ms_cg(Context &c, const double2 *b, int nrhs, int nshift, const doubl
       dslash_dispatch<R, LINK, DS_HOP, RED_NONE>(c, hop);
       dslash_dispatch<R, LINK, DS_MDAGM, RED_DOT>(c, mdm);
       finalize(c, mdm.red, nrhs, false);
-      hipLaunchKernelGGL(k_ms_update_r<R>, dim3(nbx, nrhs), dim3(kBlock), 0, c.stream, ba);
+      hipLaunchKernelGGL(k_ms_update_r<R>, dim3(nbr, nrhs), dim3(kBlock), 0, c.stream, ba);
       finalize(c, red_c, nrhs, true);
       hipLaunchKernelGGL(k_ms_update_p<R>, dim3(nbx, nrhs), dim3(kBlock), 0, c.stream, ba);
     }
@@ -749,7 +825,9 @@ static int block_cg32(Context &c, const double2 *r64, int nsys, const double *sh
   ba.Vh = Vh; ba.nvec = nsys; ba.nshift = 1;
   ba.b = r64; ba.r = r; ba.p = p; ba.s = s; ba.x = x32; ba.cg = cg;
   ba.red = make_red(c, nsys, EPI_CG_INIT, cg, nullptr);
-  hipLaunchKernelGGL(k_cg_init<float>, dim3(nbx, nsys), dim3(kBlock), 0, c.stream, ba);
+  const unsigned nbr = reduce_blocks(c, nsys);
+  ba.red.nbr = (int)nbr;
+  hipLaunchKernelGGL(k_cg_init<float>, dim3(nbr, nsys), dim3(kBlock), 0, c.stream, ba);
   finalize(c, ba.red, nsys, true);
   RedArgs red_c = ba.red;
   red_c.epi = EPI_CG_C;
@@ -767,7 +845,7 @@ static int block_cg32(Context &c, const double2 *r64, int nsys, const double *sh
       dslash_dispatch<float, LINK_S12, DS_HOP, RED_NONE>(c, hop);
       dslash_dispatch<float, LINK_S12, DS_MDAGM, RED_DOT>(c, mdm);
       finalize(c, mdm.red, nsys, false);
-      hipLaunchKernelGGL(k_cg_update_xr<float>, dim3(nbx, nsys), dim3(kBlock), 0, c.stream, ba);
+      hipLaunchKernelGGL(k_cg_update_xr<float>, dim3(nbr, nsys), dim3(kBlock), 0, c.stream, ba);
       finalize(c, red_c, nsys, true);
       hipLaunchKernelGGL(k_cg_update_p<float>, dim3(nbx, nsys), dim3(kBlock), 0, c.stream, ba);
     }
@@ -827,8 +905,9 @@ int multishift_solve(Context &c, const double2 *b, int nrhs, int nshift, const d
     c.last_iters_fp32 = 0;
     return rc;
   }
-  // mixed: FP32 multi-shift to ~1e-6 relative, then FP64 defect correction
-  const double rel32 = 2e-6;
+  // mixed: FP32 multi-shift to ~1e-6 relative, then FP64 defect correction;
+  // FP32 only: the FP32 recursion to the requested residual
+  const double rel32 = prec == PREC_FP32 ? 0.0 : 2e-6;
   std::vector<double> res32(nrhs);
   for (int j = 0; j < nrhs; j++) res32[j] = std::max(res_abs[j], rel32 * sqrt(bn2[j]));
   float2 *x32 = (float2 *)ws_get(c, S_X32, sizeof(float2) * 3 * Vh * nrhs * nshift);
@@ -838,6 +917,13 @@ int multishift_solve(Context &c, const double2 *b, int nrhs, int nshift, const d
   if (rc < 0) return rc;
   const long n = 3 * Vh * (long)nrhs * nshift;
   hipLaunchKernelGGL(k_copy_f2d, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, c.stream, x, x32, n);
+  if (prec == PREC_FP32) {
+    if (!hip_ok(c, hipStreamSynchronize(c.stream), "fp32 solve")) return -1;
+    c.last_iters_fp32 = it32;
+    c.last_iters = it32;
+    *iters = it32;
+    return rc;
+  }
   const int nsys = nrhs * nshift;
   std::vector<double> sh(nsys), nr2(nsys), tol(nsys);
   for (int j = 0; j < nsys; j++) sh[j] = shifts[j % nshift];
@@ -884,23 +970,53 @@ int multishift_solve(Context &c, const double2 *b, int nrhs, int nshift, const d
 
 // ---------------------------------------------------------- norms / scaling
 __global__ __launch_bounds__(kBlock) void k_norms(const double2 *v, long vs, long Vh, int nvec, RedArgs red) {
-  const int bx = xcd_block(blockIdx.x, gridDim.x);
-  const long h = (long)bx * kBlock + threadIdx.x;
+  const unsigned bx = xcd_block(blockIdx.x, gridDim.x);
   const int j = blockIdx.y;
+  long h0, h1;
+  site_range(Vh, bx, gridDim.x, h0, h1);
   double s[1] = {0.0};
-  if (h < Vh)
+  for (long h = h0 + threadIdx.x; h < h1; h += kBlock)
     for (int i = 0; i < 3; i++) {
       const double2 a = v[(long)j * vs + (long)i * Vh + h];
       s[0] += a.x * a.x + a.y * a.y;
     }
-  block_partials<1>(s, red.partial, j, nvec, gridDim.x, blockIdx.x);
+  block_partials<1>(s, red.partial, j, nvec, gridDim.x, bx);
+}
+
+__global__ __launch_bounds__(kBlock) void k_dots(const double2 *a, const double2 *b, long vs, long Vh, int nvec,
+                                                 RedArgs red) {
+  const unsigned bx = xcd_block(blockIdx.x, gridDim.x);
+  const int j = blockIdx.y;
+  long h0, h1;
+  site_range(Vh, bx, gridDim.x, h0, h1);
+  double s[1] = {0.0};
+  for (long h = h0 + threadIdx.x; h < h1; h += kBlock)
+    for (int i = 0; i < 3; i++) {
+      const double2 u = a[(long)j * vs + (long)i * Vh + h], v = b[(long)j * vs + (long)i * Vh + h];
+      s[0] += u.x * v.x + u.y * v.y;
+    }
+  block_partials<1>(s, red.partial, j, nvec, gridDim.x, bx);
+}
+
+int vec_dots(Context &c, const double2 *a, const double2 *b, int nvec, long vs, double *out) {
+  if (nvec > kMaxRed) return -1;
+  double *sums = (double *)ws_get(c, S_SUMS, sizeof(double) * 256);
+  RedArgs red = make_red(c, nvec, EPI_STORE, nullptr, sums);
+  const unsigned nbr = reduce_blocks(c, nvec);
+  red.nbr = (int)nbr;
+  hipLaunchKernelGGL(k_dots, dim3(nbr, nvec), dim3(kBlock), 0, c.stream, a, b, vs, c.Vh, nvec, red);
+  finalize(c, red, nvec, false);
+  hipMemcpyAsync(out, sums, sizeof(double) * nvec, hipMemcpyDeviceToHost, c.stream);
+  return hip_ok(c, hipStreamSynchronize(c.stream), "dots") ? 0 : -1;
 }
 
 int vec_norms2(Context &c, const double2 *v, int nvec, long vs, double *out) {
   if (nvec > kMaxRed) return -1;
   double *sums = (double *)ws_get(c, S_SUMS, sizeof(double) * 256);
   RedArgs red = make_red(c, nvec, EPI_STORE, nullptr, sums);
-  hipLaunchKernelGGL(k_norms, dim3(nbx_of(c), nvec), dim3(kBlock), 0, c.stream, v, vs, c.Vh, nvec, red);
+  const unsigned nbr = reduce_blocks(c, nvec);
+  red.nbr = (int)nbr;
+  hipLaunchKernelGGL(k_norms, dim3(nbr, nvec), dim3(kBlock), 0, c.stream, v, vs, c.Vh, nvec, red);
   finalize(c, red, nvec, false);
   hipMemcpyAsync(out, sums, sizeof(double) * nvec, hipMemcpyDeviceToHost, c.stream);
   return hip_ok(c, hipStreamSynchronize(c.stream), "norms") ? 0 : -1;

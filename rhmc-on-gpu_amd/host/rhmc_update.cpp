@@ -15,6 +15,7 @@
 #include <chrono>
 #include <cmath>
 #include <cstring>
+#include <utility>
 #include <vector>
 
 #include "../csrc/context.hpp"
@@ -59,11 +60,18 @@ struct Traj {
 
   double secs(clk::time_point t0) { return std::chrono::duration<double>(clk::now() - t0).count(); }
 
-  int solve(const double2 *b, const Coeffs &co, double res) {
+  // mixed runs solves at or above fp32_threshold in FP32 only, like the
+  // reference's inv_single_double_prec dispatch (cu_inverter.cc:45-60)
+  int prec_for(double res) const {
+    if (p.precision == PREC_MIXED && p.fp32_threshold > 0.0 && res >= p.fp32_threshold) return PREC_FP32;
+    return p.precision;
+  }
+
+  int solve(const double2 *b, const Coeffs &co, double res, int prec = -1, double2 *x = nullptr) {
     auto t0 = clk::now();
     int it = 0;
-    int rc = multishift_solve(c, b, c.n_pf, (int)co.b.size(), co.b.data(), res, p.residual_relative, p.precision,
-                              p.max_cg, c.smf, &it);
+    int rc = multishift_solve(c, b, c.n_pf, (int)co.b.size(), co.b.data(), res, p.residual_relative,
+                              prec < 0 ? prec_for(res) : prec, p.max_cg, x ? x : c.smf, &it);
     c.smf_order = (int)co.b.size();
     r.cg_iterations += it;
     r.n_solves++;
@@ -221,6 +229,35 @@ int run_trajectory(Context &c, const rhmc_traj_params &p, rhmc_traj_result &r) {
   double s = 0.0;
   if (action_sum(c, &s)) return RHMC_EHIP;
   r.delta_h = -s;
+  r.delta_h_fp64 = r.delta_h;
+  if (p.verify_fp64 && !p.pure_gauge) {
+    // Recompute the fermion action of the same endpoints with FP64 solves:
+    // chi64 = r_first(MdagM[U_start]) phi, phi_new64 = r_last(MdagM[U_end]) chi64.
+    // Gauge and momentum parts are identical, |phi|^2 needs no solve.
+    const size_t bytes = sizeof(double2) * sp * c.n_pf;
+    double2 *chi64 = nullptr, *phn64 = nullptr;
+    if (!hip_ok(c, hipMalloc(&chi64, bytes), "verify") || !hip_ok(c, hipMalloc(&phn64, bytes), "verify"))
+      return RHMC_EHIP;
+    Coeffs cf = rescale(p.order_first, p.first_a0, p.first_a, p.first_b, mn, mx, e_first);
+    Coeffs cl = rescale(p.order_last, p.last_a0, p.last_a, p.last_b, mn, mx, e_md);
+    std::swap(c.U64, c.U64_save);
+    int rc1 = T.solve(c.phi0, cf, p.res_metro, PREC_FP64);
+    std::swap(c.U64, c.U64_save);
+    hipMemcpyAsync(chi64, c.phi0, bytes, hipMemcpyDeviceToDevice, c.stream);
+    if (!rc1) sol_sum(c, chi64, c.smf, p.order_first, cf.a0, cf.a.data());
+    int rc2 = rc1 ? rc1 : T.solve(chi64, cl, p.res_metro, PREC_FP64);
+    hipMemcpyAsync(phn64, chi64, bytes, hipMemcpyDeviceToDevice, c.stream);
+    if (!rc2) sol_sum(c, phn64, c.smf, p.order_last, cl.a0, cl.a.data());
+    std::vector<double> d_mixed(c.n_pf), d_64(c.n_pf);
+    int rc3 = rc2 ? rc2 : vec_dots(c, c.phinew, c.chi, c.n_pf, sp, d_mixed.data());
+    int rc4 = rc3 ? rc3 : vec_dots(c, phn64, chi64, c.n_pf, sp, d_64.data());
+    hipFree(chi64);
+    hipFree(phn64);
+    if (rc1 || rc2 || rc3 || rc4) return RHMC_EHIP;
+    double diff = 0.0;
+    for (int ps = 0; ps < c.n_pf; ps++) diff += d_64[ps] - d_mixed[ps];
+    r.delta_h_fp64 = r.delta_h + diff;
+  }
   r.accepted = 1;
   if (p.metropolis) {
     const double dh = r.delta_h;

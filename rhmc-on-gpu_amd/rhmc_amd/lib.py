@@ -57,7 +57,8 @@ class Params(C.Structure):
 class TrajParams(C.Structure):
     _fields_ = [("n_flavours", C.c_int), ("no_md", C.c_int), ("gauge_scale", C.c_int), ("integrator", C.c_int),
                 ("res_md", C.c_double), ("res_metro", C.c_double), ("residual_relative", C.c_int),
-                ("precision", C.c_int), ("max_cg", C.c_int),
+                ("precision", C.c_int), ("fp32_threshold", C.c_double), ("verify_fp64", C.c_int),
+                ("max_cg", C.c_int),
                 ("order_first", C.c_int), ("order_md", C.c_int), ("order_last", C.c_int),
                 ("first_a", C.c_void_p), ("first_b", C.c_void_p), ("first_a0", C.c_double),
                 ("md_a", C.c_void_p), ("md_b", C.c_void_p), ("md_a0", C.c_double),
@@ -69,7 +70,7 @@ class TrajParams(C.Structure):
 
 
 class TrajResult(C.Structure):
-    _fields_ = [("delta_h", C.c_double), ("accepted", C.c_int), ("cg_iterations", C.c_int),
+    _fields_ = [("delta_h", C.c_double), ("delta_h_fp64", C.c_double), ("accepted", C.c_int), ("cg_iterations", C.c_int),
                 ("n_solves", C.c_int), ("eig_min", C.c_double), ("eig_max", C.c_double),
                 ("seconds_total", C.c_double), ("seconds_solver", C.c_double),
                 ("seconds_force", C.c_double), ("seconds_update", C.c_double)]
@@ -265,7 +266,8 @@ class Gpu:
         self._check(self.L.gpu_gauss_phi(self.h, seed, stream), "gpu_gauss_phi")
 
     def trajectory(self, *, n_flavours, first, md, last, no_md, integrator="omelyan", gauge_scale=1,
-                   res_md=1e-10, res_metro=1e-10, relative=False, precision=0, max_cg=100000,
+                   res_md=1e-10, res_metro=1e-10, relative=False, precision=0, fp32_threshold=0.0,
+                   verify_fp64=False, max_cg=100000,
                    lambda_min=1e-5, bounds=None, power_tol=1e-5, draw_fields=False, seed=1,
                    metropolis=False, metropolis_u=0.5, pure_gauge=False):
         """first/md/last: (a0, a[], b[]) normalised coefficients; bounds=(min,max)
@@ -279,7 +281,8 @@ class Gpu:
 
         t = TrajParams(n_flavours=n_flavours, no_md=no_md, gauge_scale=gauge_scale,
                        integrator=INTEGRATORS[integrator], res_md=res_md, res_metro=res_metro,
-                       residual_relative=int(relative), precision=int(precision), max_cg=max_cg,
+                       residual_relative=int(relative), precision=int(precision),
+                       fp32_threshold=float(fp32_threshold), verify_fp64=int(verify_fp64), max_cg=max_cg,
                        order_first=len(first[1]), order_md=len(md[1]), order_last=len(last[1]),
                        first_a=arr(first[1]), first_b=arr(first[2]), first_a0=first[0],
                        md_a=arr(md[1]), md_b=arr(md[2]), md_a0=md[0],
