@@ -189,6 +189,7 @@ def main():
                      "achieved": gbs32, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs32 / HBM_PEAK_GBS,
                      "bytes_per_launch": b32, "traffic": None},
         "cpu_baseline": cpu,
+        "dslash_autotune": g.tuning_table(),
     }
     print(json.dumps(out))
     g.close()

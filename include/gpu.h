@@ -114,6 +114,10 @@ int gpu_dslash(rhmc_gpu_t h, int par_out, int precision, int nvec, const double 
  * events on the library's stream. */
 int gpu_time_dslash(rhmc_gpu_t h, int precision, int nvec, int reps, double *ms);
 
+/* Autotuner choices made so far (Dslash right-hand sides per thread) as a
+ * JSON array; returns the length written (truncated to buflen-1). */
+int gpu_tuning_table(rhmc_gpu_t h, char *buf, int buflen);
+
 /* --------------------------------------------------------------------- */
 /* One RHMC trajectory on the device: update.cc:58-152 with the integrators
  * of lib/Update/MD_integrators. */

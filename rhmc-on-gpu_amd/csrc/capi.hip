@@ -2,6 +2,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <algorithm>
 #include <cstring>
 #include <new>
 #include <string>
@@ -20,6 +21,7 @@ int upload_spinors(Context &c, const double *host, double2 *dst, int n);
 int download_spinors(Context &c, const double2 *src, double *host, int n);
 int hop_s12(Context &c, int par_out, const float2 *in, float2 *out, int nvec);
 int run_trajectory(Context &c, const rhmc_traj_params &p, rhmc_traj_result &r);
+std::string tuning_table(Context &c);
 
 bool hip_ok(Context &c, hipError_t e, const char *what) {
   if (e == hipSuccess) return true;
@@ -390,6 +392,15 @@ int cu_last_solve_stats(rhmc_gpu_t h, int *i32, int *iref, int *cyc) {
   if (iref) *iref = h->c.last_iters_refine;
   if (cyc) *cyc = h->c.last_refine_cycles;
   return RHMC_OK;
+}
+
+int gpu_tuning_table(rhmc_gpu_t h, char *buf, int buflen) {
+  if (!h || !buf || buflen < 1) return -1;
+  const std::string t = tuning_table(h->c);
+  const int n = (int)std::min<size_t>(t.size(), (size_t)buflen - 1);
+  memcpy(buf, t.data(), n);
+  buf[n] = 0;
+  return n;
 }
 
 int gpu_trajectory(rhmc_gpu_t h, const rhmc_traj_params *p, rhmc_traj_result *r) {

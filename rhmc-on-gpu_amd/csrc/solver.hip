@@ -21,7 +21,10 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <map>
+#include <string>
 #include <vector>
 
 #include "context.hpp"
@@ -40,6 +43,7 @@ enum Slot {
 };
 
 struct Workspace {
+  std::map<unsigned long long, int> tuned_nr;  // Dslash RHS-per-thread choice
   void *buf[S_COUNT] = {};
   size_t cap[S_COUNT] = {};
   unsigned *counter = nullptr;
@@ -448,8 +452,7 @@ static unsigned reduce_blocks(Context &c, int ny) {
 }
 
 template <typename R, int LINK, int MODE, int RED>
-static void dslash_dispatch(Context &c, DsArgs &a) {
-  const int nr = pick_nr(a.nvec);
+static void dslash_launch(Context &c, DsArgs &a, int nr) {
   a.L = c.L;
   a.nch = (unsigned)((a.nvec + nr - 1) / nr);
   a.nsb = RED == RED_NONE ? nbx_of(c) : reduce_blocks(c, a.nch);
@@ -461,6 +464,67 @@ static void dslash_dispatch(Context &c, DsArgs &a) {
     case 2: hipLaunchKernelGGL((k_dslash<R, LINK, 2, MODE, RED>), grid, dim3(kBlock), 0, c.stream, a); break;
     default: hipLaunchKernelGGL((k_dslash<R, LINK, 1, MODE, RED>), grid, dim3(kBlock), 0, c.stream, a); break;
   }
+}
+
+// Autotuner: the number of right-hand sides per thread trades link reuse in
+// registers against waves in flight (a single 8-RHS thread per site leaves a
+// 32^3x8 half lattice at ~2 waves per SIMD).  On the first launch of each
+// (precision, links, mode, reduction, nvec) every candidate is timed in place
+// with HIP events (these launches are idempotent) and the fastest is kept for
+// the lifetime of the context.  RHMC_DSLASH_NR overrides the choice.
+template <typename R, int LINK, int MODE, int RED>
+static void dslash_dispatch(Context &c, DsArgs &a) {
+  const unsigned long long key = ((unsigned long long)sizeof(R) << 40) | ((unsigned long long)LINK << 36) |
+                                 ((unsigned long long)MODE << 32) | ((unsigned long long)RED << 28) |
+                                 (unsigned long long)a.nvec;
+  auto &tab = c.ws->tuned_nr;
+  auto it = tab.find(key);
+  int nr;
+  if (it != tab.end()) {
+    nr = it->second;
+  } else if (const char *env = getenv("RHMC_DSLASH_NR")) {
+    nr = std::max(1, std::min(8, atoi(env)));
+    nr = nr >= 8 ? 8 : nr >= 4 ? 4 : nr >= 2 ? 2 : 1;
+    tab[key] = nr;
+  } else {
+    hipEvent_t e0, e1;
+    hipEventCreate(&e0);
+    hipEventCreate(&e1);
+    float best = 1e30f;
+    nr = pick_nr(a.nvec);
+    for (int cand = 1; cand <= 8 && cand <= std::max(1, a.nvec); cand *= 2) {
+      dslash_launch<R, LINK, MODE, RED>(c, a, cand);
+      hipEventRecord(e0, c.stream);
+      for (int k = 0; k < 3; k++) dslash_launch<R, LINK, MODE, RED>(c, a, cand);
+      hipEventRecord(e1, c.stream);
+      hipEventSynchronize(e1);
+      float t = 0.f;
+      hipEventElapsedTime(&t, e0, e1);
+      if (t < best) {
+        best = t;
+        nr = cand;
+      }
+    }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    tab[key] = nr;
+  }
+  dslash_launch<R, LINK, MODE, RED>(c, a, nr);
+}
+
+std::string tuning_table(Context &c) {
+  std::string out = "[";
+  bool first = true;
+  for (auto &kv : c.ws->tuned_nr) {
+    char buf[160];
+    const unsigned long long k = kv.first;
+    snprintf(buf, sizeof buf, "%s{\"real_bytes\":%llu,\"links\":\"%s\",\"mode\":%llu,\"reduction\":%llu,\"nvec\":%llu,\"rhs_per_thread\":%d}",
+             first ? "" : ",", k >> 40, ((k >> 36) & 15) ? "fp32_12" : "fp64_18", (k >> 32) & 15, (k >> 28) & 15,
+             k & 0xfffffff, kv.second);
+    out += buf;
+    first = false;
+  }
+  return out + "]";
 }
 
 static RedArgs make_red(Context &c, int nvec, int epi, void *state, double *sums) {

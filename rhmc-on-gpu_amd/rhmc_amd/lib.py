@@ -108,6 +108,7 @@ def _declare(L):
         "gpu_dslash": (I, [VP, I, I, I, VP, VP]),
         "gpu_time_dslash": (I, [VP, I, I, I, PD]),
         "gpu_trajectory": (I, [VP, C.POINTER(TrajParams), C.POINTER(TrajResult)]),
+        "gpu_tuning_table": (I, [VP, C.c_char_p, I]),
         "cu_multips_shifted_invert": (I, [VP, D, I, I, VP, I, I, PI]),
         "cuda_shifted_inverter": (I, [VP, D, VP, I, I, I, PI]),
         "cu_shifted_residuals": (I, [VP, I, VP, VP]),
@@ -198,6 +199,12 @@ class Gpu:
         out = np.zeros_like(inp)
         self._check(self.L.gpu_dslash(self.h, par_out, precision, inp.shape[0], _ptr(inp), _ptr(out)), "gpu_dslash")
         return out
+
+    def tuning_table(self):
+        import json
+        buf = C.create_string_buffer(1 << 16)
+        self.L.gpu_tuning_table(self.h, buf, len(buf))
+        return json.loads(buf.value.decode())
 
     def time_dslash(self, precision, nvec, reps):
         ms = C.c_double()
