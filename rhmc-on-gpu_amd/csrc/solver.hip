@@ -39,6 +39,7 @@ enum Slot {
   S_CR64, S_CX32, S_CR32, S_CP32, S_CS32, S_CH32,
   S_PW1, S_PW2, S_PW3,
   S_FH64,
+  S_PARTIAL2, S_BASIS, S_BSH, S_HIST, S_COEF,
   S_COUNT
 };
 
@@ -765,6 +766,274 @@ __global__ __launch_bounds__(kBlock) void k_copy_f2d(double2 *x64, const float2 
   }
 }
 
+// ------------------------------------------- multi-shift CG on a stored basis
+// The residuals r^(0..K) of the base CG are the (scaled) Lanczos vectors of
+// MdagM; every shifted solution is a combination of them,
+//   x_s = sum_{i<K_s} d_{s,i} r^(i),
+// with d_{s,i} from the same Jegerlehner scalar recurrences.  Keeping the
+// basis resident in HBM (one vector per iteration and right-hand side) makes
+// the per-iteration traffic independent of the number of shifts: 6 vectors
+// per rhs instead of 6 + 4 x (active shifts); the shifted solutions are
+// assembled in one pass at the end.
+//
+// Per iteration k four launches (no separate finalize): A h = D_oe p;
+// B s = m^2 p - D_eo h, partials of <p,s>; C every workgroup sums B's
+// partials (fixed order) for omega and writes r^(k) = r^(k-1) + omega s into
+// basis slot k, partials of |r^(k)|^2; D every workgroup sums C's partials for
+// gamma and updates p = r^(k) + gamma p, and workgroup (0,j) advances the
+// scalar recurrences of rhs j.  Values read by every workgroup (delta, stop)
+// are double-buffered by the parity of k, so no workgroup reads a value
+// another one writes in the same launch.
+struct BmsShared {
+  double delta[2];
+  int stop[2];
+};
+
+struct BmsHist {
+  double om, z, ga;
+};
+
+struct BmsArgs {
+  long Vh;
+  int nvec, nshift, iter, cap;
+  const double2 *b;
+  void *basis;          // [rhs][cap][3][Vh]
+  void *p, *s;          // [rhs][3][Vh]
+  double *partB, *partC;
+  int nbrB, nbrC;
+  BmsShared *sh;
+  MsState *ms;
+  BmsHist *hist;        // [rhs][cap][nshift]
+  int *active;          // [2]
+};
+
+// fixed-order workgroup sum of n partials
+__device__ __forceinline__ double block_sum_partials(const double *p, int n) {
+  __shared__ double sh[kBlock / 64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  double v = 0.0;
+  for (int i = threadIdx.x; i < n; i += kBlock) v += p[i];
+  v = wave_sum(v);
+  if (lane == 0) sh[w] = v;
+  __syncthreads();
+  double s = 0.0;
+#pragma unroll
+  for (int k = 0; k < kBlock / 64; k++) s += sh[k];
+  __syncthreads();
+  return s;
+}
+
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_bms_init(BmsArgs a) {
+  using C = Cx<R>;
+  const unsigned bx = xcd_block(blockIdx.x, gridDim.x);
+  const int j = blockIdx.y;
+  const long Vh = a.Vh;
+  long h0, h1;
+  site_range(Vh, bx, gridDim.x, h0, h1);
+  double v[1] = {0.0};
+  C *r0 = (C *)a.basis + (long)j * a.cap * 3 * Vh;
+  C *p = (C *)a.p + (long)j * 3 * Vh;
+  for (long h = h0 + threadIdx.x; h < h1; h += kBlock)
+    for (int i = 0; i < 3; i++) {
+      const double2 bv = a.b[((long)j * 3 + i) * Vh + h];
+      const C cv = mk<R>((R)bv.x, (R)bv.y);
+      r0[(long)i * Vh + h] = cv;
+      p[(long)i * Vh + h] = cv;
+      v[0] += (double)cv.x * (double)cv.x + (double)cv.y * (double)cv.y;
+    }
+  block_partials<1>(v, a.partC, j, a.nvec, gridDim.x, bx);
+}
+
+// scalar state for iteration 1 (slot 1): inverter.cc:166-198
+__global__ __launch_bounds__(kBlock) void k_bms_init_fin(BmsArgs a) {
+  __shared__ double sums[kMaxRed];
+  final_sums(a.partC, a.nvec, a.nbrC, sums);
+  for (int j = threadIdx.x; j < a.nvec; j += blockDim.x) {
+    MsState &s = a.ms[j];
+    s.omega = 1.0;
+    s.omega_prev = 1.0;
+    s.gammag = 0.0;
+    s.iter = 0;
+    s.delta = sums[j];
+    s.nact = s.nshift;
+    for (int k = 0; k < s.nshift; k++) {
+      s.z1[k] = 1.0; s.z2[k] = 1.0; s.z3[k] = 1.0;
+      s.om[k] = 0.0; s.ga[k] = 0.0;
+      s.flag[k] = 1; s.flag_next[k] = 1; s.its[k] = 0;
+    }
+    a.sh[j].delta[1] = sums[j];
+    a.sh[j].stop[1] = sums[j] == 0.0;  // zero source: x = 0
+    s.done = s.stop = a.sh[j].stop[1];
+  }
+  __syncthreads();
+  const int t = threadIdx.x;
+  const int n = __syncthreads_count(t < a.nvec && !a.sh[t].stop[1]);
+  if (t == 0) a.active[1] = n;
+}
+
+// C: r^(k) = r^(k-1) + omega s, |r^(k)|^2
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_bms_c(BmsArgs a) {
+  using C = Cx<R>;
+  const unsigned bx = xcd_block(blockIdx.x, gridDim.x);
+  const int j = blockIdx.y;
+  const int sl = a.iter & 1;
+  const long Vh = a.Vh;
+  if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) a.active[(a.iter + 1) & 1] = 0;
+  double v[1] = {0.0};
+  if (!a.sh[j].stop[sl]) {
+    const double alpha = block_sum_partials(a.partB + (long)j * a.nbrB, a.nbrB);
+    const R om = (R)(-a.sh[j].delta[sl] / alpha);
+    long h0, h1;
+    site_range(Vh, bx, gridDim.x, h0, h1);
+    const C *ro = (const C *)a.basis + ((long)j * a.cap + a.iter - 1) * 3 * Vh;
+    C *rn = (C *)a.basis + ((long)j * a.cap + a.iter) * 3 * Vh;
+    const C *sv = (const C *)a.s + (long)j * 3 * Vh;
+    for (long h = h0 + threadIdx.x; h < h1; h += kBlock)
+#pragma unroll
+      for (int i = 0; i < 3; i++) {
+        C r = ro[(long)i * Vh + h];
+        const C t = sv[(long)i * Vh + h];
+        r.x += om * t.x;
+        r.y += om * t.y;
+        rn[(long)i * Vh + h] = r;
+        v[0] += (double)r.x * (double)r.x + (double)r.y * (double)r.y;
+      }
+  }
+  block_partials<1>(v, a.partC, j, a.nvec, gridDim.x, bx);
+}
+
+// D: p = r^(k) + gamma p; workgroup (0,j) advances the recurrences of rhs j
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_bms_d(BmsArgs a) {
+  using C = Cx<R>;
+  const unsigned bx = xcd_block(blockIdx.x, gridDim.x);
+  const int j = blockIdx.y;
+  const int sl = a.iter & 1, sn = (a.iter + 1) & 1;
+  const long Vh = a.Vh;
+  BmsShared &sh = a.sh[j];
+  if (sh.stop[sl]) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+      sh.stop[sn] = 1;
+      sh.delta[sn] = sh.delta[sl];
+    }
+    return;
+  }
+  const double delta = sh.delta[sl];
+  const double lambda = block_sum_partials(a.partC + (long)j * a.nbrC, a.nbrC);
+  const double gam = lambda / delta;
+  {
+    const R g = (R)gam;
+    long h0, h1;
+    site_range(Vh, bx, gridDim.x, h0, h1);
+    const C *r = (const C *)a.basis + ((long)j * a.cap + a.iter) * 3 * Vh;
+    C *p = (C *)a.p + (long)j * 3 * Vh;
+    for (long h = h0 + threadIdx.x; h < h1; h += kBlock)
+#pragma unroll
+      for (int i = 0; i < 3; i++) {
+        const C rv = r[(long)i * Vh + h];
+        C pv = p[(long)i * Vh + h];
+        pv.x = rv.x + g * pv.x;
+        pv.y = rv.y + g * pv.y;
+        p[(long)i * Vh + h] = pv;
+      }
+  }
+  if (blockIdx.x != 0) return;
+  // ---- scalar recurrences of inverter.cc:224-287 for rhs j
+  const double alpha = block_sum_partials(a.partB + (long)j * a.nbrB, a.nbrB);
+  MsState &s = a.ms[j];
+  const double omp = s.omega, gold = s.gammag;
+  const double om = -delta / alpha;
+  const int t = threadIdx.x;
+  if (t < s.nshift && s.flag[t]) {
+    const double z1 = s.z1[t], z2 = s.z2[t];
+    const double z3 = (z1 * z2 * omp) / (om * gold * (z1 - z2) + z1 * omp * (1.0 - s.shift[t] * om));
+    const double oms = om * z3 / z2;
+    const double gas = gam * z3 * oms / (z2 * om);
+    BmsHist &hh = a.hist[((long)j * a.cap + a.iter) * a.nshift + t];
+    hh.om = oms;
+    hh.z = z3;
+    hh.ga = gas;
+    if (sqrt(delta * z2 * z2) < s.res) {
+      s.flag[t] = 0;
+      s.its[t] = a.iter;
+    }
+    s.z1[t] = z2;
+    s.z2[t] = z3;
+  }
+  const int nact = __syncthreads_count(t < s.nshift && s.flag[t]);
+  if (t == 0) {
+    s.omega_prev = omp;
+    s.omega = om;
+    s.gammag = gam;
+    s.delta = lambda;
+    s.iter = a.iter;
+    s.nact = nact;
+    const int stop = !(sqrt(lambda) > s.res && a.iter < s.max_iter) || nact == 0 || a.iter + 1 >= a.cap;
+    s.stop = s.done = stop;
+    sh.delta[sn] = lambda;
+    sh.stop[sn] = stop;
+    if (!stop) atomicAdd(&a.active[sn], 1);
+  }
+}
+
+// x[j][s] (FP64) = sum_{i<K} coef[j][s][i] basis[j][i], NS shifts per pass
+// klen[j]: basis vectors of rhs j (slots beyond were never written)
+template <typename R, int NS>
+__global__ __launch_bounds__(kBlock) void k_bms_assemble(const void *basis, const double *coef, const int *klen,
+                                                         double2 *x, long Vh, int cap, int nshift, int s0) {
+  using C = Cx<R>;
+  const int bx = xcd_block(blockIdx.x, gridDim.x);
+  const long h = (long)bx * kBlock + threadIdx.x;
+  const int j = blockIdx.y;
+  if (h >= Vh) return;
+  const int kmax = klen[j];
+  const C *bj = (const C *)basis + (long)j * cap * 3 * Vh;
+  const double *cj = coef + ((long)j * nshift + s0) * cap;
+  const int ns = nshift - s0 < NS ? nshift - s0 : NS;
+  for (int c = 0; c < 3; c++) {
+    double2 acc[NS];
+#pragma unroll
+    for (int q = 0; q < NS; q++) acc[q] = make_double2(0.0, 0.0);
+    const C *bc = bj + (long)c * Vh + h;
+    int i = 0;
+    // 8 basis loads in flight per thread before the FMAs
+    for (; i + 8 <= kmax; i += 8) {
+      C v[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++) v[u] = bc[(long)(i + u) * 3 * Vh];
+#pragma unroll
+      for (int u = 0; u < 8; u++) {
+        const double vx = (double)v[u].x, vy = (double)v[u].y;
+#pragma unroll
+        for (int q = 0; q < NS; q++) {
+          if (q < ns) {
+            const double d = cj[(long)q * cap + i + u];
+            acc[q].x += d * vx;
+            acc[q].y += d * vy;
+          }
+        }
+      }
+    }
+    for (; i < kmax; i++) {
+      const C v = bc[(long)i * 3 * Vh];
+      const double vx = (double)v.x, vy = (double)v.y;
+#pragma unroll
+      for (int q = 0; q < NS; q++) {
+        if (q < ns) {
+          const double d = cj[(long)q * cap + i];
+          acc[q].x += d * vx;
+          acc[q].y += d * vy;
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < NS; q++)
+      if (q < ns) x[(((long)j * nshift + s0 + q) * 3 + c) * Vh + h] = acc[q];
+  }
+}
+
 // ------------------------------------------------------------- host drivers
 // systems still running, written by every solver finalize launch
 static int *active_ptr(Context &c) { return (int *)(c.ws->counter + 1); }
@@ -852,6 +1121,137 @@ static int ms_cg(Context &c, const double2 *b, int nrhs, int nshift, const doubl
   int maxit = 0;
   for (int j = 0; j < nrhs; j++) maxit = std::max(maxit, hs[j].iter);
   *iters_out = maxit;
+  for (int j = 0; j < nrhs; j++)
+    if (hs[j].iter >= max_iter && hs[j].nact > 0) {
+      c.last_error = "multishift: max_cg reached";
+      return 2;
+    }
+  return 0;
+}
+
+// multi-shift CG with the stored residual basis; x (FP64) receives
+// [nrhs][nshift] solutions.  Returns 3 when the basis does not fit (the
+// caller then runs the explicit-shift algorithm).
+template <typename R, int LINK>
+static int ms_cg_basis(Context &c, const double2 *b, int nrhs, int nshift, const double *shifts,
+                       const double *res_abs, int max_iter, double2 *x, int *iters_out) {
+  using C = Cx<R>;
+  const long Vh = c.Vh;
+  const size_t vb = sizeof(C) * 3 * Vh;
+  const bool dbl = sizeof(R) == 8;
+  // basis capacity: at most max_iter+1 slots and a share of free HBM
+  size_t fr = 0, tot = 0;
+  hipMemGetInfo(&fr, &tot);
+  const size_t have = c.ws->cap[S_BASIS];
+  const size_t budget = std::min<size_t>((size_t)24 << 30, (size_t)((fr + have) * 0.45));
+  long cap = (long)std::min<size_t>(std::min<size_t>((size_t)max_iter + 2, 4096), budget / (vb * nrhs));
+  if (cap < 64) return 3;
+  void *basis = ws_get(c, S_BASIS, vb * nrhs * cap);
+  void *p = ws_get(c, dbl ? S_P64 : S_P32, vb * nrhs);
+  void *sv = ws_get(c, dbl ? S_S64 : S_S32, vb * nrhs);
+  void *hh = ws_get(c, dbl ? S_H64 : S_H32, vb * nrhs);
+  MsState *ms = (MsState *)ws_get(c, S_MS, sizeof(MsState) * nrhs);
+  BmsShared *bsh = (BmsShared *)ws_get(c, S_BSH, sizeof(BmsShared) * nrhs);
+  BmsHist *hist = (BmsHist *)ws_get(c, S_HIST, sizeof(BmsHist) * nrhs * cap * nshift);
+  double *diag = (double *)ws_get(c, S_DIAG, sizeof(double) * 256);
+  double *partC = (double *)ws_get(c, S_PARTIAL2, sizeof(double) * nbx_of(c) * 256);
+  if (!basis || !p || !sv || !hh || !ms || !bsh || !hist || !diag || !partC) return -1;
+
+  std::vector<MsState> hs(nrhs);
+  for (int j = 0; j < nrhs; j++) {
+    MsState &st = hs[j];
+    memset(&st, 0, sizeof(st));
+    st.nshift = nshift;
+    st.res = res_abs[j];
+    st.max_iter = max_iter;
+    for (int k = 0; k < nshift; k++) st.shift[k] = shifts[k];
+  }
+  hipMemcpyAsync(ms, hs.data(), sizeof(MsState) * nrhs, hipMemcpyHostToDevice, c.stream);
+  std::vector<double> hd(nrhs, c.mass * c.mass);
+  hipMemcpyAsync(diag, hd.data(), sizeof(double) * nrhs, hipMemcpyHostToDevice, c.stream);
+
+  BmsArgs a = {};
+  a.Vh = Vh; a.nvec = nrhs; a.nshift = nshift; a.cap = (int)cap;
+  a.b = b; a.basis = basis; a.p = p; a.s = sv;
+  a.partC = partC;
+  a.sh = bsh; a.ms = ms; a.hist = hist;
+  a.active = active_ptr(c);
+  const unsigned nbr = reduce_blocks(c, nrhs);
+  a.nbrC = (int)nbr;
+  hipLaunchKernelGGL(k_bms_init<R>, dim3(nbr, nrhs), dim3(kBlock), 0, c.stream, a);
+  hipLaunchKernelGGL(k_bms_init_fin, dim3(1), dim3(kBlock), 0, c.stream, a);
+
+  DsArgs hop = {};
+  hop.par_out = 1; hop.U = LINK == LINK_D18 ? (const void *)c.U64 : (const void *)c.U32;
+  hop.in = p; hop.in_vs = 3 * Vh; hop.out = hh; hop.out_vs = 3 * Vh; hop.nvec = nrhs;
+  hop.skip_stride = sizeof(BmsShared) / sizeof(int);
+  DsArgs mdm = hop;
+  mdm.par_out = 0; mdm.in = hh; mdm.out = sv; mdm.pin = p; mdm.pin_vs = 3 * Vh; mdm.diag = diag;
+  mdm.red = make_red(c, nrhs, EPI_STORE, nullptr, nullptr);
+  a.partB = mdm.red.partial;
+
+  const int chunk = 8;
+  int k = 1;
+  bool converged = false;
+  while (true) {
+    for (int q = 0; q < chunk && k < cap; q++, k++) {
+      hop.skip = &bsh[0].stop[k & 1];
+      mdm.skip = hop.skip;
+      dslash_dispatch<R, LINK, DS_HOP, RED_NONE>(c, hop);
+      dslash_dispatch<R, LINK, DS_MDAGM, RED_DOT>(c, mdm);
+      a.nbrB = mdm.red.nbr;
+      a.iter = k;
+      hipLaunchKernelGGL(k_bms_c<R>, dim3(nbr, nrhs), dim3(kBlock), 0, c.stream, a);
+      hipLaunchKernelGGL(k_bms_d<R>, dim3(nbr, nrhs), dim3(kBlock), 0, c.stream, a);
+    }
+    int *hf = c.ws->h_flags;
+    hipMemcpyAsync(hf, a.active + (k & 1), sizeof(int), hipMemcpyDeviceToHost, c.stream);
+    hipStreamSynchronize(c.stream);
+    if (hf[0] == 0) { converged = true; break; }
+    if (k >= cap) break;
+  }
+  hipMemcpyAsync(hs.data(), ms, sizeof(MsState) * nrhs, hipMemcpyDeviceToHost, c.stream);
+  if (!hip_ok(c, hipStreamSynchronize(c.stream), "basis multishift")) return -1;
+  (void)converged;
+  int kmax = 0;
+  for (int j = 0; j < nrhs; j++) kmax = std::max(kmax, hs[j].iter);
+  // basis exhausted before convergence: let the caller use the explicit algorithm
+  for (int j = 0; j < nrhs; j++)
+    if (hs[j].nact > 0 && hs[j].iter + 1 >= cap && hs[j].iter < max_iter) return 3;
+  // coefficients d_{s,i} = -z_s^(i) S_i, S_{K-1} = om^(K), S_i = om^(i+1) + ga^(i+1) S_{i+1}
+  std::vector<BmsHist> hh_host((size_t)nrhs * cap * nshift);
+  for (int j = 0; j < nrhs; j++)
+    hipMemcpyAsync(hh_host.data() + (size_t)j * cap * nshift, hist + (size_t)j * cap * nshift,
+                   sizeof(BmsHist) * (size_t)(kmax + 1) * nshift, hipMemcpyDeviceToHost, c.stream);
+  if (!hip_ok(c, hipStreamSynchronize(c.stream), "basis history")) return -1;
+  std::vector<double> coef((size_t)nrhs * nshift * cap, 0.0);
+  for (int j = 0; j < nrhs; j++)
+    for (int sft = 0; sft < nshift; sft++) {
+      const int K = hs[j].flag[sft] ? hs[j].iter : hs[j].its[sft];
+      double S = 0.0;
+      double *cf = coef.data() + ((size_t)j * nshift + sft) * cap;
+      for (int i = K - 1; i >= 0; i--) {
+        const BmsHist &n1 = hh_host[((size_t)j * cap + i + 1) * nshift + sft];
+        S = n1.om + n1.ga * S;
+        const double z = i == 0 ? 1.0 : hh_host[((size_t)j * cap + i) * nshift + sft].z;
+        cf[i] = -z * S;
+      }
+    }
+  // per-rhs basis length = largest K over its shifts
+  std::vector<int> klen(nrhs, 0);
+  for (int j = 0; j < nrhs; j++)
+    for (int sft = 0; sft < nshift; sft++)
+      klen[j] = std::max(klen[j], hs[j].flag[sft] ? hs[j].iter : hs[j].its[sft]);
+  double *dcoef = (double *)ws_get(c, S_COEF, sizeof(double) * coef.size() + sizeof(int) * 256);
+  if (!dcoef) return -1;
+  int *dklen = (int *)(dcoef + coef.size());
+  hipMemcpyAsync(dcoef, coef.data(), sizeof(double) * coef.size(), hipMemcpyHostToDevice, c.stream);
+  hipMemcpyAsync(dklen, klen.data(), sizeof(int) * nrhs, hipMemcpyHostToDevice, c.stream);
+  for (int s0 = 0; s0 < nshift; s0 += 16)
+    hipLaunchKernelGGL((k_bms_assemble<R, 16>), dim3(nbx_of(c), nrhs), dim3(kBlock), 0, c.stream, basis, dcoef, dklen,
+                       x, Vh, (int)cap, nshift, s0);
+  if (!hip_ok(c, hipStreamSynchronize(c.stream), "basis assemble")) return -1;
+  *iters_out = kmax;
   for (int j = 0; j < nrhs; j++)
     if (hs[j].iter >= max_iter && hs[j].nact > 0) {
       c.last_error = "multishift: max_cg reached";
@@ -963,8 +1363,10 @@ int multishift_solve(Context &c, const double2 *b, int nrhs, int nshift, const d
   for (int j = 0; j < nrhs; j++) res_abs[j] = rel ? res * sqrt(bn2[j]) : res;
   c.last_refine_cycles = 0;
   c.last_iters_refine = 0;
+  const bool explicit_shifts = getenv("RHMC_MS_EXPLICIT") != nullptr;
   if (prec == PREC_FP64) {
-    int rc = ms_cg<double, LINK_D18>(c, b, nrhs, nshift, shifts, res_abs.data(), max_iter, x, iters);
+    int rc = explicit_shifts ? 3 : ms_cg_basis<double, LINK_D18>(c, b, nrhs, nshift, shifts, res_abs.data(), max_iter, x, iters);
+    if (rc == 3) rc = ms_cg<double, LINK_D18>(c, b, nrhs, nshift, shifts, res_abs.data(), max_iter, x, iters);
     c.last_iters = *iters;
     c.last_iters_fp32 = 0;
     return rc;
@@ -974,13 +1376,17 @@ int multishift_solve(Context &c, const double2 *b, int nrhs, int nshift, const d
   const double rel32 = prec == PREC_FP32 ? 0.0 : 2e-6;
   std::vector<double> res32(nrhs);
   for (int j = 0; j < nrhs; j++) res32[j] = std::max(res_abs[j], rel32 * sqrt(bn2[j]));
-  float2 *x32 = (float2 *)ws_get(c, S_X32, sizeof(float2) * 3 * Vh * nrhs * nshift);
-  if (!x32) return -1;
   int it32 = 0;
-  int rc = ms_cg<float, LINK_S12>(c, b, nrhs, nshift, shifts, res32.data(), max_iter, x32, &it32);
-  if (rc < 0) return rc;
   const long n = 3 * Vh * (long)nrhs * nshift;
-  hipLaunchKernelGGL(k_copy_f2d, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, c.stream, x, x32, n);
+  int rc = explicit_shifts ? 3 : ms_cg_basis<float, LINK_S12>(c, b, nrhs, nshift, shifts, res32.data(), max_iter, x, &it32);
+  if (rc == 3) {
+    float2 *x32 = (float2 *)ws_get(c, S_X32, sizeof(float2) * 3 * Vh * nrhs * nshift);
+    if (!x32) return -1;
+    rc = ms_cg<float, LINK_S12>(c, b, nrhs, nshift, shifts, res32.data(), max_iter, x32, &it32);
+    if (rc < 0) return rc;
+    hipLaunchKernelGGL(k_copy_f2d, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, c.stream, x, x32, n);
+  }
+  if (rc < 0) return rc;
   if (prec == PREC_FP32) {
     if (!hip_ok(c, hipStreamSynchronize(c.stream), "fp32 solve")) return -1;
     c.last_iters_fp32 = it32;
