@@ -81,6 +81,8 @@ def main():
     ap.add_argument("--npf", type=int, default=8)
     ap.add_argument("--res-md", type=float, default=1e-6)
     ap.add_argument("--res-metro", type=float, default=1e-10)
+    ap.add_argument("--dslash-only", action="store_true",
+                    help="only the Dslash and copy-probe timings (for counter runs)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported"}))
@@ -95,6 +97,21 @@ def main():
     g = Gpu(dims, mass=mass, beta=beta, n_pf=n_pf, max_order=order)
     g.init0(U0)
     # spectral bounds of MdagM on the start configuration (power iterations)
+    if args.dslash_only:
+        reps = 50
+        res = {}
+        for name, prec, nv, lb, sb in (("fp32_12real_8rhs", 1, n_pf, 48, 24), ("fp64_18real_1rhs", 0, 1, 144, 48),
+                                       ("fp64_18real_8rhs", 0, n_pf, 144, 48)):
+            ms = g.time_dslash(prec, nv, reps)
+            res[name] = {"ms": ms, "GBps": dslash_bytes(lat.Vh, nv, lb, sb) / (ms * 1e-3) / 1e9,
+                         "bytes": dslash_bytes(lat.Vh, nv, lb, sb)}
+        cp = 1 << 30
+        ms = g.time_copy(cp, 20)
+        res["copy_1GiB"] = {"ms": ms, "GBps": 2 * cp / (ms * 1e-3) / 1e9, "bytes": 2 * cp}
+        res["autotune"] = g.tuning_table()
+        print(json.dumps(res))
+        g.close()
+        return
     mx = g.find_max(1e-5, seed=11)
     mn = g.find_min(mx, 1e-5, seed=12)
     mn = max(mn, mass * mass)

@@ -114,6 +114,11 @@ int gpu_dslash(rhmc_gpu_t h, int par_out, int precision, int nvec, const double 
  * events on the library's stream. */
 int gpu_time_dslash(rhmc_gpu_t h, int precision, int nvec, int reps, double *ms);
 
+/* Timing helper: a plain device copy of `bytes` (read + write), mean ms per
+ * copy over `reps` launches (HIP events); the achievable-bandwidth reference
+ * and the byte-counter calibration for the roofline. */
+int gpu_time_copy(rhmc_gpu_t h, double bytes, int reps, double *ms);
+
 /* Autotuner choices made so far (Dslash right-hand sides per thread) as a
  * JSON array; returns the length written (truncated to buflen-1). */
 int gpu_tuning_table(rhmc_gpu_t h, char *buf, int buflen);

@@ -22,6 +22,7 @@ int download_spinors(Context &c, const double2 *src, double *host, int n);
 int hop_s12(Context &c, int par_out, const float2 *in, float2 *out, int nvec);
 int run_trajectory(Context &c, const rhmc_traj_params &p, rhmc_traj_result &r);
 std::string tuning_table(Context &c);
+int time_copy(Context &c, size_t bytes, int reps, double *ms);
 
 bool hip_ok(Context &c, hipError_t e, const char *what) {
   if (e == hipSuccess) return true;
@@ -392,6 +393,12 @@ int cu_last_solve_stats(rhmc_gpu_t h, int *i32, int *iref, int *cyc) {
   if (iref) *iref = h->c.last_iters_refine;
   if (cyc) *cyc = h->c.last_refine_cycles;
   return RHMC_OK;
+}
+
+int gpu_time_copy(rhmc_gpu_t h, double bytes, int reps, double *ms) {
+  if (!h || !ms || bytes < 1024 || reps < 1) return RHMC_EINVAL;
+  hipSetDevice(h->c.device);
+  return time_copy(h->c, (size_t)bytes, reps, ms) ? RHMC_EHIP : RHMC_OK;
 }
 
 int gpu_tuning_table(rhmc_gpu_t h, char *buf, int buflen) {

@@ -446,6 +446,41 @@ int gauss_spinor(Context &c, double2 *out, int n, unsigned long long seed, unsig
   return hip_ok(c, hipGetLastError(), "gauss spinor") ? 0 : -1;
 }
 
+// ------------------------------------------------------- copy probe
+// plain streaming copy of known size: calibrates the HBM byte counters and
+// gives the achievable-bandwidth reference for the roofline
+__global__ __launch_bounds__(kBlock) void k_copy_probe(const double4 *__restrict__ in, double4 *__restrict__ out,
+                                                       long n) {
+  const long stride = (long)gridDim.x * kBlock;
+  for (long i = (long)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) out[i] = in[i];
+}
+
+int time_copy(Context &c, size_t bytes, int reps, double *ms) {
+  const long n = (long)(bytes / sizeof(double4));
+  double4 *a = nullptr, *b = nullptr;
+  if (!hip_ok(c, hipMalloc(&a, n * sizeof(double4)), "copy probe")) return -1;
+  if (!hip_ok(c, hipMalloc(&b, n * sizeof(double4)), "copy probe")) { hipFree(a); return -1; }
+  hipMemsetAsync(a, 0, n * sizeof(double4), c.stream);
+  const unsigned nb = 256 * 8 * 4;
+  hipLaunchKernelGGL(k_copy_probe, dim3(nb), dim3(kBlock), 0, c.stream, a, b, n);
+  hipEvent_t e0, e1;
+  hipEventCreate(&e0);
+  hipEventCreate(&e1);
+  hipEventRecord(e0, c.stream);
+  for (int r = 0; r < reps; r++)
+    hipLaunchKernelGGL(k_copy_probe, dim3(nb), dim3(kBlock), 0, c.stream, (r & 1) ? b : a, (r & 1) ? a : b, n);
+  hipEventRecord(e1, c.stream);
+  hipEventSynchronize(e1);
+  float t = 0.f;
+  hipEventElapsedTime(&t, e0, e1);
+  *ms = (double)t / reps;
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  hipFree(a);
+  hipFree(b);
+  return hip_ok(c, hipGetLastError(), "copy probe") ? 0 : -1;
+}
+
 // ----------------------------------------------------- host-layout transfers
 // host links/algebra: [4V][3][3] complex at mu*V + par*Vh + h  ==  [(mu*2+par)*Vh + h]
 __global__ __launch_bounds__(kBlock) void k_links_in(const double2 *aos, double2 *U, long Vh) {

@@ -109,6 +109,7 @@ def _declare(L):
         "gpu_time_dslash": (I, [VP, I, I, I, PD]),
         "gpu_trajectory": (I, [VP, C.POINTER(TrajParams), C.POINTER(TrajResult)]),
         "gpu_tuning_table": (I, [VP, C.c_char_p, I]),
+        "gpu_time_copy": (I, [VP, D, I, PD]),
         "cu_multips_shifted_invert": (I, [VP, D, I, I, VP, I, I, PI]),
         "cuda_shifted_inverter": (I, [VP, D, VP, I, I, I, PI]),
         "cu_shifted_residuals": (I, [VP, I, VP, VP]),
@@ -205,6 +206,11 @@ class Gpu:
         buf = C.create_string_buffer(1 << 16)
         self.L.gpu_tuning_table(self.h, buf, len(buf))
         return json.loads(buf.value.decode())
+
+    def time_copy(self, nbytes, reps):
+        ms = C.c_double()
+        self._check(self.L.gpu_time_copy(self.h, float(nbytes), int(reps), C.byref(ms)), "gpu_time_copy")
+        return ms.value
 
     def time_dslash(self, precision, nvec, reps):
         ms = C.c_double()

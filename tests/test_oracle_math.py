@@ -1,0 +1,212 @@
+"""Pins the oracle against mathematics (closed forms, invariants, brute force),
+independently of the reference's golden vectors.  Each check is chosen so a
+plausible mistake (a sign, a conjugation, a boundary phase, a transposed
+product) fails it."""
+import numpy as np
+import pytest
+from scipy.linalg import expm
+
+from oracle import Oracle
+from rhmc_amd import fields
+
+
+def dense_mdagm(o, U, mass):
+    n = o.Vh * 3
+    M = np.zeros((n, n), np.complex128)
+    e = np.zeros((o.Vh, 3), np.complex128)
+    for k in range(n):
+        e[:] = 0
+        e.flat[k] = 1.0
+        M[:, k] = o.mdagm(U, mass, e).ravel()
+    return M
+
+
+def dense_deo_doe(o, U):
+    n = o.Vh * 3
+    Deo = np.zeros((n, n), np.complex128)
+    Doe = np.zeros((n, n), np.complex128)
+    e = np.zeros((o.Vh, 3), np.complex128)
+    for k in range(n):
+        e[:] = 0
+        e.flat[k] = 1.0
+        Deo[:, k] = o.deo(U, e).ravel()
+        Doe[:, k] = o.doe(U, e).ravel()
+    return Deo, Doe
+
+
+def test_free_field_spectrum():
+    """U = eta*1: MdagM eigenvalues are m^2 + sum_mu sin^2 p_mu with periodic
+    momenta in x,y,z and antiperiodic in t, each with the staggered
+    degeneracy; the antiperiodic phase sits on eta_t at t = nt-1."""
+    dims = (4, 4, 4, 4)
+    mass = 0.1
+    lat = fields.Lattice(*dims)
+    o = Oracle(dims)
+    U = (np.eye(3)[None] * lat.eta()[:, None, None]).astype(np.complex128)
+    ev = np.sort(np.linalg.eigvalsh(dense_mdagm(o, U, mass)))
+    px = 2 * np.pi * np.arange(4) / 4
+    pt = (2 * np.arange(4) + 1) * np.pi / 4
+    lam = mass ** 2 + (np.sin(px)[:, None, None, None] ** 2 + np.sin(px)[None, :, None, None] ** 2
+                       + np.sin(px)[None, None, :, None] ** 2 + np.sin(pt)[None, None, None, :] ** 2)
+    # V momenta on the full lattice -> V/2 even-site modes x 3 colours
+    expect = np.sort(np.repeat(lam.ravel(), 3))
+    expect = np.sort(np.concatenate([expect[::2], expect[1::2]]))[: len(ev)]
+    # compare as multisets of distinct values with multiplicities
+    vals, counts = np.unique(np.round(ev, 10), return_counts=True)
+    evals, ecounts = np.unique(np.round(np.repeat(lam.ravel(), 3), 10), return_counts=True)
+    assert np.allclose(vals, evals, atol=1e-10)
+    assert np.all(counts * 2 == ecounts)
+    # periodic time would move the lowest mode to m^2 exactly: make sure not
+    assert ev.min() > mass ** 2 + 0.4
+
+
+def test_antihermiticity_and_gauge_covariance():
+    dims = (4, 4, 2, 4)
+    lat = fields.Lattice(*dims)
+    o = Oracle(dims)
+    U = fields.random_links(lat, 3)
+    rng = np.random.default_rng(0)
+    x = fields.gauss_spinor(lat, 1)[0]
+    y = fields.gauss_spinor(lat, 2)[0]
+    # <x, D_eo y> = -<D_oe x, y>
+    lhs = np.vdot(x, o.deo(U, y))
+    rhs = -np.vdot(o.doe(U, x), y)
+    assert abs(lhs - rhs) < 1e-12 * abs(lhs)
+    # gauge covariance: U_mu(x) -> g(x) U_mu(x) g(x+mu)^+, psi -> g psi
+    g = fields.haar_su3(rng, lat.V)
+    nnp, _ = lat.neighbours()
+    Ug = U.copy()
+    for mu in range(4):
+        sl = slice(mu * lat.V, (mu + 1) * lat.V)
+        Ug[sl] = g @ U[sl] @ np.conj(np.swapaxes(g[nnp[:, mu]], 1, 2))
+    ge, go = g[: lat.Vh], g[lat.Vh:]
+    psi_e = fields.gauss_spinor(lat, 5)[0]
+    out = o.doe(Ug, np.einsum("nij,nj->ni", ge, psi_e))
+    ref = np.einsum("nij,nj->ni", go, o.doe(U, psi_e))
+    assert np.linalg.norm(out - ref) < 1e-12 * np.linalg.norm(ref)
+
+
+def test_multishift_against_dense_solve():
+    dims = (4, 4, 4, 4)
+    mass = 0.2
+    lat = fields.Lattice(*dims)
+    o = Oracle(dims)
+    U = fields.random_links(lat, 4)
+    M = dense_mdagm(o, U, mass)
+    b = fields.gauss_spinor(lat, 6)[0]
+    shifts = np.array([0.0, 0.01, 0.3, 2.0])
+    x, it = o.multishift(U, mass, b, shifts, 1e-12)
+    for k, s in enumerate(shifts):
+        xs = np.linalg.solve(M + s * np.eye(M.shape[0]), b.ravel())
+        assert np.linalg.norm(x[k].ravel() - xs) < 1e-10 * np.linalg.norm(xs)
+    # MdagM = m^2 - D_eo D_oe is hermitian positive definite
+    assert np.allclose(M, M.conj().T, atol=1e-13)
+    Deo, Doe = dense_deo_doe(o, U)
+    assert np.allclose(M, mass ** 2 * np.eye(M.shape[0]) - Deo @ Doe, atol=1e-13)
+
+
+def test_su3_exp_and_projection():
+    rng = np.random.default_rng(7)
+    lat = fields.Lattice(2, 2, 2, 2)
+    o = Oracle(lat.dims)
+    U = fields.random_links(lat, 1)
+    P = fields.gauss_momenta(lat, 2)
+    step = 0.05
+    Un = o.exp_update(U, P, step)
+    eta = lat.eta()
+    for r in range(0, 4 * lat.V, 7):
+        E = expm(1j * step * P[r])
+        ref = E @ U[r]
+        bound = np.linalg.norm(step * P[r], 2) ** 7 / 5040 * 3
+        assert np.linalg.norm(Un[r] - ref) < bound + 1e-13
+        V = Un[r] / eta[r]
+        assert np.allclose(V @ V.conj().T, np.eye(3), atol=1e-14)
+        assert abs(np.linalg.det(V) - 1) < 1e-14
+
+
+def test_unit_configuration_action_and_force():
+    """U = eta*1: every staggered plaquette has phase -1 (Re tr = -3), so the
+    per-site gauge action is 6 * beta/3 * (-3) and the gauge force is zero."""
+    dims = (4, 2, 2, 4)
+    beta = 5.5
+    lat = fields.Lattice(*dims)
+    o = Oracle(dims)
+    U = (np.eye(3)[None] * lat.eta()[:, None, None]).astype(np.complex128)
+    P = np.zeros_like(U)
+    phi = np.zeros((1, lat.Vh, 3), np.complex128)
+    v = o.action(U, P, beta, phi, phi, True, None, 0)
+    assert np.allclose(v, 6 * beta / 3 * (-3.0))
+    ip = o.ipdot_total(U, beta, None, True)
+    assert np.abs(ip).max() < 1e-14
+
+
+def _gell_mann():
+    l = np.zeros((8, 3, 3), np.complex128)
+    l[0][0, 1] = l[0][1, 0] = 1
+    l[1][0, 1], l[1][1, 0] = -1j, 1j
+    l[2][0, 0], l[2][1, 1] = 1, -1
+    l[3][0, 2] = l[3][2, 0] = 1
+    l[4][0, 2], l[4][2, 0] = -1j, 1j
+    l[5][1, 2] = l[5][2, 1] = 1
+    l[6][1, 2], l[6][2, 1] = -1j, 1j
+    l[7] = np.diag([1, 1, -2]) / np.sqrt(3)
+    return l
+
+
+def test_fermion_force_finite_differences():
+    """ipdot = -i dS/dq: for U_mu(x) -> exp(i w lambda_a) U_mu(x),
+    dS/dw = tr(lambda_a * i * ipdot) with S = sum_ps phi^+ r(MdagM) phi and
+    ipdot the fermion part (ipdot.cc:83-155).  Central differences with
+    delta = 1e-5 on tightly converged solves; truncation ~delta^2 |S'''| and
+    rounding ~1e-16 |S| / delta give the 1e-6 relative tolerance."""
+    dims = (4, 4, 2, 4)
+    mass, beta = 0.2, 5.5
+    lat = fields.Lattice(*dims)
+    o = Oracle(dims)
+    U = fields.random_links(lat, 11, 0.4)
+    phi = fields.gauss_spinor(lat, 12, 1)
+    a = np.array([0.3, 0.5, 0.7])
+    b = np.array([0.05, 0.4, 1.5])
+    a0 = 0.1
+
+    def S(Ux):
+        x, _ = o.multishift(Ux, mass, phi[0], b, 1e-14)
+        val = a0 * np.vdot(phi[0], phi[0]).real
+        for k in range(3):
+            val += a[k] * np.vdot(phi[0], x[k]).real
+        return val
+
+    ip, _ = o.full_force(U, beta, mass, phi, a0, a, b, 1e-14, with_gauge=False)
+    lam = _gell_mann()
+    rng = np.random.default_rng(5)
+    d = 1e-5
+    for _ in range(6):
+        link = int(rng.integers(4 * lat.V))
+        ga = int(rng.integers(8))
+        up, dn = U.copy(), U.copy()
+        up[link] = expm(1j * d * lam[ga]) @ U[link]
+        dn[link] = expm(-1j * d * lam[ga]) @ U[link]
+        fd = (S(up) - S(dn)) / (2 * d)
+        an = np.trace(lam[ga] @ (1j * ip[link])).real
+        assert abs(fd - an) < 1e-6 * max(1.0, abs(an)), (link, ga, fd, an)
+
+
+def test_global_sum_order():
+    """include/tools.h:25-57 pairwise fold, checked against a literal
+    restatement on an odd length."""
+    o = Oracle((2, 2, 2, 2))
+    v = np.random.default_rng(1).standard_normal(37) * 1e3
+    w = list(v)
+    n = len(w)
+    while n > 1:
+        if n % 2 == 0:
+            j = n // 2
+            for k in range(j):
+                w[k] += w[k + j]
+            n = j
+        else:
+            j1, j2 = n // 2, n // 2 + 1
+            for k in range(j1):
+                w[k] += w[k + j2]
+            n = j2
+    assert o.global_sum(v) == w[0]
