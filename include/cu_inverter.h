@@ -35,7 +35,7 @@ int cu_multips_shifted_invert(rhmc_gpu_t h, double res, int relative, int order,
                               int precision, int max_cg, int *iterations);
 
 /* lib/Cuda/cuda_inverter.h:4-14: one pseudofermion psferm only. */
-int cuda_shifted_inverter(rhmc_gpu_t h, double res, const double *shifts, int num_shifts, int psferm,
+int gpu_shifted_inverter(rhmc_gpu_t h, double res, const double *shifts, int num_shifts, int psferm,
                           int precision, int *ncount);
 
 /* True residuals |b - (MdagM + shift_k) x_k| in FP64 of the solutions held in

@@ -36,6 +36,9 @@
 #include "lib/Update/MD_integrators/md_integrators.h"
 #include "lib/Update/ipdot.h"
 #include "lib/Update/momenta.h"
+#ifdef USE_GPU
+#include "include/gpu.h"
+#endif
 #undef private
 
 void multips_shifted_invert(ShiftMultiFermion *out, MultiFermion *in, REAL res, RationalApprox approx);
@@ -97,6 +100,16 @@ int main(int argc, char **argv) {
   p_shiftferm = new ShiftFermion;
   fermion_shiftmulti = new ShiftMultiFermion;
   update_iteration = 0;
+#ifdef USE_GPU
+  // host staging arrays of the reference's device boundary (init.cc:79-96)
+  simple_fermion_packed = new float[6 * sizeh * 2];
+  chi_packed = new float[6 * sizeh * no_ps * 2];
+  psi_packed = new float[6 * sizeh * max_approx_order * no_ps * 2];
+  gauge_field_packed = new float[12 * no_links * 2];
+  ipdot_packed = new float[8 * no_links];
+  momenta_packed = new float[8 * no_links];
+  shift_table = new int[8 * size];
+#endif
 
   // inputs
   read_raw("links.bin", gauge_conf->u_work, sizeof(Su3) * no_links);
@@ -162,11 +175,19 @@ int main(int argc, char **argv) {
     // without the Metropolis draw.
     read_raw("mom.bin", gauge_momenta->momenta, sizeof(Su3) * no_links);
     read_raw("phi.bin", fermion_phi->fermion, sizeof(Vec3) * sizeh * no_ps);
+#ifdef USE_GPU
+    // the reference's device sequence: rhmc.cc:72 then update.cc:75-108
+    gpu_init0();
+    gpu_init1();
+#endif
     first_inv_approx_calc(residue_metro);
     write_raw("out_chi.bin", fermion_chi->fermion, sizeof(Vec3) * sizeh * no_ps);
     calc_action(plaq_test, 0);
     update_nometro();
     last_inv_approx_calc(residue_metro);
+#ifdef USE_GPU
+    gpu_end();
+#endif
     calc_action(plaq_test, -1);
     global_sum(plaq_test, size);
     double de = -plaq_test[0];

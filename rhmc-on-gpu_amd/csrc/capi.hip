@@ -356,7 +356,7 @@ int cu_multips_shifted_invert(rhmc_gpu_t h, double res, int relative, int order,
   return rc == 0 ? RHMC_OK : (rc == 2 ? RHMC_EMAXCG : RHMC_EHIP);
 }
 
-int cuda_shifted_inverter(rhmc_gpu_t h, double res, const double *shifts, int num_shifts, int psferm, int precision,
+int gpu_shifted_inverter(rhmc_gpu_t h, double res, const double *shifts, int num_shifts, int psferm, int precision,
                           int *ncount) {
   if (!h || !shifts || num_shifts < 1 || num_shifts > h->c.max_order || psferm < 0 || psferm >= h->c.n_pf)
     return RHMC_EINVAL;

@@ -92,13 +92,6 @@ __global__ __launch_bounds__(kBlock) void k_fermion_force(LatticeDims L, const d
   }
 }
 
-struct ForceBuf {
-  double2 *h = nullptr;
-  size_t cap = 0;
-  double *coef = nullptr;
-};
-static ForceBuf g_force_dummy;
-
 int fermion_force(Context &c, int nsys, const double *num, const double2 *x, int accumulate) {
   // nsys = n_pf * order systems, x = [n_pf][order][3][Vh]; num has nsys entries
   static thread_local std::vector<double> hcoef;

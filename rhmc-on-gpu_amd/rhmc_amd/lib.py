@@ -111,7 +111,7 @@ def _declare(L):
         "gpu_tuning_table": (I, [VP, C.c_char_p, I]),
         "gpu_time_copy": (I, [VP, D, I, PD]),
         "cu_multips_shifted_invert": (I, [VP, D, I, I, VP, I, I, PI]),
-        "cuda_shifted_inverter": (I, [VP, D, VP, I, I, I, PI]),
+        "gpu_shifted_inverter": (I, [VP, D, VP, I, I, I, PI]),
         "cu_shifted_residuals": (I, [VP, I, VP, VP]),
         "cu_last_solve_stats": (I, [VP, PI, PI, PI]),
     }
