@@ -7,11 +7,17 @@ Workload (one "step" = one RHMC trajectory, all fields resident in HBM):
   Omelyan 2MN with 10 steps of tau=1, links exp(i 0.3 H) seeded,
   momenta/pseudofermions drawn on the device from the seed.  Heatbath and
   Metropolis solves to relative residual 1e-10, MD force solves to 1e-6
-  relative.  Mixed precision: MD solves in FP32 (12-real links), heatbath and
-  Metropolis solves FP32 + FP64 refinement; Delta H of every mixed trajectory
-  is recomputed on the same endpoints with FP64 solves.  The FP64 path runs
-  the same trajectories from the same start with every solve in FP64.  The
-  value is the mean mixed-precision trajectory time.
+  relative.
+
+Timed paths, each from the same start configuration and seed:
+  mixed       MD solves FP32 (12-real links) on all 8 pseudofermions as one
+              multi-RHS solve; heatbath/Metropolis solves FP64 (the
+              reference's precision dispatch).  value = its mean time.
+  fp64        every solve FP64, same multi-RHS batching.
+  fp64_plain  every solve FP64, one pseudofermion per solve with per-shift
+              search directions: the reference's solve layout.
+Delta H of every mixed trajectory is recomputed on the same endpoints with
+FP64 solves to 1e-13 (untimed rerun); the difference is the Delta H error.
 
 Prints ONE JSON line.  --gpus above 1 is refused (one-GPU build).
 """
@@ -69,6 +75,171 @@ def cpu_baseline(dims, mass, beta, n_pf, shifts, md_a, chi, U, iters_total, n_fo
     return est, sample
 
 
+def _bounds(g, mass):
+    mx = g.find_max(1e-6, seed=11)
+    mn = max(g.find_min(mx, 1e-6, seed=12), mass * mass)
+    return mn, mx
+
+
+def _timed(fn, reps):
+    ts = []
+    out = None
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        out = fn()
+        ts.append(time.perf_counter() - t0)
+    return float(np.median(ts)), out
+
+
+def wl_ms16(args):
+    """Dslash + multi-shift CG on 16^4: exp(i 0.3 H) links, am=0.02, one
+    even-site gaussian source, 15 shifts of the Remez approximation of
+    x^(-1/2), relative residual 1e-10.  FP64 (18-real links) against FP32
+    iterations on 12-real links with FP64 defect correction."""
+    dims, mass = (16, 16, 16, 16), 0.02
+    lat = fields.Lattice(*dims)
+    U = fields.random_links(lat, 2024, 0.3)
+    src = fields.gauss_spinor(lat, 7, 1)
+    _, md, _ = rhmc_approximations(2, 1, 15, 15, 15)
+    g = Gpu(dims, mass=mass, beta=5.5, n_pf=1, max_order=15)
+    g.init0(U)
+    mn, mx = _bounds(g, mass)
+    _, _, b, _ = md.rescaled(mn, mx, -0.5)
+    b = np.array(b)
+    out = {"metric": "multi-shift CG seconds and Dslash GB/s, FP64 vs FP32+FP64 refinement, 16^4", "unit": "s/solve",
+           "config": {"workload": "16^4, am=0.02, 15 shifts, relative residual 1e-10, 1 rhs"},
+           "spectral_bounds": [mn, mx]}
+    bn = np.linalg.norm(src)
+    sols = {}
+    for name, prec in (("fp64", 0), ("mixed", 1)):
+        g.put_fermions("mf", src)
+        g.multishift(b, 1e-10, relative=True, precision=prec)
+
+        def solve():
+            g.put_fermions("mf", src)
+            return g.multishift(b, 1e-10, relative=True, precision=prec)
+        t, it = _timed(solve, 5)
+        res = g.shifted_residuals(b)[0] / bn
+        sols[name] = g.get_fermions("smf")
+        out[name] = {"s_per_solve": t, "iterations": it, "solve_stats": list(g.solve_stats()),
+                     "max_true_relative_residual": float(res.max())}
+    d = sols["mixed"] - sols["fp64"]
+    out["mixed_vs_fp64_solution_rel_diff"] = float(np.linalg.norm(d) / np.linalg.norm(sols["fp64"]))
+    out["value"] = out["mixed"]["s_per_solve"]
+    out["mixed_vs_fp64_speedup"] = out["fp64"]["s_per_solve"] / out["mixed"]["s_per_solve"]
+    ds = {}
+    for name, prec, lb, sb in (("fp32_12real_1rhs", 1, 48, 24), ("fp64_18real_1rhs", 0, 144, 48)):
+        ms = g.time_dslash(prec, 1, 200)
+        ds[name] = {"ms": ms, "GBps": dslash_bytes(lat.Vh, 1, lb, sb) / (ms * 1e-3) / 1e9}
+    out["dslash"] = ds
+    g.close()
+    return out
+
+
+def wl_force24(args):
+    """Fermion force as the adjoint of S = phi^+ r(MdagM) phi on 24^4:
+    exp(i 0.3 H) links, am=0.05, one pseudofermion, 12 poles.  Times the
+    solve + force with all per-pole solutions stored against recomputing them
+    in groups of poles, and checks the adjoint against central differences of
+    S (delta=1e-5) at 64 random (link, generator) pairs."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import torch
+    from force_autograd import PfModel, PseudofermionAction
+    dims, mass = (24, 24, 24, 24), 0.05
+    lat = fields.Lattice(*dims)
+    U = fields.random_links(lat, 31, 0.3)
+    phi = fields.gauss_spinor(lat, 32, 1)
+    _, md, _ = rhmc_approximations(2, 1, 12, 12, 12)
+    g = Gpu(dims, mass=mass, beta=5.5, n_pf=1, max_order=12)
+    g.init0(U)
+    mn, mx = _bounds(g, mass)
+    a0, a, b, _ = md.rescaled(mn, mx, -0.5)
+    out = {"metric": "fermion force seconds (store vs recompute) and adjoint-vs-finite-difference deviation, 24^4",
+           "unit": "s/force", "config": {"workload": "24^4, am=0.05, 1 pseudofermion, 12 poles, residual 1e-10"}}
+    times = {}
+    for chunk in (12, 4, 1):
+        m = PfModel(g, U, phi, a0, a, b, res=1e-10, precision=0, pole_chunk=chunk)
+        m._upload(U)
+
+        def force():
+            g.put_fermions("mf", phi)
+            return g.fermionforce(b, a, 1e-10, relative=True, precision=0, pole_chunk=chunk)
+        force()
+        t, it = _timed(force, 3)
+        times[f"pole_chunk_{chunk}"] = {"s": t, "iterations": it,
+                                        "solutions_held_bytes": lat.Vh * 48 * chunk}
+    out["store_vs_recompute"] = times
+    out["value"] = times["pole_chunk_12"]["s"]
+    model = PfModel(g, U, phi, a0, a, b, res=1e-14)
+    omega = torch.zeros((4 * lat.V, 8), dtype=torch.float64, requires_grad=True)
+    S = PseudofermionAction.apply(omega, model)
+    S.backward()
+    grad = omega.grad.numpy()
+    rng = np.random.default_rng(9)
+    d, dev = 1e-5, []
+    for _ in range(64):
+        link, gen = int(rng.integers(4 * lat.V)), int(rng.integers(8))
+        w = np.zeros((4 * lat.V, 8))
+        w[link, gen] = d
+        fd = (model.action(w) - model.action(-w)) / (2 * d)
+        dev.append(abs(fd - grad[link, gen]) / max(1.0, abs(grad[link, gen])))
+    out["finite_difference_max_rel_dev"] = float(max(dev))
+    out["action"] = float(S)
+    g.close()
+    return out
+
+
+def wl_traj48c12(args):
+    """Mixed-precision RHMC on 48^3x12: Nf=2, am=0.01, beta=5.6, 2
+    pseudofermions, Omelyan 20 steps of tau=1, 16 MD poles / 20 heatbath and
+    Metropolis poles, exp(i 0.3 H) start.  A Markov chain of --steps
+    trajectories with Metropolis accept/reject; Delta H of each is recomputed
+    with FP64 solves to 1e-13; the FP64 trajectory time is taken on the same
+    start configuration."""
+    dims, mass, beta, n_pf = (48, 48, 48, 12), 0.01, 5.6, 2
+    lat = fields.Lattice(*dims)
+    U0 = fields.random_links(lat, 4812, 0.3)
+    first, md, last = rhmc_approximations(2, n_pf, 20, 16, 20)
+    co = dict(first=(first.a0, first.a, first.b), md=(md.a0, md.a, md.b), last=(last.a0, last.a, last.b))
+    g = Gpu(dims, mass=mass, beta=beta, n_pf=n_pf, max_order=20)
+    g.init0(U0)
+    mn, mx = _bounds(g, mass)
+    kw = dict(n_flavours=2, no_md=20, integrator="omelyan", res_md=1e-6, res_metro=1e-10, relative=True,
+              bounds=(mn, mx), draw_fields=True, fp32_threshold=1e-7, **co)
+    rng = np.random.default_rng(48)
+    chain = []
+    for k in range(args.steps):
+        u = float(rng.random())
+        r = g.trajectory(precision=1, seed=5000 + k, verify_fp64=True, verify_res=1e-13, metropolis=True,
+                         metropolis_u=u, **kw)
+        chain.append({"s": r.seconds_total, "delta_h": r.delta_h, "delta_h_err": abs(r.delta_h - r.delta_h_fp64),
+                      "accepted": bool(r.accepted), "iters": r.cg_iterations})
+        print(f"[bench] 48^3x12 trajectory {k} {r.seconds_total:.3f}s dH={r.delta_h:.4e} "
+              f"err={abs(r.delta_h - r.delta_h_fp64):.2e} acc={r.accepted}", file=sys.stderr, flush=True)
+    t64 = []
+    for k in range(2):
+        g.init0(U0)
+        r = g.trajectory(precision=0, seed=7000 + k, **kw)
+        t64.append(r.seconds_total)
+        g.init0(U0)
+        r = g.trajectory(precision=1, seed=7000 + k, **kw)
+        t64.append(-r.seconds_total)
+    fp64 = float(np.mean([t for t in t64 if t > 0]))
+    mixed_same = float(np.mean([-t for t in t64 if t < 0]))
+    g.close()
+    return {"metric": "RHMC trajectory seconds, mixed vs FP64, 48^3x12", "unit": "s/traj",
+            "value": float(np.mean([c["s"] for c in chain])),
+            "config": {"workload": "48^3x12, Nf=2, am=0.01, beta=5.6, 2 pseudofermions, Omelyan 20 steps, "
+                                   "16 MD / 20 heatbath+Metropolis poles"},
+            "acceptance": float(np.mean([c["accepted"] for c in chain])),
+            "max_delta_h_error_vs_fp64_1e-13": float(max(c["delta_h_err"] for c in chain)),
+            "chain": chain, "fp64_s_per_traj_same_start": fp64, "mixed_s_per_traj_same_start": mixed_same,
+            "mixed_vs_fp64_speedup": fp64 / mixed_same, "spectral_bounds": [mn, mx]}
+
+
+WORKLOADS = {"traj32c8": None, "ms16": wl_ms16, "force24": wl_force24, "traj48c12": wl_traj48c12}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -81,12 +252,19 @@ def main():
     ap.add_argument("--npf", type=int, default=8)
     ap.add_argument("--res-md", type=float, default=1e-6)
     ap.add_argument("--res-metro", type=float, default=1e-10)
+    ap.add_argument("--metro-refined", action="store_true",
+                    help="also time mixed with FP32 + FP64 defect-correction heatbath/Metropolis solves")
     ap.add_argument("--dslash-only", action="store_true",
                     help="only the Dslash and copy-probe timings (for counter runs)")
+    ap.add_argument("--workload", default="traj32c8", choices=sorted(WORKLOADS),
+                    help="traj32c8 is the headline; the others are the per-configuration studies")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported"}))
         sys.exit(2)
+    if args.workload != "traj32c8":
+        print(json.dumps(WORKLOADS[args.workload](args)))
+        return
 
     dims = tuple(int(x) for x in args.dims.split(","))
     mass, beta, n_pf, order = 0.01, 5.6, args.npf, 16
@@ -119,37 +297,43 @@ def main():
     traj_kw = dict(n_flavours=2, no_md=args.no_md, integrator="omelyan", res_md=args.res_md,
                    res_metro=args.res_metro, relative=True, bounds=(mn, mx), draw_fields=True,
                    metropolis=False, fp32_threshold=1e-7, **co)
+    modes = {"mixed": dict(precision=1), "fp64": dict(precision=0),
+             "fp64_plain": dict(precision=0, per_pf_solves=True),
+             "mixed_metro_refined": dict(precision=1, metro_precision=1)}
 
-    def run(precision, seed, verify=False):
+    def run(mode, seed, verify=False):
         g.init0(U0)
         t0 = time.perf_counter()
-        r = g.trajectory(precision=precision, seed=seed, verify_fp64=verify, **traj_kw)
+        r = g.trajectory(seed=seed, verify_fp64=verify, verify_res=1e-13, **modes[mode], **traj_kw)
         dt = time.perf_counter() - t0
-        print(f"[bench] trajectory precision={'mixed' if precision else 'fp64'} seed={seed} {dt:.3f}s "
-              f"dH={r.delta_h:.3e} iters={r.cg_iterations} solves={r.n_solves} "
+        print(f"[bench] trajectory {mode}{' +verify' if verify else ''} seed={seed} {dt:.3f}s "
+              f"dH={r.delta_h:.6e} dH64={r.delta_h_fp64:.6e} iters={r.cg_iterations} solves={r.n_solves} "
               f"solver={r.seconds_solver:.3f}s force={r.seconds_force:.3f}s", file=sys.stderr, flush=True)
         return dt, r
 
+    legs = ["mixed"] + ([] if args.no_fp64 else ["fp64", "fp64_plain"]) + \
+        (["mixed_metro_refined"] if args.metro_refined else [])
     for w in range(args.warmup):
-        run(1, 100 + w)
-    t_mixed, t_fp64, dh_m, dh_d, iters_m, iters_d = [], [], [], [], [], []
-    dh_m64 = []
-    r_d = None
+        for leg in legs:
+            run(leg, 100 + w)
+    times = {leg: [] for leg in legs}
+    dh = {leg: [] for leg in legs}
+    iters = {leg: [] for leg in legs}
+    dh_err = {leg: [] for leg in legs if leg.startswith("mixed")}
     for s in range(args.steps):
-        dt, r = run(1, 1000 + s)
-        t_mixed.append(dt)
-        dh_m.append(r.delta_h)
-        iters_m.append(r.cg_iterations)
-        # same trajectory again (identical), now with the FP64 recomputation of Delta H
-        _, rv = run(1, 1000 + s, verify=True)
-        dh_m64.append(rv.delta_h_fp64)
-        if not args.no_fp64:
-            dt, r_d = run(0, 1000 + s)
-            t_fp64.append(dt)
-            dh_d.append(r_d.delta_h)
-            iters_d.append(r_d.cg_iterations)
-    val = float(np.mean(t_mixed))
-    fp64 = float(np.mean(t_fp64)) if t_fp64 else None
+        for leg in legs:
+            dt, r = run(leg, 1000 + s)
+            times[leg].append(dt)
+            dh[leg].append(r.delta_h)
+            iters[leg].append(r.cg_iterations)
+            if leg in dh_err:
+                # same trajectory again (identical), untimed, with the tight FP64 recomputation
+                _, rv = run(leg, 1000 + s, verify=True)
+                dh_err[leg].append(abs(rv.delta_h - rv.delta_h_fp64))
+    val = float(np.mean(times["mixed"]))
+    fp64 = float(np.mean(times["fp64"])) if times.get("fp64") else None
+    fp64_plain = float(np.mean(times["fp64_plain"])) if times.get("fp64_plain") else None
+    iters_d = iters.get("fp64", [])
 
     # Dslash roofline: FP32 12-real links, 8 right-hand sides (the mixed
     # solver's operator) and FP64 18-real, timed with HIP events on the
@@ -163,7 +347,7 @@ def main():
     gbs64 = b64 / (ms64 * 1e-3) / 1e9
 
     cpu = None
-    if not args.no_cpu and r_d is not None:
+    if not args.no_cpu and iters_d:
         n_force = 2 * args.no_md
         n_q = 2 * args.no_md + 1
         chi = fields.gauss_spinor(lat, 5)[0]
@@ -190,14 +374,13 @@ def main():
                                f"residual 1e-10 (heatbath/Metropolis), {args.res_md:g} (MD), relative",
                    "lattice": list(dims), "n_pf": n_pf, "poles": order, "no_md": args.no_md},
         "fp64_s_per_traj": fp64,
+        "fp64_plain_s_per_traj": fp64_plain,
         "mixed_vs_fp64_speedup": (fp64 / val) if fp64 else None,
-        "delta_h_mixed": dh_m,
-        "delta_h_fp64": dh_d,
-        "delta_h_mixed_fp64_recomputed": dh_m64,
-        "delta_h_error_vs_fp64_recomputation": [abs(a - b) for a, b in zip(dh_m, dh_m64)],
-        "delta_h_abs_diff_mixed_vs_fp64_trajectory": [abs(a - b) for a, b in zip(dh_m, dh_d)],
-        "cg_iterations_mixed": iters_m,
-        "cg_iterations_fp64": iters_d,
+        "mixed_vs_fp64_plain_speedup": (fp64_plain / val) if fp64_plain else None,
+        "s_per_traj": {leg: float(np.mean(v)) for leg, v in times.items()},
+        "delta_h": dh,
+        "delta_h_error_vs_fp64_1e-13": dh_err,
+        "cg_iterations": iters,
         "spectral_bounds": [mn, mx],
         "dslash": {"fp32_12real_8rhs_ms": ms32, "fp32_12real_8rhs_GBps": gbs32,
                    "fp64_18real_1rhs_ms": ms64, "fp64_18real_1rhs_GBps": gbs64,

@@ -86,6 +86,24 @@ int gpu_fermion_force(rhmc_gpu_t h, int num_shifts, const double *numerators, in
  * mf <- a0*mf + sum_k num[k]*smf[k] for every pseudofermion. */
 int gpu_sol_sum(rhmc_gpu_t h, int num_shifts, double mult_const, const double *numerators);
 
+/* lib/Action/fermionforce.cc:10-45 (the USE_GPU branch: the MD multi-shift
+ * solve of the pseudofermion right-hand sides in "mf", then
+ * cuda_fermion_force): ipdot <- fermion force of
+ * S = sum_ps phi^+ (a0 + sum_i num_i (MdagM + shifts_i)^-1) phi, the adjoint
+ * dS/dU.  pole_chunk < order solves the poles in groups of pole_chunk and
+ * accumulates the force group by group, so only n_pf*pole_chunk solutions are
+ * held at once (recompute instead of store; pole_chunk <= max_order);
+ * pole_chunk >= order is a single solve.  iterations: summed CG iterations. */
+int gpu_fermionforce(rhmc_gpu_t h, double res, int relative, int order, const double *shifts,
+                     const double *numerators, int precision, int max_cg, int pole_chunk, int *iterations);
+
+/* Pseudofermion action S = sum_ps Re <mf_ps, (a0 + sum_i num_i
+ * (MdagM + shifts_i)^-1) mf_ps> with one multi-shift solve; the fermion part
+ * of calc_action(-1) after last_inv_approx_calc (action.cc:87-101,
+ * rationalapprox.cc:330-369).  FP64 reductions. */
+int gpu_fermion_action(rhmc_gpu_t h, double res, int relative, int order, const double *shifts, double a0,
+                       const double *numerators, int precision, int max_cg, double *action);
+
 /* lib/Cuda/cuda_find_min_max.h:6-7 (findminmax.cc:140-183): power iterations
  * for the extreme eigenvalues of MdagM from a gaussian start drawn with seed;
  * tolerance on the relative change (reference: 1e-5). */
@@ -126,14 +144,14 @@ int gpu_tuning_table(rhmc_gpu_t h, char *buf, int buflen);
 /* --------------------------------------------------------------------- */
 /* One RHMC trajectory on the device: update.cc:58-152 with the integrators
  * of lib/Update/MD_integrators. */
-enum { RHMC_INT_OMELYAN = 0, RHMC_INT_MULTISTEP_2MN = 1, RHMC_INT_LEAPFROG = 2 };
+enum { RHMC_INT_OMELYAN = 0, RHMC_INT_MULTISTEP_2MN = 1, RHMC_INT_LEAPFROG = 2, RHMC_INT_MULTISTEP_4MN = 3 };
 
 typedef struct {
   int n_flavours;
   int no_md, gauge_scale, integrator;
   double res_md, res_metro;       /* multi-shift stopping residuals           */
   int residual_relative;          /* 1: residuals relative to |rhs|           */
-  int precision;                  /* 0 FP64, 1 mixed FP32 + FP64 refinement   */
+  int precision;                  /* 0 FP64, 1 mixed (see fp32_threshold)     */
   double fp32_threshold;          /* mixed: solves with res >= this run FP32
                                      only (the reference's
                                      inv_single_double_prec dispatch,
@@ -154,6 +172,18 @@ typedef struct {
   int metropolis;                 /* 1: accept/reject with u (below)          */
   double metropolis_u;
   int pure_gauge;
+  int metro_precision;            /* mixed: solves with res < fp32_threshold
+                                     (heatbath, Metropolis): 0 FP64 as the
+                                     reference dispatches them
+                                     (cu_inverter.cc:45-52), 1 FP32 iterations
+                                     + FP64 defect correction               */
+  int per_pf_solves;              /* 1: one pseudofermion per solve with
+                                     per-shift search directions, the
+                                     reference's loop (cu_inverter.cc:42-75);
+                                     0: all pseudofermions as one multi-RHS
+                                     solve on the stored residual basis     */
+  double verify_res;              /* verify_fp64: residual of the FP64
+                                     recomputation (0: res_metro)           */
 } rhmc_traj_params;
 
 typedef struct {
@@ -163,7 +193,8 @@ typedef struct {
   int cg_iterations;              /* summed over all solves                   */
   int n_solves;
   double eig_min, eig_max;
-  double seconds_total, seconds_solver, seconds_force, seconds_update;
+  double seconds_total, seconds_solver, seconds_force, seconds_update;  /* without the verification */
+  double seconds_verify;          /* the FP64 recomputation (verify_fp64=1)   */
 } rhmc_traj_result;
 
 int gpu_trajectory(rhmc_gpu_t h, const rhmc_traj_params *p, rhmc_traj_result *r);

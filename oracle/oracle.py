@@ -46,7 +46,7 @@ class _Traj(C.Structure):
                 ("eig_min", C.c_double), ("eig_max", C.c_double), ("pure_gauge", C.c_int)]
 
 
-INTEGRATORS = {"omelyan": 0, "multistep_2mn": 1, "leapfrog": 2}
+INTEGRATORS = {"omelyan": 0, "multistep_2mn": 1, "leapfrog": 2, "multistep_4mn": 3}
 
 
 def _p(a):

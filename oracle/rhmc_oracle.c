@@ -651,6 +651,52 @@ static void orc_multistep_2MN(orc_md *m) {
   md_P_fermion(m, -eps * l);
 }
 
+/* 4MN coefficients, multistep_4MN.cc:37-40 (Takaishi, de Forcrand) */
+static const double MN4_THETA = 0.08398315262876693, MN4_RHO = 0.2539785108410595,
+                    MN4_LAMBDA = 0.6822365335719091, MN4_MU = -0.03230286765269967;
+
+/* multistep_4MN.cc:30-165: gauge-only 4MN at step eps*scale; the trailing
+ * theta kick of one step merges with the leading one of the next */
+static void orc_multistep_4MN_gauge(orc_md *m, double scale) {
+  const double e = scale / m->prm->no_md;
+  const double th = MN4_THETA, rh = MN4_RHO, la = MN4_LAMBDA, mu = MN4_MU;
+  const double hf = 0.5 * (1.0 - 2.0 * (la + th)), mid = 1.0 - 2.0 * (mu + rh);
+  md_P_gauge(m, -e * th);
+  for (int md = 1; md < m->prm->gauge_scale; md++) {
+    md_Q(m, e * rh); md_P_gauge(m, -e * la);
+    md_Q(m, e * mu); md_P_gauge(m, -e * hf);
+    md_Q(m, e * mid); md_P_gauge(m, -e * hf);
+    md_Q(m, e * mu); md_P_gauge(m, -e * la);
+    md_Q(m, e * rh); md_P_gauge(m, -e * 2.0 * th);
+  }
+  md_Q(m, e * rh); md_P_gauge(m, -e * la);
+  md_Q(m, e * mu); md_P_gauge(m, -e * hf);
+  md_Q(m, e * mid); md_P_gauge(m, -e * hf);
+  md_Q(m, e * mu); md_P_gauge(m, -e * la);
+  md_Q(m, e * rh); md_P_gauge(m, -e * th);
+}
+
+/* multistep_4MN.cc:167-300: fermion kicks around nested gauge 4MN with
+ * gs = 1/gauge_scale */
+static void orc_multistep_4MN(orc_md *m) {
+  const double eps = 1.0 / m->prm->no_md, gs = 1.0 / (double)m->prm->gauge_scale;
+  const double th = MN4_THETA, rh = MN4_RHO, la = MN4_LAMBDA, mu = MN4_MU;
+  const double hf = 0.5 * (1.0 - 2.0 * (la + th)), mid = 1.0 - 2.0 * (mu + rh);
+  md_P_fermion(m, -eps * th);
+  for (int md = 1; md < m->prm->no_md; md++) {
+    orc_multistep_4MN_gauge(m, rh * gs); md_P_fermion(m, -eps * la);
+    orc_multistep_4MN_gauge(m, mu * gs); md_P_fermion(m, -eps * hf);
+    orc_multistep_4MN_gauge(m, mid * gs); md_P_fermion(m, -eps * hf);
+    orc_multistep_4MN_gauge(m, mu * gs); md_P_fermion(m, -eps * la);
+    orc_multistep_4MN_gauge(m, rh * gs); md_P_fermion(m, -eps * 2.0 * th);
+  }
+  orc_multistep_4MN_gauge(m, rh * gs); md_P_fermion(m, -eps * la);
+  orc_multistep_4MN_gauge(m, mu * gs); md_P_fermion(m, -eps * hf);
+  orc_multistep_4MN_gauge(m, mid * gs); md_P_fermion(m, -eps * hf);
+  orc_multistep_4MN_gauge(m, mu * gs); md_P_fermion(m, -eps * la);
+  orc_multistep_4MN_gauge(m, rh * gs); md_P_fermion(m, -eps * th);
+}
+
 /* leapfrog.cc:18-62 with the momentum sign of the live integrators
  * (minimum_norm2.cc:52): P <- P - i eps ipdot. */
 static void orc_leapfrog(orc_md *m) {
@@ -704,6 +750,7 @@ double orc_trajectory(const orc_lattice *L, const orc_traj_params *q, orc_c (*U)
                        -(double)q->n_flavours / 4.0 / (double)npf, 1, &m.md_a0, m.md_a, m.md_b);
   if (q->integrator == ORC_INT_OMELYAN) orc_minimum_norm2_A(&m);
   else if (q->integrator == ORC_INT_MULTISTEP_2MN) orc_multistep_2MN(&m);
+  else if (q->integrator == ORC_INT_MULTISTEP_4MN) orc_multistep_4MN(&m);
   else orc_leapfrog(&m);
 
   if (!q->pure_gauge) {

@@ -117,7 +117,7 @@ void orc_rescale_approx(int order, double a0_in, const double *a_in, const doubl
 /* Trajectory (update.cc:58-152) with the MD integrators of
  * lib/Update/MD_integrators.  The state is supplied explicitly. */
 
-enum { ORC_INT_OMELYAN = 0, ORC_INT_MULTISTEP_2MN = 1, ORC_INT_LEAPFROG = 2 };
+enum { ORC_INT_OMELYAN = 0, ORC_INT_MULTISTEP_2MN = 1, ORC_INT_LEAPFROG = 2, ORC_INT_MULTISTEP_4MN = 3 };
 
 typedef struct {
   double beta, mass;

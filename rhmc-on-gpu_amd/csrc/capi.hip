@@ -110,7 +110,7 @@ int gpu_create(const rhmc_gpu_params *p, rhmc_gpu_t *out) {
     return RHMC_EHIP;
   }
   c.ws = workspace_create(c);
-  if (!c.ws) { gpu_destroy(h); return RHMC_EHIP; }
+  if (!c.ws || build_tile_order(c)) { gpu_destroy(h); return RHMC_EHIP; }
   *out = h;
   return RHMC_OK;
 }
@@ -122,6 +122,7 @@ int gpu_destroy(rhmc_gpu_t h) {
   if (c.stream) hipStreamSynchronize(c.stream);
   dev_free(c.U64); dev_free(c.U64_save); dev_free(c.U32); dev_free(c.P); dev_free(c.ipdot);
   dev_free(c.mf); dev_free(c.smf); dev_free(c.phi0); dev_free(c.chi); dev_free(c.phinew); dev_free(c.action);
+  dev_free(c.tiles);
   workspace_destroy(c.ws);
   if (c.stream) hipStreamDestroy(c.stream);
   delete h;
@@ -228,6 +229,56 @@ int gpu_sol_sum(rhmc_gpu_t h, int num_shifts, double a0, const double *num) {
   if (num_shifts != c.smf_order) return fail(h, "gpu_sol_sum: order differs from the last solve");
   hipSetDevice(c.device);
   return sol_sum(c, c.mf, c.smf, num_shifts, a0, num) ? RHMC_EHIP : RHMC_OK;
+}
+
+static int solve_status(int rc) { return rc == 0 ? RHMC_OK : rc == 1 ? RHMC_EINVAL : rc == 2 ? RHMC_EMAXCG : RHMC_EHIP; }
+
+int gpu_fermionforce(rhmc_gpu_t h, double res, int relative, int order, const double *shifts, const double *num,
+                     int precision, int max_cg, int pole_chunk, int *iterations) {
+  if (!h || !shifts || !num || order < 1 || order > kMaxShifts) return RHMC_EINVAL;
+  Context &c = h->c;
+  if (pole_chunk < 1 || pole_chunk > order) pole_chunk = order;
+  if (pole_chunk > c.max_order) return fail(h, "gpu_fermionforce: pole_chunk above max_order");
+  hipSetDevice(c.device);
+  int total = 0;
+  for (int k0 = 0; k0 < order; k0 += pole_chunk) {
+    const int nk = order - k0 < pole_chunk ? order - k0 : pole_chunk;
+    int it = 0;
+    const int rc = multishift_solve(c, c.mf, c.n_pf, nk, shifts + k0, res, relative, precision, max_cg, c.smf, &it);
+    total += it;
+    if (rc) return solve_status(rc);
+    std::vector<double> coef((size_t)c.n_pf * nk);
+    for (int ps = 0; ps < c.n_pf; ps++)
+      for (int k = 0; k < nk; k++) coef[(size_t)ps * nk + k] = num[k0 + k];
+    if (fermion_force(c, c.n_pf * nk, coef.data(), c.smf, k0 > 0)) return RHMC_EHIP;
+    c.smf_order = nk;
+  }
+  if (iterations) *iterations = total;
+  return hip_ok(c, hipStreamSynchronize(c.stream), "fermionforce") ? RHMC_OK : RHMC_EHIP;
+}
+
+int gpu_fermion_action(rhmc_gpu_t h, double res, int relative, int order, const double *shifts, double a0,
+                       const double *num, int precision, int max_cg, double *action) {
+  if (!h || !shifts || !num || !action || order < 1 || order > h->c.max_order) return RHMC_EINVAL;
+  Context &c = h->c;
+  hipSetDevice(c.device);
+  int it = 0;
+  const int rc = multishift_solve(c, c.mf, c.n_pf, order, shifts, res, relative, precision, max_cg, c.smf, &it);
+  if (rc) return solve_status(rc);
+  c.smf_order = order;
+  const long sp = 3 * c.Vh;
+  double2 *tmp = (double2 *)dev_alloc(c, sizeof(double2) * sp * c.n_pf);
+  if (!tmp) return RHMC_EHIP;
+  hipMemcpyAsync(tmp, c.mf, sizeof(double2) * sp * c.n_pf, hipMemcpyDeviceToDevice, c.stream);
+  std::vector<double> d(c.n_pf);
+  int err = sol_sum(c, tmp, c.smf, order, a0, num);
+  if (!err) err = vec_dots(c, c.mf, tmp, c.n_pf, sp, d.data());
+  dev_free(tmp);
+  if (err) return RHMC_EHIP;
+  double s = 0.0;
+  for (int ps = 0; ps < c.n_pf; ps++) s += d[ps];
+  *action = s;
+  return RHMC_OK;
 }
 
 int gpu_find_max(rhmc_gpu_t h, double *mx, double tol, unsigned long long seed) {
@@ -353,7 +404,7 @@ int cu_multips_shifted_invert(rhmc_gpu_t h, double res, int relative, int order,
   int rc = multishift_solve(c, c.mf, c.n_pf, order, shifts, res, relative, precision, max_cg, c.smf, &it);
   c.smf_order = order;
   if (iterations) *iterations = it;
-  return rc == 0 ? RHMC_OK : (rc == 2 ? RHMC_EMAXCG : RHMC_EHIP);
+  return solve_status(rc);
 }
 
 int gpu_shifted_inverter(rhmc_gpu_t h, double res, const double *shifts, int num_shifts, int psferm, int precision,

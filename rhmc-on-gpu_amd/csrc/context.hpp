@@ -57,6 +57,17 @@ struct Context {
   double *action = nullptr;  // [V] per-site action differences
   int smf_order = 0;         // poles held in smf
 
+  // Dslash traversal order: kBlock-site tiles listed so that each eighth of
+  // the list (one XCD's share) is a compact sub-block of the lattice
+  int *tiles = nullptr;
+  int ntiles = 0;
+  std::vector<int> tiles_host;
+  int tile_split[3] = {1, 1, 1};  // sub-blocks in y, z, t
+
+  // multi-shift algorithm: 0 stored residual basis (falls back to 1 when the
+  // basis does not fit), 1 per-shift search directions (inverter.cc:142-339)
+  int ms_algorithm = 0;
+
   Workspace *ws = nullptr;
   std::string last_error;
 
@@ -103,6 +114,9 @@ int refresh_links32(Context &c);
 int vec_norms2(Context &c, const double2 *v, int nvec, long vstride, double *out);
 // out[j] = Re<a_j, b_j> (fixed-order FP64 reduction)
 int vec_dots(Context &c, const double2 *a, const double2 *b, int nvec, long vstride, double *out);
+
+// Dslash tile order for the lattice of c (solver.hip); allocates c.tiles
+int build_tile_order(Context &c);
 
 // host helpers
 bool hip_ok(Context &c, hipError_t e, const char *what);

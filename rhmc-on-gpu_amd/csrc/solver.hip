@@ -311,6 +311,7 @@ struct DsArgs {
   const int *skip; int skip_stride;    // stop flags (ints), may be null
   int nvec;
   unsigned nsb, nch;                   // site blocks x vector chunks
+  const int *tiles; int ntiles;        // traversal order of the kBlock-site tiles
   RedArgs red;
 };
 
@@ -334,8 +335,10 @@ __global__ __launch_bounds__(kBlock) void k_dslash(DsArgs a) {
   unsigned sb, ch;
   xcd_split(blockIdx.x, a.nsb, a.nch, sb, ch);
   const int jbase = ch * NR;
-  long h0, h1;
-  site_range(Vh, sb, a.nsb, h0, h1);
+  // this workgroup's run of tiles in the traversal order
+  const int per = (a.ntiles + (int)a.nsb - 1) / (int)a.nsb;
+  const int q0 = (int)sb * per;
+  const int q1 = q0 + per < a.ntiles ? q0 + per : a.ntiles;
 
   bool act[NR];
   bool any = false;
@@ -349,7 +352,9 @@ __global__ __launch_bounds__(kBlock) void k_dslash(DsArgs a) {
 #pragma unroll
   for (int r = 0; r < NR; r++) red[r] = 0.0;
 
-  for (long h = h0 + threadIdx.x; any && h < h1; h += kBlock) {
+  for (int q = q0; any && q < q1; q++) {
+    const long h = (long)(a.tiles ? a.tiles[q] : q) * kBlock + threadIdx.x;
+    if (h >= Vh) continue;
     const SiteCoord c = site_coords(L, a.par_out, h);
     const int pin_ = a.par_out, pnb = 1 - a.par_out;
     C acc[NR][3];
@@ -432,6 +437,77 @@ __global__ __launch_bounds__(kBlock) void k_dslash(DsArgs a) {
   if (RED != RED_NONE) block_partials<NR>(red, a.red.partial, jbase, a.nvec, a.nsb, sb);
 }
 
+// Dslash traversal order.  The hardware hands consecutive workgroups to the
+// eight XCDs in turn and xcd_split gives each XCD one contiguous eighth of the
+// site blocks, so each eighth of the tile list is one XCD's share.  Listing
+// the kBlock-site tiles by sub-block of a (y, z, t) decomposition into eight
+// equal sub-blocks makes each share compact: the neighbours a share reads
+// from other XCDs' L2 slices shrink from two full t-slices (share = one
+// t-slice) to the sub-block surface.  The split with the smallest surface
+// among those that give every sub-block the same number of whole tiles is
+// taken; without one the identity order is kept.
+int build_tile_order(Context &c) {
+  const LatticeDims &L = c.L;
+  const int T = (int)((c.Vh + kBlock - 1) / kBlock);
+  std::vector<int> order(T);
+  for (int q = 0; q < T; q++) order[q] = q;
+  int best[3] = {1, 1, 1};
+  double best_surface = 1e300;
+  const int dims[3] = {L.ny, L.nz, L.nt};
+  if (c.Vh % kBlock == 0 && T % 8 == 0) {
+    for (int sy = 1; sy <= 8; sy *= 2)
+      for (int sz = 1; sy * sz <= 8; sz *= 2) {
+        const int st = 8 / (sy * sz);
+        const int sp[3] = {sy, sz, st};
+        bool ok = true;
+        double surface = 0.0;
+        for (int d = 0; d < 3; d++) {
+          if (dims[d] % sp[d]) ok = false;
+          if (sp[d] > 1) surface += 2.0 * sp[d] / dims[d];
+        }
+        if (!ok || surface >= best_surface) continue;
+        // every tile inside one sub-block, T/8 tiles per sub-block
+        std::vector<int> count(8, 0);
+        for (int q = 0; q < T && ok; q++) {
+          int id0 = -1;
+          for (long h = (long)q * kBlock; h < (long)(q + 1) * kBlock; h += L.nxh) {
+            const long row = h / L.nxh;
+            const int y = (int)(row % L.ny), z = (int)(row / L.ny % L.nz), t = (int)(row / L.ny / L.nz);
+            const int id = y / (L.ny / sy) + sy * (z / (L.nz / sz) + sz * (t / (L.nt / st)));
+            if (id0 < 0) id0 = id;
+            else if (id != id0) { ok = false; break; }
+          }
+          if (ok) count[id0]++;
+        }
+        for (int k = 0; k < 8 && ok; k++) ok = count[k] == T / 8;
+        if (!ok) continue;
+        best_surface = surface;
+        best[0] = sy; best[1] = sz; best[2] = st;
+      }
+  }
+  if (best_surface < 1e300) {
+    const int sy = best[0], sz = best[1], st = best[2];
+    std::vector<int> key(T);
+    for (int q = 0; q < T; q++) {
+      const long row = (long)q * kBlock / L.nxh;
+      const int y = (int)(row % L.ny), z = (int)(row / L.ny % L.nz), t = (int)(row / L.ny / L.nz);
+      key[q] = y / (L.ny / sy) + sy * (z / (L.nz / sz) + sz * (t / (L.nt / st)));
+    }
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return key[a] < key[b]; });
+  }
+  c.tile_split[0] = best[0]; c.tile_split[1] = best[1]; c.tile_split[2] = best[2];
+  c.tiles_host = order;
+  c.ntiles = T;
+  if (getenv("RHMC_TILE_IDENTITY")) {
+    c.tiles = nullptr;
+    return 0;
+  }
+  c.tiles = (int *)dev_alloc(c, sizeof(int) * T);
+  if (!c.tiles) return -1;
+  if (!hip_ok(c, hipMemcpy(c.tiles, order.data(), sizeof(int) * T, hipMemcpyHostToDevice), "tile order")) return -1;
+  return 0;
+}
+
 // number of right-hand sides handled per thread for a launch with nvec vectors
 static int pick_nr(int nvec) {
   if (nvec >= 8) return 8;
@@ -455,6 +531,8 @@ static unsigned reduce_blocks(Context &c, int ny) {
 template <typename R, int LINK, int MODE, int RED>
 static void dslash_launch(Context &c, DsArgs &a, int nr) {
   a.L = c.L;
+  a.tiles = c.tiles;
+  a.ntiles = c.ntiles;
   a.nch = (unsigned)((a.nvec + nr - 1) / nr);
   a.nsb = RED == RED_NONE ? nbx_of(c) : reduce_blocks(c, a.nch);
   a.red.nbr = (int)a.nsb;
@@ -978,60 +1056,75 @@ __global__ __launch_bounds__(kBlock) void k_bms_d(BmsArgs a) {
   }
 }
 
-// x[j][s] (FP64) = sum_{i<K} coef[j][s][i] basis[j][i], NS shifts per pass
-// klen[j]: basis vectors of rhs j (slots beyond were never written)
+// x[j][s] (FP64) = sum_{i<K} coef[j][s][i] basis[j][i] for NS shifts per
+// pass; klen[j] = basis vectors of rhs j (slots beyond were never written).
+// A tall-skinny product (3 Vh complex rows x K) . (K x NS real): one thread
+// per (colour, site) element streams its K basis values once, while the
+// workgroup stages the coefficients of kTileK basis indices in LDS, where
+// every lane reads the same entry (broadcast).  Coefficients of shifts at or
+// beyond nshift are staged as zero, so the inner loop has no per-shift
+// branches.
+constexpr int kTileK = 32;
+
 template <typename R, int NS>
 __global__ __launch_bounds__(kBlock) void k_bms_assemble(const void *basis, const double *coef, const int *klen,
                                                          double2 *x, long Vh, int cap, int nshift, int s0) {
   using C = Cx<R>;
-  const int bx = xcd_block(blockIdx.x, gridDim.x);
-  const long h = (long)bx * kBlock + threadIdx.x;
+  __shared__ double2 cs[kTileK][NS / 2];
+  const long n3 = 3 * Vh;
+  const long e = (long)xcd_block(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
   const int j = blockIdx.y;
-  if (h >= Vh) return;
   const int kmax = klen[j];
-  const C *bj = (const C *)basis + (long)j * cap * 3 * Vh;
-  const double *cj = coef + ((long)j * nshift + s0) * cap;
   const int ns = nshift - s0 < NS ? nshift - s0 : NS;
-  for (int c = 0; c < 3; c++) {
-    double2 acc[NS];
+  const C *bj = (const C *)basis + (long)j * cap * n3 + (e < n3 ? e : 0);
+  const double *cj = coef + ((long)j * nshift + s0) * cap;
+  double2 acc[NS];
 #pragma unroll
-    for (int q = 0; q < NS; q++) acc[q] = make_double2(0.0, 0.0);
-    const C *bc = bj + (long)c * Vh + h;
-    int i = 0;
-    // 8 basis loads in flight per thread before the FMAs
-    for (; i + 8 <= kmax; i += 8) {
+  for (int q = 0; q < NS; q++) acc[q] = make_double2(0.0, 0.0);
+  for (int i0 = 0; i0 < kmax; i0 += kTileK) {
+    const int nk = kmax - i0 < kTileK ? kmax - i0 : kTileK;
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < kTileK * NS; idx += kBlock) {
+      const int u = idx / NS, q = idx - u * NS;
+      ((double *)cs[u])[q] = (u < nk && q < ns) ? cj[(long)q * cap + i0 + u] : 0.0;
+    }
+    __syncthreads();
+    if (e >= n3) continue;
+    int u = 0;
+    for (; u + 8 <= nk; u += 8) {
       C v[8];
 #pragma unroll
-      for (int u = 0; u < 8; u++) v[u] = bc[(long)(i + u) * 3 * Vh];
+      for (int w = 0; w < 8; w++) v[w] = bj[(long)(i0 + u + w) * n3];
 #pragma unroll
-      for (int u = 0; u < 8; u++) {
-        const double vx = (double)v[u].x, vy = (double)v[u].y;
+      for (int w = 0; w < 8; w++) {
+        const double vx = (double)v[w].x, vy = (double)v[w].y;
 #pragma unroll
-        for (int q = 0; q < NS; q++) {
-          if (q < ns) {
-            const double d = cj[(long)q * cap + i + u];
-            acc[q].x += d * vx;
-            acc[q].y += d * vy;
-          }
+        for (int q2 = 0; q2 < NS / 2; q2++) {
+          const double2 d = cs[u + w][q2];
+          acc[2 * q2].x += d.x * vx;
+          acc[2 * q2].y += d.x * vy;
+          acc[2 * q2 + 1].x += d.y * vx;
+          acc[2 * q2 + 1].y += d.y * vy;
         }
       }
     }
-    for (; i < kmax; i++) {
-      const C v = bc[(long)i * 3 * Vh];
+    for (; u < nk; u++) {
+      const C v = bj[(long)(i0 + u) * n3];
       const double vx = (double)v.x, vy = (double)v.y;
 #pragma unroll
-      for (int q = 0; q < NS; q++) {
-        if (q < ns) {
-          const double d = cj[(long)q * cap + i];
-          acc[q].x += d * vx;
-          acc[q].y += d * vy;
-        }
+      for (int q2 = 0; q2 < NS / 2; q2++) {
+        const double2 d = cs[u][q2];
+        acc[2 * q2].x += d.x * vx;
+        acc[2 * q2].y += d.x * vy;
+        acc[2 * q2 + 1].x += d.y * vx;
+        acc[2 * q2 + 1].y += d.y * vy;
       }
     }
-#pragma unroll
-    for (int q = 0; q < NS; q++)
-      if (q < ns) x[(((long)j * nshift + s0 + q) * 3 + c) * Vh + h] = acc[q];
   }
+  if (e >= n3) return;
+#pragma unroll
+  for (int q = 0; q < NS; q++)
+    if (q < ns) x[((long)j * nshift + s0 + q) * n3 + e] = acc[q];
 }
 
 // ------------------------------------------------------------- host drivers
@@ -1248,8 +1341,8 @@ static int ms_cg_basis(Context &c, const double2 *b, int nrhs, int nshift, const
   hipMemcpyAsync(dcoef, coef.data(), sizeof(double) * coef.size(), hipMemcpyHostToDevice, c.stream);
   hipMemcpyAsync(dklen, klen.data(), sizeof(int) * nrhs, hipMemcpyHostToDevice, c.stream);
   for (int s0 = 0; s0 < nshift; s0 += 16)
-    hipLaunchKernelGGL((k_bms_assemble<R, 16>), dim3(nbx_of(c), nrhs), dim3(kBlock), 0, c.stream, basis, dcoef, dklen,
-                       x, Vh, (int)cap, nshift, s0);
+    hipLaunchKernelGGL((k_bms_assemble<R, 16>), dim3((unsigned)((3 * Vh + kBlock - 1) / kBlock), nrhs),
+                       dim3(kBlock), 0, c.stream, basis, dcoef, dklen, x, Vh, (int)cap, nshift, s0);
   if (!hip_ok(c, hipStreamSynchronize(c.stream), "basis assemble")) return -1;
   *iters_out = kmax;
   for (int j = 0; j < nrhs; j++)
@@ -1363,7 +1456,7 @@ int multishift_solve(Context &c, const double2 *b, int nrhs, int nshift, const d
   for (int j = 0; j < nrhs; j++) res_abs[j] = rel ? res * sqrt(bn2[j]) : res;
   c.last_refine_cycles = 0;
   c.last_iters_refine = 0;
-  const bool explicit_shifts = getenv("RHMC_MS_EXPLICIT") != nullptr;
+  const bool explicit_shifts = c.ms_algorithm == 1 || getenv("RHMC_MS_EXPLICIT") != nullptr;
   if (prec == PREC_FP64) {
     int rc = explicit_shifts ? 3 : ms_cg_basis<double, LINK_D18>(c, b, nrhs, nshift, shifts, res_abs.data(), max_iter, x, iters);
     if (rc == 3) rc = ms_cg<double, LINK_D18>(c, b, nrhs, nshift, shifts, res_abs.data(), max_iter, x, iters);

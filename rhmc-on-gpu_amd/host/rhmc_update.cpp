@@ -5,6 +5,7 @@
 //   lib/Tools/RationalApprox/rationalapprox.cc:134-369  rescaling, chi / phi_new
 //   lib/Update/MD_integrators/minimum_norm2.cc:28-105    Omelyan 2MN
 //   lib/Update/MD_integrators/multistep_2MN.cc:31-168    multistep 2MN
+//   lib/Update/MD_integrators/multistep_4MN.cc:30-300    multistep 4MN
 //   lib/Update/MD_integrators/leapfrog.cc:18-62          leapfrog, with the
 //       momentum sign of the live integrators (P <- P - i eps ipdot)
 //   lib/Update/ipdot.cc:83-196, lib/Update/momenta.cc:42-115
@@ -60,20 +61,40 @@ struct Traj {
 
   double secs(clk::time_point t0) { return std::chrono::duration<double>(clk::now() - t0).count(); }
 
-  // mixed runs solves at or above fp32_threshold in FP32 only, like the
-  // reference's inv_single_double_prec dispatch (cu_inverter.cc:45-60)
+  // mixed runs solves at or above fp32_threshold in FP32 only and the rest
+  // in FP64, like the reference's inv_single_double_prec dispatch
+  // (cu_inverter.cc:45-60), or with metro_precision=1 the rest as FP32
+  // iterations with FP64 defect correction
   int prec_for(double res) const {
-    if (p.precision == PREC_MIXED && p.fp32_threshold > 0.0 && res >= p.fp32_threshold) return PREC_FP32;
-    return p.precision;
+    if (p.precision != PREC_MIXED) return p.precision;
+    if (p.fp32_threshold > 0.0 && res >= p.fp32_threshold) return PREC_FP32;
+    return p.metro_precision == 1 ? PREC_MIXED : PREC_FP64;
   }
 
   int solve(const double2 *b, const Coeffs &co, double res, int prec = -1, double2 *x = nullptr) {
     auto t0 = clk::now();
-    int it = 0;
-    int rc = multishift_solve(c, b, c.n_pf, (int)co.b.size(), co.b.data(), res, p.residual_relative,
-                              prec < 0 ? prec_for(res) : prec, p.max_cg, x ? x : c.smf, &it);
-    c.smf_order = (int)co.b.size();
-    r.cg_iterations += it;
+    const int order = (int)co.b.size();
+    const int pr = prec < 0 ? prec_for(res) : prec;
+    double2 *xo = x ? x : c.smf;
+    int rc = 0;
+    if (p.per_pf_solves) {
+      // cu_inverter.cc:42-75: one pseudofermion at a time
+      const long sp = 3 * c.Vh;
+      const int saved = c.ms_algorithm;
+      c.ms_algorithm = 1;
+      for (int ps = 0; ps < c.n_pf && !rc; ps++) {
+        int it = 0;
+        rc = multishift_solve(c, b + ps * sp, 1, order, co.b.data(), res, p.residual_relative, pr, p.max_cg,
+                              xo + (size_t)ps * order * sp, &it);
+        r.cg_iterations += it;
+      }
+      c.ms_algorithm = saved;
+    } else {
+      int it = 0;
+      rc = multishift_solve(c, b, c.n_pf, order, co.b.data(), res, p.residual_relative, pr, p.max_cg, xo, &it);
+      r.cg_iterations += it;
+    }
+    c.smf_order = order;
     r.n_solves++;
     t_solve += secs(t0);
     if (rc && !status) status = rc;
@@ -157,6 +178,44 @@ struct Traj {
     ms2mn_gauge(gs);
     P(-eps * l, true, false);
   }
+  // multistep_4MN.cc:37-40 (Takaishi, de Forcrand)
+  static constexpr double k4Theta = 0.08398315262876693, k4Rho = 0.2539785108410595,
+                          k4Lambda = 0.6822365335719091, k4Mu = -0.03230286765269967;
+  void ms4mn_gauge(double scale) {  // multistep_4MN.cc:30-165
+    const double e = scale / p.no_md, th = k4Theta, rh = k4Rho, la = k4Lambda, mu = k4Mu;
+    const double hf = 0.5 * (1.0 - 2.0 * (la + th)), mid = 1.0 - 2.0 * (mu + rh);
+    P(-e * th, false, true);
+    for (int md_ = 1; md_ < p.gauge_scale; md_++) {
+      Q(e * rh); P(-e * la, false, true);
+      Q(e * mu); P(-e * hf, false, true);
+      Q(e * mid); P(-e * hf, false, true);
+      Q(e * mu); P(-e * la, false, true);
+      Q(e * rh); P(-e * 2.0 * th, false, true);
+    }
+    Q(e * rh); P(-e * la, false, true);
+    Q(e * mu); P(-e * hf, false, true);
+    Q(e * mid); P(-e * hf, false, true);
+    Q(e * mu); P(-e * la, false, true);
+    Q(e * rh); P(-e * th, false, true);
+  }
+  void ms4mn() {  // multistep_4MN.cc:167-300
+    const double eps = 1.0 / p.no_md, gs = 1.0 / (double)p.gauge_scale;
+    const double th = k4Theta, rh = k4Rho, la = k4Lambda, mu = k4Mu;
+    const double hf = 0.5 * (1.0 - 2.0 * (la + th)), mid = 1.0 - 2.0 * (mu + rh);
+    P(-eps * th, true, false);
+    for (int md_ = 1; md_ < p.no_md; md_++) {
+      ms4mn_gauge(rh * gs); P(-eps * la, true, false);
+      ms4mn_gauge(mu * gs); P(-eps * hf, true, false);
+      ms4mn_gauge(mid * gs); P(-eps * hf, true, false);
+      ms4mn_gauge(mu * gs); P(-eps * la, true, false);
+      ms4mn_gauge(rh * gs); P(-eps * 2.0 * th, true, false);
+    }
+    ms4mn_gauge(rh * gs); P(-eps * la, true, false);
+    ms4mn_gauge(mu * gs); P(-eps * hf, true, false);
+    ms4mn_gauge(mid * gs); P(-eps * hf, true, false);
+    ms4mn_gauge(mu * gs); P(-eps * la, true, false);
+    ms4mn_gauge(rh * gs); P(-eps * th, true, false);
+  }
   void leapfrog() {  // leapfrog.cc:18-62 with the live momentum sign
     const double eps = 1.0 / p.no_md;
     const bool f = !p.pure_gauge;
@@ -174,7 +233,8 @@ struct Traj {
 
 int run_trajectory(Context &c, const rhmc_traj_params &p, rhmc_traj_result &r) {
   memset(&r, 0, sizeof(r));
-  if (p.no_md < 1 || (p.integrator == RHMC_INT_MULTISTEP_2MN && p.gauge_scale < 1)) {
+  if (p.no_md < 1 || p.integrator < 0 || p.integrator > RHMC_INT_MULTISTEP_4MN ||
+      ((p.integrator == RHMC_INT_MULTISTEP_2MN || p.integrator == RHMC_INT_MULTISTEP_4MN) && p.gauge_scale < 1)) {
     c.last_error = "trajectory: bad MD parameters";
     return RHMC_EINVAL;
   }
@@ -215,6 +275,7 @@ int run_trajectory(Context &c, const rhmc_traj_params &p, rhmc_traj_result &r) {
   action_local(c, c.phi0, c.chi, !p.pure_gauge, 0);
   if (p.integrator == RHMC_INT_OMELYAN) T.omelyan();
   else if (p.integrator == RHMC_INT_MULTISTEP_2MN) T.ms2mn();
+  else if (p.integrator == RHMC_INT_MULTISTEP_4MN) T.ms4mn();
   else T.leapfrog();
   if (T.status > 0) return RHMC_EMAXCG;
   if (T.status < 0) return RHMC_EHIP;
@@ -230,9 +291,15 @@ int run_trajectory(Context &c, const rhmc_traj_params &p, rhmc_traj_result &r) {
   if (action_sum(c, &s)) return RHMC_EHIP;
   r.delta_h = -s;
   r.delta_h_fp64 = r.delta_h;
+  r.seconds_total = std::chrono::duration<double>(clk::now() - t0).count();
+  r.seconds_solver = T.t_solve;
+  r.seconds_force = T.t_force;
+  r.seconds_update = T.t_update;
   if (p.verify_fp64 && !p.pure_gauge) {
-    // Recompute the fermion action of the same endpoints with FP64 solves:
-    // chi64 = r_first(MdagM[U_start]) phi, phi_new64 = r_last(MdagM[U_end]) chi64.
+    const auto tv = clk::now();
+    // Recompute the fermion action of the same endpoints with FP64 solves to
+    // verify_res: chi64 = r_first(MdagM[U_start]) phi,
+    // phi_new64 = r_last(MdagM[U_end]) chi64.
     // Gauge and momentum parts are identical, |phi|^2 needs no solve.
     const size_t bytes = sizeof(double2) * sp * c.n_pf;
     double2 *chi64 = nullptr, *phn64 = nullptr;
@@ -240,12 +307,13 @@ int run_trajectory(Context &c, const rhmc_traj_params &p, rhmc_traj_result &r) {
       return RHMC_EHIP;
     Coeffs cf = rescale(p.order_first, p.first_a0, p.first_a, p.first_b, mn, mx, e_first);
     Coeffs cl = rescale(p.order_last, p.last_a0, p.last_a, p.last_b, mn, mx, e_md);
+    const double vres = p.verify_res > 0.0 ? p.verify_res : p.res_metro;
     std::swap(c.U64, c.U64_save);
-    int rc1 = T.solve(c.phi0, cf, p.res_metro, PREC_FP64);
+    int rc1 = T.solve(c.phi0, cf, vres, PREC_FP64);
     std::swap(c.U64, c.U64_save);
     hipMemcpyAsync(chi64, c.phi0, bytes, hipMemcpyDeviceToDevice, c.stream);
     if (!rc1) sol_sum(c, chi64, c.smf, p.order_first, cf.a0, cf.a.data());
-    int rc2 = rc1 ? rc1 : T.solve(chi64, cl, p.res_metro, PREC_FP64);
+    int rc2 = rc1 ? rc1 : T.solve(chi64, cl, vres, PREC_FP64);
     hipMemcpyAsync(phn64, chi64, bytes, hipMemcpyDeviceToDevice, c.stream);
     if (!rc2) sol_sum(c, phn64, c.smf, p.order_last, cl.a0, cl.a.data());
     std::vector<double> d_mixed(c.n_pf), d_64(c.n_pf);
@@ -257,6 +325,7 @@ int run_trajectory(Context &c, const rhmc_traj_params &p, rhmc_traj_result &r) {
     double diff = 0.0;
     for (int ps = 0; ps < c.n_pf; ps++) diff += d_64[ps] - d_mixed[ps];
     r.delta_h_fp64 = r.delta_h + diff;
+    r.seconds_verify = std::chrono::duration<double>(clk::now() - tv).count();
   }
   r.accepted = 1;
   if (p.metropolis) {
@@ -268,10 +337,6 @@ int run_trajectory(Context &c, const rhmc_traj_params &p, rhmc_traj_result &r) {
     }
   }
   if (!hip_ok(c, hipStreamSynchronize(c.stream), "trajectory")) return RHMC_EHIP;
-  r.seconds_total = std::chrono::duration<double>(clk::now() - t0).count();
-  r.seconds_solver = T.t_solve;
-  r.seconds_force = T.t_force;
-  r.seconds_update = T.t_update;
   return RHMC_OK;
 }
 

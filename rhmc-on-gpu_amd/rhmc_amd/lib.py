@@ -66,14 +66,15 @@ class TrajParams(C.Structure):
                 ("lambda_min", C.c_double), ("bounds_mode", C.c_int), ("eig_min", C.c_double),
                 ("eig_max", C.c_double), ("power_tol", C.c_double), ("draw_fields", C.c_int),
                 ("seed", C.c_ulonglong), ("metropolis", C.c_int), ("metropolis_u", C.c_double),
-                ("pure_gauge", C.c_int)]
+                ("pure_gauge", C.c_int), ("metro_precision", C.c_int), ("per_pf_solves", C.c_int),
+                ("verify_res", C.c_double)]
 
 
 class TrajResult(C.Structure):
     _fields_ = [("delta_h", C.c_double), ("delta_h_fp64", C.c_double), ("accepted", C.c_int), ("cg_iterations", C.c_int),
                 ("n_solves", C.c_int), ("eig_min", C.c_double), ("eig_max", C.c_double),
                 ("seconds_total", C.c_double), ("seconds_solver", C.c_double),
-                ("seconds_force", C.c_double), ("seconds_update", C.c_double)]
+                ("seconds_force", C.c_double), ("seconds_update", C.c_double), ("seconds_verify", C.c_double)]
 
 
 VP = C.c_void_p
@@ -99,6 +100,8 @@ def _declare(L):
         "gpu_gauge_deriv": (I, [VP, I]),
         "gpu_fermion_force": (I, [VP, I, VP, I]),
         "gpu_sol_sum": (I, [VP, I, D, VP]),
+        "gpu_fermionforce": (I, [VP, D, I, I, VP, VP, I, I, I, PI]),
+        "gpu_fermion_action": (I, [VP, D, I, I, VP, D, VP, I, I, PD]),
         "gpu_find_max": (I, [VP, PD, D, U64]),
         "gpu_find_min": (I, [VP, PD, D, D, U64]),
         "gpu_action": (I, [VP, I, I]),
@@ -129,7 +132,7 @@ def _ptr(a: np.ndarray):
     return a.ctypes.data_as(C.c_void_p)
 
 
-INTEGRATORS = {"omelyan": 0, "multistep_2mn": 1, "leapfrog": 2}
+INTEGRATORS = {"omelyan": 0, "multistep_2mn": 1, "leapfrog": 2, "multistep_4mn": 3}
 FERMION_BUFS = {"mf": 0, "smf": 1, "phi": 2, "chi": 3, "phinew": 4}
 
 
@@ -254,6 +257,28 @@ class Gpu:
         num = np.ascontiguousarray(numerators, np.float64)
         self._check(self.L.gpu_sol_sum(self.h, len(num), float(a0), _ptr(num)), "gpu_sol_sum")
 
+    def fermionforce(self, shifts, numerators, res, relative=False, precision=0, max_cg=100000, pole_chunk=0):
+        """MD solve of the right-hand sides in "mf" and the fermion force into
+        ipdot (fermionforce.cc:10-45); pole_chunk < order recomputes the
+        solutions group by group instead of holding all of them."""
+        sh = np.ascontiguousarray(shifts, np.float64)
+        num = np.ascontiguousarray(numerators, np.float64)
+        it = C.c_int()
+        self._check(self.L.gpu_fermionforce(self.h, float(res), int(relative), len(sh), _ptr(sh), _ptr(num),
+                                            int(precision), int(max_cg), int(pole_chunk), C.byref(it)),
+                    "gpu_fermionforce")
+        return it.value
+
+    def fermion_action(self, shifts, a0, numerators, res, relative=False, precision=0, max_cg=100000):
+        """sum_ps Re <mf, (a0 + sum_i num_i (MdagM + shift_i)^-1) mf>."""
+        sh = np.ascontiguousarray(shifts, np.float64)
+        num = np.ascontiguousarray(numerators, np.float64)
+        v = C.c_double()
+        self._check(self.L.gpu_fermion_action(self.h, float(res), int(relative), len(sh), _ptr(sh), float(a0),
+                                              _ptr(num), int(precision), int(max_cg), C.byref(v)),
+                    "gpu_fermion_action")
+        return v.value
+
     def find_max(self, tol=1e-5, seed=1):
         v = C.c_double()
         self._check(self.L.gpu_find_max(self.h, C.byref(v), float(tol), seed), "gpu_find_max")
@@ -282,9 +307,15 @@ class Gpu:
                    res_md=1e-10, res_metro=1e-10, relative=False, precision=0, fp32_threshold=0.0,
                    verify_fp64=False, max_cg=100000,
                    lambda_min=1e-5, bounds=None, power_tol=1e-5, draw_fields=False, seed=1,
-                   metropolis=False, metropolis_u=0.5, pure_gauge=False):
+                   metropolis=False, metropolis_u=0.5, pure_gauge=False, metro_precision=0,
+                   per_pf_solves=False, verify_res=0.0):
         """first/md/last: (a0, a[], b[]) normalised coefficients; bounds=(min,max)
-        for fixed spectral bounds, None to measure them with power iterations."""
+        for fixed spectral bounds, None to measure them with power iterations.
+        precision=1 (mixed): solves with res >= fp32_threshold run in FP32, the
+        others in FP64 (metro_precision=0) or as FP32 iterations with FP64
+        defect correction (metro_precision=1).  per_pf_solves: one
+        pseudofermion per solve with per-shift search directions (the
+        reference's loop) instead of one multi-RHS solve."""
         keep = []
 
         def arr(x):
@@ -303,7 +334,9 @@ class Gpu:
                        lambda_min=lambda_min, bounds_mode=0 if bounds else 1,
                        eig_min=bounds[0] if bounds else 0.0, eig_max=bounds[1] if bounds else 0.0,
                        power_tol=power_tol, draw_fields=int(draw_fields), seed=seed,
-                       metropolis=int(metropolis), metropolis_u=metropolis_u, pure_gauge=int(pure_gauge))
+                       metropolis=int(metropolis), metropolis_u=metropolis_u, pure_gauge=int(pure_gauge),
+                       metro_precision=int(metro_precision), per_pf_solves=int(per_pf_solves),
+                       verify_res=float(verify_res))
         r = TrajResult()
         self._check(self.L.gpu_trajectory(self.h, C.byref(t), C.byref(r)), "gpu_trajectory")
         return r

@@ -38,7 +38,7 @@ def _load(name, path):
 build_ref = _load("build_ref", os.path.join(ROOT, "oracle", "build_ref.py"))
 
 SEEDS = {"links": 101, "mom": 202, "phi": 303, "in": 404, "ipdot": 505, "chi": 606}
-LINK_EPS = {"t4": None, "t4o": 0.3, "c1": None}
+LINK_EPS = {"t4": None, "t4o": 0.3, "t4m": 0.3, "c1": None}
 
 
 def write_approx(d, eig_min, eig_max, apps):

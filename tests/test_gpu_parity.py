@@ -171,24 +171,29 @@ def _traj_args(gd):
                 md=(float(gd["md_a0"]), gd["md_a"], gd["md_b"]),
                 last=(float(gd["last_a0"]), gd["last_a"], gd["last_b"]),
                 no_md=int(gd["no_md"]),
-                integrator="multistep_2mn" if int(gd["use_multistep"]) == 1 else "omelyan",
+                integrator=("omelyan", "multistep_2mn", "multistep_4mn")[int(gd["use_multistep"])],
                 gauge_scale=int(gd["gauge_scale"]), res_md=float(gd["res_md"]),
                 res_metro=float(gd["res_metro"]), bounds=(float(gd["eig_min"]), float(gd["eig_max"])))
 
 
-@pytest.mark.parametrize("name", ["t4", "t4o"])
-@pytest.mark.parametrize("precision", [0, 1])
-def test_trajectory_golden_reference(name, precision):
-    """Whole trajectory on the device against the reference's own update()."""
+@pytest.mark.parametrize("name", ["t4", "t4o", "t4m"])
+@pytest.mark.parametrize("mode", ["fp64", "mixed", "fp64_per_pf"])
+def test_trajectory_golden_reference(name, mode):
+    """Whole trajectory on the device against the reference's own update().
+    mixed: every solve (residuals 1e-12 here) as FP32 iterations with FP64
+    defect correction; fp64_per_pf: one pseudofermion per solve with per-shift
+    search directions, the reference's own solve layout."""
     gd = dict(np.load(os.path.join(GOLD, f"golden_{name}.npz")))
     dims = tuple(gd["dims"])
     npf = int(gd["no_ps"])
-    say("trajectory", name, precision)
+    say("trajectory", name, mode)
     g = Gpu(dims, mass=float(gd["mass"]), beta=float(gd["beta"]), n_pf=npf, max_order=8)
     g.init0(gd["links"])
     g.init1(momenta=gd["mom"])
     g.put_fermions("phi", gd["phi"])
-    r = g.trajectory(precision=precision, **_traj_args(gd))
+    kw = dict(fp64=dict(precision=0), mixed=dict(precision=1, metro_precision=1),
+              fp64_per_pf=dict(precision=0, per_pf_solves=True))[mode]
+    r = g.trajectory(**kw, **_traj_args(gd))
     U, _, P = g.end(links=True, ipdot=False, momenta=True)
     chi = g.get_fermions("chi")
     phin = g.get_fermions("phinew")

@@ -21,7 +21,7 @@ def rel(a, b):
     return np.linalg.norm((a - b).ravel()) / max(np.linalg.norm(b.ravel()), 1e-300)
 
 
-@pytest.fixture(scope="module", params=["t4", "t4o"])
+@pytest.fixture(scope="module", params=["t4", "t4o", "t4m"])
 def g(request):
     return load(request.param)
 
@@ -84,7 +84,7 @@ def test_action(g):
 
 def test_trajectory(g):
     o = Oracle(tuple(g["dims"]))
-    integ = "multistep_2mn" if int(g["use_multistep"]) == 1 else "omelyan"
+    integ = ("omelyan", "multistep_2mn", "multistep_4mn")[int(g["use_multistep"])]
     dh, U, P, chi, phin = o.trajectory(
         g["links"], g["mom"], g["phi"], beta=float(g["beta"]), mass=float(g["mass"]),
         n_flavours=int(g["n_flavours"]),
@@ -99,3 +99,41 @@ def test_trajectory(g):
     assert rel(P, g["traj_mom"]) < 1e-11
     assert rel(phin, g["traj_phinew"]) < 1e-11
     assert abs(dh - float(g["traj_dh"])) < 1e-9
+
+
+def test_c1_configuration():
+    """8^3x4, am=0.05, beta=5.5, one pseudofermion, 8 MD / 12 action poles,
+    10 Omelyan steps (the reference's update() dispatches only Omelyan, 2MN and
+    4MN, update.cc:29-43).  The fixture holds outputs only: the inputs are
+    regenerated from their seeds and pinned by a checksum."""
+    from rhmc_amd import fields
+    sys_path_make = os.path.join(GOLD, "make_golden.py")
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("make_golden_c1", sys_path_make)
+    mg = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mg)
+    g = load("c1")
+    dims = tuple(g["dims"])
+    lat = fields.Lattice(*dims)
+    U = fields.random_links(lat, mg.SEEDS["links"], None)
+    P = fields.gauss_momenta(lat, mg.SEEDS["mom"])
+    phi = fields.gauss_spinor(lat, mg.SEEDS["phi"], 1)
+    chk = np.array([np.abs(U).sum(), np.abs(P).sum(), np.abs(phi).sum()])
+    np.testing.assert_allclose(chk, g["input_checksum"], rtol=1e-13)
+    o = Oracle(dims)
+    assert rel(o.doe(U, g["dslash_in"][0]), g["doe"]) < 1e-14
+    assert rel(o.deo(U, g["dslash_in"][1]), g["deo"]) < 1e-14
+    dh, U1, P1, chi, phin = o.trajectory(
+        U, P, phi, beta=float(g["beta"]), mass=float(g["mass"]), n_flavours=int(g["n_flavours"]),
+        first=(float(g["first_a0"]), g["first_a"], g["first_b"]),
+        md=(float(g["md_a0"]), g["md_a"], g["md_b"]),
+        last=(float(g["last_a0"]), g["last_a"], g["last_b"]),
+        eig_min=float(g["eig_min"]), eig_max=float(g["eig_max"]), no_md=int(g["no_md"]),
+        integrator="omelyan", gauge_scale=int(g["gauge_scale"]), res_md=float(g["res_md"]),
+        res_metro=float(g["res_metro"]))
+    sel = g["traj_sel"]
+    assert rel(chi, g["traj_chi"]) < 1e-10
+    assert rel(U1[sel], g["traj_links_sel"]) < 1e-10
+    assert rel(P1[sel], g["traj_mom_sel"]) < 1e-10
+    assert rel(phin, g["traj_phinew"]) < 1e-9
+    assert abs(dh - float(g["traj_dh"])) < 1e-8
