@@ -237,7 +237,73 @@ def wl_traj48c12(args):
             "mixed_vs_fp64_speedup": fp64 / mixed_same, "spectral_bounds": [mn, mx]}
 
 
-WORKLOADS = {"traj32c8": None, "ms16": wl_ms16, "force24": wl_force24, "traj48c12": wl_traj48c12}
+def wl_multi(args):
+    """Six independent Markov chains on lattices of uneven size (8^4, 12^4 x2,
+    16^4 x2, 8^3x16): Nf=2, am=0.01, beta=5.6, one pseudofermion, 12/8/12
+    poles, Omelyan 10 steps, mixed precision, exp(i 0.3 H) starts.  Each
+    lattice's trajectory is timed alone, then all six run concurrently
+    (gpu_multi_trajectory: one host thread and stream per lattice, solver
+    iteration chunks replayed as HIP graphs).  Dslash bytes are the
+    algorithmic half-Dslash bytes of every CG iteration (two per iteration,
+    FP32 12-real for the FP32 solves, FP64 18-real for the rest)."""
+    from rhmc_amd import multi_trajectory
+    lats = [(8, 8, 8, 8), (12, 12, 12, 12), (12, 12, 12, 12), (16, 16, 16, 16), (16, 16, 16, 16), (8, 8, 8, 16)]
+    mass, beta = 0.01, 5.6
+    first, md, last = rhmc_approximations(2, 1, 12, 8, 12)
+    co = dict(first=(first.a0, first.a, first.b), md=(md.a0, md.a, md.b), last=(last.a0, last.a, last.b))
+    gpus, starts, kws = [], [], []
+    for i, d in enumerate(lats):
+        lat = fields.Lattice(*d)
+        U = fields.random_links(lat, 900 + i, 0.3)
+        g = Gpu(d, mass=mass, beta=beta, n_pf=1, max_order=12)
+        g.init0(U)
+        mn, mx = _bounds(g, mass)
+        gpus.append(g)
+        starts.append(U)
+        kws.append(dict(n_flavours=2, no_md=10, integrator="omelyan", res_md=1e-6, res_metro=1e-10, relative=True,
+                        bounds=(mn, mx), draw_fields=True, precision=1, fp32_threshold=1e-7, **co))
+
+    def dbytes(d, it32, it_all):
+        vh = int(np.prod(d)) // 2
+        return 2 * (it32 * dslash_bytes(vh, 1, 48, 24) + (it_all - it32) * dslash_bytes(vh, 1, 144, 48))
+
+    # warm-up (autotuner, graph capture), then each lattice alone
+    for g, U, kw in zip(gpus, starts, kws):
+        g.init0(U)
+        g.trajectory(seed=1, **kw)
+    alone = []
+    for g, U, kw, d in zip(gpus, starts, kws, lats):
+        g.init0(U)
+        t0 = time.perf_counter()
+        r = g.trajectory(seed=2, **kw)
+        alone.append({"lattice": list(d), "s": time.perf_counter() - t0, "iters": r.cg_iterations,
+                      "delta_h": r.delta_h})
+    seq_total = sum(a["s"] for a in alone)
+    walls, per = [], None
+    for rep in range(max(1, args.steps)):
+        for g, U in zip(gpus, starts):
+            g.init0(U)
+        t0 = time.perf_counter()
+        res, st = multi_trajectory(gpus, [dict(seed=2, **kw) for kw in kws])
+        walls.append(time.perf_counter() - t0)
+        per = res
+    wall = float(np.median(walls))
+    total_bytes = sum(dbytes(d, r.cg_iterations_fp32, r.cg_iterations) for d, r in zip(lats, per))
+    for g in gpus:
+        g.close()
+    return {"metric": "six uneven lattices: per-lattice trajectory seconds, concurrent wall seconds, "
+                      "aggregate Dslash GB/s", "unit": "s/6 traj", "value": wall,
+            "config": {"workload": "8^4, 12^4 x2, 16^4 x2, 8^3x16; Nf=2, am=0.01, 1 pseudofermion, 12/8/12 poles, "
+                                   "Omelyan 10, mixed"},
+            "alone": alone, "sequential_sum_s": seq_total, "concurrent_wall_s": wall,
+            "concurrent_speedup_vs_sequential": seq_total / wall,
+            "concurrent_per_lattice_s": [r.seconds_total for r in per],
+            "delta_h_concurrent_equals_alone": [abs(r.delta_h - a["delta_h"]) for r, a in zip(per, alone)],
+            "aggregate_dslash_GBps": total_bytes / wall / 1e9}
+
+
+WORKLOADS = {"traj32c8": None, "ms16": wl_ms16, "force24": wl_force24, "traj48c12": wl_traj48c12,
+             "multi": wl_multi}
 
 
 def main():

@@ -191,6 +191,7 @@ typedef struct {
   double delta_h_fp64;            /* FP64 recomputation (verify_fp64=1)       */
   int accepted;
   int cg_iterations;              /* summed over all solves                   */
+  int cg_iterations_fp32;         /* of which in FP32 solves                  */
   int n_solves;
   double eig_min, eig_max;
   double seconds_total, seconds_solver, seconds_force, seconds_update;  /* without the verification */
@@ -198,6 +199,14 @@ typedef struct {
 } rhmc_traj_result;
 
 int gpu_trajectory(rhmc_gpu_t h, const rhmc_traj_params *p, rhmc_traj_result *r);
+
+/* Several independent lattices (distinct handles, any sizes), one trajectory
+ * each, run concurrently on the device: independent Markov chains as in the
+ * reference's update() per lattice.  status (may be NULL) receives each
+ * lattice's return code; the return value is the first failure or RHMC_OK.
+ * No counterpart in the reference, which runs one lattice per process. */
+int gpu_multi_trajectory(rhmc_gpu_t *handles, int n, const rhmc_traj_params *params, rhmc_traj_result *results,
+                         int *status);
 
 #ifdef __cplusplus
 }

@@ -1,6 +1,9 @@
 // capi.hip -- the extern "C" boundary (include/gpu.h, include/cu_inverter.h).
 #include <hip/hip_runtime.h>
 
+#include <cstdlib>
+#include <thread>
+
 #include <cmath>
 #include <algorithm>
 #include <cstring>
@@ -30,10 +33,16 @@ bool hip_ok(Context &c, hipError_t e, const char *what) {
   return false;
 }
 
+// Zero-filled allocation, ordered on the context's stream.  The stream is
+// non-blocking, so work on the legacy null stream (a plain hipMemset) would
+// not be ordered before later launches on it.
 void *dev_alloc(Context &c, size_t bytes) {
   void *p = nullptr;
   if (!hip_ok(c, hipMalloc(&p, bytes), "hipMalloc")) return nullptr;
-  if (!hip_ok(c, hipMemset(p, 0, bytes), "hipMemset")) return nullptr;
+  if (!hip_ok(c, hipMemsetAsync(p, 0, bytes, c.stream), "hipMemsetAsync")) {
+    hipFree(p);
+    return nullptr;
+  }
   return p;
 }
 
@@ -48,6 +57,9 @@ void fill_dims(Context &c, int nx, int ny, int nz, int nt, int antiperiodic) {
   c.Vh = c.V / 2;
   c.L.Vh = c.Vh;
   c.L.antiperiodic_t = antiperiodic;
+  c.L.inv_nxh = 1.0 / c.L.nxh;
+  c.L.inv_ny = 1.0 / ny;
+  c.L.inv_nz = 1.0 / nz;
 }
 
 __global__ void k_f2d(double2 *o, const float2 *i, long n) {
@@ -78,6 +90,8 @@ int gpu_create(const rhmc_gpu_params *p, rhmc_gpu_t *out) {
   if (!p || !out) return RHMC_EINVAL;
   *out = nullptr;
   if (p->nx <= 0 || p->ny <= 0 || p->nz <= 0 || p->nt <= 0 || (p->nx | p->ny | p->nz | p->nt) & 1) return RHMC_EINVAL;
+  // 32-bit site arithmetic in the kernels: V/2 < 2^31
+  if ((double)p->nx * p->ny * p->nz * p->nt / 2 >= 2147483647.0) return RHMC_EINVAL;
   if (p->n_pf < 1 || p->max_order < 1 || p->max_order > kMaxShifts || p->n_pf * p->max_order > 256)
     return RHMC_EINVAL;
   rhmc_gpu_t h = new (std::nothrow) rhmc_gpu_ctx();
@@ -111,6 +125,7 @@ int gpu_create(const rhmc_gpu_params *p, rhmc_gpu_t *out) {
   }
   c.ws = workspace_create(c);
   if (!c.ws || build_tile_order(c)) { gpu_destroy(h); return RHMC_EHIP; }
+  c.use_graphs = getenv("RHMC_NO_GRAPHS") == nullptr;
   *out = h;
   return RHMC_OK;
 }
@@ -123,6 +138,7 @@ int gpu_destroy(rhmc_gpu_t h) {
   dev_free(c.U64); dev_free(c.U64_save); dev_free(c.U32); dev_free(c.P); dev_free(c.ipdot);
   dev_free(c.mf); dev_free(c.smf); dev_free(c.phi0); dev_free(c.chi); dev_free(c.phinew); dev_free(c.action);
   dev_free(c.tiles);
+  dev_free(c.smf32);
   workspace_destroy(c.ws);
   if (c.stream) hipStreamDestroy(c.stream);
   delete h;
@@ -465,6 +481,30 @@ int gpu_trajectory(rhmc_gpu_t h, const rhmc_traj_params *p, rhmc_traj_result *r)
   if (!h || !p || !r) return RHMC_EINVAL;
   hipSetDevice(h->c.device);
   return run_trajectory(h->c, *p, *r);
+}
+
+int gpu_multi_trajectory(rhmc_gpu_t *hs, int n, const rhmc_traj_params *ps, rhmc_traj_result *rs, int *status) {
+  if (!hs || !ps || !rs || n < 1) return RHMC_EINVAL;
+  for (int i = 0; i < n; i++)
+    for (int k = 0; k < i; k++)
+      if (!hs[i] || hs[i] == hs[k]) return RHMC_EINVAL;
+  // one host thread per lattice, each sequencing its own stream; the device
+  // runs the lattices' kernels (and replayed solver graphs) concurrently
+  std::vector<int> rc(n, RHMC_OK);
+  std::vector<std::thread> th;
+  th.reserve(n);
+  for (int i = 0; i < n; i++)
+    th.emplace_back([&, i]() {
+      hipSetDevice(hs[i]->c.device);
+      rc[i] = run_trajectory(hs[i]->c, ps[i], rs[i]);
+    });
+  for (auto &t : th) t.join();
+  int worst = RHMC_OK;
+  for (int i = 0; i < n; i++) {
+    if (status) status[i] = rc[i];
+    if (rc[i] != RHMC_OK && worst == RHMC_OK) worst = rc[i];
+  }
+  return worst;
 }
 
 }  // extern "C"

@@ -55,6 +55,7 @@ struct Context {
   double2 *chi = nullptr;    // chi = r_first(MdagM) phi0
   double2 *phinew = nullptr; // r_last(MdagM) chi at the end of the MD
   double *action = nullptr;  // [V] per-site action differences
+  float2 *smf32 = nullptr;   // FP32 MD solutions [n_pf][max_order][3][Vh] (allocated on first use)
   int smf_order = 0;         // poles held in smf
 
   // Dslash traversal order: kBlock-site tiles listed so that each eighth of
@@ -67,6 +68,8 @@ struct Context {
   // multi-shift algorithm: 0 stored residual basis (falls back to 1 when the
   // basis does not fit), 1 per-shift search directions (inverter.cc:142-339)
   int ms_algorithm = 0;
+  // replay the stored-basis solver's iteration chunks as captured HIP graphs
+  int use_graphs = 1;
 
   Workspace *ws = nullptr;
   std::string last_error;
@@ -93,8 +96,12 @@ int residual_d(Context &c, const double2 *b, int b_group, const double2 *x, cons
 // (FP32 iterations only; the target is floored at the FP32 accuracy).
 // rel=0: res is an absolute residual (reference convention);
 // rel=1: res is relative to |b| per right-hand side.
+// x32 (PREC_FP32 only): FP32 solutions written there instead of x.
 int multishift_solve(Context &c, const double2 *b, int nrhs, int nshift, const double *shifts, double res,
-                     int rel, int prec, int max_iter, double2 *x, int *iters);
+                     int rel, int prec, int max_iter, double2 *x, int *iters, float2 *x32 = nullptr);
+// out[j] = 0.5 * hop(in[j]) to parity par_out, FP32 12-real links
+void launch_hop_s(Context &c, int par_out, const float2 *in, float2 *out, int nvec, long vstride_in,
+                  long vstride_out);
 
 // spectral bounds of MdagM by power iterations (findminmax.cc:12-137)
 int power_max(Context &c, const double2 *start, double tol, double *out, int *loops);
@@ -102,6 +109,9 @@ int power_min(Context &c, const double2 *start, double tol, double mx, double *o
 
 // fields.hip
 int fermion_force(Context &c, int nsys_per_pf, const double *num, const double2 *x, int accumulate);
+// the same from FP32 solutions: FP32 hop on 12-real links, FP32 loads, FP64
+// accumulation and projection
+int fermion_force32(Context &c, int nsys, const double *num, const float2 *x32, int accumulate);
 int gauge_force(Context &c, int accumulate);
 int exp_update(Context &c, double step);
 int momenta_update(Context &c, double step);

@@ -33,9 +33,12 @@ __device__ __forceinline__ void shift_coord(const LatticeDims &L, int &x, int &y
 //   F_mu(o) = -sum_j a_j s_j(o+mu) (x) h_j(o)^*     (odd sites)
 //   ipdot_mu(x) = TA(U_mu(x) F_mu(x))   [+= existing ipdot if accumulate]
 // s_j: shifted solutions (even), h_j = D_oe s_j (odd); one thread per site
-// holds the four F_mu in registers and reads each spinor of its site once.
-__global__ __launch_bounds__(kBlock) void k_fermion_force(LatticeDims L, const double2 *U, const double2 *s,
-                                                          const double2 *hv, const double *coef, int nsys,
+// holds the four F_mu in registers (FP64) and reads each spinor of its site
+// once.  C = double2 (FP64 solutions) or float2 (FP32 solutions of the FP32
+// MD solves).
+template <typename C>
+__global__ __launch_bounds__(kBlock) void k_fermion_force(LatticeDims L, const double2 *U, const C *s,
+                                                          const C *hv, const double *coef, int nsys,
                                                           double2 *ipdot, int accumulate) {
   const long Vh = L.Vh;
   const int bx = xcd_block(blockIdx.x, gridDim.x);
@@ -53,8 +56,8 @@ __global__ __launch_bounds__(kBlock) void k_fermion_force(LatticeDims L, const d
     for (int i = 0; i < 3; i++)
 #pragma unroll
       for (int k = 0; k < 3; k++) F[mu].e[i][k] = make_double2(0.0, 0.0);
-  const double2 *own_v = par == 0 ? s : hv;
-  const double2 *nb_v = par == 0 ? hv : s;
+  const C *own_v = par == 0 ? s : hv;
+  const C *nb_v = par == 0 ? hv : s;
   const double sgn = par == 0 ? 1.0 : -1.0;
   for (int j = 0; j < nsys; j++) {
     const double a = sgn * coef[j];
@@ -62,16 +65,15 @@ __global__ __launch_bounds__(kBlock) void k_fermion_force(LatticeDims L, const d
     double2 own[3];
 #pragma unroll
     for (int k = 0; k < 3; k++) {
-      own[k] = own_v[base + (long)k * Vh + h];
-      own[k].y = -own[k].y;  // conjugate
+      const C o = own_v[base + (long)k * Vh + h];
+      own[k] = make_double2((double)o.x, -(double)o.y);  // conjugate
     }
 #pragma unroll
     for (int mu = 0; mu < 4; mu++) {
 #pragma unroll
       for (int i = 0; i < 3; i++) {
-        double2 v = nb_v[base + (long)i * Vh + nb[mu]];
-        v.x *= a;
-        v.y *= a;
+        const C w = nb_v[base + (long)i * Vh + nb[mu]];
+        double2 v = make_double2(a * (double)w.x, a * (double)w.y);
 #pragma unroll
         for (int k = 0; k < 3; k++) cfma(F[mu].e[i][k], v, own[k]);
       }
@@ -105,8 +107,26 @@ int fermion_force(Context &c, int nsys, const double *num, const double2 *x, int
   hcoef.assign(num, num + nsys);
   hipMemcpyAsync(coef, hcoef.data(), sizeof(double) * nsys, hipMemcpyHostToDevice, c.stream);
   launch_hop_d(c, 1, x, hv, nsys, 3 * Vh, 3 * Vh);
-  hipLaunchKernelGGL(k_fermion_force, dim3(nblk(Vh), 2), dim3(kBlock), 0, c.stream, c.L, c.U64, x, hv, coef, nsys,
-                     c.ipdot, accumulate);
+  hipLaunchKernelGGL(k_fermion_force<double2>, dim3(nblk(Vh), 2), dim3(kBlock), 0, c.stream, c.L, c.U64, x, hv, coef,
+                     nsys, c.ipdot, accumulate);
+  hipFreeAsync(hv, c.stream);
+  hipFreeAsync(coef, c.stream);
+  return hip_ok(c, hipStreamSynchronize(c.stream), "fermion force") ? 0 : -1;
+}
+
+int fermion_force32(Context &c, int nsys, const double *num, const float2 *x32, int accumulate) {
+  static thread_local std::vector<double> hcoef;
+  const long Vh = c.Vh;
+  float2 *hv = nullptr;
+  double *coef = nullptr;
+  if (!hip_ok(c, hipMallocAsync((void **)&hv, sizeof(float2) * 3 * Vh * (size_t)nsys, c.stream), "force buffer"))
+    return -1;
+  if (!hip_ok(c, hipMallocAsync((void **)&coef, sizeof(double) * nsys, c.stream), "force coef")) return -1;
+  hcoef.assign(num, num + nsys);
+  hipMemcpyAsync(coef, hcoef.data(), sizeof(double) * nsys, hipMemcpyHostToDevice, c.stream);
+  launch_hop_s(c, 1, x32, hv, nsys, 3 * Vh, 3 * Vh);
+  hipLaunchKernelGGL(k_fermion_force<float2>, dim3(nblk(Vh), 2), dim3(kBlock), 0, c.stream, c.L, c.U64, x32, hv, coef,
+                     nsys, c.ipdot, accumulate);
   hipFreeAsync(hv, c.stream);
   hipFreeAsync(coef, c.stream);
   return hip_ok(c, hipStreamSynchronize(c.stream), "fermion force") ? 0 : -1;

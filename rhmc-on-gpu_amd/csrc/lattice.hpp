@@ -29,7 +29,20 @@ struct LatticeDims {
   int nxh;   // nx/2
   long Vh;   // half volume
   int antiperiodic_t;
+  double inv_nxh, inv_ny, inv_nz;  // reciprocals for the 32-bit coordinate split
 };
+
+// q = n / d, rem = n % d for 0 <= n < 2^31: the FP64 product with the
+// reciprocal is within one of the quotient, one exact correction fixes it
+// (an integer division by a run-time divisor is a long instruction sequence).
+__device__ __forceinline__ int fdivmod(int n, int d, double inv, int &rem) {
+  int q = (int)((double)n * inv);
+  int r = n - q * d;
+  if (r < 0) { q -= 1; r += d; }
+  else if (r >= d) { q += 1; r -= d; }
+  rem = r;
+  return q;
+}
 
 struct SiteCoord {
   int x, y, z, t;
@@ -49,6 +62,32 @@ __device__ __forceinline__ SiteCoord site_coords(const LatticeDims &L, int par, 
 
 __device__ __forceinline__ long half_index(const LatticeDims &L, int x, int y, int z, int t) {
   return (long)(x >> 1) + (long)L.nxh * (y + (long)L.ny * (z + (long)L.nz * t));
+}
+
+// 32-bit forms (Vh < 2^31, checked at context creation)
+__device__ __forceinline__ SiteCoord site_coords_i(const LatticeDims &L, int par, int h) {
+  SiteCoord c;
+  int xh, y, z;
+  const int r = fdivmod(h, L.nxh, L.inv_nxh, xh);
+  const int r2 = fdivmod(r, L.ny, L.inv_ny, y);
+  c.t = fdivmod(r2, L.nz, L.inv_nz, z);
+  c.y = y;
+  c.z = z;
+  c.x = 2 * xh + ((par + y + z + c.t) & 1);
+  return c;
+}
+
+__device__ __forceinline__ int half_index_i(const LatticeDims &L, int x, int y, int z, int t) {
+  return (x >> 1) + L.nxh * (y + L.ny * (z + L.nz * t));
+}
+
+__device__ __forceinline__ int neighbour_i(const LatticeDims &L, SiteCoord c, int mu, int dir) {
+  int x = c.x, y = c.y, z = c.z, t = c.t;
+  if (mu == 0) x = dir > 0 ? (x + 1 == L.nx ? 0 : x + 1) : (x == 0 ? L.nx - 1 : x - 1);
+  else if (mu == 1) y = dir > 0 ? (y + 1 == L.ny ? 0 : y + 1) : (y == 0 ? L.ny - 1 : y - 1);
+  else if (mu == 2) z = dir > 0 ? (z + 1 == L.nz ? 0 : z + 1) : (z == 0 ? L.nz - 1 : z - 1);
+  else t = dir > 0 ? (t + 1 == L.nt ? 0 : t + 1) : (t == 0 ? L.nt - 1 : t - 1);
+  return half_index_i(L, x, y, z, t);
 }
 
 // half index of the neighbour x +/- mu (which has the opposite parity)
@@ -99,13 +138,24 @@ template <typename C> __device__ __forceinline__ C cmulc(C a, C b) {
 }
 template <typename C> __device__ __forceinline__ C cconj(C a) { a.y = -a.y; return a; }
 template <typename C, typename R> __device__ __forceinline__ C cscale(C a, R s) { a.x *= s; a.y *= s; return a; }
+__device__ __forceinline__ float fmar(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fmar(double a, double b, double c) { return __builtin_fma(a, b, c); }
+// Complex multiply-accumulates as chains of four fused multiply-adds
+// (operand negation is free on the FMA).
 // acc += a*b
 template <typename C> __device__ __forceinline__ void cfma(C &acc, C a, C b) {
-  acc.x += a.x * b.x - a.y * b.y; acc.y += a.x * b.y + a.y * b.x;
+  acc.x = fmar(a.x, b.x, acc.x); acc.x = fmar(-a.y, b.y, acc.x);
+  acc.y = fmar(a.x, b.y, acc.y); acc.y = fmar(a.y, b.x, acc.y);
 }
 // acc += conj(a)*b
 template <typename C> __device__ __forceinline__ void cfmac(C &acc, C a, C b) {
-  acc.x += a.x * b.x + a.y * b.y; acc.y += a.x * b.y - a.y * b.x;
+  acc.x = fmar(a.x, b.x, acc.x); acc.x = fmar(a.y, b.y, acc.x);
+  acc.y = fmar(a.x, b.y, acc.y); acc.y = fmar(-a.y, b.x, acc.y);
+}
+// acc -= conj(a)*b
+template <typename C> __device__ __forceinline__ void cfmsc(C &acc, C a, C b) {
+  acc.x = fmar(-a.x, b.x, acc.x); acc.x = fmar(-a.y, b.y, acc.x);
+  acc.y = fmar(-a.x, b.y, acc.y); acc.y = fmar(a.y, b.x, acc.y);
 }
 
 template <typename R> struct Mat { Cx<R> e[3][3]; };

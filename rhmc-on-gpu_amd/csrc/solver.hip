@@ -43,8 +43,16 @@ enum Slot {
   S_COUNT
 };
 
+// an instantiated graph of one solver chunk and the launch arguments it was
+// captured with
+struct GraphSlot {
+  hipGraphExec_t exec = nullptr;
+  unsigned char key[2048];
+};
+
 struct Workspace {
   std::map<unsigned long long, int> tuned_nr;  // Dslash RHS-per-thread choice
+  GraphSlot bms_graph[2];                      // FP32 / FP64 stored-basis chunk
   void *buf[S_COUNT] = {};
   size_t cap[S_COUNT] = {};
   unsigned *counter = nullptr;
@@ -55,7 +63,7 @@ struct Workspace {
 Workspace *workspace_create(Context &c) {
   Workspace *w = new Workspace();
   if (!hip_ok(c, hipMalloc(&w->counter, 64 * sizeof(unsigned)), "hipMalloc counter")) return nullptr;
-  hipMemset(w->counter, 0, 64 * sizeof(unsigned));
+  if (!hip_ok(c, hipMemsetAsync(w->counter, 0, 64 * sizeof(unsigned), c.stream), "counter")) return nullptr;
   if (!hip_ok(c, hipHostMalloc(&w->h_flags, 4096 * sizeof(int), hipHostMallocDefault), "hipHostMalloc")) return nullptr;
   if (!hip_ok(c, hipHostMalloc(&w->h_sums, 4096 * sizeof(double), hipHostMallocDefault), "hipHostMalloc")) return nullptr;
   // reduction partials for the largest launch (kMaxRed vectors), allocated once
@@ -67,6 +75,8 @@ Workspace *workspace_create(Context &c) {
 
 void workspace_destroy(Workspace *w) {
   if (!w) return;
+  for (auto &g : w->bms_graph)
+    if (g.exec) hipGraphExecDestroy(g.exec);
   for (int i = 0; i < S_COUNT; i++)
     if (w->buf[i]) hipFree(w->buf[i]);
   if (w->counter) hipFree(w->counter);
@@ -352,28 +362,31 @@ __global__ __launch_bounds__(kBlock) void k_dslash(DsArgs a) {
 #pragma unroll
   for (int r = 0; r < NR; r++) red[r] = 0.0;
 
+  const int Vh32 = (int)Vh;
+  const C *inr[NR];
+#pragma unroll
+  for (int r = 0; r < NR; r++) inr[r] = (const C *)a.in + (long)(jbase + r) * a.in_vs;
   for (int q = q0; any && q < q1; q++) {
-    const long h = (long)(a.tiles ? a.tiles[q] : q) * kBlock + threadIdx.x;
-    if (h >= Vh) continue;
-    const SiteCoord c = site_coords(L, a.par_out, h);
+    const int h = (a.tiles ? a.tiles[q] : q) * kBlock + (int)threadIdx.x;
+    if (h >= Vh32) continue;
+    const SiteCoord c = site_coords_i(L, a.par_out, h);
     const int pin_ = a.par_out, pnb = 1 - a.par_out;
     C acc[NR][3];
 #pragma unroll
     for (int r = 0; r < NR; r++)
 #pragma unroll
       for (int k = 0; k < 3; k++) acc[r][k] = mk<R>(0, 0);
-    const C *in = (const C *)a.in;
 #pragma unroll
     for (int mu = 0; mu < 4; mu++) {
       // forward hop: U_mu(x) psi(x+mu)
       {
-        const long hp = neighbour(L, c, mu, +1);
+        const int hp = neighbour_i(L, c, mu, +1);
         const R eta = (R)eta_phase(L, c.x, c.y, c.z, c.t, mu);
         const Mat<R> U = ds_link<R, LINK>(a, mu, pin_, h, eta);
 #pragma unroll
         for (int r = 0; r < NR; r++) {
           if (!act[r]) continue;
-          const C *v = in + (long)(jbase + r) * a.in_vs + hp;
+          const C *v = inr[r] + hp;
           const C v0 = v[0], v1 = v[Vh], v2 = v[2 * Vh];
 #pragma unroll
           for (int i = 0; i < 3; i++) {
@@ -385,21 +398,20 @@ __global__ __launch_bounds__(kBlock) void k_dslash(DsArgs a) {
       }
       // backward hop: -U_mu(x-mu)^dag psi(x-mu)
       {
-        const long hm = neighbour(L, c, mu, -1);
+        const int hm = neighbour_i(L, c, mu, -1);
         const int tm = (mu == 3) ? (c.t == 0 ? L.nt - 1 : c.t - 1) : c.t;
         const R eta = (R)eta_phase(L, c.x, c.y, c.z, tm, mu);
         const Mat<R> U = ds_link<R, LINK>(a, mu, pnb, hm, eta);
 #pragma unroll
         for (int r = 0; r < NR; r++) {
           if (!act[r]) continue;
-          const C *v = in + (long)(jbase + r) * a.in_vs + hm;
+          const C *v = inr[r] + hm;
           const C v0 = v[0], v1 = v[Vh], v2 = v[2 * Vh];
 #pragma unroll
           for (int i = 0; i < 3; i++) {
-            C t = cmulc(U.e[0][i], v0);
-            cfmac(t, U.e[1][i], v1);
-            cfmac(t, U.e[2][i], v2);
-            acc[r][i] = csub(acc[r][i], t);
+            cfmsc(acc[r][i], U.e[0][i], v0);
+            cfmsc(acc[r][i], U.e[1][i], v1);
+            cfmsc(acc[r][i], U.e[2][i], v2);
           }
         }
       }
@@ -419,7 +431,8 @@ __global__ __launch_bounds__(kBlock) void k_dslash(DsArgs a) {
 #pragma unroll
         for (int i = 0; i < 3; i++) {
           C pv = p[(long)i * Vh];
-          res[i] = csub(cscale(pv, d), cscale(acc[r][i], (R)0.5));
+          res[i].x = fmar(d, pv.x, (R)-0.5 * acc[r][i].x);
+          res[i].y = fmar(d, pv.y, (R)-0.5 * acc[r][i].y);
           if (MODE == DS_RESID) {
             const double2 bv = a.b[(long)(j / a.b_group) * a.b_vs + (long)i * Vh + h];
             res[i] = csub(mk<R>((R)bv.x, (R)bv.y), res[i]);
@@ -504,8 +517,8 @@ int build_tile_order(Context &c) {
   }
   c.tiles = (int *)dev_alloc(c, sizeof(int) * T);
   if (!c.tiles) return -1;
-  if (!hip_ok(c, hipMemcpy(c.tiles, order.data(), sizeof(int) * T, hipMemcpyHostToDevice), "tile order")) return -1;
-  return 0;
+  hipMemcpyAsync(c.tiles, order.data(), sizeof(int) * T, hipMemcpyHostToDevice, c.stream);
+  return hip_ok(c, hipStreamSynchronize(c.stream), "tile order") ? 0 : -1;
 }
 
 // number of right-hand sides handled per thread for a launch with nvec vectors
@@ -626,6 +639,16 @@ void launch_hop_d(Context &c, int par_out, const double2 *in, double2 *out, int 
   a.out = out; a.out_vs = vs_out;
   a.nvec = nvec;
   dslash_dispatch<double, LINK_D18, DS_HOP, RED_NONE>(c, a);
+}
+
+void launch_hop_s(Context &c, int par_out, const float2 *in, float2 *out, int nvec, long vs_in, long vs_out) {
+  DsArgs a = {};
+  a.par_out = par_out;
+  a.U = c.U32;
+  a.in = in; a.in_vs = vs_in;
+  a.out = out; a.out_vs = vs_out;
+  a.nvec = nvec;
+  dslash_dispatch<float, LINK_S12, DS_HOP, RED_NONE>(c, a);
 }
 
 // --------------------------------------------------------------- BLAS kernels
@@ -873,7 +896,8 @@ struct BmsHist {
 
 struct BmsArgs {
   long Vh;
-  int nvec, nshift, iter, cap;
+  int nvec, nshift, iter_off, cap;
+  const int *iterc;     // device iteration base: iteration = *iterc + iter_off
   const double2 *b;
   void *basis;          // [rhs][cap][3][Vh]
   void *p, *s;          // [rhs][3][Vh]
@@ -884,6 +908,15 @@ struct BmsArgs {
   BmsHist *hist;        // [rhs][cap][nshift]
   int *active;          // [2]
 };
+
+constexpr int kChunk = 8;  // solver iterations between host polls
+
+struct BmsGraphKey {
+  BmsArgs a;
+  DsArgs hop, mdm;
+  unsigned nbr;
+};
+static_assert(sizeof(BmsGraphKey) <= sizeof(GraphSlot::key), "graph key");
 
 // fixed-order workgroup sum of n partials
 __device__ __forceinline__ double block_sum_partials(const double *p, int n) {
@@ -956,17 +989,18 @@ __global__ __launch_bounds__(kBlock) void k_bms_c(BmsArgs a) {
   using C = Cx<R>;
   const unsigned bx = xcd_block(blockIdx.x, gridDim.x);
   const int j = blockIdx.y;
-  const int sl = a.iter & 1;
+  const int iter = *a.iterc + a.iter_off;
+  const int sl = iter & 1;
   const long Vh = a.Vh;
-  if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) a.active[(a.iter + 1) & 1] = 0;
+  if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) a.active[(iter + 1) & 1] = 0;
   double v[1] = {0.0};
   if (!a.sh[j].stop[sl]) {
     const double alpha = block_sum_partials(a.partB + (long)j * a.nbrB, a.nbrB);
     const R om = (R)(-a.sh[j].delta[sl] / alpha);
     long h0, h1;
     site_range(Vh, bx, gridDim.x, h0, h1);
-    const C *ro = (const C *)a.basis + ((long)j * a.cap + a.iter - 1) * 3 * Vh;
-    C *rn = (C *)a.basis + ((long)j * a.cap + a.iter) * 3 * Vh;
+    const C *ro = (const C *)a.basis + ((long)j * a.cap + iter - 1) * 3 * Vh;
+    C *rn = (C *)a.basis + ((long)j * a.cap + iter) * 3 * Vh;
     const C *sv = (const C *)a.s + (long)j * 3 * Vh;
     for (long h = h0 + threadIdx.x; h < h1; h += kBlock)
 #pragma unroll
@@ -988,7 +1022,8 @@ __global__ __launch_bounds__(kBlock) void k_bms_d(BmsArgs a) {
   using C = Cx<R>;
   const unsigned bx = xcd_block(blockIdx.x, gridDim.x);
   const int j = blockIdx.y;
-  const int sl = a.iter & 1, sn = (a.iter + 1) & 1;
+  const int iter = *a.iterc + a.iter_off;
+  const int sl = iter & 1, sn = (iter + 1) & 1;
   const long Vh = a.Vh;
   BmsShared &sh = a.sh[j];
   if (sh.stop[sl]) {
@@ -1005,7 +1040,7 @@ __global__ __launch_bounds__(kBlock) void k_bms_d(BmsArgs a) {
     const R g = (R)gam;
     long h0, h1;
     site_range(Vh, bx, gridDim.x, h0, h1);
-    const C *r = (const C *)a.basis + ((long)j * a.cap + a.iter) * 3 * Vh;
+    const C *r = (const C *)a.basis + ((long)j * a.cap + iter) * 3 * Vh;
     C *p = (C *)a.p + (long)j * 3 * Vh;
     for (long h = h0 + threadIdx.x; h < h1; h += kBlock)
 #pragma unroll
@@ -1029,13 +1064,13 @@ __global__ __launch_bounds__(kBlock) void k_bms_d(BmsArgs a) {
     const double z3 = (z1 * z2 * omp) / (om * gold * (z1 - z2) + z1 * omp * (1.0 - s.shift[t] * om));
     const double oms = om * z3 / z2;
     const double gas = gam * z3 * oms / (z2 * om);
-    BmsHist &hh = a.hist[((long)j * a.cap + a.iter) * a.nshift + t];
+    BmsHist &hh = a.hist[((long)j * a.cap + iter) * a.nshift + t];
     hh.om = oms;
     hh.z = z3;
     hh.ga = gas;
     if (sqrt(delta * z2 * z2) < s.res) {
       s.flag[t] = 0;
-      s.its[t] = a.iter;
+      s.its[t] = iter;
     }
     s.z1[t] = z2;
     s.z2[t] = z3;
@@ -1046,9 +1081,9 @@ __global__ __launch_bounds__(kBlock) void k_bms_d(BmsArgs a) {
     s.omega = om;
     s.gammag = gam;
     s.delta = lambda;
-    s.iter = a.iter;
+    s.iter = iter;
     s.nact = nact;
-    const int stop = !(sqrt(lambda) > s.res && a.iter < s.max_iter) || nact == 0 || a.iter + 1 >= a.cap;
+    const int stop = !(sqrt(lambda) > s.res && iter < s.max_iter) || nact == 0 || iter + 1 >= a.cap;
     s.stop = s.done = stop;
     sh.delta[sn] = lambda;
     sh.stop[sn] = stop;
@@ -1066,9 +1101,9 @@ __global__ __launch_bounds__(kBlock) void k_bms_d(BmsArgs a) {
 // branches.
 constexpr int kTileK = 32;
 
-template <typename R, int NS>
+template <typename R, int NS, typename XO>
 __global__ __launch_bounds__(kBlock) void k_bms_assemble(const void *basis, const double *coef, const int *klen,
-                                                         double2 *x, long Vh, int cap, int nshift, int s0) {
+                                                         XO *x, long Vh, int cap, int nshift, int s0) {
   using C = Cx<R>;
   __shared__ double2 cs[kTileK][NS / 2];
   const long n3 = 3 * Vh;
@@ -1124,12 +1159,28 @@ __global__ __launch_bounds__(kBlock) void k_bms_assemble(const void *basis, cons
   if (e >= n3) return;
 #pragma unroll
   for (int q = 0; q < NS; q++)
-    if (q < ns) x[((long)j * nshift + s0 + q) * n3 + e] = acc[q];
+    if (q < ns) {
+      XO v;
+      v.x = acc[q].x;
+      v.y = acc[q].y;
+      x[((long)j * nshift + s0 + q) * n3 + e] = v;
+    }
+}
+
+// single-value kernels: set / advance the device iteration base (kernels,
+// not memsets, so that they can sit in a captured graph)
+__global__ void k_set_int(int *p, int v) {
+  if (threadIdx.x == 0) *p = v;
+}
+__global__ void k_add_int(int *p, int v) {
+  if (threadIdx.x == 0) *p += v;
 }
 
 // ------------------------------------------------------------- host drivers
 // systems still running, written by every solver finalize launch
 static int *active_ptr(Context &c) { return (int *)(c.ws->counter + 1); }
+// device iteration base of the stored-basis solver
+static int *iter_ptr(Context &c) { return (int *)(c.ws->counter + 8); }
 
 static void finalize(Context &c, const RedArgs &ra, int nvec, bool count_active) {
   hipLaunchKernelGGL(k_finalize, dim3(1), dim3(kBlock), 0, c.stream, ra, nvec,
@@ -1227,7 +1278,7 @@ static int ms_cg(Context &c, const double2 *b, int nrhs, int nshift, const doubl
 // caller then runs the explicit-shift algorithm).
 template <typename R, int LINK>
 static int ms_cg_basis(Context &c, const double2 *b, int nrhs, int nshift, const double *shifts,
-                       const double *res_abs, int max_iter, double2 *x, int *iters_out) {
+                       const double *res_abs, int max_iter, double2 *x, int *iters_out, float2 *x32 = nullptr) {
   using C = Cx<R>;
   const long Vh = c.Vh;
   const size_t vb = sizeof(C) * 3 * Vh;
@@ -1283,20 +1334,53 @@ static int ms_cg_basis(Context &c, const double2 *b, int nrhs, int nshift, const
   mdm.red = make_red(c, nrhs, EPI_STORE, nullptr, nullptr);
   a.partB = mdm.red.partial;
 
-  const int chunk = 8;
-  int k = 1;
-  bool converged = false;
-  while (true) {
-    for (int q = 0; q < chunk && k < cap; q++, k++) {
-      hop.skip = &bsh[0].stop[k & 1];
+  // Iterations run in chunks of kChunk with the iteration number read on the
+  // device (base + offset), so a chunk is a fixed launch sequence: the first
+  // chunk of a solve runs eagerly (the Dslash autotuner may time candidates
+  // there), later chunks replay a captured graph of it.  Iterations past the
+  // basis capacity are no-ops: k_bms_d stops a system at iteration cap-1.
+  int *iterc = iter_ptr(c);
+  hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, c.stream, iterc, 1);
+  a.iterc = iterc;
+  auto chunk_launch = [&]() {
+    for (int q = 0; q < kChunk; q++) {
+      hop.skip = &bsh[0].stop[(1 + q) & 1];
       mdm.skip = hop.skip;
       dslash_dispatch<R, LINK, DS_HOP, RED_NONE>(c, hop);
       dslash_dispatch<R, LINK, DS_MDAGM, RED_DOT>(c, mdm);
       a.nbrB = mdm.red.nbr;
-      a.iter = k;
+      a.iter_off = q;
       hipLaunchKernelGGL(k_bms_c<R>, dim3(nbr, nrhs), dim3(kBlock), 0, c.stream, a);
       hipLaunchKernelGGL(k_bms_d<R>, dim3(nbr, nrhs), dim3(kBlock), 0, c.stream, a);
     }
+    hipLaunchKernelGGL(k_add_int, dim3(1), dim3(64), 0, c.stream, iterc, kChunk);
+  };
+  int k = 1;
+  bool converged = false;
+  GraphSlot &gs = c.ws->bms_graph[sizeof(R) == 8 ? 1 : 0];
+  while (true) {
+    if (k == 1 || !c.use_graphs) {
+      chunk_launch();
+    } else {
+      BmsGraphKey key;
+      memset(&key, 0, sizeof(key));
+      key.a = a; key.hop = hop; key.mdm = mdm; key.nbr = nbr;
+      key.a.b = nullptr;  // the right-hand sides are read only by k_bms_init, outside the chunk
+      if (!gs.exec || memcmp(&key, &gs.key, sizeof(key)) != 0) {
+        if (gs.exec) hipGraphExecDestroy(gs.exec);
+        gs.exec = nullptr;
+        hipGraph_t g = nullptr;
+        if (!hip_ok(c, hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal), "capture")) return -1;
+        chunk_launch();
+        if (!hip_ok(c, hipStreamEndCapture(c.stream, &g), "capture end")) return -1;
+        const bool ok = hip_ok(c, hipGraphInstantiate(&gs.exec, g, nullptr, nullptr, 0), "instantiate");
+        hipGraphDestroy(g);
+        if (!ok) { gs.exec = nullptr; return -1; }
+        memcpy(&gs.key, &key, sizeof(key));
+      }
+      if (!hip_ok(c, hipGraphLaunch(gs.exec, c.stream), "graph launch")) return -1;
+    }
+    k += kChunk;
     int *hf = c.ws->h_flags;
     hipMemcpyAsync(hf, a.active + (k & 1), sizeof(int), hipMemcpyDeviceToHost, c.stream);
     hipStreamSynchronize(c.stream);
@@ -1341,8 +1425,12 @@ static int ms_cg_basis(Context &c, const double2 *b, int nrhs, int nshift, const
   hipMemcpyAsync(dcoef, coef.data(), sizeof(double) * coef.size(), hipMemcpyHostToDevice, c.stream);
   hipMemcpyAsync(dklen, klen.data(), sizeof(int) * nrhs, hipMemcpyHostToDevice, c.stream);
   for (int s0 = 0; s0 < nshift; s0 += 16)
-    hipLaunchKernelGGL((k_bms_assemble<R, 16>), dim3((unsigned)((3 * Vh + kBlock - 1) / kBlock), nrhs),
-                       dim3(kBlock), 0, c.stream, basis, dcoef, dklen, x, Vh, (int)cap, nshift, s0);
+    if (x32)
+      hipLaunchKernelGGL((k_bms_assemble<R, 16, float2>), dim3((unsigned)((3 * Vh + kBlock - 1) / kBlock), nrhs),
+                         dim3(kBlock), 0, c.stream, basis, dcoef, dklen, x32, Vh, (int)cap, nshift, s0);
+    else
+      hipLaunchKernelGGL((k_bms_assemble<R, 16, double2>), dim3((unsigned)((3 * Vh + kBlock - 1) / kBlock), nrhs),
+                         dim3(kBlock), 0, c.stream, basis, dcoef, dklen, x, Vh, (int)cap, nshift, s0);
   if (!hip_ok(c, hipStreamSynchronize(c.stream), "basis assemble")) return -1;
   *iters_out = kmax;
   for (int j = 0; j < nrhs; j++)
@@ -1444,7 +1532,11 @@ int residual_d(Context &c, const double2 *b, int b_group, const double2 *x, cons
 }
 
 int multishift_solve(Context &c, const double2 *b, int nrhs, int nshift, const double *shifts, double res, int rel,
-                     int prec, int max_iter, double2 *x, int *iters) {
+                     int prec, int max_iter, double2 *x, int *iters, float2 *x32) {
+  if (x32 && prec != PREC_FP32) {
+    c.last_error = "multishift: FP32 solutions only from the FP32 solver";
+    return 1;
+  }
   if (nshift < 1 || nshift > kMaxShifts || nrhs < 1 || nrhs * nshift > kMaxRed) {
     c.last_error = "multishift: unsupported number of shifts/rhs";
     return 1;
@@ -1471,13 +1563,16 @@ int multishift_solve(Context &c, const double2 *b, int nrhs, int nshift, const d
   for (int j = 0; j < nrhs; j++) res32[j] = std::max(res_abs[j], rel32 * sqrt(bn2[j]));
   int it32 = 0;
   const long n = 3 * Vh * (long)nrhs * nshift;
-  int rc = explicit_shifts ? 3 : ms_cg_basis<float, LINK_S12>(c, b, nrhs, nshift, shifts, res32.data(), max_iter, x, &it32);
+  int rc = explicit_shifts ? 3
+                           : ms_cg_basis<float, LINK_S12>(c, b, nrhs, nshift, shifts, res32.data(), max_iter, x, &it32,
+                                                          x32);
   if (rc == 3) {
-    float2 *x32 = (float2 *)ws_get(c, S_X32, sizeof(float2) * 3 * Vh * nrhs * nshift);
-    if (!x32) return -1;
-    rc = ms_cg<float, LINK_S12>(c, b, nrhs, nshift, shifts, res32.data(), max_iter, x32, &it32);
+    float2 *xs = x32 ? x32 : (float2 *)ws_get(c, S_X32, sizeof(float2) * 3 * Vh * nrhs * nshift);
+    if (!xs) return -1;
+    rc = ms_cg<float, LINK_S12>(c, b, nrhs, nshift, shifts, res32.data(), max_iter, xs, &it32);
     if (rc < 0) return rc;
-    hipLaunchKernelGGL(k_copy_f2d, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, c.stream, x, x32, n);
+    if (!x32)
+      hipLaunchKernelGGL(k_copy_f2d, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, c.stream, x, xs, n);
   }
   if (rc < 0) return rc;
   if (prec == PREC_FP32) {

@@ -87,14 +87,37 @@ struct Traj {
         rc = multishift_solve(c, b + ps * sp, 1, order, co.b.data(), res, p.residual_relative, pr, p.max_cg,
                               xo + (size_t)ps * order * sp, &it);
         r.cg_iterations += it;
+        if (pr == PREC_FP32) r.cg_iterations_fp32 += it;
       }
       c.ms_algorithm = saved;
     } else {
       int it = 0;
       rc = multishift_solve(c, b, c.n_pf, order, co.b.data(), res, p.residual_relative, pr, p.max_cg, xo, &it);
       r.cg_iterations += it;
+      if (pr == PREC_FP32) r.cg_iterations_fp32 += it;
     }
     c.smf_order = order;
+    r.n_solves++;
+    t_solve += secs(t0);
+    if (rc && !status) status = rc;
+    return rc;
+  }
+
+  // MD solve in FP32 with FP32 solutions and the FP32-input force
+  bool md_fp32() const { return prec_for(p.res_md) == PREC_FP32 && !p.per_pf_solves; }
+
+  int solve32(const double2 *b, const Coeffs &co, double res) {
+    auto t0 = clk::now();
+    const int order = (int)co.b.size();
+    if (!c.smf32) {
+      c.smf32 = (float2 *)dev_alloc(c, sizeof(float2) * 3 * c.Vh * (size_t)c.n_pf * c.max_order);
+      if (!c.smf32) { if (!status) status = -1; return -1; }
+    }
+    int it = 0;
+    int rc = multishift_solve(c, b, c.n_pf, order, co.b.data(), res, p.residual_relative, PREC_FP32, p.max_cg,
+                              c.smf, &it, c.smf32);
+    r.cg_iterations += it;
+    r.cg_iterations_fp32 += it;
     r.n_solves++;
     t_solve += secs(t0);
     if (rc && !status) status = rc;
@@ -104,13 +127,16 @@ struct Traj {
   // ipdot from the fermion force (+ gauge force when with_gauge)
   void force(bool fermion, bool gauge) {
     if (fermion && !p.pure_gauge) {
-      if (solve(c.chi, md, p.res_md)) return;
+      const bool f32 = md_fp32();
+      if (f32 ? solve32(c.chi, md, p.res_md) : solve(c.chi, md, p.res_md)) return;
       auto t0 = clk::now();
       const int order = (int)md.a.size();
       std::vector<double> coef((size_t)c.n_pf * order);
       for (int ps = 0; ps < c.n_pf; ps++)
         for (int k = 0; k < order; k++) coef[(size_t)ps * order + k] = md.a[k];
-      if (fermion_force(c, c.n_pf * order, coef.data(), c.smf, 0) && !status) status = -1;
+      const int rcf = f32 ? fermion_force32(c, c.n_pf * order, coef.data(), c.smf32, 0)
+                          : fermion_force(c, c.n_pf * order, coef.data(), c.smf, 0);
+      if (rcf && !status) status = -1;
       if (gauge) gauge_force(c, 1);
       hipStreamSynchronize(c.stream);
       t_force += secs(t0);

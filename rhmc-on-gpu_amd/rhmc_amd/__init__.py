@@ -2,4 +2,4 @@
 force and SU(3) link/momentum update behind the C ABI of include/gpu.h and
 include/cu_inverter.h.  This package holds the ctypes binding, synthetic
 field generators and the rational-approximation generator."""
-from .lib import Gpu, RhmcError, load, header_functions  # noqa: F401
+from .lib import Gpu, RhmcError, load, header_functions, multi_trajectory, traj_params  # noqa: F401

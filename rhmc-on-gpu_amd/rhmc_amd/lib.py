@@ -72,6 +72,7 @@ class TrajParams(C.Structure):
 
 class TrajResult(C.Structure):
     _fields_ = [("delta_h", C.c_double), ("delta_h_fp64", C.c_double), ("accepted", C.c_int), ("cg_iterations", C.c_int),
+                ("cg_iterations_fp32", C.c_int),
                 ("n_solves", C.c_int), ("eig_min", C.c_double), ("eig_max", C.c_double),
                 ("seconds_total", C.c_double), ("seconds_solver", C.c_double),
                 ("seconds_force", C.c_double), ("seconds_update", C.c_double), ("seconds_verify", C.c_double)]
@@ -101,6 +102,7 @@ def _declare(L):
         "gpu_fermion_force": (I, [VP, I, VP, I]),
         "gpu_sol_sum": (I, [VP, I, D, VP]),
         "gpu_fermionforce": (I, [VP, D, I, I, VP, VP, I, I, I, PI]),
+        "gpu_multi_trajectory": (I, [VP, I, VP, VP, VP]),
         "gpu_fermion_action": (I, [VP, D, I, I, VP, D, VP, I, I, PD]),
         "gpu_find_max": (I, [VP, PD, D, U64]),
         "gpu_find_min": (I, [VP, PD, D, D, U64]),
@@ -136,6 +138,68 @@ INTEGRATORS = {"omelyan": 0, "multistep_2mn": 1, "leapfrog": 2, "multistep_4mn":
 FERMION_BUFS = {"mf": 0, "smf": 1, "phi": 2, "chi": 3, "phinew": 4}
 
 
+def traj_params(*, n_flavours, first, md, last, no_md, integrator="omelyan", gauge_scale=1,
+               res_md=1e-10, res_metro=1e-10, relative=False, precision=0, fp32_threshold=0.0,
+               verify_fp64=False, max_cg=100000,
+               lambda_min=1e-5, bounds=None, power_tol=1e-5, draw_fields=False, seed=1,
+               metropolis=False, metropolis_u=0.5, pure_gauge=False, metro_precision=0,
+               per_pf_solves=False, verify_res=0.0):
+    """rhmc_traj_params from keywords; returns (params, arrays to keep alive).
+    first/md/last: (a0, a[], b[]) normalised coefficients; bounds=(min,max)
+    for fixed spectral bounds, None to measure them with power iterations.
+    precision=1 (mixed): solves with res >= fp32_threshold run in FP32, the
+    others in FP64 (metro_precision=0) or as FP32 iterations with FP64
+    defect correction (metro_precision=1).  per_pf_solves: one
+    pseudofermion per solve with per-shift search directions (the
+    reference's loop) instead of one multi-RHS solve."""
+    keep = []
+
+    def arr(x):
+        a = np.ascontiguousarray(x, np.float64)
+        keep.append(a)
+        return _ptr(a)
+
+    t = TrajParams(n_flavours=n_flavours, no_md=no_md, gauge_scale=gauge_scale,
+                   integrator=INTEGRATORS[integrator], res_md=res_md, res_metro=res_metro,
+                   residual_relative=int(relative), precision=int(precision),
+                   fp32_threshold=float(fp32_threshold), verify_fp64=int(verify_fp64), max_cg=max_cg,
+                   order_first=len(first[1]), order_md=len(md[1]), order_last=len(last[1]),
+                   first_a=arr(first[1]), first_b=arr(first[2]), first_a0=first[0],
+                   md_a=arr(md[1]), md_b=arr(md[2]), md_a0=md[0],
+                   last_a=arr(last[1]), last_b=arr(last[2]), last_a0=last[0],
+                   lambda_min=lambda_min, bounds_mode=0 if bounds else 1,
+                   eig_min=bounds[0] if bounds else 0.0, eig_max=bounds[1] if bounds else 0.0,
+                   power_tol=power_tol, draw_fields=int(draw_fields), seed=seed,
+                   metropolis=int(metropolis), metropolis_u=metropolis_u, pure_gauge=int(pure_gauge),
+                   metro_precision=int(metro_precision), per_pf_solves=int(per_pf_solves),
+                   verify_res=float(verify_res))
+    return t, keep
+
+
+def multi_trajectory(gpus, kwargs_list):
+    """One trajectory on each of several handles (independent lattices of any
+    size) run concurrently (gpu_multi_trajectory); returns the results and the
+    per-lattice status codes."""
+    n = len(gpus)
+    if n != len(kwargs_list) or n == 0:
+        raise ValueError("one keyword set per handle")
+    L = gpus[0].L
+    keep = []
+    params = (TrajParams * n)()
+    for i, kw in enumerate(kwargs_list):
+        t, k = traj_params(**kw)
+        params[i] = t
+        keep.append(k)
+    results = (TrajResult * n)()
+    status = (C.c_int * n)()
+    handles = (C.c_void_p * n)(*[g.h for g in gpus])
+    rc = L.gpu_multi_trajectory(handles, n, params, results, status)
+    if rc not in (0,):
+        msgs = [gpus[i]._err() for i in range(n) if status[i]]
+        raise RhmcError(f"gpu_multi_trajectory: rc={rc} {msgs}")
+    return list(results), list(status)
+
+
 class Gpu:
     """One lattice on one GPU (an rhmc_gpu_t handle)."""
 
@@ -165,10 +229,13 @@ class Gpu:
         except Exception:
             pass
 
+    def _err(self):
+        msg = self.L.gpu_last_error(self.h)
+        return msg.decode() if msg else ""
+
     def _check(self, rc, what):
         if rc:
-            msg = self.L.gpu_last_error(self.h)
-            raise RhmcError(f"{what}: rc={rc} {msg.decode() if msg else ''}")
+            raise RhmcError(f"{what}: rc={rc} {self._err()}")
 
     # -- transfers
     def init0(self, links):
@@ -303,40 +370,9 @@ class Gpu:
     def gauss_phi(self, seed, stream=1):
         self._check(self.L.gpu_gauss_phi(self.h, seed, stream), "gpu_gauss_phi")
 
-    def trajectory(self, *, n_flavours, first, md, last, no_md, integrator="omelyan", gauge_scale=1,
-                   res_md=1e-10, res_metro=1e-10, relative=False, precision=0, fp32_threshold=0.0,
-                   verify_fp64=False, max_cg=100000,
-                   lambda_min=1e-5, bounds=None, power_tol=1e-5, draw_fields=False, seed=1,
-                   metropolis=False, metropolis_u=0.5, pure_gauge=False, metro_precision=0,
-                   per_pf_solves=False, verify_res=0.0):
-        """first/md/last: (a0, a[], b[]) normalised coefficients; bounds=(min,max)
-        for fixed spectral bounds, None to measure them with power iterations.
-        precision=1 (mixed): solves with res >= fp32_threshold run in FP32, the
-        others in FP64 (metro_precision=0) or as FP32 iterations with FP64
-        defect correction (metro_precision=1).  per_pf_solves: one
-        pseudofermion per solve with per-shift search directions (the
-        reference's loop) instead of one multi-RHS solve."""
-        keep = []
-
-        def arr(x):
-            a = np.ascontiguousarray(x, np.float64)
-            keep.append(a)
-            return _ptr(a)
-
-        t = TrajParams(n_flavours=n_flavours, no_md=no_md, gauge_scale=gauge_scale,
-                       integrator=INTEGRATORS[integrator], res_md=res_md, res_metro=res_metro,
-                       residual_relative=int(relative), precision=int(precision),
-                       fp32_threshold=float(fp32_threshold), verify_fp64=int(verify_fp64), max_cg=max_cg,
-                       order_first=len(first[1]), order_md=len(md[1]), order_last=len(last[1]),
-                       first_a=arr(first[1]), first_b=arr(first[2]), first_a0=first[0],
-                       md_a=arr(md[1]), md_b=arr(md[2]), md_a0=md[0],
-                       last_a=arr(last[1]), last_b=arr(last[2]), last_a0=last[0],
-                       lambda_min=lambda_min, bounds_mode=0 if bounds else 1,
-                       eig_min=bounds[0] if bounds else 0.0, eig_max=bounds[1] if bounds else 0.0,
-                       power_tol=power_tol, draw_fields=int(draw_fields), seed=seed,
-                       metropolis=int(metropolis), metropolis_u=metropolis_u, pure_gauge=int(pure_gauge),
-                       metro_precision=int(metro_precision), per_pf_solves=int(per_pf_solves),
-                       verify_res=float(verify_res))
+    def trajectory(self, **kw):
+        """One RHMC trajectory (gpu_trajectory); keyword arguments as traj_params."""
+        t, keep = traj_params(**kw)
         r = TrajResult()
         self._check(self.L.gpu_trajectory(self.h, C.byref(t), C.byref(r)), "gpu_trajectory")
         return r

@@ -244,3 +244,58 @@ def test_free_field_spectrum():
     ray = np.vdot(v, mdm).real / np.vdot(v, v).real
     assert mass * mass <= ray <= 4 + mass * mass
     g.close()
+
+
+@pytest.mark.parametrize("name", ["t4", "t4o", "t4m"])
+def test_trajectory_fp32_md_against_reference(name):
+    """Mixed trajectory as benchmarked: MD solves in FP32 (12-real links) with
+    FP32 solutions feeding the FP32-input force, heatbath and Metropolis solves
+    in FP64.  FP32 MD moves the end point by FP32 rounding only, so links and
+    Delta H agree with the reference's FP64 update() to ~1e-6."""
+    gd = dict(np.load(os.path.join(GOLD, f"golden_{name}.npz")))
+    dims = tuple(gd["dims"])
+    say("trajectory fp32 md", name)
+    g = Gpu(dims, mass=float(gd["mass"]), beta=float(gd["beta"]), n_pf=int(gd["no_ps"]), max_order=8)
+    g.init0(gd["links"])
+    g.init1(momenta=gd["mom"])
+    g.put_fermions("phi", gd["phi"])
+    kw = _traj_args(gd)
+    kw["res_md"] = 1e-9  # FP32 MD solves (>= fp32_threshold); Metropolis stays at 1e-12 in FP64
+    r = g.trajectory(precision=1, fp32_threshold=1e-10, **kw)
+    U, _, P = g.end(links=True, ipdot=False, momenta=True)
+    say("dH", r.delta_h, "ref", float(gd["traj_dh"]), "links", rel(U, gd["traj_links"]))
+    assert rel(U, gd["traj_links"]) < 1e-5
+    assert rel(P, gd["traj_mom"]) < 1e-4
+    assert abs(r.delta_h - float(gd["traj_dh"])) < 1e-4 * max(1.0, abs(float(gd["traj_dh"])))
+    g.close()
+
+
+def test_multi_trajectory_matches_single():
+    """Independent lattices of different sizes run concurrently
+    (gpu_multi_trajectory) give the same trajectories as one at a time, and
+    each matches the reference's update()."""
+    from rhmc_amd import multi_trajectory
+    names = ["t4", "t4o", "t4m"]
+    gds = [dict(np.load(os.path.join(GOLD, f"golden_{n}.npz"))) for n in names]
+    gpus = []
+    for gd in gds:
+        g = Gpu(tuple(gd["dims"]), mass=float(gd["mass"]), beta=float(gd["beta"]), n_pf=int(gd["no_ps"]),
+                max_order=8)
+        g.init0(gd["links"])
+        g.init1(momenta=gd["mom"])
+        g.put_fermions("phi", gd["phi"])
+        gpus.append(g)
+    say("multi trajectory", names)
+    res, st = multi_trajectory(gpus, [_traj_args(gd) for gd in gds])
+    assert st == [0, 0, 0]
+    for g, gd, r in zip(gpus, gds, res):
+        U, _, _ = g.end(links=True, ipdot=False, momenta=False)
+        assert abs(r.delta_h - float(gd["traj_dh"])) < 1e-9
+        assert rel(U, gd["traj_links"]) < 1e-10
+        # the same lattice alone, from the same start
+        g.init0(gd["links"])
+        g.init1(momenta=gd["mom"])
+        g.put_fermions("phi", gd["phi"])
+        r1 = g.trajectory(**_traj_args(gd))
+        assert r1.delta_h == r.delta_h
+        g.close()
