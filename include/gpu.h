@@ -58,6 +58,19 @@ int gpu_init1(rhmc_gpu_t h, const double *phi, const double *ipdot, const double
  * the host; NULL skips a field.  Device state is kept (no free). */
 int gpu_end(rhmc_gpu_t h, double *links, double *ipdot, double *momenta);
 
+/* lib/Main/Conf/conf.cc:143-204 (Conf::write / write_last): the device links
+ * to a text file in the reference's format (header "nx ny nz nt beta mass
+ * no_flavours update_iteration", then every link as three rows of "(re,im)"
+ * at precision 16); written to path.tmp and renamed. */
+int gpu_write_conf(rhmc_gpu_t h, const char *path, double beta, double mass, double n_flavours,
+                   int update_iteration);
+
+/* lib/Main/Conf/conf.cc:55-96 (Conf(init=2)): links from such a file to the
+ * device, as stored; the extents must match the handle's lattice.  Header
+ * values are returned through the pointers (each may be NULL). */
+int gpu_read_conf(rhmc_gpu_t h, const char *path, int *update_iteration, double *beta, double *mass,
+                  double *n_flavours);
+
 /* Pseudofermion buffers: which = 0 right-hand sides "mf" (reference
  * mf_device / chi_packed), 1 shifted solutions "smf" (psi_packed, n_pf*order
  * vectors of the last solve), 2 heatbath phi, 3 chi, 4 phi_new. */
@@ -131,6 +144,19 @@ int gpu_dslash(rhmc_gpu_t h, int par_out, int precision, int nvec, const double 
  * nvec vectors and return the mean kernel time in ms, measured with HIP
  * events on the library's stream. */
 int gpu_time_dslash(rhmc_gpu_t h, int precision, int nvec, int reps, double *ms);
+
+/* The same with every timed launch preceded by a write pass over
+ * flush_bytes (<= 8e9) of other device memory, so the operands come from HBM
+ * rather than the L2s or the Infinity Cache; mean of the per-launch event
+ * times. */
+int gpu_time_dslash_cold(rhmc_gpu_t h, int precision, int nvec, int reps, double flush_bytes, double *ms);
+
+/* Live kernel timing inside real work: while enabled, every FP32 12-real
+ * half-lattice Dslash on n_pf right-hand sides (the MD solver's hop) is
+ * bracketed by HIP events on the library stream.  Returns the mean duration
+ * and count recorded since the last reset; enable = 1/0 then switches it on
+ * or off and resets the counters, enable = -1 only reads. */
+int gpu_kernel_timing(rhmc_gpu_t h, int enable, double *mean_ms, long long *count);
 
 /* Timing helper: a plain device copy of `bytes` (read + write), mean ms per
  * copy over `reps` launches (HIP events); the achievable-bandwidth reference

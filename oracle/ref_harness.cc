@@ -23,6 +23,7 @@
 #include <fstream>
 #include <iostream>
 #include <string>
+#include <unistd.h>
 
 #define private public
 #define DEFINE_GLOBALS
@@ -195,6 +196,19 @@ int main(int argc, char **argv) {
     write_raw("out_mom.bin", gauge_momenta->momenta, sizeof(Su3) * no_links);
     write_raw("out_phinew.bin", fermion_phi->fermion, sizeof(Vec3) * sizeh * no_ps);
     write_raw("out_dh.bin", &de, sizeof(double));
+  } else if (op == "conf_write") {
+    // Conf::write_last (conf.cc:179-204) of the input links, update_iteration = arg
+    update_iteration = (int)arg;
+    memcpy(gauge_conf->u_save, gauge_conf->u_work, sizeof(Su3) * no_links);
+    if (chdir(g_dir.c_str()) != 0) return 2;
+    gauge_conf->write_last();
+  } else if (op == "conf_read") {
+    // Conf(init=2) (conf.cc:55-96) on the file "config" in the directory
+    if (chdir(g_dir.c_str()) != 0) return 2;
+    Conf *c2 = new Conf(2);
+    write_raw("out_links.bin", c2->u_work, sizeof(Su3) * no_links);
+    double it = update_iteration;
+    write_raw("out_iter.bin", &it, sizeof(double));
   } else {
     fprintf(stderr, "unknown op %s\n", op.c_str());
     return 1;

@@ -3,6 +3,7 @@
 
 #include <cstdlib>
 #include <thread>
+#include <cstdio>
 
 #include <cmath>
 #include <algorithm>
@@ -139,6 +140,7 @@ int gpu_destroy(rhmc_gpu_t h) {
   dev_free(c.mf); dev_free(c.smf); dev_free(c.phi0); dev_free(c.chi); dev_free(c.phinew); dev_free(c.action);
   dev_free(c.tiles);
   dev_free(c.smf32);
+  kernel_timing_release(c);
   workspace_destroy(c.ws);
   if (c.stream) hipStreamDestroy(c.stream);
   delete h;
@@ -173,6 +175,105 @@ int gpu_end(rhmc_gpu_t h, double *links, double *ipdot, double *momenta) {
   if (links && download_links(c, c.U64, links)) return RHMC_EHIP;
   if (ipdot && download_alg(c, c.ipdot, ipdot, 0)) return RHMC_EHIP;
   if (momenta && download_alg(c, c.P, momenta, 1)) return RHMC_EHIP;
+  return RHMC_OK;
+}
+
+// ---- gauge configuration files in the reference's text format
+// conf.cc:143-204 (Conf::write_last): header "nx ny nz nt beta mass
+// no_flavours update_iteration", then every link (site order mu*V + i,
+// phases folded in) as three rows of "(re,im)" at precision 16
+// (su3.cc:225-232), a blank line after each matrix.
+static void put_g16(std::string &o, double v) {
+  char b[40];
+  snprintf(b, sizeof b, "%.16g", v);
+  o += b;
+}
+
+int gpu_write_conf(rhmc_gpu_t h, const char *path, double beta, double mass, double n_flavours, int update_iteration) {
+  if (!h || !path) return RHMC_EINVAL;
+  Context &c = h->c;
+  hipSetDevice(c.device);
+  const long nl = 4 * c.V;
+  std::vector<double> u((size_t)nl * 18);
+  if (download_links(c, c.U64, u.data())) return RHMC_EHIP;
+  std::string o;
+  o.reserve((size_t)nl * 9 * 48 + 128);
+  o += std::to_string(c.L.nx) + " " + std::to_string(c.L.ny) + " " + std::to_string(c.L.nz) + " " +
+       std::to_string(c.L.nt) + " ";
+  put_g16(o, beta); o += " ";
+  put_g16(o, mass); o += " ";
+  put_g16(o, n_flavours); o += " ";
+  o += std::to_string(update_iteration) + "\n";
+  for (long r = 0; r < nl; r++)
+    for (int i = 0; i < 3; i++) {
+      for (int k = 0; k < 3; k++) {
+        const double *z = &u[(size_t)r * 18 + (i * 3 + k) * 2];
+        if (k) o += " ";
+        o += "(";
+        put_g16(o, z[0]);
+        o += ",";
+        put_g16(o, z[1]);
+        o += ")";
+      }
+      o += i < 2 ? "\n" : "\n\n";
+    }
+  const std::string tmp = std::string(path) + ".tmp";
+  FILE *f = fopen(tmp.c_str(), "wb");
+  if (!f) return fail(h, "gpu_write_conf: cannot open file");
+  const bool ok = fwrite(o.data(), 1, o.size(), f) == o.size();
+  if (fclose(f) != 0 || !ok || rename(tmp.c_str(), path) != 0) return fail(h, "gpu_write_conf: write failed");
+  return RHMC_OK;
+}
+
+// conf.cc:55-96 (Conf(init=2)): extents must match; links used as stored
+int gpu_read_conf(rhmc_gpu_t h, const char *path, int *update_iteration, double *beta, double *mass,
+                  double *n_flavours) {
+  if (!h || !path) return RHMC_EINVAL;
+  Context &c = h->c;
+  hipSetDevice(c.device);
+  FILE *f = fopen(path, "rb");
+  if (!f) return fail(h, "gpu_read_conf: no stored configuration");
+  std::string txt;
+  char buf[1 << 16];
+  size_t n;
+  while ((n = fread(buf, 1, sizeof buf, f)) > 0) txt.append(buf, n);
+  fclose(f);
+  const char *p = txt.c_str();
+  char *e = nullptr;
+  long d[4];
+  for (int k = 0; k < 4; k++) {
+    d[k] = strtol(p, &e, 10);
+    if (e == p) return fail(h, "gpu_read_conf: bad header");
+    p = e;
+  }
+  if (d[0] != c.L.nx || d[1] != c.L.ny || d[2] != c.L.nz || d[3] != c.L.nt)
+    return fail(h, "gpu_read_conf: stored configuration does not fit the lattice");
+  double hv[3];
+  for (int k = 0; k < 3; k++) {
+    hv[k] = strtod(p, &e);
+    if (e == p) return fail(h, "gpu_read_conf: bad header");
+    p = e;
+  }
+  const long it = strtol(p, &e, 10);
+  if (e == p) return fail(h, "gpu_read_conf: bad header");
+  p = e;
+  const long nl = 4 * c.V;
+  std::vector<double> u((size_t)nl * 18);
+  for (size_t q = 0; q < u.size(); q += 2) {
+    while (*p && *p != '(') p++;
+    if (!*p) return fail(h, "gpu_read_conf: truncated file");
+    u[q] = strtod(p + 1, &e);
+    if (*e != ',') return fail(h, "gpu_read_conf: bad complex number");
+    u[q + 1] = strtod(e + 1, &e);
+    if (*e != ')') return fail(h, "gpu_read_conf: bad complex number");
+    p = e + 1;
+  }
+  if (upload_links(c, u.data(), c.U64) || refresh_links32(c)) return RHMC_EHIP;
+  if (!hip_ok(c, hipStreamSynchronize(c.stream), "read conf")) return RHMC_EHIP;
+  if (update_iteration) *update_iteration = (int)it;
+  if (beta) *beta = hv[0];
+  if (mass) *mass = hv[1];
+  if (n_flavours) *n_flavours = hv[2];
   return RHMC_OK;
 }
 
@@ -408,6 +509,71 @@ int gpu_time_dslash(rhmc_gpu_t h, int precision, int nvec, int reps, double *ms)
   hipEventDestroy(e1);
   hipFree(a);
   hipFree(b);
+  return hip_ok(c, hipGetLastError(), "time dslash") ? RHMC_OK : RHMC_EHIP;
+}
+
+int gpu_kernel_timing(rhmc_gpu_t h, int enable, double *mean_ms, long long *count) {
+  if (!h) return RHMC_EINVAL;
+  Context &c = h->c;
+  hipSetDevice(c.device);
+  kernel_timing_harvest(c);
+  if (mean_ms) *mean_ms = c.kt_count ? c.kt_sum_ms / c.kt_count : 0.0;
+  if (count) *count = c.kt_count;
+  if (enable >= 0) {
+    c.time_kernels = enable;
+    c.kt_sum_ms = 0.0;
+    c.kt_count = 0;
+  }
+  return RHMC_OK;
+}
+
+__global__ void k_flush_fill(double4 *p, long n, double v) {
+  const long stride = (long)gridDim.x * blockDim.x;
+  for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) p[i] = make_double4(v, v, v, v);
+}
+
+int gpu_time_dslash_cold(rhmc_gpu_t h, int precision, int nvec, int reps, double flush_bytes, double *ms) {
+  if (!h || !ms || nvec < 1 || reps < 1 || flush_bytes < 0 || flush_bytes > 8e9) return RHMC_EINVAL;
+  Context &c = h->c;
+  hipSetDevice(c.device);
+  const long n = 3 * c.Vh * (long)nvec;
+  const size_t es = precision == 0 ? sizeof(double2) : sizeof(float2);
+  const long nf = (long)(flush_bytes / sizeof(double4));
+  void *a = nullptr, *b = nullptr, *f = nullptr;
+  if (!hip_ok(c, hipMalloc(&a, es * n), "malloc")) return RHMC_EHIP;
+  if (!hip_ok(c, hipMalloc(&b, es * n), "malloc")) { hipFree(a); return RHMC_EHIP; }
+  if (nf > 0 && !hip_ok(c, hipMalloc(&f, nf * sizeof(double4)), "malloc")) { hipFree(a); hipFree(b); return RHMC_EHIP; }
+  hipMemsetAsync(a, 0, es * n, c.stream);
+  std::vector<hipEvent_t> ev(2 * reps);
+  for (auto &e : ev) hipEventCreate(&e);
+  auto launch = [&](int k) {
+    const int par = k & 1;
+    if (precision == 0)
+      launch_hop_d(c, par, (const double2 *)((k & 1) ? b : a), (double2 *)((k & 1) ? a : b), nvec, 3 * c.Vh, 3 * c.Vh);
+    else
+      hop_s12(c, par, (const float2 *)((k & 1) ? b : a), (float2 *)((k & 1) ? a : b), nvec);
+  };
+  launch(0);
+  // each timed launch follows a write pass over flush_bytes of other memory,
+  // which evicts the operands from the L2s and the Infinity Cache
+  for (int k = 0; k < reps; k++) {
+    if (nf > 0) hipLaunchKernelGGL(k_flush_fill, dim3(4096), dim3(256), 0, c.stream, (double4 *)f, nf, (double)k);
+    hipEventRecord(ev[2 * k], c.stream);
+    launch(k);
+    hipEventRecord(ev[2 * k + 1], c.stream);
+  }
+  hipStreamSynchronize(c.stream);
+  double tot = 0.0;
+  for (int k = 0; k < reps; k++) {
+    float t = 0.f;
+    hipEventElapsedTime(&t, ev[2 * k], ev[2 * k + 1]);
+    tot += t;
+  }
+  *ms = tot / reps;
+  for (auto &e : ev) hipEventDestroy(e);
+  hipFree(a);
+  hipFree(b);
+  if (f) hipFree(f);
   return hip_ok(c, hipGetLastError(), "time dslash") ? RHMC_OK : RHMC_EHIP;
 }
 

@@ -71,6 +71,15 @@ struct Context {
   // replay the stored-basis solver's iteration chunks as captured HIP graphs
   int use_graphs = 1;
 
+  // live kernel timing: HIP events around every FP32 multi-RHS hop with
+  // n_pf right-hand sides (the MD solver's Dslash); graphs are not replayed
+  // while it is on
+  int time_kernels = 0;
+  std::vector<hipEvent_t> kt_free;
+  std::vector<hipEvent_t> kt_pending;  // start/stop pairs
+  double kt_sum_ms = 0.0;
+  long kt_count = 0;
+
   Workspace *ws = nullptr;
   std::string last_error;
 
@@ -127,6 +136,9 @@ int vec_dots(Context &c, const double2 *a, const double2 *b, int nvec, long vstr
 
 // Dslash tile order for the lattice of c (solver.hip); allocates c.tiles
 int build_tile_order(Context &c);
+// fold the recorded kernel-timing event pairs into kt_sum_ms / kt_count
+void kernel_timing_harvest(Context &c);
+void kernel_timing_release(Context &c);
 
 // host helpers
 bool hip_ok(Context &c, hipError_t e, const char *what);

@@ -601,7 +601,45 @@ static void dslash_dispatch(Context &c, DsArgs &a) {
     hipEventDestroy(e1);
     tab[key] = nr;
   }
+  if (c.time_kernels && MODE == DS_HOP && sizeof(R) == 4 && a.nvec == c.n_pf) {
+    hipEvent_t ev[2];
+    for (auto &e : ev) {
+      if (c.kt_free.empty()) {
+        hipEventCreate(&e);
+      } else {
+        e = c.kt_free.back();
+        c.kt_free.pop_back();
+      }
+    }
+    hipEventRecord(ev[0], c.stream);
+    dslash_launch<R, LINK, MODE, RED>(c, a, nr);
+    hipEventRecord(ev[1], c.stream);
+    c.kt_pending.push_back(ev[0]);
+    c.kt_pending.push_back(ev[1]);
+    return;
+  }
   dslash_launch<R, LINK, MODE, RED>(c, a, nr);
+}
+
+void kernel_timing_harvest(Context &c) {
+  if (c.kt_pending.empty()) return;
+  hipStreamSynchronize(c.stream);
+  for (size_t i = 0; i + 1 < c.kt_pending.size(); i += 2) {
+    float t = 0.f;
+    if (hipEventElapsedTime(&t, c.kt_pending[i], c.kt_pending[i + 1]) == hipSuccess) {
+      c.kt_sum_ms += t;
+      c.kt_count++;
+    }
+    c.kt_free.push_back(c.kt_pending[i]);
+    c.kt_free.push_back(c.kt_pending[i + 1]);
+  }
+  c.kt_pending.clear();
+}
+
+void kernel_timing_release(Context &c) {
+  kernel_timing_harvest(c);
+  for (auto e : c.kt_free) hipEventDestroy(e);
+  c.kt_free.clear();
 }
 
 std::string tuning_table(Context &c) {
@@ -1359,7 +1397,7 @@ static int ms_cg_basis(Context &c, const double2 *b, int nrhs, int nshift, const
   bool converged = false;
   GraphSlot &gs = c.ws->bms_graph[sizeof(R) == 8 ? 1 : 0];
   while (true) {
-    if (k == 1 || !c.use_graphs) {
+    if (k == 1 || !c.use_graphs || c.time_kernels) {
       chunk_launch();
     } else {
       BmsGraphKey key;

@@ -103,6 +103,10 @@ def _declare(L):
         "gpu_sol_sum": (I, [VP, I, D, VP]),
         "gpu_fermionforce": (I, [VP, D, I, I, VP, VP, I, I, I, PI]),
         "gpu_multi_trajectory": (I, [VP, I, VP, VP, VP]),
+        "gpu_write_conf": (I, [VP, C.c_char_p, D, D, D, I]),
+        "gpu_read_conf": (I, [VP, C.c_char_p, PI, PD, PD, PD]),
+        "gpu_time_dslash_cold": (I, [VP, I, I, I, D, PD]),
+        "gpu_kernel_timing": (I, [VP, I, PD, C.POINTER(C.c_longlong)]),
         "gpu_fermion_action": (I, [VP, D, I, I, VP, D, VP, I, I, PD]),
         "gpu_find_max": (I, [VP, PD, D, U64]),
         "gpu_find_min": (I, [VP, PD, D, D, U64]),
@@ -270,6 +274,32 @@ class Gpu:
         out = np.zeros_like(inp)
         self._check(self.L.gpu_dslash(self.h, par_out, precision, inp.shape[0], _ptr(inp), _ptr(out)), "gpu_dslash")
         return out
+
+    def time_dslash_cold(self, precision, nvec, reps, flush_bytes=512e6):
+        v = C.c_double()
+        self._check(self.L.gpu_time_dslash_cold(self.h, int(precision), int(nvec), int(reps), float(flush_bytes),
+                                                C.byref(v)), "gpu_time_dslash_cold")
+        return v.value
+
+    def write_conf(self, path, beta, mass, n_flavours, update_iteration):
+        """Device links to a configuration file in the reference's text format."""
+        self._check(self.L.gpu_write_conf(self.h, str(path).encode(), float(beta), float(mass), float(n_flavours),
+                                          int(update_iteration)), "gpu_write_conf")
+
+    def read_conf(self, path):
+        """Links from a configuration file to the device; returns the header
+        (update_iteration, beta, mass, n_flavours)."""
+        it, b, m, nf = C.c_int(), C.c_double(), C.c_double(), C.c_double()
+        self._check(self.L.gpu_read_conf(self.h, str(path).encode(), C.byref(it), C.byref(b), C.byref(m),
+                                         C.byref(nf)), "gpu_read_conf")
+        return it.value, b.value, m.value, nf.value
+
+    def kernel_timing(self, enable=-1):
+        """(mean ms, count) of the timed MD hop launches since the last reset;
+        enable=1/0 then switches live timing on/off and resets."""
+        m, n = C.c_double(), C.c_longlong()
+        self._check(self.L.gpu_kernel_timing(self.h, int(enable), C.byref(m), C.byref(n)), "gpu_kernel_timing")
+        return m.value, n.value
 
     def tuning_table(self):
         import json

@@ -13,6 +13,7 @@ fixed subset of sites plus full spinors and Delta H.
 from __future__ import annotations
 
 import importlib.util
+import gzip
 import os
 import subprocess
 import sys
@@ -125,6 +126,15 @@ def make(cfg_name):
             run(exe, d, "action")
             out["action0"] = rd(d, "out_action0.bin", (V,), np.float64)
             out["action1"] = rd(d, "out_action1.bin", (V,), np.float64)
+        if cfg_name == "t4":
+            # the reference's text configuration writer and reader
+            run(exe, d, "conf_write", 7)
+            with open(os.path.join(d, "config"), "rb") as fh:
+                txt = fh.read()
+            with gzip.open(os.path.join(HERE, "config_t4.txt.gz"), "wb") as fh:
+                fh.write(txt)
+            run(exe, d, "conf_read")
+            out["conf_read_links"] = rd(d, "out_links.bin", (4 * V, 3, 3))
         run(exe, d, "traj")
         out["traj_dh"] = rd(d, "out_dh.bin", (1,), np.float64)[0]
         out["traj_chi"] = rd(d, "out_chi.bin", (npf, Vh, 3))

@@ -299,3 +299,65 @@ def test_multi_trajectory_matches_single():
         r1 = g.trajectory(**_traj_args(gd))
         assert r1.delta_h == r.delta_h
         g.close()
+
+
+def test_conf_write_read_reference_format(tmp_path):
+    """gpu_write_conf writes the file the reference's Conf::write_last wrote
+    for the same links, byte for byte; gpu_read_conf reads it back exactly."""
+    import gzip
+    gd = dict(np.load(os.path.join(GOLD, "golden_t4.npz")))
+    with gzip.open(os.path.join(GOLD, "config_t4.txt.gz"), "rt") as fh:
+        ref = fh.read()
+    g = Gpu(tuple(gd["dims"]), mass=float(gd["mass"]), beta=float(gd["beta"]), n_pf=1, max_order=4)
+    g.init0(gd["links"])
+    p = tmp_path / "config"
+    g.write_conf(p, float(gd["beta"]), float(gd["mass"]), float(gd["n_flavours"]), 7)
+    assert p.read_text() == ref
+    q = tmp_path / "config_ref"
+    q.write_text(ref)
+    g.init0(np.zeros_like(gd["links"]))
+    it, beta, mass, nf = g.read_conf(q)
+    assert (it, beta, mass, nf) == (7, float(gd["beta"]), float(gd["mass"]), float(gd["n_flavours"]))
+    U, _, _ = g.end(links=True, ipdot=False, momenta=False)
+    assert np.array_equal(U, gd["conf_read_links"])
+    g.close()
+
+
+def test_chain_checkpoint_and_resume(tmp_path):
+    """rhmc.cc:64-150 semantics: alternating config_0/config_1, config after
+    the last iteration, the stop file, and a restart from config that
+    continues the same chain (same links as an uninterrupted run)."""
+    from rhmc_amd.chain import Chain
+    gd = dict(np.load(os.path.join(GOLD, "golden_t4o.npz")))
+    kw = _traj_args(gd)
+    kw["draw_fields"] = True
+    mk = lambda: Gpu(tuple(gd["dims"]), mass=float(gd["mass"]), beta=float(gd["beta"]), n_pf=1, max_order=8)
+    a = tmp_path / "a"
+    g = mk()
+    ch = Chain(g, str(a), float(gd["beta"]), float(gd["mass"]), 2, links=gd["links"], therm_updates=1, rng_seed=5)
+    log = ch.run(4, 2, kw)
+    assert [r["written"] for r in log] == [[], ["config_0"], [], ["config"]]
+    U4, _, _ = g.end(links=True, ipdot=False, momenta=False)
+    g.close()
+    # the same chain in two runs of 2 with a restart from config
+    b = tmp_path / "b"
+    g = mk()
+    ch = Chain(g, str(b), float(gd["beta"]), float(gd["mass"]), 2, links=gd["links"], therm_updates=1, rng_seed=5)
+    ch.run(2, 3, kw)
+    assert (b / "config").exists()
+    g.close()
+    g = mk()
+    ch2 = Chain(g, str(b), float(gd["beta"]), float(gd["mass"]), 2, therm_updates=1, rng_seed=5)
+    assert ch2.resumed and ch2.update_iteration == 2
+    ch2.rng = np.random.default_rng(5)
+    ch2.rng.random(2)  # the accept draws the first run used
+    (b / "stop").write_text("")
+    log2 = ch2.run(2, 3, kw)
+    # iteration 3 is a multiple of save_conf_every (config_0), then the stop file (config)
+    assert len(log2) == 1 and log2[0]["written"] == ["config_0", "config"]
+    (b / "stop").unlink()
+    ch2.run(1, 3, kw)
+    U4b, _, _ = g.end(links=True, ipdot=False, momenta=False)
+    # the restart reads the 16-digit text: same chain to that precision
+    assert rel(U4b, U4) < 1e-12
+    g.close()
