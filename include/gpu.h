@@ -140,6 +140,12 @@ int gpu_gauss_phi(rhmc_gpu_t h, unsigned long long seed, unsigned long long stre
  * 1 = FP32 12-real links (vectors rounded to float). */
 int gpu_dslash(rhmc_gpu_t h, int par_out, int precision, int nvec, const double *in, double *out);
 
+/* lib/Measurements/gaugemeas.cc:6-104 (Conf::calc_plaq, Conf::calc_poly):
+ * spatial and temporal plaquettes and the Polyakov loop of the device links,
+ * with the reference's signs (staggered phases folded in), pairwise sums and
+ * normalisation. */
+int gpu_gauge_meas(rhmc_gpu_t h, double *plaq_spatial, double *plaq_temporal, double *poly_re, double *poly_im);
+
 /* Timing helper: run `reps` device-resident half-lattice Dslash launches on
  * nvec vectors and return the mean kernel time in ms, measured with HIP
  * events on the library's stream. */

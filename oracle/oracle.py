@@ -153,6 +153,14 @@ class Oracle:
                                _p(phi), _p(chi), C.c_int(int(with_fermions)), _p(value), C.c_int(init))
         return value
 
+    def gauge_meas(self, U):
+        """(plaq_spatial, plaq_temporal, poly_re, poly_im), gaugemeas.cc:6-104."""
+        U = np.ascontiguousarray(U, np.complex128)
+        a, b, c, d = C.c_double(), C.c_double(), C.c_double(), C.c_double()
+        self.L.orc_plaquette(C.byref(self.lat), _p(U), C.byref(a), C.byref(b))
+        self.L.orc_polyakov(C.byref(self.lat), _p(U), C.byref(c), C.byref(d))
+        return a.value, b.value, c.value, d.value
+
     def global_sum(self, v):
         v = np.array(v, np.float64)
         return self.L.orc_global_sum(_p(v), C.c_long(len(v)))

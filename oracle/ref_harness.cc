@@ -196,6 +196,13 @@ int main(int argc, char **argv) {
     write_raw("out_mom.bin", gauge_momenta->momenta, sizeof(Su3) * no_links);
     write_raw("out_phinew.bin", fermion_phi->fermion, sizeof(Vec3) * sizeh * no_ps);
     write_raw("out_dh.bin", &de, sizeof(double));
+  } else if (op == "meas_gauge") {
+    // Conf::calc_plaq / calc_poly (gaugemeas.cc:6-104) on the stored links
+    REAL ps, pt, pr, pi;
+    gauge_conf->calc_plaq(ps, pt);
+    gauge_conf->calc_poly(pr, pi);
+    double v[4] = {ps, pt, pr, pi};
+    write_raw("out_meas.bin", v, sizeof v);
   } else if (op == "conf_write") {
     // Conf::write_last (conf.cc:179-204) of the input links, update_iteration = arg
     update_iteration = (int)arg;

@@ -480,6 +480,61 @@ void orc_calc_action(const orc_lattice *L, const orc_c (*U)[3][3], const orc_c (
       for (long i = 0; i < Vh; i++) value[i] -= v_rdot(phi[ps * Vh + i], chi[ps * Vh + i]);
 }
 
+/* gaugemeas.cc:6-66 (Conf::calc_plaq): -Re tr of the plaquettes, temporal
+ * (nu = 3) and spatial, pairwise-summed, / (3 V) / 3 */
+void orc_plaquette(const orc_lattice *L, const orc_c (*U)[3][3], double *pls, double *plt) {
+  const long V = L->V;
+  double *dt = (double *)malloc(sizeof(double) * V), *ds = (double *)malloc(sizeof(double) * V);
+  for (long r = 0; r < V; r++) {
+    dt[r] = 0.0;
+    ds[r] = 0.0;
+    for (int mu = 0; mu < 3; mu++)
+      for (int nu = mu + 1; nu < 4; nu++) {
+        su3 a, b;
+        m_mul(a, U[mu * V + r], U[nu * V + L->nnp[r * 4 + mu]]);
+        m_dag(b, U[mu * V + L->nnp[r * 4 + nu]]);
+        m_mul(a, a, b);
+        m_dag(b, U[nu * V + r]);
+        m_mul(a, a, b);
+        if (nu == 3) dt[r] -= m_retr(a);
+        else ds[r] -= m_retr(a);
+      }
+  }
+  const double inv_size_by_three = 1.0 / (double)V / 3.;
+  const double one_by_three = 0.3333333333333333333333333333333333333333333333333333333333333333333;
+  *plt = orc_global_sum(dt, V) * inv_size_by_three * one_by_three;
+  *pls = orc_global_sum(ds, V) * inv_size_by_three * one_by_three;
+  free(dt);
+  free(ds);
+}
+
+/* gaugemeas.cc:70-104 (Conf::calc_poly): lines start at the sites r < vol3
+ * (even parity, t = 0 or 1: one per spatial point), product of the t links
+ * along t, minus the trace (antiperiodic phase), pairwise-summed, / vol3 / 3 */
+void orc_polyakov(const orc_lattice *L, const orc_c (*U)[3][3], double *re, double *im) {
+  const long V = L->V, vol3 = (long)L->nx * L->ny * L->nz;
+  double *d1 = (double *)malloc(sizeof(double) * vol3), *d2 = (double *)malloc(sizeof(double) * vol3);
+  for (long r = 0; r < vol3; r++) {
+    su3 a;
+    memset(a, 0, sizeof(a));
+    a[0][0] = a[1][1] = a[2][2] = 1.0;
+    long r2 = r;
+    for (int t = 0; t < L->nt; t++) {
+      m_mul(a, a, U[r2 + 3 * V]);
+      r2 = L->nnp[r2 * 4 + 3];
+    }
+    const orc_c tr = a[0][0] + a[1][1] + a[2][2];
+    d1[r] = -creal(tr);
+    d2[r] = -cimag(tr);
+  }
+  const double inv_vol3 = 1.0 / (double)vol3;
+  const double one_by_three = 0.3333333333333333333333333333333333333333333333333333333333333333333;
+  *re = orc_global_sum(d1, vol3) * inv_vol3 * one_by_three;
+  *im = orc_global_sum(d2, vol3) * inv_vol3 * one_by_three;
+  free(d1);
+  free(d2);
+}
+
 /* findminmax.cc:12-72 */
 int orc_findmaxeig(const orc_lattice *L, const orc_c (*U)[3][3], double mass, const orc_c (*start)[3],
                    double tol, double *max) {

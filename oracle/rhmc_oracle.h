@@ -97,6 +97,9 @@ void orc_momenta_sum_multiply(const orc_lattice *lat, orc_c (*P)[3][3],
                               const orc_c (*ipdot)[3][3], double step);
 
 /* action.cc:6-156 per-site action accumulation (init 0: set, -1: subtract) */
+/* gaugemeas.cc:6-104: plaquettes and Polyakov loop of the stored links */
+void orc_plaquette(const orc_lattice *lat, const orc_c (*U)[3][3], double *pls, double *plt);
+void orc_polyakov(const orc_lattice *lat, const orc_c (*U)[3][3], double *re, double *im);
 void orc_calc_action(const orc_lattice *lat, const orc_c (*U)[3][3], const orc_c (*P)[3][3],
                      double beta, int n_pf, const orc_c (*phi)[3], const orc_c (*chi)[3],
                      int with_fermions, double *value, int init);

@@ -512,6 +512,12 @@ int gpu_time_dslash(rhmc_gpu_t h, int precision, int nvec, int reps, double *ms)
   return hip_ok(c, hipGetLastError(), "time dslash") ? RHMC_OK : RHMC_EHIP;
 }
 
+int gpu_gauge_meas(rhmc_gpu_t h, double *plaq_spatial, double *plaq_temporal, double *poly_re, double *poly_im) {
+  if (!h || !plaq_spatial || !plaq_temporal || !poly_re || !poly_im) return RHMC_EINVAL;
+  hipSetDevice(h->c.device);
+  return gauge_meas(h->c, plaq_spatial, plaq_temporal, poly_re, poly_im) ? RHMC_EHIP : RHMC_OK;
+}
+
 int gpu_kernel_timing(rhmc_gpu_t h, int enable, double *mean_ms, long long *count) {
   if (!h) return RHMC_EINVAL;
   Context &c = h->c;

@@ -127,6 +127,8 @@ int momenta_update(Context &c, double step);
 int sol_sum(Context &c, double2 *io, const double2 *x, int nshift, double a0, const double *num);
 int action_local(Context &c, const double2 *phi, const double2 *chi, int with_fermions, int init);
 int action_sum(Context &c, double *out);
+// plaquettes (spatial, temporal) and Polyakov loop of the device links
+int gauge_meas(Context &c, double *pls, double *plt, double *pre, double *pim);
 int gauss_momenta(Context &c, unsigned long long seed, unsigned long long stream_id);
 int gauss_spinor(Context &c, double2 *out, int n, unsigned long long seed, unsigned long long stream_id);
 int refresh_links32(Context &c);

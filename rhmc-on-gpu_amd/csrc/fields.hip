@@ -374,17 +374,95 @@ __global__ __launch_bounds__(kBlock) void k_fold(double *v, long j, long off) {
   if (k < j) v[k] += v[k + off];
 }
 
-int action_sum(Context &c, double *out) {
-  long n = c.V;
+// the reference's pairwise tree sum (include/tools.h:25-57) of v[0..n), in place
+static void pairwise_sum(Context &c, double *v, long n) {
   while (n > 1) {
     long j, off, next;
     if (n % 2 == 0) { j = n / 2; off = j; next = j; }
     else { j = n / 2; off = j + 1; next = j + 1; }
-    hipLaunchKernelGGL(k_fold, dim3(nblk(j)), dim3(kBlock), 0, c.stream, c.action, j, off);
+    hipLaunchKernelGGL(k_fold, dim3(nblk(j)), dim3(kBlock), 0, c.stream, v, j, off);
     n = next;
   }
+}
+
+int action_sum(Context &c, double *out) {
+  pairwise_sum(c, c.action, c.V);
   hipMemcpyAsync(out, c.action, sizeof(double), hipMemcpyDeviceToHost, c.stream);
   return hip_ok(c, hipStreamSynchronize(c.stream), "action sum") ? 0 : -1;
+}
+
+// ------------------------------------------------------------ measurements
+// gaugemeas.cc:6-66 (Conf::calc_plaq): per site minus Re tr of its three
+// temporal (nu = 3) and three spatial plaquettes, at the reference site index
+__global__ __launch_bounds__(kBlock) void k_plaq(LatticeDims L, const double2 *U, double *vt, double *vs) {
+  const long Vh = L.Vh;
+  const long h = (long)blockIdx.x * kBlock + threadIdx.x;
+  const int par = blockIdx.y;
+  if (h >= Vh) return;
+  const SiteCoord c = site_coords(L, par, h);
+  double st = 0.0, ss = 0.0;
+  for (int mu = 0; mu < 3; mu++)
+    for (int nu = mu + 1; nu < 4; nu++) {
+      int x1 = c.x, y1 = c.y, z1 = c.z, t1 = c.t;
+      shift_coord(L, x1, y1, z1, t1, mu, +1);
+      int x2 = c.x, y2 = c.y, z2 = c.z, t2 = c.t;
+      shift_coord(L, x2, y2, z2, t2, nu, +1);
+      Mat<double> a = mmul(link_at(U, L, c.x, c.y, c.z, c.t, mu), link_at(U, L, x1, y1, z1, t1, nu));
+      a = mmul_nd(a, link_at(U, L, x2, y2, z2, t2, mu));
+      a = mmul_nd(a, link_at(U, L, c.x, c.y, c.z, c.t, nu));
+      const double re = a.e[0][0].x + a.e[1][1].x + a.e[2][2].x;
+      if (nu == 3) st -= re;
+      else ss -= re;
+    }
+  const long idx = (long)par * Vh + h;
+  vt[idx] = st;
+  vs[idx] = ss;
+}
+
+// gaugemeas.cc:70-104 (Conf::calc_poly): one line per spatial point, from
+// the sites r < vol3 (even parity, t = 0 or 1), minus the trace
+__global__ __launch_bounds__(kBlock) void k_poly(LatticeDims L, const double2 *U, double *vre, double *vim,
+                                                 long vol3) {
+  const long r = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (r >= vol3) return;
+  const SiteCoord c = site_coords(L, 0, r);
+  Mat<double> a = link_at(U, L, c.x, c.y, c.z, c.t, 3);
+  int t = c.t;
+  for (int k = 1; k < L.nt; k++) {
+    t = t + 1 == L.nt ? 0 : t + 1;
+    a = mmul(a, link_at(U, L, c.x, c.y, c.z, t, 3));
+  }
+  vre[r] = -(a.e[0][0].x + a.e[1][1].x + a.e[2][2].x);
+  vim[r] = -(a.e[0][0].y + a.e[1][1].y + a.e[2][2].y);
+}
+
+int gauge_meas(Context &c, double *pls, double *plt, double *pre, double *pim) {
+  double *w = nullptr;
+  const long vol3 = (long)c.L.nx * c.L.ny * c.L.nz;
+  if (!hip_ok(c, hipMallocAsync((void **)&w, sizeof(double) * (2 * c.V + 2 * vol3), c.stream), "meas")) return -1;
+  double *vt = w, *vs = w + c.V, *vre = w + 2 * c.V, *vim = vre + vol3;
+  hipLaunchKernelGGL(k_plaq, dim3(nblk(c.Vh), 2), dim3(kBlock), 0, c.stream, c.L, c.U64, vt, vs);
+  hipLaunchKernelGGL(k_poly, dim3(nblk(vol3)), dim3(kBlock), 0, c.stream, c.L, c.U64, vre, vim, vol3);
+  pairwise_sum(c, vt, c.V);
+  pairwise_sum(c, vs, c.V);
+  pairwise_sum(c, vre, vol3);
+  pairwise_sum(c, vim, vol3);
+  double h[4];
+  hipMemcpyAsync(&h[0], vt, sizeof(double), hipMemcpyDeviceToHost, c.stream);
+  hipMemcpyAsync(&h[1], vs, sizeof(double), hipMemcpyDeviceToHost, c.stream);
+  hipMemcpyAsync(&h[2], vre, sizeof(double), hipMemcpyDeviceToHost, c.stream);
+  hipMemcpyAsync(&h[3], vim, sizeof(double), hipMemcpyDeviceToHost, c.stream);
+  hipFreeAsync(w, c.stream);
+  if (!hip_ok(c, hipStreamSynchronize(c.stream), "meas")) return -1;
+  // the reference's normalisation order: sum * (1/V/3.) * one_by_three
+  const double inv_size_by_three = 1.0 / (double)c.V / 3.;
+  const double inv_vol3 = 1.0 / (double)vol3;
+  const double one_by_three = 0.3333333333333333333333333333333333333333333333333333333333333333333;
+  *plt = h[0] * inv_size_by_three * one_by_three;
+  *pls = h[1] * inv_size_by_three * one_by_three;
+  *pre = h[2] * inv_vol3 * one_by_three;
+  *pim = h[3] * inv_vol3 * one_by_three;
+  return 0;
 }
 
 // --------------------------------------------------------------------- RNG

@@ -521,6 +521,159 @@ int build_tile_order(Context &c) {
   return hip_ok(c, hipStreamSynchronize(c.stream), "tile order") ? 0 : -1;
 }
 
+// ---- lane-split Dslash: four lanes per site, lane q takes direction mu = q
+// (forward and backward hop), the partial sums are combined across the quad
+// with DPP moves and lanes 0-2 write one colour each.  Four times the waves
+// of the one-lane form at the same right-hand sides per lane, with each link
+// still loaded once per site for all NR right-hand sides.
+__device__ __forceinline__ int dpp_quad_xor1(int v) { return __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, false); }
+__device__ __forceinline__ int dpp_quad_xor2(int v) { return __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, false); }
+__device__ __forceinline__ float quad_sum(float v) {
+  v += __int_as_float(dpp_quad_xor1(__float_as_int(v)));
+  v += __int_as_float(dpp_quad_xor2(__float_as_int(v)));
+  return v;
+}
+__device__ __forceinline__ double quad_xor(double v, bool two) {
+  const long long b = __double_as_longlong(v);
+  int lo = (int)(b & 0xffffffffLL), hi = (int)(b >> 32);
+  lo = two ? dpp_quad_xor2(lo) : dpp_quad_xor1(lo);
+  hi = two ? dpp_quad_xor2(hi) : dpp_quad_xor1(hi);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ double quad_sum(double v) {
+  v += quad_xor(v, false);
+  v += quad_xor(v, true);
+  return v;
+}
+
+constexpr int kLsSites = kBlock / 4;  // sites per workgroup pass
+
+// value select among three (by value, so that the accumulators stay in
+// registers instead of an indexed private array)
+template <typename C>
+__device__ __forceinline__ C pick3(C a0, C a1, C a2, int k) {
+  C r;
+  r.x = k == 0 ? a0.x : (k == 1 ? a1.x : a2.x);
+  r.y = k == 0 ? a0.y : (k == 1 ? a1.y : a2.y);
+  return r;
+}
+
+template <typename R, int LINK, int NR, int MODE, int RED>
+__global__ __launch_bounds__(kBlock) void k_dslash_ls(DsArgs a) {
+  using C = Cx<R>;
+  const LatticeDims &L = a.L;
+  const long Vh = L.Vh;
+  const int Vh32 = (int)Vh;
+  unsigned sb, ch;
+  xcd_split(blockIdx.x, a.nsb, a.nch, sb, ch);
+  const int jbase = ch * NR;
+  const int nunits = a.ntiles * (kBlock / kLsSites);
+  const int per = (nunits + (int)a.nsb - 1) / (int)a.nsb;
+  const int q0 = (int)sb * per;
+  const int q1 = q0 + per < nunits ? q0 + per : nunits;
+  const int mu = threadIdx.x & 3;
+  const int slot = threadIdx.x >> 2;
+
+  bool act[NR];
+  bool any = false;
+#pragma unroll
+  for (int r = 0; r < NR; r++) {
+    const int j = jbase + r;
+    act[r] = (j < a.nvec) && (a.skip == nullptr || !a.skip[(long)j * a.skip_stride]);
+    any |= act[r];
+  }
+  double red[NR];
+#pragma unroll
+  for (int r = 0; r < NR; r++) red[r] = 0.0;
+  const C *inr[NR];
+#pragma unroll
+  for (int r = 0; r < NR; r++) inr[r] = (const C *)a.in + (long)(jbase + r) * a.in_vs;
+
+  for (int q = q0; any && q < q1; q++) {
+    const int tile = a.tiles ? a.tiles[q >> 2] : (q >> 2);
+    const int h = tile * kBlock + (q & 3) * kLsSites + slot;
+    const bool valid = h < Vh32;
+    const int hs = valid ? h : 0;
+    C acc[NR][3];
+#pragma unroll
+    for (int r = 0; r < NR; r++)
+#pragma unroll
+      for (int k = 0; k < 3; k++) acc[r][k] = mk<R>(0, 0);
+    if (valid) {
+      const SiteCoord c = site_coords_i(L, a.par_out, hs);
+      const int pin_ = a.par_out, pnb = 1 - a.par_out;
+      {
+        const int hp = neighbour_i(L, c, mu, +1);
+        const R eta = (R)eta_phase(L, c.x, c.y, c.z, c.t, mu);
+        const Mat<R> U = ds_link<R, LINK>(a, mu, pin_, hs, eta);
+#pragma unroll
+        for (int r = 0; r < NR; r++) {
+          if (!act[r]) continue;
+          const C *v = inr[r] + hp;
+          const C v0 = v[0], v1 = v[Vh], v2 = v[2 * Vh];
+#pragma unroll
+          for (int i = 0; i < 3; i++) {
+            cfma(acc[r][i], U.e[i][0], v0);
+            cfma(acc[r][i], U.e[i][1], v1);
+            cfma(acc[r][i], U.e[i][2], v2);
+          }
+        }
+      }
+      {
+        const int hm = neighbour_i(L, c, mu, -1);
+        const int tm = (mu == 3) ? (c.t == 0 ? L.nt - 1 : c.t - 1) : c.t;
+        const R eta = (R)eta_phase(L, c.x, c.y, c.z, tm, mu);
+        const Mat<R> U = ds_link<R, LINK>(a, mu, pnb, hm, eta);
+#pragma unroll
+        for (int r = 0; r < NR; r++) {
+          if (!act[r]) continue;
+          const C *v = inr[r] + hm;
+          const C v0 = v[0], v1 = v[Vh], v2 = v[2 * Vh];
+#pragma unroll
+          for (int i = 0; i < 3; i++) {
+            cfmsc(acc[r][i], U.e[0][i], v0);
+            cfmsc(acc[r][i], U.e[1][i], v1);
+            cfmsc(acc[r][i], U.e[2][i], v2);
+          }
+        }
+      }
+    }
+    // all four lanes of every quad take part in the exchange
+#pragma unroll
+    for (int r = 0; r < NR; r++)
+#pragma unroll
+      for (int i = 0; i < 3; i++) {
+        acc[r][i].x = quad_sum(acc[r][i].x);
+        acc[r][i].y = quad_sum(acc[r][i].y);
+      }
+    if (!valid || mu == 3) continue;
+    const int col = mu;  // lanes 0-2: one colour each
+#pragma unroll
+    for (int r = 0; r < NR; r++) {
+      if (!act[r]) continue;
+      const int j = jbase + r;
+      const C av = pick3(acc[r][0], acc[r][1], acc[r][2], col);
+      C res;
+      if (MODE == DS_HOP) {
+        res = cscale(av, (R)0.5);
+      } else {
+        const C pv = ((const C *)a.pin + (long)j * a.pin_vs + h)[(long)col * Vh];
+        const R d = (R)a.diag[j];
+        res.x = fmar(d, pv.x, (R)-0.5 * av.x);
+        res.y = fmar(d, pv.y, (R)-0.5 * av.y);
+        if (MODE == DS_RESID) {
+          const double2 bv = a.b[(long)(j / a.b_group) * a.b_vs + (long)col * Vh + h];
+          res = csub(mk<R>((R)bv.x, (R)bv.y), res);
+        }
+        if (RED == RED_DOT) red[r] += (double)pv.x * (double)res.x + (double)pv.y * (double)res.y;
+      }
+      ((C *)a.out + (long)j * a.out_vs + h)[(long)col * Vh] = res;
+      if (RED == RED_NORM) red[r] += (double)res.x * (double)res.x + (double)res.y * (double)res.y;
+    }
+  }
+  if (RED != RED_NONE) block_partials<NR>(red, a.red.partial, jbase, a.nvec, a.nsb, sb);
+}
+
 // number of right-hand sides handled per thread for a launch with nvec vectors
 static int pick_nr(int nvec) {
   if (nvec >= 8) return 8;
@@ -541,15 +694,29 @@ static unsigned reduce_blocks(Context &c, int ny) {
   return want < nbx ? want : nbx;
 }
 
+// launch shape code: rhs per thread (1,2,4,8) + 16 * lanes per site (1 or 4)
+static inline int shape_nr(int code) { return code & 15; }
+static inline int shape_lanes(int code) { return code >> 4; }
+
 template <typename R, int LINK, int MODE, int RED>
-static void dslash_launch(Context &c, DsArgs &a, int nr) {
+static void dslash_launch(Context &c, DsArgs &a, int code) {
+  const int nr = shape_nr(code), lanes = shape_lanes(code) == 4 ? 4 : 1;
   a.L = c.L;
   a.tiles = c.tiles;
   a.ntiles = c.ntiles;
   a.nch = (unsigned)((a.nvec + nr - 1) / nr);
-  a.nsb = RED == RED_NONE ? nbx_of(c) : reduce_blocks(c, a.nch);
+  a.nsb = RED == RED_NONE ? nbx_of(c) * lanes : reduce_blocks(c, a.nch);
   a.red.nbr = (int)a.nsb;
   dim3 grid(a.nsb * a.nch);
+  if (lanes == 4) {
+    switch (nr) {
+      case 8: hipLaunchKernelGGL((k_dslash_ls<R, LINK, 8, MODE, RED>), grid, dim3(kBlock), 0, c.stream, a); break;
+      case 4: hipLaunchKernelGGL((k_dslash_ls<R, LINK, 4, MODE, RED>), grid, dim3(kBlock), 0, c.stream, a); break;
+      case 2: hipLaunchKernelGGL((k_dslash_ls<R, LINK, 2, MODE, RED>), grid, dim3(kBlock), 0, c.stream, a); break;
+      default: hipLaunchKernelGGL((k_dslash_ls<R, LINK, 1, MODE, RED>), grid, dim3(kBlock), 0, c.stream, a); break;
+    }
+    return;
+  }
   switch (nr) {
     case 8: hipLaunchKernelGGL((k_dslash<R, LINK, 8, MODE, RED>), grid, dim3(kBlock), 0, c.stream, a); break;
     case 4: hipLaunchKernelGGL((k_dslash<R, LINK, 4, MODE, RED>), grid, dim3(kBlock), 0, c.stream, a); break;
@@ -558,12 +725,13 @@ static void dslash_launch(Context &c, DsArgs &a, int nr) {
   }
 }
 
-// Autotuner: the number of right-hand sides per thread trades link reuse in
+// Autotuner: the launch shape -- right-hand sides per thread (1, 2, 4, 8) and
+// lanes per site (1, or 4 with the lane-split kernel) -- trades link reuse in
 // registers against waves in flight (a single 8-RHS thread per site leaves a
 // 32^3x8 half lattice at ~2 waves per SIMD).  On the first launch of each
 // (precision, links, mode, reduction, nvec) every candidate is timed in place
 // with HIP events (these launches are idempotent) and the fastest is kept for
-// the lifetime of the context.  RHMC_DSLASH_NR overrides the choice.
+// the lifetime of the context.  RHMC_DSLASH_NR ("4" or "4x4") overrides it.
 template <typename R, int LINK, int MODE, int RED>
 static void dslash_dispatch(Context &c, DsArgs &a) {
   const unsigned long long key = ((unsigned long long)sizeof(R) << 40) | ((unsigned long long)LINK << 36) |
@@ -575,28 +743,32 @@ static void dslash_dispatch(Context &c, DsArgs &a) {
   if (it != tab.end()) {
     nr = it->second;
   } else if (const char *env = getenv("RHMC_DSLASH_NR")) {
-    nr = std::max(1, std::min(8, atoi(env)));
-    nr = nr >= 8 ? 8 : nr >= 4 ? 4 : nr >= 2 ? 2 : 1;
+    // "<rhs per thread>" or "<rhs per thread>x4" for four lanes per site
+    int v = std::max(1, std::min(8, atoi(env)));
+    v = v >= 8 ? 8 : v >= 4 ? 4 : v >= 2 ? 2 : 1;
+    nr = (strchr(env, 'x') ? 64 : 16) + v;
     tab[key] = nr;
   } else {
     hipEvent_t e0, e1;
     hipEventCreate(&e0);
     hipEventCreate(&e1);
     float best = 1e30f;
-    nr = pick_nr(a.nvec);
-    for (int cand = 1; cand <= 8 && cand <= std::max(1, a.nvec); cand *= 2) {
-      dslash_launch<R, LINK, MODE, RED>(c, a, cand);
-      hipEventRecord(e0, c.stream);
-      for (int k = 0; k < 3; k++) dslash_launch<R, LINK, MODE, RED>(c, a, cand);
-      hipEventRecord(e1, c.stream);
-      hipEventSynchronize(e1);
-      float t = 0.f;
-      hipEventElapsedTime(&t, e0, e1);
-      if (t < best) {
-        best = t;
-        nr = cand;
+    nr = 16 + pick_nr(a.nvec);
+    for (int lanes = 1; lanes <= 4; lanes *= 4)
+      for (int v = 1; v <= 8 && v <= std::max(1, a.nvec); v *= 2) {
+        const int cand = lanes * 16 + v;
+        dslash_launch<R, LINK, MODE, RED>(c, a, cand);
+        hipEventRecord(e0, c.stream);
+        for (int k = 0; k < 3; k++) dslash_launch<R, LINK, MODE, RED>(c, a, cand);
+        hipEventRecord(e1, c.stream);
+        hipEventSynchronize(e1);
+        float t = 0.f;
+        hipEventElapsedTime(&t, e0, e1);
+        if (t < best) {
+          best = t;
+          nr = cand;
+        }
       }
-    }
     hipEventDestroy(e0);
     hipEventDestroy(e1);
     tab[key] = nr;
@@ -646,11 +818,13 @@ std::string tuning_table(Context &c) {
   std::string out = "[";
   bool first = true;
   for (auto &kv : c.ws->tuned_nr) {
-    char buf[160];
+    char buf[200];
     const unsigned long long k = kv.first;
-    snprintf(buf, sizeof buf, "%s{\"real_bytes\":%llu,\"links\":\"%s\",\"mode\":%llu,\"reduction\":%llu,\"nvec\":%llu,\"rhs_per_thread\":%d}",
+    snprintf(buf, sizeof buf,
+             "%s{\"real_bytes\":%llu,\"links\":\"%s\",\"mode\":%llu,\"reduction\":%llu,\"nvec\":%llu,"
+             "\"rhs_per_thread\":%d,\"lanes_per_site\":%d}",
              first ? "" : ",", k >> 40, ((k >> 36) & 15) ? "fp32_12" : "fp64_18", (k >> 32) & 15, (k >> 28) & 15,
-             k & 0xfffffff, kv.second);
+             k & 0xfffffff, shape_nr(kv.second), shape_lanes(kv.second));
     out += buf;
     first = false;
   }

@@ -106,6 +106,7 @@ def _declare(L):
         "gpu_write_conf": (I, [VP, C.c_char_p, D, D, D, I]),
         "gpu_read_conf": (I, [VP, C.c_char_p, PI, PD, PD, PD]),
         "gpu_time_dslash_cold": (I, [VP, I, I, I, D, PD]),
+        "gpu_gauge_meas": (I, [VP, PD, PD, PD, PD]),
         "gpu_kernel_timing": (I, [VP, I, PD, C.POINTER(C.c_longlong)]),
         "gpu_fermion_action": (I, [VP, D, I, I, VP, D, VP, I, I, PD]),
         "gpu_find_max": (I, [VP, PD, D, U64]),
@@ -280,6 +281,12 @@ class Gpu:
         self._check(self.L.gpu_time_dslash_cold(self.h, int(precision), int(nvec), int(reps), float(flush_bytes),
                                                 C.byref(v)), "gpu_time_dslash_cold")
         return v.value
+
+    def gauge_meas(self):
+        """(plaq_spatial, plaq_temporal, poly_re, poly_im) of the device links."""
+        v = [C.c_double() for _ in range(4)]
+        self._check(self.L.gpu_gauge_meas(self.h, *[C.byref(x) for x in v]), "gpu_gauge_meas")
+        return tuple(x.value for x in v)
 
     def write_conf(self, path, beta, mass, n_flavours, update_iteration):
         """Device links to a configuration file in the reference's text format."""

@@ -240,12 +240,17 @@ def _generate(n: int, pnum: int, pden: int, lo: float, inverse: bool, dps: int, 
 
 
 def remez_power(n: int, pnum: int, pden: int, lam_low: float = 1e-5, inverse: bool = False,
-                dps: int = 60, tol: float = 1e-15) -> RationalApprox:
+                dps: int | None = None, tol: float = 1e-15) -> RationalApprox:
     """(n,n) minimax approximation of x^(pnum/pden) on [lam_low, 1].
 
     inverse=False: partial fractions of r ~ x^(pnum/pden)   (getPFE,  alg_remez.cc:205-245)
     inverse=True : partial fractions of 1/r ~ x^(-pnum/pden) (getIPFE, alg_remez.cc:249-290)
+
+    The monomial basis on [lam_low, 1] spans ~|log10 lam_low| * n decades, so
+    the working precision grows with the order (dps = max(60, 6n) unless given).
     """
+    if dps is None:
+        dps = max(60, 6 * n)
     g = Fraction(pnum, pden)
     c, a, b, err, meta = _generate(n, g.numerator, g.denominator, float(lam_low), bool(inverse), dps, tol)
     sign = -1 if inverse else 1

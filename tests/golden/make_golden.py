@@ -126,12 +126,15 @@ def make(cfg_name):
             run(exe, d, "action")
             out["action0"] = rd(d, "out_action0.bin", (V,), np.float64)
             out["action1"] = rd(d, "out_action1.bin", (V,), np.float64)
+        run(exe, d, "meas_gauge")
+        out["meas_gauge"] = np.fromfile(os.path.join(d, "out_meas.bin"), np.float64)
         if cfg_name == "t4":
             # the reference's text configuration writer and reader
             run(exe, d, "conf_write", 7)
             with open(os.path.join(d, "config"), "rb") as fh:
                 txt = fh.read()
-            with gzip.open(os.path.join(HERE, "config_t4.txt.gz"), "wb") as fh:
+            with open(os.path.join(HERE, "config_t4.txt.gz"), "wb") as raw, \
+                    gzip.GzipFile(fileobj=raw, mode="wb", mtime=0) as fh:
                 fh.write(txt)
             run(exe, d, "conf_read")
             out["conf_read_links"] = rd(d, "out_links.bin", (4 * V, 3, 3))

@@ -361,3 +361,15 @@ def test_chain_checkpoint_and_resume(tmp_path):
     # the restart reads the 16-digit text: same chain to that precision
     assert rel(U4b, U4) < 1e-12
     g.close()
+
+
+@pytest.mark.parametrize("name", ["t4", "t4o"])
+def test_gauge_measurements_reference(name):
+    """gpu_gauge_meas against the reference's Conf::calc_plaq / calc_poly."""
+    gd = dict(np.load(os.path.join(GOLD, f"golden_{name}.npz")))
+    g = Gpu(tuple(gd["dims"]), mass=float(gd["mass"]), beta=float(gd["beta"]), n_pf=1, max_order=4)
+    g.init0(gd["links"])
+    v = g.gauge_meas()
+    say("gauge meas", v, gd["meas_gauge"])
+    np.testing.assert_allclose(v, gd["meas_gauge"], rtol=1e-13, atol=1e-15)
+    g.close()

@@ -137,3 +137,23 @@ def test_c1_configuration():
     assert rel(P1[sel], g["traj_mom_sel"]) < 1e-10
     assert rel(phin, g["traj_phinew"]) < 1e-9
     assert abs(dh - float(g["traj_dh"])) < 1e-8
+
+
+@pytest.mark.parametrize("name", ["t4", "t4o", "c1"])
+def test_gauge_measurements(name):
+    """Plaquettes and Polyakov loop (gaugemeas.cc:6-104) against the
+    reference's Conf::calc_plaq / calc_poly, and the unit configuration
+    U = eta * 1, where both plaquettes and Re P are exactly 1."""
+    g = load(name)
+    dims = tuple(g["dims"])
+    o = Oracle(dims)
+    if "links" in g:
+        U = g["links"]
+    else:
+        from rhmc_amd import fields
+        U = fields.random_links(fields.Lattice(*dims), 101, None)
+    np.testing.assert_allclose(o.gauge_meas(U), g["meas_gauge"], rtol=1e-13, atol=1e-15)
+    from rhmc_amd import fields
+    lat = fields.Lattice(*dims)
+    unit = (np.eye(3)[None] * lat.eta()[:, None, None]).astype(np.complex128)
+    np.testing.assert_allclose(o.gauge_meas(unit), [1.0, 1.0, 1.0, 0.0], atol=1e-14)
