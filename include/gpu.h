@@ -146,6 +146,16 @@ int gpu_dslash(rhmc_gpu_t h, int par_out, int precision, int nvec, const double 
  * normalisation. */
 int gpu_gauge_meas(rhmc_gpu_t h, double *plaq_spatial, double *plaq_temporal, double *poly_re, double *poly_im);
 
+/* lib/Measurements/chiralmeas.cc:18-205 for one Z2 noise vector, without an
+ * imaginary chemical potential: chi_e = (MdagM)^-1 (m rnd_e - D_eo rnd_o)
+ * (FP64 CG to the absolute residual res), chi_o = (rnd_o - D_oe chi_e)/m;
+ * out[8] = chiral condensate, energy, baryon and pressure densities as
+ * (re, im) pairs with the reference's normalisation.  rnd_e/rnd_o: host
+ * noise vectors (V/2 x 3 complex each), or both NULL to draw Z2 noise on the
+ * device from seed. */
+int gpu_chiral_meas(rhmc_gpu_t h, const double *rnd_e, const double *rnd_o, unsigned long long seed, double res,
+                    int max_cg, double n_flavours, double *out, int *iterations);
+
 /* Timing helper: run `reps` device-resident half-lattice Dslash launches on
  * nvec vectors and return the mean kernel time in ms, measured with HIP
  * events on the library's stream. */

@@ -161,6 +161,16 @@ class Oracle:
         self.L.orc_polyakov(C.byref(self.lat), _p(U), C.byref(c), C.byref(d))
         return a.value, b.value, c.value, d.value
 
+    def chiral_meas(self, U, mass, n_flavours, rnd_e, rnd_o, res, max_cg=100000):
+        """chiralmeas.cc:18-205 for one noise vector: complex (chiral, e_dens,
+        b_dens, p_dens)."""
+        out = np.zeros(8)
+        self.L.orc_chiral_meas(C.byref(self.lat), _p(np.ascontiguousarray(U, np.complex128)), C.c_double(mass),
+                               C.c_double(n_flavours), _p(np.ascontiguousarray(rnd_e, np.complex128)),
+                               _p(np.ascontiguousarray(rnd_o, np.complex128)), C.c_double(res), C.c_int(max_cg),
+                               _p(out))
+        return out[0::2] + 1j * out[1::2]
+
     def global_sum(self, v):
         v = np.array(v, np.float64)
         return self.L.orc_global_sum(_p(v), C.c_long(len(v)))

@@ -43,6 +43,7 @@
 #undef private
 
 void multips_shifted_invert(ShiftMultiFermion *out, MultiFermion *in, REAL res, RationalApprox approx);
+void chiral_meas(complex<REAL> &chiral, complex<REAL> &e_dens, complex<REAL> &b_dens, complex<REAL> &p_dens);
 void update_nometro();
 void fermionforce(void);
 
@@ -203,6 +204,17 @@ int main(int argc, char **argv) {
     gauge_conf->calc_poly(pr, pi);
     double v[4] = {ps, pt, pr, pi};
     write_raw("out_meas.bin", v, sizeof v);
+  } else if (op == "meas_chiral") {
+    // chiral_meas (chiralmeas.cc:18-205), rand_vect = 1: dumps the Z2 noise
+    // it drew and its four complex results
+    chi_e = new Fermion; chi_o = new Fermion; phi_e = new Fermion; phi_o = new Fermion;
+    rnd_e = new Fermion; rnd_o = new Fermion;
+    complex<REAL> ch, ed, bd, pd;
+    chiral_meas(ch, ed, bd, pd);
+    double v[8] = {ch.real(), ch.imag(), ed.real(), ed.imag(), bd.real(), bd.imag(), pd.real(), pd.imag()};
+    write_raw("out_chiral.bin", v, sizeof v);
+    write_raw("out_rnd_e.bin", rnd_e->fermion, sizeof(Vec3) * sizeh);
+    write_raw("out_rnd_o.bin", rnd_o->fermion, sizeof(Vec3) * sizeh);
   } else if (op == "conf_write") {
     // Conf::write_last (conf.cc:179-204) of the input links, update_iteration = arg
     update_iteration = (int)arg;

@@ -535,6 +535,98 @@ void orc_polyakov(const orc_lattice *L, const orc_c (*U)[3][3], double *re, doub
   free(d2);
 }
 
+/* chiralmeas.cc:18-205 for one Z2 noise vector (rnd_e, rnd_o), without the
+ * imaginary chemical potential: chi_e = (MdagM)^-1 (m rnd_e - D_eo rnd_o),
+ * chi_o = (rnd_o - D_oe chi_e)/m; out = [chiral, energy density, baryon
+ * density, pressure density] as (re, im) pairs.  Sums are the pairwise
+ * global_sum over the half lattice on real and imaginary parts. */
+static void c_sum_pair(double *re, double *im, long n, orc_c *out) {
+  const double a = orc_global_sum(re, n), b = orc_global_sum(im, n);
+  *out = a + I * b;
+}
+static orc_c c_dot(const vec3 a, const vec3 b) { /* c_scalprod, vec3.h:227-235 */
+  orc_c r = 0.0;
+  for (int c = 0; c < 3; c++) r += conj(a[c]) * b[c];
+  return r;
+}
+int orc_chiral_meas(const orc_lattice *L, const orc_c (*U)[3][3], double mass, double n_flavours,
+                    const orc_c (*rnd_e)[3], const orc_c (*rnd_o)[3], double res, int max_cg, double *out) {
+  const long V = L->V, Vh = L->Vh;
+  vec3 *phi = (vec3 *)malloc(sizeof(vec3) * Vh), *chi_e = (vec3 *)calloc(Vh, sizeof(vec3));
+  vec3 *chi_o = (vec3 *)malloc(sizeof(vec3) * Vh), *tmp = (vec3 *)malloc(sizeof(vec3) * Vh);
+  double *re = (double *)malloc(sizeof(double) * Vh), *im = (double *)malloc(sizeof(double) * Vh);
+  double *re2 = (double *)malloc(sizeof(double) * Vh), *im2 = (double *)malloc(sizeof(double) * Vh);
+  orc_deo(L, U, phi, rnd_o);
+  for (long i = 0; i < Vh; i++)
+    for (int c = 0; c < 3; c++) phi[i][c] = mass * rnd_e[i][c] - phi[i][c];
+  const int iters = orc_cg(L, U, mass, (const orc_c(*)[3])phi, res, max_cg, chi_e);
+  orc_doe(L, U, tmp, (const orc_c(*)[3])chi_e);
+  const double one_by_mass = 1.0 / mass;
+  for (long i = 0; i < Vh; i++)
+    for (int c = 0; c < 3; c++) chi_o[i][c] = one_by_mass * (rnd_o[i][c] - tmp[i][c]);
+  const double inv_size = 1.0 / (double)V;
+  orc_c chiral, sum1, sum2, e, b, pd;
+  for (long i = 0; i < Vh; i++) {
+    const orc_c v = c_dot(rnd_o[i], chi_o[i]) + c_dot(rnd_e[i], chi_e[i]);
+    re[i] = creal(v);
+    im[i] = cimag(v);
+  }
+  c_sum_pair(re, im, Vh, &sum1);
+  chiral = sum1 * (n_flavours * 0.25 * inv_size);
+  /* energy and baryon density: temporal hops */
+  for (long i = 0; i < Vh; i++) {
+    const long j = i + Vh;
+    vec3 ve, we, vo, wo;
+    long i2 = L->nnp[i * 4 + 3] - Vh;
+    mv(ve, U[3 * V + i], chi_o[i2]);
+    mv(we, U[3 * V + i], rnd_o[i2]);
+    orc_c p1 = c_dot(rnd_e[i], ve), p2 = c_dot(we, chi_e[i]);
+    orc_c c1 = p1 - p2, c2 = p1 + p2;
+    i2 = L->nnp[j * 4 + 3];
+    mv(vo, U[3 * V + j], chi_e[i2]);
+    mv(wo, U[3 * V + j], rnd_e[i2]);
+    p1 = c_dot(rnd_o[i], vo);
+    p2 = c_dot(wo, chi_o[i]);
+    c1 += p1 - p2;
+    c2 += p1 + p2;
+    re[i] = creal(c1); im[i] = cimag(c1);
+    re2[i] = creal(c2); im2[i] = cimag(c2);
+  }
+  c_sum_pair(re, im, Vh, &sum1);
+  c_sum_pair(re2, im2, Vh, &sum2);
+  e = sum1 * (0.5 * n_flavours * 0.25 * inv_size);
+  b = sum2 * (0.5 * n_flavours * 0.25 * inv_size);
+  /* pressure density: spatial hops */
+  for (long i = 0; i < Vh; i++) {
+    const long j = i + Vh;
+    orc_c c1 = 0.0;
+    for (int k = 0; k < 3; k++) {
+      vec3 ve, we, vo, wo;
+      long i2 = L->nnp[i * 4 + k] - Vh;
+      mv(ve, U[k * V + i], chi_o[i2]);
+      mv(we, U[k * V + i], rnd_o[i2]);
+      c1 += c_dot(rnd_e[i], ve);
+      c1 -= c_dot(we, chi_e[i]);
+      i2 = L->nnp[j * 4 + k];
+      mv(vo, U[k * V + j], chi_e[i2]);
+      mv(wo, U[k * V + j], rnd_e[i2]);
+      c1 += c_dot(rnd_o[i], vo);
+      c1 -= c_dot(wo, chi_o[i]);
+    }
+    re[i] = creal(c1);
+    im[i] = cimag(c1);
+  }
+  c_sum_pair(re, im, Vh, &sum1);
+  pd = sum1 * (0.5 * n_flavours * 0.25 * inv_size);
+  const orc_c o4[4] = {chiral, e, b, pd};
+  for (int k = 0; k < 4; k++) {
+    out[2 * k] = creal(o4[k]);
+    out[2 * k + 1] = cimag(o4[k]);
+  }
+  free(phi); free(chi_e); free(chi_o); free(tmp); free(re); free(im); free(re2); free(im2);
+  return iters;
+}
+
 /* findminmax.cc:12-72 */
 int orc_findmaxeig(const orc_lattice *L, const orc_c (*U)[3][3], double mass, const orc_c (*start)[3],
                    double tol, double *max) {

@@ -97,6 +97,11 @@ void orc_momenta_sum_multiply(const orc_lattice *lat, orc_c (*P)[3][3],
                               const orc_c (*ipdot)[3][3], double step);
 
 /* action.cc:6-156 per-site action accumulation (init 0: set, -1: subtract) */
+/* chiralmeas.cc:18-205, one Z2 noise vector, no imaginary chemical
+ * potential; out[8] = (chiral, e_dens, b_dens, p_dens) as (re, im); returns
+ * the CG iterations */
+int orc_chiral_meas(const orc_lattice *lat, const orc_c (*U)[3][3], double mass, double n_flavours,
+                    const orc_c (*rnd_e)[3], const orc_c (*rnd_o)[3], double res, int max_cg, double *out);
 /* gaugemeas.cc:6-104: plaquettes and Polyakov loop of the stored links */
 void orc_plaquette(const orc_lattice *lat, const orc_c (*U)[3][3], double *pls, double *plt);
 void orc_polyakov(const orc_lattice *lat, const orc_c (*U)[3][3], double *re, double *im);

@@ -518,6 +518,13 @@ int gpu_gauge_meas(rhmc_gpu_t h, double *plaq_spatial, double *plaq_temporal, do
   return gauge_meas(h->c, plaq_spatial, plaq_temporal, poly_re, poly_im) ? RHMC_EHIP : RHMC_OK;
 }
 
+int gpu_chiral_meas(rhmc_gpu_t h, const double *rnd_e, const double *rnd_o, unsigned long long seed, double res,
+                    int max_cg, double n_flavours, double *out, int *iterations) {
+  if (!h || !out || (!rnd_e) != (!rnd_o) || res <= 0.0 || h->c.mass == 0.0) return RHMC_EINVAL;
+  hipSetDevice(h->c.device);
+  return solve_status(chiral_meas(h->c, rnd_e, rnd_o, seed, res, max_cg, n_flavours, out, iterations));
+}
+
 int gpu_kernel_timing(rhmc_gpu_t h, int enable, double *mean_ms, long long *count) {
   if (!h) return RHMC_EINVAL;
   Context &c = h->c;

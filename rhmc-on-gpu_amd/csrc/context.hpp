@@ -127,6 +127,11 @@ int momenta_update(Context &c, double step);
 int sol_sum(Context &c, double2 *io, const double2 *x, int nshift, double a0, const double *num);
 int action_local(Context &c, const double2 *phi, const double2 *chi, int with_fermions, int init);
 int action_sum(Context &c, double *out);
+// chiral condensate and energy / baryon / pressure densities for one Z2
+// noise vector (host arrays, or drawn on the device from seed when NULL)
+int chiral_meas(Context &c, const double *rnd_e, const double *rnd_o, unsigned long long seed, double res,
+                int max_cg, double n_flavours, double *out, int *iters);
+int upload_spinors(Context &c, const double *host, double2 *dst, int n);
 // plaquettes (spatial, temporal) and Polyakov loop of the device links
 int gauge_meas(Context &c, double *pls, double *plt, double *pre, double *pim);
 int gauss_momenta(Context &c, unsigned long long seed, unsigned long long stream_id);

@@ -436,6 +436,144 @@ __global__ __launch_bounds__(kBlock) void k_poly(LatticeDims L, const double2 *U
   vim[r] = -(a.e[0][0].y + a.e[1][1].y + a.e[2][2].y);
 }
 
+__device__ __forceinline__ uint4 philox4x32(uint4 ctr, uint2 key);
+
+// chiralmeas.cc:18-205 (one noise vector, no imaginary chemical potential)
+__global__ __launch_bounds__(kBlock) void k_axpby(double2 *out, const double2 *x, const double2 *y, double a,
+                                                  double b, long n) {
+  const long i = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const double2 u = x[i], v = y[i];
+  out[i] = make_double2(a * u.x + b * v.x, a * u.y + b * v.y);
+}
+
+__global__ __launch_bounds__(kBlock) void k_z2noise(double2 *out, long n, unsigned long long seed,
+                                                    unsigned long long stream) {
+  const long i = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const uint4 r = philox4x32(make_uint4((unsigned)i, (unsigned)(i >> 32), 7u, (unsigned)stream),
+                             make_uint2((unsigned)seed, (unsigned)(seed >> 32)));
+  out[i] = make_double2((r.x & 1u) ? -1.0 : 1.0, 0.0);
+}
+
+__device__ __forceinline__ double2 cdot3(const double2 *a, long as, const double2 *b, long bs) {
+  double2 r = make_double2(0.0, 0.0);
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    const double2 u = a[k * as], v = b[k * bs];
+    r.x += u.x * v.x + u.y * v.y;  // conj(u) * v
+    r.y += u.x * v.y - u.y * v.x;
+  }
+  return r;
+}
+// <w, U v> with U the link at (mu, par, h) and v the spinor at half index hn
+__device__ __forceinline__ double2 dot_link(const double2 *w, long ws, const double2 *U, long Vh, int mu, int par,
+                                            long h, const double2 *v, long hn) {
+  const Mat<double> m = load_link_d18(U, Vh, mu, par, h);
+  double2 r = make_double2(0.0, 0.0);
+#pragma unroll
+  for (int i = 0; i < 3; i++) {
+    double2 t = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int k = 0; k < 3; k++) cfma(t, m.e[i][k], v[k * Vh + hn]);
+    const double2 u = w[i * ws];
+    r.x += u.x * t.x + u.y * t.y;
+    r.y += u.x * t.y - u.y * t.x;
+  }
+  return r;
+}
+// <U w, v> = conj(<v, U w>)
+__device__ __forceinline__ double2 dot_link_left(const double2 *U, long Vh, int mu, int par, long h,
+                                                 const double2 *w, long hn, const double2 *v, long vs) {
+  const double2 r = dot_link(v, vs, U, Vh, mu, par, h, w, hn);
+  return make_double2(r.x, -r.y);
+}
+
+// per half index i: [chiral re, im, energy re, im, baryon re, im, pressure re, im]
+__global__ __launch_bounds__(kBlock) void k_chiral_terms(LatticeDims L, const double2 *U, const double2 *re_,
+                                                         const double2 *ro, const double2 *ce, const double2 *co,
+                                                         double *w) {
+  const long Vh = L.Vh;
+  const long i = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= Vh) return;
+  const SiteCoord ce_ = site_coords(L, 0, i), co_ = site_coords(L, 1, i);
+  const double2 a = cdot3(ro + i, Vh, co + i, Vh), b = cdot3(re_ + i, Vh, ce + i, Vh);
+  double v[8];
+  v[0] = a.x + b.x;
+  v[1] = a.y + b.y;
+  // temporal hops: even i -> odd neighbour, odd i -> even neighbour
+  {
+    const long ne = neighbour(L, ce_, 3, +1), no = neighbour(L, co_, 3, +1);
+    double2 p1 = dot_link(re_ + i, Vh, U, Vh, 3, 0, i, co, ne);
+    double2 p2 = dot_link_left(U, Vh, 3, 0, i, ro, ne, ce + i, Vh);
+    double2 c1 = make_double2(p1.x - p2.x, p1.y - p2.y), c2 = make_double2(p1.x + p2.x, p1.y + p2.y);
+    p1 = dot_link(ro + i, Vh, U, Vh, 3, 1, i, ce, no);
+    p2 = dot_link_left(U, Vh, 3, 1, i, re_, no, co + i, Vh);
+    c1.x += p1.x - p2.x; c1.y += p1.y - p2.y;
+    c2.x += p1.x + p2.x; c2.y += p1.y + p2.y;
+    v[2] = c1.x; v[3] = c1.y; v[4] = c2.x; v[5] = c2.y;
+  }
+  double2 c1 = make_double2(0.0, 0.0);
+  for (int k = 0; k < 3; k++) {
+    const long ne = neighbour(L, ce_, k, +1), no = neighbour(L, co_, k, +1);
+    double2 p = dot_link(re_ + i, Vh, U, Vh, k, 0, i, co, ne);
+    c1.x += p.x; c1.y += p.y;
+    p = dot_link_left(U, Vh, k, 0, i, ro, ne, ce + i, Vh);
+    c1.x -= p.x; c1.y -= p.y;
+    p = dot_link(ro + i, Vh, U, Vh, k, 1, i, ce, no);
+    c1.x += p.x; c1.y += p.y;
+    p = dot_link_left(U, Vh, k, 1, i, re_, no, co + i, Vh);
+    c1.x -= p.x; c1.y -= p.y;
+  }
+  v[6] = c1.x;
+  v[7] = c1.y;
+#pragma unroll
+  for (int k = 0; k < 8; k++) w[k * Vh + i] = v[k];
+}
+
+int chiral_meas(Context &c, const double *rnd_e_h, const double *rnd_o_h, unsigned long long seed, double res,
+                int max_cg, double n_flavours, double *out, int *iters) {
+  const long Vh = c.Vh, n3 = 3 * Vh;
+  double2 *buf = nullptr;
+  double *w = nullptr;
+  if (!hip_ok(c, hipMallocAsync((void **)&buf, sizeof(double2) * n3 * 6, c.stream), "chiral")) return -1;
+  if (!hip_ok(c, hipMallocAsync((void **)&w, sizeof(double) * 8 * Vh, c.stream), "chiral")) return -1;
+  double2 *re_ = buf, *ro = buf + n3, *phi = buf + 2 * n3, *ce = buf + 3 * n3, *co = buf + 4 * n3, *tmp = buf + 5 * n3;
+  if (rnd_e_h && rnd_o_h) {
+    if (upload_spinors(c, rnd_e_h, re_, 1) || upload_spinors(c, rnd_o_h, ro, 1)) return -1;
+  } else {
+    hipLaunchKernelGGL(k_z2noise, dim3(nblk(n3)), dim3(kBlock), 0, c.stream, re_, n3, seed, 0ull);
+    hipLaunchKernelGGL(k_z2noise, dim3(nblk(n3)), dim3(kBlock), 0, c.stream, ro, n3, seed, 1ull);
+  }
+  launch_hop_d(c, 0, ro, tmp, 1, n3, n3);  // D_eo rnd_o
+  hipLaunchKernelGGL(k_axpby, dim3(nblk(n3)), dim3(kBlock), 0, c.stream, phi, re_, tmp, c.mass, -1.0, n3);
+  const double shift0 = 0.0;
+  int it = 0;
+  const int rc = multishift_solve(c, phi, 1, 1, &shift0, res, 0, PREC_FP64, max_cg, ce, &it);
+  if (rc) {
+    hipFreeAsync(buf, c.stream);
+    hipFreeAsync(w, c.stream);
+    hipStreamSynchronize(c.stream);
+    return rc;
+  }
+  launch_hop_d(c, 1, ce, tmp, 1, n3, n3);  // D_oe chi_e
+  const double im = 1.0 / c.mass;
+  hipLaunchKernelGGL(k_axpby, dim3(nblk(n3)), dim3(kBlock), 0, c.stream, co, ro, tmp, im, -im, n3);
+  hipLaunchKernelGGL(k_chiral_terms, dim3(nblk(Vh)), dim3(kBlock), 0, c.stream, c.L, c.U64, re_, ro, ce, co, w);
+  for (int k = 0; k < 8; k++) pairwise_sum(c, w + k * Vh, Vh);
+  double h[8];
+  for (int k = 0; k < 8; k++) hipMemcpyAsync(&h[k], w + k * Vh, sizeof(double), hipMemcpyDeviceToHost, c.stream);
+  hipFreeAsync(buf, c.stream);
+  hipFreeAsync(w, c.stream);
+  if (!hip_ok(c, hipStreamSynchronize(c.stream), "chiral")) return -1;
+  const double inv_size = 1.0 / (double)c.V;
+  const double f0 = n_flavours * 0.25 * inv_size, f1 = 0.5 * n_flavours * 0.25 * inv_size;
+  out[0] = h[0] * f0; out[1] = h[1] * f0;
+  for (int k = 2; k < 8; k++) out[k] = h[k] * f1;
+  if (iters) *iters = it;
+  return 0;
+}
+
 int gauge_meas(Context &c, double *pls, double *plt, double *pre, double *pim) {
   double *w = nullptr;
   const long vol3 = (long)c.L.nx * c.L.ny * c.L.nz;

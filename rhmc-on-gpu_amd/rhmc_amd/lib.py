@@ -107,6 +107,7 @@ def _declare(L):
         "gpu_read_conf": (I, [VP, C.c_char_p, PI, PD, PD, PD]),
         "gpu_time_dslash_cold": (I, [VP, I, I, I, D, PD]),
         "gpu_gauge_meas": (I, [VP, PD, PD, PD, PD]),
+        "gpu_chiral_meas": (I, [VP, VP, VP, U64, D, I, D, PD, PI]),
         "gpu_kernel_timing": (I, [VP, I, PD, C.POINTER(C.c_longlong)]),
         "gpu_fermion_action": (I, [VP, D, I, I, VP, D, VP, I, I, PD]),
         "gpu_find_max": (I, [VP, PD, D, U64]),
@@ -287,6 +288,17 @@ class Gpu:
         v = [C.c_double() for _ in range(4)]
         self._check(self.L.gpu_gauge_meas(self.h, *[C.byref(x) for x in v]), "gpu_gauge_meas")
         return tuple(x.value for x in v)
+
+    def chiral_meas(self, n_flavours, res, rnd_e=None, rnd_o=None, seed=1, max_cg=100000):
+        """Complex (chiral condensate, energy, baryon, pressure density) for one
+        Z2 noise vector (given, or drawn on the device from seed)."""
+        out = np.zeros(8)
+        it = C.c_int()
+        a = None if rnd_e is None else np.ascontiguousarray(rnd_e, np.complex128)
+        b = None if rnd_o is None else np.ascontiguousarray(rnd_o, np.complex128)
+        self._check(self.L.gpu_chiral_meas(self.h, _ptr(a), _ptr(b), int(seed), float(res), int(max_cg),
+                                           float(n_flavours), _ptr(out), C.byref(it)), "gpu_chiral_meas")
+        return out[0::2] + 1j * out[1::2], it.value
 
     def write_conf(self, path, beta, mass, n_flavours, update_iteration):
         """Device links to a configuration file in the reference's text format."""

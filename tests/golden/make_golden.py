@@ -128,6 +128,11 @@ def make(cfg_name):
             out["action1"] = rd(d, "out_action1.bin", (V,), np.float64)
         run(exe, d, "meas_gauge")
         out["meas_gauge"] = np.fromfile(os.path.join(d, "out_meas.bin"), np.float64)
+        run(exe, d, "meas_chiral")
+        ch = np.fromfile(os.path.join(d, "out_chiral.bin"), np.float64)
+        out["meas_chiral"] = ch[0::2] + 1j * ch[1::2]
+        out["meas_rnd_e"] = rd(d, "out_rnd_e.bin", (Vh, 3))
+        out["meas_rnd_o"] = rd(d, "out_rnd_o.bin", (Vh, 3))
         if cfg_name == "t4":
             # the reference's text configuration writer and reader
             run(exe, d, "conf_write", 7)

@@ -373,3 +373,18 @@ def test_gauge_measurements_reference(name):
     say("gauge meas", v, gd["meas_gauge"])
     np.testing.assert_allclose(v, gd["meas_gauge"], rtol=1e-13, atol=1e-15)
     g.close()
+
+
+@pytest.mark.parametrize("name", ["t4", "t4o"])
+def test_chiral_measurements_reference(name):
+    """gpu_chiral_meas with the reference's Z2 noise against its chiral_meas."""
+    gd = dict(np.load(os.path.join(GOLD, f"golden_{name}.npz")))
+    g = Gpu(tuple(gd["dims"]), mass=float(gd["mass"]), beta=float(gd["beta"]), n_pf=1, max_order=4)
+    g.init0(gd["links"])
+    v, it = g.chiral_meas(float(gd["n_flavours"]), float(gd["res_metro"]), gd["meas_rnd_e"], gd["meas_rnd_o"])
+    say("chiral", v, gd["meas_chiral"], it)
+    np.testing.assert_allclose(v, gd["meas_chiral"], rtol=1e-11, atol=1e-12)
+    # device-drawn noise: Z2 entries, finite results
+    v2, _ = g.chiral_meas(float(gd["n_flavours"]), float(gd["res_metro"]), seed=3)
+    assert np.all(np.isfinite(v2))
+    g.close()

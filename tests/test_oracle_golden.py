@@ -157,3 +157,14 @@ def test_gauge_measurements(name):
     lat = fields.Lattice(*dims)
     unit = (np.eye(3)[None] * lat.eta()[:, None, None]).astype(np.complex128)
     np.testing.assert_allclose(o.gauge_meas(unit), [1.0, 1.0, 1.0, 0.0], atol=1e-14)
+
+
+@pytest.mark.parametrize("name", ["t4", "t4o"])
+def test_chiral_measurements(name):
+    """Chiral condensate, energy, baryon and pressure densities
+    (chiralmeas.cc:18-205) for the reference's own Z2 noise vector."""
+    g = load(name)
+    o = Oracle(tuple(g["dims"]))
+    v = o.chiral_meas(g["links"], float(g["mass"]), float(g["n_flavours"]), g["meas_rnd_e"], g["meas_rnd_o"],
+                      float(g["res_metro"]))
+    np.testing.assert_allclose(v, g["meas_chiral"], rtol=1e-11, atol=1e-12)
