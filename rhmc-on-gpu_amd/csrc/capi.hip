@@ -664,9 +664,11 @@ int gpu_trajectory(rhmc_gpu_t h, const rhmc_traj_params *p, rhmc_traj_result *r)
 
 int gpu_multi_trajectory(rhmc_gpu_t *hs, int n, const rhmc_traj_params *ps, rhmc_traj_result *rs, int *status) {
   if (!hs || !ps || !rs || n < 1) return RHMC_EINVAL;
-  for (int i = 0; i < n; i++)
+  for (int i = 0; i < n; i++) {
+    if (!hs[i]) return RHMC_EINVAL;
     for (int k = 0; k < i; k++)
-      if (!hs[i] || hs[i] == hs[k]) return RHMC_EINVAL;
+      if (hs[i] == hs[k]) return RHMC_EINVAL;
+  }
   // one host thread per lattice, each sequencing its own stream; the device
   // runs the lattices' kernels (and replayed solver graphs) concurrently
   std::vector<int> rc(n, RHMC_OK);

@@ -107,7 +107,7 @@ def _declare(L):
         "gpu_read_conf": (I, [VP, C.c_char_p, PI, PD, PD, PD]),
         "gpu_time_dslash_cold": (I, [VP, I, I, I, D, PD]),
         "gpu_gauge_meas": (I, [VP, PD, PD, PD, PD]),
-        "gpu_chiral_meas": (I, [VP, VP, VP, U64, D, I, D, PD, PI]),
+        "gpu_chiral_meas": (I, [VP, VP, VP, U64, D, I, D, VP, PI]),
         "gpu_kernel_timing": (I, [VP, I, PD, C.POINTER(C.c_longlong)]),
         "gpu_fermion_action": (I, [VP, D, I, I, VP, D, VP, I, I, PD]),
         "gpu_find_max": (I, [VP, PD, D, U64]),
