@@ -110,6 +110,7 @@ int gpu_create(const rhmc_gpu_params *p, rhmc_gpu_t *out) {
   c.U64 = (double2 *)dev_alloc(c, sizeof(double2) * 72 * Vh);
   c.U64_save = (double2 *)dev_alloc(c, sizeof(double2) * 72 * Vh);
   c.U32 = (float4 *)dev_alloc(c, sizeof(float4) * 24 * Vh);
+  c.U64c = (double2 *)dev_alloc(c, sizeof(double2) * 48 * Vh);
   c.P = (double2 *)dev_alloc(c, sizeof(double2) * 32 * Vh);
   c.ipdot = (double2 *)dev_alloc(c, sizeof(double2) * 32 * Vh);
   const size_t sp = sizeof(double2) * 3 * Vh;
@@ -119,7 +120,7 @@ int gpu_create(const rhmc_gpu_params *p, rhmc_gpu_t *out) {
   c.chi = (double2 *)dev_alloc(c, sp * c.n_pf);
   c.phinew = (double2 *)dev_alloc(c, sp * c.n_pf);
   c.action = (double *)dev_alloc(c, sizeof(double) * c.V);
-  if (!c.U64 || !c.U64_save || !c.U32 || !c.P || !c.ipdot || !c.mf || !c.smf || !c.phi0 || !c.chi || !c.phinew ||
+  if (!c.U64 || !c.U64_save || !c.U32 || !c.U64c || !c.P || !c.ipdot || !c.mf || !c.smf || !c.phi0 || !c.chi || !c.phinew ||
       !c.action) {
     gpu_destroy(h);
     return RHMC_EHIP;
@@ -136,7 +137,7 @@ int gpu_destroy(rhmc_gpu_t h) {
   Context &c = h->c;
   hipSetDevice(c.device);
   if (c.stream) hipStreamSynchronize(c.stream);
-  dev_free(c.U64); dev_free(c.U64_save); dev_free(c.U32); dev_free(c.P); dev_free(c.ipdot);
+  dev_free(c.U64); dev_free(c.U64_save); dev_free(c.U32); dev_free(c.U64c); dev_free(c.P); dev_free(c.ipdot);
   dev_free(c.mf); dev_free(c.smf); dev_free(c.phi0); dev_free(c.chi); dev_free(c.phinew); dev_free(c.action);
   dev_free(c.tiles);
   dev_free(c.smf32);

@@ -44,6 +44,7 @@ struct Context {
   // gauge field (eta folded in)
   double2 *U64 = nullptr;       // FP64 18-real
   float4 *U32 = nullptr;        // FP32 12-real copy, refreshed with U64
+  double2 *U64c = nullptr;      // FP64 12-real copy, refreshed with U64
   double2 *U64_save = nullptr;  // Metropolis backup
   // algebra fields (8 reals/link)
   double2 *P = nullptr;      // momenta (hermitian)
@@ -140,6 +141,8 @@ int refresh_links32(Context &c);
 int vec_norms2(Context &c, const double2 *v, int nvec, long vstride, double *out);
 // out[j] = Re<a_j, b_j> (fixed-order FP64 reduction)
 int vec_dots(Context &c, const double2 *a, const double2 *b, int nvec, long vstride, double *out);
+int vec_dot_diffs(Context &c, const double2 *a1, const double2 *b1, const double2 *a2, const double2 *b2, int nvec,
+                  long vs, double *out);
 
 // Dslash tile order for the lattice of c (solver.hip); allocates c.tiles
 int build_tile_order(Context &c);

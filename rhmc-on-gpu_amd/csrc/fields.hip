@@ -202,7 +202,7 @@ int gauge_force(Context &c, int accumulate) {
 //   U <- Pi_SU3[ exp6(i step P) ] U, then Gram-Schmidt and row 2 *= eta.
 // Also refreshes the FP32 12-real copy used by the mixed-precision Dslash.
 __global__ __launch_bounds__(kBlock) void k_exp_update(LatticeDims L, const double2 *P, double step, double2 *U,
-                                                       float4 *U32) {
+                                                       float4 *U32, double2 *U64c) {
   const long Vh = L.Vh;
   const int bx = xcd_block(blockIdx.x, gridDim.x);
   const long h = (long)bx * kBlock + threadIdx.x;
@@ -243,23 +243,28 @@ __global__ __launch_bounds__(kBlock) void k_exp_update(LatticeDims L, const doub
   for (int k = 0; k < 3; k++) Un.e[2][k] = cscale(Un.e[2][k], eta);
   store_link_d18(U, Vh, mu, par, h, Un);
   store_link_s12(U32, Vh, mu, par, h, Un);
+  store_link_d12(U64c, Vh, mu, par, h, Un);
 }
 
 int exp_update(Context &c, double step) {
-  hipLaunchKernelGGL(k_exp_update, dim3(nblk(c.Vh), 2, 4), dim3(kBlock), 0, c.stream, c.L, c.P, step, c.U64, c.U32);
+  hipLaunchKernelGGL(k_exp_update, dim3(nblk(c.Vh), 2, 4), dim3(kBlock), 0, c.stream, c.L, c.P, step, c.U64, c.U32,
+                     c.U64c);
   return hip_ok(c, hipGetLastError(), "exp update") ? 0 : -1;
 }
 
-__global__ __launch_bounds__(kBlock) void k_refresh32(LatticeDims L, const double2 *U, float4 *U32) {
+__global__ __launch_bounds__(kBlock) void k_refresh32(LatticeDims L, const double2 *U, float4 *U32, double2 *U64c) {
   const long Vh = L.Vh;
   const long h = (long)blockIdx.x * kBlock + threadIdx.x;
   const int par = blockIdx.y, mu = blockIdx.z;
   if (h >= Vh) return;
-  store_link_s12(U32, Vh, mu, par, h, load_link_d18(U, Vh, mu, par, h));
+  const Mat<double> m = load_link_d18(U, Vh, mu, par, h);
+  store_link_s12(U32, Vh, mu, par, h, m);
+  store_link_d12(U64c, Vh, mu, par, h, m);
 }
 
+// the FP32 12-real and FP64 12-real copies of the FP64 links
 int refresh_links32(Context &c) {
-  hipLaunchKernelGGL(k_refresh32, dim3(nblk(c.Vh), 2, 4), dim3(kBlock), 0, c.stream, c.L, c.U64, c.U32);
+  hipLaunchKernelGGL(k_refresh32, dim3(nblk(c.Vh), 2, 4), dim3(kBlock), 0, c.stream, c.L, c.U64, c.U32, c.U64c);
   return hip_ok(c, hipGetLastError(), "refresh links32") ? 0 : -1;
 }
 

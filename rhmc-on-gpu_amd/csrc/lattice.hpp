@@ -266,11 +266,31 @@ __device__ __forceinline__ void store_link_d18(double2 *__restrict__ U, long Vh,
   for (int c = 0; c < 9; c++) b[(long)c * Vh] = m.e[c / 3][c % 3];
 }
 
+// FP64 12-real: rows 0 and 1 as six double2 [(mu*2+par)*6 + k][Vh]; row 2 is
+// rebuilt as eta * conj(row0 x row1), exact to rounding for the unitarised
+// links (96 instead of 144 bytes per link)
+__device__ __forceinline__ void store_link_d12(double2 *__restrict__ U, long Vh, int mu, int par, long h,
+                                               const Mat<double> &m) {
+  double2 *b = U + (long)(mu * 2 + par) * 6 * Vh + h;
+#pragma unroll
+  for (int c = 0; c < 6; c++) b[(long)c * Vh] = m.e[c / 3][c % 3];
+}
+
 template <typename R>
 __device__ __forceinline__ void rebuild_row2(Mat<R> &m, R eta) {
   m.e[2][0] = cscale(cconj(csub(cmul(m.e[0][1], m.e[1][2]), cmul(m.e[0][2], m.e[1][1]))), eta);
   m.e[2][1] = cscale(cconj(csub(cmul(m.e[0][2], m.e[1][0]), cmul(m.e[0][0], m.e[1][2]))), eta);
   m.e[2][2] = cscale(cconj(csub(cmul(m.e[0][0], m.e[1][1]), cmul(m.e[0][1], m.e[1][0]))), eta);
+}
+
+__device__ __forceinline__ Mat<double> load_link_d12(const double2 *__restrict__ U, long Vh, int mu, int par, long h,
+                                                     double eta) {
+  Mat<double> m;
+  const double2 *b = U + (long)(mu * 2 + par) * 6 * Vh + h;
+#pragma unroll
+  for (int c = 0; c < 6; c++) m.e[c / 3][c % 3] = b[(long)c * Vh];
+  rebuild_row2(m, eta);
+  return m;
 }
 
 __device__ __forceinline__ Mat<float> load_link_s12(const float4 *__restrict__ U, long Vh, int mu, int par, long h,

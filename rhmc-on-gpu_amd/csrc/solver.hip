@@ -25,6 +25,7 @@
 #include <cstring>
 #include <map>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "context.hpp"
@@ -307,7 +308,7 @@ __global__ __launch_bounds__(kBlock) void k_finalize(RedArgs ra, int nvec, int *
 // ------------------------------------------------------------------ Dslash
 enum DsMode { DS_HOP = 0, DS_MDAGM = 1, DS_RESID = 2 };
 enum DsRed { RED_NONE = 0, RED_DOT = 1, RED_NORM = 2 };
-enum LinkFmt { LINK_D18 = 0, LINK_S12 = 1 };
+enum LinkFmt { LINK_D18 = 0, LINK_S12 = 1, LINK_D12 = 2 };
 
 struct DsArgs {
   LatticeDims L;
@@ -325,20 +326,144 @@ struct DsArgs {
   RedArgs red;
 };
 
+// A link as stored: rows 0-1 for the 12-real formats (row 2 is rebuilt when
+// the link is used, so the loads can be issued a hop ahead of the FMAs),
+// all nine entries for FP64-18.
 template <typename R, int LINK>
-__device__ __forceinline__ Mat<R> ds_link(const DsArgs &a, int mu, int par, long h, R eta);
+struct LinkRaw {
+  static constexpr int N = LINK == LINK_D18 ? 9 : LINK == LINK_D12 ? 6 : 3;
+  using T = typename std::conditional<LINK == LINK_S12, float4, double2>::type;
+  T e[N];
+};
 
-template <>
-__device__ __forceinline__ Mat<double> ds_link<double, LINK_D18>(const DsArgs &a, int mu, int par, long h, double) {
-  return load_link_d18((const double2 *)a.U, a.L.Vh, mu, par, h);
+// load at a workgroup-uniform base plus a 32-bit per-lane byte offset (one
+// offset register serves every load of a hop)
+template <typename T>
+__device__ __forceinline__ T ld_off(const T *base, unsigned byte_off) {
+  return *(const T *)((const char *)base + byte_off);
 }
-template <>
-__device__ __forceinline__ Mat<float> ds_link<float, LINK_S12>(const DsArgs &a, int mu, int par, long h, float eta) {
-  return load_link_s12((const float4 *)a.U, a.L.Vh, mu, par, h, eta);
+
+template <typename R, int LINK>
+__device__ __forceinline__ void link_raw_load(const DsArgs &a, int mu, int par, int h, LinkRaw<R, LINK> &w) {
+  using T = typename LinkRaw<R, LINK>::T;
+  constexpr int N = LinkRaw<R, LINK>::N;
+  const T *b = (const T *)a.U + (long)(mu * 2 + par) * N * a.L.Vh;
+  const unsigned off = (unsigned)h * (unsigned)sizeof(T);
+#pragma unroll
+  for (int k = 0; k < N; k++) w.e[k] = ld_off(b + (long)k * a.L.Vh, off);
+}
+
+template <typename R, int LINK>
+__device__ __forceinline__ Mat<R> link_raw_mat(const LinkRaw<R, LINK> &w, R eta) {
+  Mat<R> m;
+  if constexpr (LINK == LINK_S12) {
+    m.e[0][0] = make_float2(w.e[0].x, w.e[0].y);
+    m.e[0][1] = make_float2(w.e[0].z, w.e[0].w);
+    m.e[0][2] = make_float2(w.e[1].x, w.e[1].y);
+    m.e[1][0] = make_float2(w.e[1].z, w.e[1].w);
+    m.e[1][1] = make_float2(w.e[2].x, w.e[2].y);
+    m.e[1][2] = make_float2(w.e[2].z, w.e[2].w);
+    rebuild_row2(m, eta);
+  } else if constexpr (LINK == LINK_D12) {
+#pragma unroll
+    for (int k = 0; k < 6; k++) m.e[k / 3][k % 3] = w.e[k];
+    rebuild_row2(m, eta);
+  } else {
+#pragma unroll
+    for (int k = 0; k < 9; k++) m.e[k / 3][k % 3] = w.e[k];
+  }
+  return m;
+}
+
+// One hop's operands: the link and the neighbour's colour vectors of every
+// right-hand side of the chunk.
+template <typename R, int LINK, int NR>
+struct HopIn {
+  LinkRaw<R, LINK> U;
+  Cx<R> v[NR][3];
+};
+
+// hop k = 2 mu + (0 forward, 1 backward) of the site c (half index h):
+// forward  U_mu(x) psi(x+mu), link at x (output parity);
+// backward -U_mu(x-mu)^dag psi(x-mu), link at x-mu (the other parity)
+template <typename R, int LINK, int NR>
+__device__ __forceinline__ void hop_load(const DsArgs &a, const Cx<R> *const *inr, const SiteCoord &c, int h, int k,
+                                         HopIn<R, LINK, NR> &d) {
+  const int mu = k >> 1;
+  const bool fwd = (k & 1) == 0;
+  const int hn = neighbour_i(a.L, c, mu, fwd ? +1 : -1);
+  link_raw_load<R, LINK>(a, mu, fwd ? a.par_out : 1 - a.par_out, fwd ? h : hn, d.U);
+  const long Vh = a.L.Vh;
+  const unsigned off = (unsigned)hn * (unsigned)sizeof(Cx<R>);
+#pragma unroll
+  for (int r = 0; r < NR; r++) {
+    d.v[r][0] = ld_off(inr[r], off);
+    d.v[r][1] = ld_off(inr[r] + Vh, off);
+    d.v[r][2] = ld_off(inr[r] + 2 * Vh, off);
+  }
+}
+
+// Ends a hop: the accumulators pass through an empty asm statement (in-out
+// operands, memory clobber), so the hop's multiply-adds are issued before
+// the loads that follow it in program order instead of being sunk to the
+// end of the site with every hop's operands live at once.
+template <typename R, int NR>
+__device__ __forceinline__ void hop_fence(Cx<R> (&acc)[NR][3]) {
+#pragma unroll
+  for (int r = 0; r < NR; r++)
+#pragma unroll
+    for (int i = 0; i < 3; i++) asm volatile("" : "+v"(acc[r][i].x), "+v"(acc[r][i].y) : : "memory");
+}
+
+template <typename R, int LINK, int NR>
+__device__ __forceinline__ void hop_apply(const LatticeDims &L, const SiteCoord &c, int k, const HopIn<R, LINK, NR> &d,
+                                          Cx<R> (&acc)[NR][3]) {
+  const int mu = k >> 1;
+  if ((k & 1) == 0) {
+    const Mat<R> U = link_raw_mat<R, LINK>(d.U, (R)eta_phase(L, c.x, c.y, c.z, c.t, mu));
+#pragma unroll
+    for (int r = 0; r < NR; r++)
+#pragma unroll
+      for (int i = 0; i < 3; i++) {
+        cfma(acc[r][i], U.e[i][0], d.v[r][0]);
+        cfma(acc[r][i], U.e[i][1], d.v[r][1]);
+        cfma(acc[r][i], U.e[i][2], d.v[r][2]);
+      }
+  } else {
+    const int tm = (mu == 3) ? (c.t == 0 ? L.nt - 1 : c.t - 1) : c.t;
+    const Mat<R> U = link_raw_mat<R, LINK>(d.U, (R)eta_phase(L, c.x, c.y, c.z, tm, mu));
+#pragma unroll
+    for (int r = 0; r < NR; r++)
+#pragma unroll
+      for (int i = 0; i < 3; i++) {
+        cfmsc(acc[r][i], U.e[0][i], d.v[r][0]);
+        cfmsc(acc[r][i], U.e[1][i], d.v[r][1]);
+        cfmsc(acc[r][i], U.e[2][i], d.v[r][2]);
+      }
+  }
+}
+
+// device link array for a link format
+template <int LINK>
+static const void *link_ptr(Context &c) {
+  return LINK == LINK_D18 ? (const void *)c.U64 : LINK == LINK_D12 ? (const void *)c.U64c : (const void *)c.U32;
+}
+
+// Waves per SIMD the Dslash kernels are compiled for: the registers of two
+// hops in flight (link + NR neighbour vectors each), the NR accumulators and
+// ~32 for addresses and coordinates must fit the wave's share of the 512.
+template <typename R, int LINK, int NR>
+constexpr int ds_waves() {
+  constexpr int rs = (int)sizeof(R) / 4;
+  constexpr int link = LINK == LINK_D18 ? 36 : LINK == LINK_D12 ? 24 : 12;
+  constexpr int hop = link + NR * 6 * rs;
+  constexpr int need = 2 * hop + NR * 6 * rs + 32;
+  constexpr int w = 512 / need;
+  return w < 1 ? 1 : (w > 8 ? 8 : w);
 }
 
 template <typename R, int LINK, int NR, int MODE, int RED>
-__global__ __launch_bounds__(kBlock) void k_dslash(DsArgs a) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(ds_waves<R, LINK, NR>()))) void k_dslash(DsArgs a) {
   using C = Cx<R>;
   const LatticeDims &L = a.L;
   const long Vh = L.Vh;
@@ -363,58 +488,34 @@ __global__ __launch_bounds__(kBlock) void k_dslash(DsArgs a) {
   for (int r = 0; r < NR; r++) red[r] = 0.0;
 
   const int Vh32 = (int)Vh;
+  // every right-hand side of the chunk is loaded and multiplied without a
+  // branch, so the loads of a whole hop are in flight together; a slot past
+  // nvec reads the last vector and a stopped system is computed but not
+  // stored
   const C *inr[NR];
 #pragma unroll
-  for (int r = 0; r < NR; r++) inr[r] = (const C *)a.in + (long)(jbase + r) * a.in_vs;
+  for (int r = 0; r < NR; r++) {
+    const int j = jbase + r < a.nvec ? jbase + r : a.nvec - 1;
+    inr[r] = (const C *)a.in + (long)j * a.in_vs;
+  }
   for (int q = q0; any && q < q1; q++) {
     const int h = (a.tiles ? a.tiles[q] : q) * kBlock + (int)threadIdx.x;
     if (h >= Vh32) continue;
     const SiteCoord c = site_coords_i(L, a.par_out, h);
-    const int pin_ = a.par_out, pnb = 1 - a.par_out;
     C acc[NR][3];
 #pragma unroll
     for (int r = 0; r < NR; r++)
 #pragma unroll
       for (int k = 0; k < 3; k++) acc[r][k] = mk<R>(0, 0);
+    // software pipeline over the eight hops: the operands of hop k+1 are in
+    // flight while hop k is multiplied
+    HopIn<R, LINK, NR> buf[2];
+    hop_load<R, LINK, NR>(a, inr, c, h, 0, buf[0]);
 #pragma unroll
-    for (int mu = 0; mu < 4; mu++) {
-      // forward hop: U_mu(x) psi(x+mu)
-      {
-        const int hp = neighbour_i(L, c, mu, +1);
-        const R eta = (R)eta_phase(L, c.x, c.y, c.z, c.t, mu);
-        const Mat<R> U = ds_link<R, LINK>(a, mu, pin_, h, eta);
-#pragma unroll
-        for (int r = 0; r < NR; r++) {
-          if (!act[r]) continue;
-          const C *v = inr[r] + hp;
-          const C v0 = v[0], v1 = v[Vh], v2 = v[2 * Vh];
-#pragma unroll
-          for (int i = 0; i < 3; i++) {
-            cfma(acc[r][i], U.e[i][0], v0);
-            cfma(acc[r][i], U.e[i][1], v1);
-            cfma(acc[r][i], U.e[i][2], v2);
-          }
-        }
-      }
-      // backward hop: -U_mu(x-mu)^dag psi(x-mu)
-      {
-        const int hm = neighbour_i(L, c, mu, -1);
-        const int tm = (mu == 3) ? (c.t == 0 ? L.nt - 1 : c.t - 1) : c.t;
-        const R eta = (R)eta_phase(L, c.x, c.y, c.z, tm, mu);
-        const Mat<R> U = ds_link<R, LINK>(a, mu, pnb, hm, eta);
-#pragma unroll
-        for (int r = 0; r < NR; r++) {
-          if (!act[r]) continue;
-          const C *v = inr[r] + hm;
-          const C v0 = v[0], v1 = v[Vh], v2 = v[2 * Vh];
-#pragma unroll
-          for (int i = 0; i < 3; i++) {
-            cfmsc(acc[r][i], U.e[0][i], v0);
-            cfmsc(acc[r][i], U.e[1][i], v1);
-            cfmsc(acc[r][i], U.e[2][i], v2);
-          }
-        }
-      }
+    for (int k = 0; k < 8; k++) {
+      if (k + 1 < 8) hop_load<R, LINK, NR>(a, inr, c, h, k + 1, buf[(k + 1) & 1]);
+      hop_apply<R, LINK, NR>(L, c, k, buf[k & 1], acc);
+      hop_fence<R, NR>(acc);
     }
 #pragma unroll
     for (int r = 0; r < NR; r++) {
@@ -559,7 +660,7 @@ __device__ __forceinline__ C pick3(C a0, C a1, C a2, int k) {
 }
 
 template <typename R, int LINK, int NR, int MODE, int RED>
-__global__ __launch_bounds__(kBlock) void k_dslash_ls(DsArgs a) {
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(ds_waves<R, LINK, NR>()))) void k_dslash_ls(DsArgs a) {
   using C = Cx<R>;
   const LatticeDims &L = a.L;
   const long Vh = L.Vh;
@@ -585,9 +686,16 @@ __global__ __launch_bounds__(kBlock) void k_dslash_ls(DsArgs a) {
   double red[NR];
 #pragma unroll
   for (int r = 0; r < NR; r++) red[r] = 0.0;
+  // every right-hand side of the chunk is loaded and multiplied without a
+  // branch, so the loads of a whole hop are in flight together; a slot past
+  // nvec reads the last vector and a stopped system is computed but not
+  // stored
   const C *inr[NR];
 #pragma unroll
-  for (int r = 0; r < NR; r++) inr[r] = (const C *)a.in + (long)(jbase + r) * a.in_vs;
+  for (int r = 0; r < NR; r++) {
+    const int j = jbase + r < a.nvec ? jbase + r : a.nvec - 1;
+    inr[r] = (const C *)a.in + (long)j * a.in_vs;
+  }
 
   for (int q = q0; any && q < q1; q++) {
     const int tile = a.tiles ? a.tiles[q >> 2] : (q >> 2);
@@ -601,42 +709,11 @@ __global__ __launch_bounds__(kBlock) void k_dslash_ls(DsArgs a) {
       for (int k = 0; k < 3; k++) acc[r][k] = mk<R>(0, 0);
     if (valid) {
       const SiteCoord c = site_coords_i(L, a.par_out, hs);
-      const int pin_ = a.par_out, pnb = 1 - a.par_out;
-      {
-        const int hp = neighbour_i(L, c, mu, +1);
-        const R eta = (R)eta_phase(L, c.x, c.y, c.z, c.t, mu);
-        const Mat<R> U = ds_link<R, LINK>(a, mu, pin_, hs, eta);
-#pragma unroll
-        for (int r = 0; r < NR; r++) {
-          if (!act[r]) continue;
-          const C *v = inr[r] + hp;
-          const C v0 = v[0], v1 = v[Vh], v2 = v[2 * Vh];
-#pragma unroll
-          for (int i = 0; i < 3; i++) {
-            cfma(acc[r][i], U.e[i][0], v0);
-            cfma(acc[r][i], U.e[i][1], v1);
-            cfma(acc[r][i], U.e[i][2], v2);
-          }
-        }
-      }
-      {
-        const int hm = neighbour_i(L, c, mu, -1);
-        const int tm = (mu == 3) ? (c.t == 0 ? L.nt - 1 : c.t - 1) : c.t;
-        const R eta = (R)eta_phase(L, c.x, c.y, c.z, tm, mu);
-        const Mat<R> U = ds_link<R, LINK>(a, mu, pnb, hm, eta);
-#pragma unroll
-        for (int r = 0; r < NR; r++) {
-          if (!act[r]) continue;
-          const C *v = inr[r] + hm;
-          const C v0 = v[0], v1 = v[Vh], v2 = v[2 * Vh];
-#pragma unroll
-          for (int i = 0; i < 3; i++) {
-            cfmsc(acc[r][i], U.e[0][i], v0);
-            cfmsc(acc[r][i], U.e[1][i], v1);
-            cfmsc(acc[r][i], U.e[2][i], v2);
-          }
-        }
-      }
+      HopIn<R, LINK, NR> fw, bw;
+      hop_load<R, LINK, NR>(a, inr, c, hs, 2 * mu, fw);
+      hop_load<R, LINK, NR>(a, inr, c, hs, 2 * mu + 1, bw);
+      hop_apply<R, LINK, NR>(L, c, 2 * mu, fw, acc);
+      hop_apply<R, LINK, NR>(L, c, 2 * mu + 1, bw, acc);
     }
     // all four lanes of every quad take part in the exchange
 #pragma unroll
@@ -823,7 +900,9 @@ std::string tuning_table(Context &c) {
     snprintf(buf, sizeof buf,
              "%s{\"real_bytes\":%llu,\"links\":\"%s\",\"mode\":%llu,\"reduction\":%llu,\"nvec\":%llu,"
              "\"rhs_per_thread\":%d,\"lanes_per_site\":%d}",
-             first ? "" : ",", k >> 40, ((k >> 36) & 15) ? "fp32_12" : "fp64_18", (k >> 32) & 15, (k >> 28) & 15,
+             first ? "" : ",", k >> 40,
+             ((k >> 36) & 15) == LINK_S12 ? "fp32_12" : ((k >> 36) & 15) == LINK_D12 ? "fp64_12" : "fp64_18",
+             (k >> 32) & 15, (k >> 28) & 15,
              k & 0xfffffff, shape_nr(kv.second), shape_lanes(kv.second));
     out += buf;
     first = false;
@@ -1451,7 +1530,7 @@ static int ms_cg(Context &c, const double2 *b, int nrhs, int nshift, const doubl
   red_c.epi = EPI_MS_C;
 
   DsArgs hop = {};
-  hop.par_out = 1; hop.U = LINK == LINK_D18 ? (const void *)c.U64 : (const void *)c.U32;
+  hop.par_out = 1; hop.U = link_ptr<LINK>(c);
   hop.in = p; hop.in_vs = 3 * Vh; hop.out = hh; hop.out_vs = 3 * Vh; hop.nvec = nrhs;
   hop.skip = &ms[0].stop; hop.skip_stride = sizeof(MsState) / sizeof(int);
   DsArgs mdm = hop;
@@ -1538,7 +1617,7 @@ static int ms_cg_basis(Context &c, const double2 *b, int nrhs, int nshift, const
   hipLaunchKernelGGL(k_bms_init_fin, dim3(1), dim3(kBlock), 0, c.stream, a);
 
   DsArgs hop = {};
-  hop.par_out = 1; hop.U = LINK == LINK_D18 ? (const void *)c.U64 : (const void *)c.U32;
+  hop.par_out = 1; hop.U = link_ptr<LINK>(c);
   hop.in = p; hop.in_vs = 3 * Vh; hop.out = hh; hop.out_vs = 3 * Vh; hop.nvec = nrhs;
   hop.skip_stride = sizeof(BmsShared) / sizeof(int);
   DsArgs mdm = hop;
@@ -1762,8 +1841,8 @@ int multishift_solve(Context &c, const double2 *b, int nrhs, int nshift, const d
   c.last_iters_refine = 0;
   const bool explicit_shifts = c.ms_algorithm == 1 || getenv("RHMC_MS_EXPLICIT") != nullptr;
   if (prec == PREC_FP64) {
-    int rc = explicit_shifts ? 3 : ms_cg_basis<double, LINK_D18>(c, b, nrhs, nshift, shifts, res_abs.data(), max_iter, x, iters);
-    if (rc == 3) rc = ms_cg<double, LINK_D18>(c, b, nrhs, nshift, shifts, res_abs.data(), max_iter, x, iters);
+    int rc = explicit_shifts ? 3 : ms_cg_basis<double, LINK_D12>(c, b, nrhs, nshift, shifts, res_abs.data(), max_iter, x, iters);
+    if (rc == 3) rc = ms_cg<double, LINK_D12>(c, b, nrhs, nshift, shifts, res_abs.data(), max_iter, x, iters);
     c.last_iters = *iters;
     c.last_iters_fp32 = 0;
     return rc;
@@ -1878,6 +1957,39 @@ int vec_dots(Context &c, const double2 *a, const double2 *b, int nvec, long vs, 
   finalize(c, red, nvec, false);
   hipMemcpyAsync(out, sums, sizeof(double) * nvec, hipMemcpyDeviceToHost, c.stream);
   return hip_ok(c, hipStreamSynchronize(c.stream), "dots") ? 0 : -1;
+}
+
+// Re<a1,b1> - Re<a2,b2> per vector, differenced per element before the
+// sum: for two nearly equal dot products of large magnitude the result
+// carries the rounding of the (small) differences, not one ulp of each
+// product.
+__global__ __launch_bounds__(kBlock) void k_dot_diffs(const double2 *a1, const double2 *b1, const double2 *a2,
+                                                      const double2 *b2, long vs, long Vh, int nvec, RedArgs red) {
+  const unsigned bx = xcd_block(blockIdx.x, gridDim.x);
+  const int j = blockIdx.y;
+  long h0, h1;
+  site_range(Vh, bx, gridDim.x, h0, h1);
+  double s[1] = {0.0};
+  for (long h = h0 + threadIdx.x; h < h1; h += kBlock)
+    for (int i = 0; i < 3; i++) {
+      const long e = (long)j * vs + (long)i * Vh + h;
+      const double2 u1 = a1[e], v1 = b1[e], u2 = a2[e], v2 = b2[e];
+      s[0] += (u1.x * v1.x + u1.y * v1.y) - (u2.x * v2.x + u2.y * v2.y);
+    }
+  block_partials<1>(s, red.partial, j, nvec, gridDim.x, bx);
+}
+
+int vec_dot_diffs(Context &c, const double2 *a1, const double2 *b1, const double2 *a2, const double2 *b2, int nvec,
+                  long vs, double *out) {
+  if (nvec > kMaxRed) return -1;
+  double *sums = (double *)ws_get(c, S_SUMS, sizeof(double) * 256);
+  RedArgs red = make_red(c, nvec, EPI_STORE, nullptr, sums);
+  const unsigned nbr = reduce_blocks(c, nvec);
+  red.nbr = (int)nbr;
+  hipLaunchKernelGGL(k_dot_diffs, dim3(nbr, nvec), dim3(kBlock), 0, c.stream, a1, b1, a2, b2, vs, c.Vh, nvec, red);
+  finalize(c, red, nvec, false);
+  hipMemcpyAsync(out, sums, sizeof(double) * nvec, hipMemcpyDeviceToHost, c.stream);
+  return hip_ok(c, hipStreamSynchronize(c.stream), "dot diffs") ? 0 : -1;
 }
 
 int vec_norms2(Context &c, const double2 *v, int nvec, long vs, double *out) {

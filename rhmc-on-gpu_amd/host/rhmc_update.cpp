@@ -334,22 +334,25 @@ int run_trajectory(Context &c, const rhmc_traj_params &p, rhmc_traj_result &r) {
     Coeffs cf = rescale(p.order_first, p.first_a0, p.first_a, p.first_b, mn, mx, e_first);
     Coeffs cl = rescale(p.order_last, p.last_a0, p.last_a, p.last_b, mn, mx, e_md);
     const double vres = p.verify_res > 0.0 ? p.verify_res : p.res_metro;
+    // the start links for the chi solve (the compressed copies follow U64)
     std::swap(c.U64, c.U64_save);
+    refresh_links32(c);
     int rc1 = T.solve(c.phi0, cf, vres, PREC_FP64);
     std::swap(c.U64, c.U64_save);
+    refresh_links32(c);
     hipMemcpyAsync(chi64, c.phi0, bytes, hipMemcpyDeviceToDevice, c.stream);
     if (!rc1) sol_sum(c, chi64, c.smf, p.order_first, cf.a0, cf.a.data());
     int rc2 = rc1 ? rc1 : T.solve(chi64, cl, vres, PREC_FP64);
     hipMemcpyAsync(phn64, chi64, bytes, hipMemcpyDeviceToDevice, c.stream);
     if (!rc2) sol_sum(c, phn64, c.smf, p.order_last, cl.a0, cl.a.data());
-    std::vector<double> d_mixed(c.n_pf), d_64(c.n_pf);
-    int rc3 = rc2 ? rc2 : vec_dots(c, c.phinew, c.chi, c.n_pf, sp, d_mixed.data());
-    int rc4 = rc3 ? rc3 : vec_dots(c, phn64, chi64, c.n_pf, sp, d_64.data());
+    // sum over sites of Re<phi_new64, chi64> - Re<phi_new, chi>, differenced per site
+    std::vector<double> d(c.n_pf);
+    int rc3 = rc2 ? rc2 : vec_dot_diffs(c, phn64, chi64, c.phinew, c.chi, c.n_pf, sp, d.data());
     hipFree(chi64);
     hipFree(phn64);
-    if (rc1 || rc2 || rc3 || rc4) return RHMC_EHIP;
+    if (rc1 || rc2 || rc3) return RHMC_EHIP;
     double diff = 0.0;
-    for (int ps = 0; ps < c.n_pf; ps++) diff += d_64[ps] - d_mixed[ps];
+    for (int ps = 0; ps < c.n_pf; ps++) diff += d[ps];
     r.delta_h_fp64 = r.delta_h + diff;
     r.seconds_verify = std::chrono::duration<double>(clk::now() - tv).count();
   }
