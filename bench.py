@@ -212,10 +212,12 @@ def wl_force24(args):
 def wl_traj48c12(args):
     """Mixed-precision RHMC on 48^3x12: Nf=2, am=0.01, beta=5.6, 2
     pseudofermions, Omelyan 20 steps of tau=1, 16 MD poles / 20 heatbath and
-    Metropolis poles (relative residual 1e-11), exp(i 0.3 H) start.  A Markov chain of --steps
-    trajectories with Metropolis accept/reject; Delta H of each is recomputed
-    with FP64 solves to 1e-13; the FP64 trajectory time is taken on the same
-    start configuration."""
+    Metropolis poles (relative residual 1e-12), exp(i 0.3 H) start.  --therm
+    thermalisation trajectories without the Metropolis test (the reference's
+    therm_updates), then a Markov chain of --steps trajectories with
+    Metropolis accept/reject; Delta H of each is recomputed with FP64 solves
+    to 1e-13; the FP64 trajectory time is taken on the same start
+    configuration."""
     dims, mass, beta, n_pf = (48, 48, 48, 12), 0.01, 5.6, 2
     lat = fields.Lattice(*dims)
     U0 = fields.random_links(lat, 4812, 0.3)
@@ -224,11 +226,17 @@ def wl_traj48c12(args):
     g = Gpu(dims, mass=mass, beta=beta, n_pf=n_pf, max_order=20)
     g.init0(U0)
     mn, mx = _bounds(g, mass)
-    # Heatbath/Metropolis residual 1e-11: at this volume the chi solve's
-    # error enters Delta H linearly at ~7e-9 for 1e-10 (FP64 CG).
-    kw = dict(n_flavours=2, no_md=20, integrator="omelyan", res_md=1e-6, res_metro=1e-11, relative=True,
+    # Heatbath/Metropolis residual 1e-12: at this volume the heatbath solve's
+    # error enters Delta H about linearly (FP64 CG: ~8e-9 at 1e-10, ~1.3e-9 at 1e-11).
+    kw = dict(n_flavours=2, no_md=20, integrator="omelyan", res_md=1e-6, res_metro=1e-12, relative=True,
               bounds=(mn, mx), draw_fields=True, fp32_threshold=1e-7, **co)
     rng = np.random.default_rng(48)
+    therm = []
+    for k in range(args.therm):
+        r = g.trajectory(precision=1, seed=4000 + k, metropolis=False, **kw)
+        therm.append({"s": r.seconds_total, "delta_h": r.delta_h})
+        print(f"[bench] 48^3x12 thermalisation {k} {r.seconds_total:.3f}s dH={r.delta_h:.4e}", file=sys.stderr,
+              flush=True)
     chain = []
     for k in range(args.steps):
         u = float(rng.random())
@@ -252,7 +260,9 @@ def wl_traj48c12(args):
     return {"metric": "RHMC trajectory seconds, mixed vs FP64, 48^3x12", "unit": "s/traj",
             "value": float(np.mean([c["s"] for c in chain])),
             "config": {"workload": "48^3x12, Nf=2, am=0.01, beta=5.6, 2 pseudofermions, Omelyan 20 steps, "
-                                   "16 MD / 20 heatbath+Metropolis poles, residuals 1e-6 (MD) / 1e-11"},
+                                   "16 MD / 20 heatbath+Metropolis poles, residuals 1e-6 (MD) / 1e-12",
+                       "thermalisation_trajectories": args.therm},
+            "thermalisation": therm,
             "acceptance": float(np.mean([c["accepted"] for c in chain])),
             "max_delta_h_error_vs_fp64_1e-13": float(max(c["delta_h_err"] for c in chain)),
             "chain": chain, "fp64_s_per_traj_same_start": fp64, "mixed_s_per_traj_same_start": mixed_same,
@@ -335,6 +345,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-md", type=int, default=10)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--therm", type=int, default=10,
+                    help="48^3x12 workload: thermalisation trajectories before the Metropolis chain")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--dims", type=str, default="32,32,32,8")
     ap.add_argument("--npf", type=int, default=8)
