@@ -66,33 +66,39 @@ def dslash_traffic(table, Vh, nrhs):
 
 
 def cpu_baseline(dims, mass, beta, n_pf, shifts, md_a, chi, U, iters_total, n_force, n_q):
-    """FP64 oracle (one core) on a bounded sample of the same workload, scaled
-    to one trajectory with the FP64 GPU trajectory's own operation counts."""
+    """FP64 oracle on a bounded sample of the same workload, scaled to one
+    trajectory with the FP64 GPU trajectory's own operation counts; timed on
+    one thread and on every host core of the process (OpenMP over the
+    per-site loops; the global sums stay serial)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    from oracle import Oracle  # checker, timed here only as the CPU baseline
+    from oracle import Oracle, set_threads  # checker, timed here only as the CPU baseline
+    cores = int(os.environ.get("OMP_NUM_THREADS", "0")) or len(os.sched_getaffinity(0))
     o = Oracle(dims)
     k = 4
-    t0 = time.perf_counter()
-    o.multishift(U, mass, chi, shifts, 0.0, max_cg=k)
-    t_it = (time.perf_counter() - t0) / k
+    P = fields.gauss_momenta(fields.Lattice(*dims), 3)
     x = np.zeros((len(shifts), o.Vh, 3), np.complex128)
     x[:] = chi
-    t0 = time.perf_counter()
-    F = o.fermion_force_accum(U, md_a, x)
-    t_facc = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    o.ipdot_total(U, beta, F, True)
-    t_ipdot = time.perf_counter() - t0
-    P = fields.gauss_momenta(fields.Lattice(*dims), 3)
-    t0 = time.perf_counter()
-    o.exp_update(U, P, 0.01)
-    t_exp = time.perf_counter() - t0
-    est = iters_total * n_pf * t_it + n_force * (n_pf * t_facc + t_ipdot) + n_q * t_exp
+    est = {}
+    for nth in (cores, 1):
+        set_threads(nth)
+        t0 = time.perf_counter()
+        o.multishift(U, mass, chi, shifts, 0.0, max_cg=k)
+        t_it = (time.perf_counter() - t0) / k
+        t0 = time.perf_counter()
+        F = o.fermion_force_accum(U, md_a, x)
+        t_facc = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        o.ipdot_total(U, beta, F, True)
+        t_ipdot = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        o.exp_update(U, P, 0.01)
+        t_exp = time.perf_counter() - t0
+        est[nth] = iters_total * n_pf * t_it + n_force * (n_pf * t_facc + t_ipdot) + n_q * t_exp
     sample = (f"oracle FP64 on 32^3x8: {k} multi-shift CG iterations (1 rhs, {len(shifts)} shifts), "
-              f"one {len(shifts)}-pole force accumulation, one gauge+TA force, one exp update; scaled by the "
-              f"FP64 GPU trajectory's {iters_total} CG iterations x {n_pf} pseudofermions, {n_force} force "
-              f"evaluations and {n_q} link updates")
-    return est, sample
+              f"one {len(shifts)}-pole force accumulation, one gauge+TA force, one exp update, each timed on "
+              f"{cores} threads and on 1 thread; scaled by the FP64 GPU trajectory's {iters_total} CG "
+              f"iterations x {n_pf} pseudofermions, {n_force} force evaluations and {n_q} link updates")
+    return est[cores], cores, est[1], sample
 
 
 def _bounds(g, mass):
@@ -466,9 +472,10 @@ def main():
         n_q = 2 * args.no_md + 1
         chi = fields.gauss_spinor(lat, 5)[0]
         a0, a, b, _ = md.rescaled(mn, mx, -2.0 / 4.0 / n_pf)
-        est, sample = cpu_baseline(dims, mass, beta, n_pf, np.array(b), np.array(a), chi, U0,
-                                   int(np.mean(iters_d)), n_force, n_q)
-        cpu = {"value": est, "unit": "s/traj", "cores": 1, "kind": "port", "sample": sample}
+        est, cores, est1, sample = cpu_baseline(dims, mass, beta, n_pf, np.array(b), np.array(a), chi, U0,
+                                                int(np.mean(iters_d)), n_force, n_q)
+        cpu = {"value": est, "unit": "s/traj", "cores": cores, "kind": "port", "sample": sample,
+               "single_thread_value": est1}
 
     out = {
         "metric": "RHMC traj sec + Dslash GB/s (% HBM), mixed vs FP64 on GPU, 32^3x8",

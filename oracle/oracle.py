@@ -29,6 +29,11 @@ def lib():
     return _LIB
 
 
+def set_threads(n):
+    """Threads of the oracle's per-site loops (results do not depend on it)."""
+    lib().orc_set_threads(int(n))
+
+
 class _Lat(C.Structure):
     _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("nt", C.c_int),
                 ("antiperiodic_t", C.c_int), ("V", C.c_long), ("Vh", C.c_long),

@@ -104,6 +104,20 @@ void orc_su3_exp(orc_c m[3][3]) {
   memcpy(m, ris, sizeof(su3));
 }
 
+/* Threads of the per-site loops (OpenMP).  Every parallel loop writes
+ * disjoint sites and every global sum stays serial, so results do not
+ * depend on the thread count. */
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+void orc_set_threads(int n) {
+#ifdef _OPENMP
+  omp_set_num_threads(n > 0 ? n : 1);
+#else
+  (void)n;
+#endif
+}
+
 /* ------------------------------------------------------------------ */
 /* geometry.cc:5-153                                                   */
 
@@ -157,6 +171,7 @@ void orc_lattice_free(orc_lattice *L) {
 
 /* conf.cc:235-251 */
 void orc_unitarize_with_eta(const orc_lattice *L, orc_c (*U)[3][3]) {
+#pragma omp parallel for schedule(static)
   for (long r = 0; r < 4 * L->V; r++) {
     orc_sunitarize(U[r]);
     for (int j = 0; j < 3; j++) U[r][2][j] *= (double)L->eta[r];
@@ -168,6 +183,7 @@ void orc_unitarize_with_eta(const orc_lattice *L, orc_c (*U)[3][3]) {
 
 void orc_deo(const orc_lattice *L, const orc_c (*U)[3][3], orc_c (*out)[3], const orc_c (*in)[3]) {
   const long V = L->V, Vh = L->Vh;
+#pragma omp parallel for schedule(static)
   for (long e = 0; e < Vh; e++) {
     vec3 acc = {0, 0, 0}, t;
     for (int mu = 0; mu < 4; mu++) {
@@ -185,6 +201,7 @@ void orc_deo(const orc_lattice *L, const orc_c (*U)[3][3], orc_c (*out)[3], cons
 
 void orc_doe(const orc_lattice *L, const orc_c (*U)[3][3], orc_c (*out)[3], const orc_c (*in)[3]) {
   const long V = L->V, Vh = L->Vh;
+#pragma omp parallel for schedule(static)
   for (long o = Vh; o < V; o++) {
     vec3 acc = {0, 0, 0}, t;
     for (int mu = 0; mu < 4; mu++) {
@@ -207,6 +224,7 @@ void orc_mdagm(const orc_lattice *L, const orc_c (*U)[3][3], double mass, orc_c 
   orc_doe(L, U, h, in);
   orc_deo(L, U, out, (const orc_c(*)[3])h);
   const double m2 = mass * mass;
+#pragma omp parallel for schedule(static)
   for (long i = 0; i < Vh; i++)
     for (int c = 0; c < 3; c++) out[i][c] = m2 * in[i][c] - out[i][c];
   free(h);
@@ -247,6 +265,7 @@ int orc_cg(const orc_lattice *L, const orc_c (*U)[3][3], double mass, const orc_
   do {
     cg++;
     orc_mdagm(L, U, mass, s, (const orc_c(*)[3])p);
+#pragma omp parallel for schedule(static)
     for (long i = 0; i < Vh; i++) dv[i] = v_rdot(p[i], s[i]);
     double alpha = orc_global_sum(dv, Vh);
     double omega = delta / alpha;
@@ -295,6 +314,7 @@ int orc_multishift(const orc_lattice *L, const orc_c (*U)[3][3], double mass, co
   do {
     cg++;
     orc_mdagm(L, U, mass, s, (const orc_c(*)[3])p);
+#pragma omp parallel for schedule(static)
     for (long i = 0; i < Vh; i++) dv[i] = v_rdot(p[i], s[i]);
     double alpha = orc_global_sum(dv, Vh);
     omega_save = omega;
@@ -305,21 +325,25 @@ int orc_multishift(const orc_lattice *L, const orc_c (*U)[3][3], double mass, co
               (omega * gammag * (z1[k] - z2[k]) + z1[k] * omega_save * (1.0 - shifts[k] * omega));
       om[k] = omega * z3[k] / z2[k];
       vec3 *xk = x + (long)k * Vh, *pk = ps + (long)k * Vh;
+#pragma omp parallel for schedule(static)
       for (long i = 0; i < Vh; i++)
         for (int c = 0; c < 3; c++) xk[i][c] = xk[i][c] - om[k] * pk[i][c];
     }
+#pragma omp parallel for schedule(static)
     for (long i = 0; i < Vh; i++) {
       for (int c = 0; c < 3; c++) r[i][c] = r[i][c] + omega * s[i][c];
       dv[i] = v_norm2(r[i]);
     }
     lambda = orc_global_sum(dv, Vh);
     gammag = lambda / delta;
+#pragma omp parallel for schedule(static)
     for (long i = 0; i < Vh; i++)
       for (int c = 0; c < 3; c++) p[i][c] = r[i][c] + gammag * p[i][c];
     for (int k = 0; k < ns; k++) {
       if (!flag[k]) continue;
       ga[k] = gammag * z3[k] * om[k] / (z2[k] * omega);
       vec3 *pk = ps + (long)k * Vh;
+#pragma omp parallel for schedule(static)
       for (long i = 0; i < Vh; i++)
         for (int c = 0; c < 3; c++) pk[i][c] = z3[k] * r[i][c] + ga[k] * pk[i][c];
       double fact = sqrt(delta * z2[k] * z2[k]);
@@ -345,6 +369,7 @@ void orc_fermion_force_accum(const orc_lattice *L, const orc_c (*U)[3][3], int n
   for (int k = 0; k < ns; k++) {
     const orc_c(*s)[3] = x + (long)k * Vh;
     orc_doe(L, U, h, s);
+#pragma omp parallel for schedule(static)
     for (long e = 0; e < Vh; e++) {
       long o = e + Vh;
       for (int mu = 0; mu < 4; mu++) {
@@ -363,6 +388,7 @@ void orc_fermion_force_accum(const orc_lattice *L, const orc_c (*U)[3][3], int n
 /* staples.cc:16-74 */
 void orc_staples(const orc_lattice *L, const orc_c (*U)[3][3], orc_c (*st)[3][3]) {
   const long V = L->V;
+#pragma omp parallel for schedule(static)
   for (long pos = 0; pos < V; pos++)
     for (int mu = 0; mu < 4; mu++) {
       su3 acc, a, b;
@@ -404,6 +430,7 @@ void orc_ipdot_total(const orc_lattice *L, const orc_c (*U)[3][3], double beta, 
     orc_staples(L, U, st);
   }
   const double b3 = beta / 3.0;
+#pragma omp parallel for schedule(static)
   for (long r = 0; r < NL; r++) {
     su3 m;
     memset(m, 0, sizeof(su3));
@@ -423,6 +450,7 @@ void orc_ipdot_total(const orc_lattice *L, const orc_c (*U)[3][3], double beta, 
 void orc_conf_left_exp_multiply(const orc_lattice *L, orc_c (*U)[3][3], const orc_c (*P)[3][3], double step) {
   const long NL = 4 * L->V;
   const orc_c p = I * step;
+#pragma omp parallel for schedule(static)
   for (long r = 0; r < NL; r++) {
     su3 a;
     for (int i = 0; i < 3; i++)

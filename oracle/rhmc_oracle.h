@@ -39,6 +39,7 @@ typedef struct {
 } orc_lattice;
 
 /* geometry.cc:26-153 */
+void orc_set_threads(int n);
 int orc_lattice_init(orc_lattice *lat, int nx, int ny, int nz, int nt, int antiperiodic_t);
 void orc_lattice_free(orc_lattice *lat);
 

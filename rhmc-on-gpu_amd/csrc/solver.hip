@@ -323,8 +323,16 @@ struct DsArgs {
   int nvec;
   unsigned nsb, nch;                   // site blocks x vector chunks
   const int *tiles; int ntiles;        // traversal order of the kBlock-site tiles
+  // iteration-indexed operand: in (iter_in) and pin (iter_pin) advance by
+  // (*iterp + iter_off) * iter_stride elements (a basis slot chosen on the device)
+  const int *iterp; int iter_off; long iter_stride; int iter_in, iter_pin;
   RedArgs red;
 };
+
+// element offset of the iteration-indexed operand
+__device__ __forceinline__ long ds_iter_shift(const DsArgs &a) {
+  return a.iterp ? (long)(*a.iterp + a.iter_off) * a.iter_stride : 0;
+}
 
 // A link as stored: rows 0-1 for the 12-real formats (row 2 is rebuilt when
 // the link is used, so the loads can be issued a hop ahead of the FMAs),
@@ -462,6 +470,46 @@ constexpr int ds_waves() {
   return w < 1 ? 1 : (w > 8 ? 8 : w);
 }
 
+// Output of one site for the chunk's right-hand sides: the hop result (times
+// 1/2), or m^2 + shift times the input minus it (M^dag M), or the residual
+// b - M^dag M x, with the dot/norm partials of the reducing modes.
+template <typename R, int NR, int MODE, int RED>
+__device__ __forceinline__ void ds_site_out(const DsArgs &a, const Cx<R> (&acc)[NR][3], const bool (&act)[NR],
+                                            double (&red)[NR], int jbase, int h, const Cx<R> *pinb) {
+  using C = Cx<R>;
+  const long Vh = a.L.Vh;
+#pragma unroll
+  for (int r = 0; r < NR; r++) {
+    if (!act[r]) continue;
+    const int j = jbase + r;
+    C *o = (C *)a.out + (long)j * a.out_vs + h;
+    C res[3];
+    if (MODE == DS_HOP) {
+#pragma unroll
+      for (int i = 0; i < 3; i++) res[i] = cscale(acc[r][i], (R)0.5);
+    } else {
+      const C *p = pinb + (long)j * a.pin_vs + h;
+      const R d = (R)a.diag[j];
+#pragma unroll
+      for (int i = 0; i < 3; i++) {
+        C pv = p[(long)i * Vh];
+        res[i].x = fmar(d, pv.x, (R)-0.5 * acc[r][i].x);
+        res[i].y = fmar(d, pv.y, (R)-0.5 * acc[r][i].y);
+        if (MODE == DS_RESID) {
+          const double2 bv = a.b[(long)(j / a.b_group) * a.b_vs + (long)i * Vh + h];
+          res[i] = csub(mk<R>((R)bv.x, (R)bv.y), res[i]);
+        }
+        if (RED == RED_DOT) red[r] += (double)pv.x * (double)res[i].x + (double)pv.y * (double)res[i].y;
+      }
+    }
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      o[(long)i * Vh] = res[i];
+      if (RED == RED_NORM) red[r] += (double)res[i].x * (double)res[i].x + (double)res[i].y * (double)res[i].y;
+    }
+  }
+}
+
 template <typename R, int LINK, int NR, int MODE, int RED>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(ds_waves<R, LINK, NR>()))) void k_dslash(DsArgs a) {
   using C = Cx<R>;
@@ -492,11 +540,14 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(ds_waves
   // branch, so the loads of a whole hop are in flight together; a slot past
   // nvec reads the last vector and a stopped system is computed but not
   // stored
+  const long ish = ds_iter_shift(a);
+  const C *inb = (const C *)a.in + (a.iter_in ? ish : 0);
+  const C *pinb = (const C *)a.pin + (a.iter_pin ? ish : 0);
   const C *inr[NR];
 #pragma unroll
   for (int r = 0; r < NR; r++) {
     const int j = jbase + r < a.nvec ? jbase + r : a.nvec - 1;
-    inr[r] = (const C *)a.in + (long)j * a.in_vs;
+    inr[r] = inb + (long)j * a.in_vs;
   }
   for (int q = q0; any && q < q1; q++) {
     const int h = (a.tiles ? a.tiles[q] : q) * kBlock + (int)threadIdx.x;
@@ -517,36 +568,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(ds_waves
       hop_apply<R, LINK, NR>(L, c, k, buf[k & 1], acc);
       hop_fence<R, NR>(acc);
     }
-#pragma unroll
-    for (int r = 0; r < NR; r++) {
-      if (!act[r]) continue;
-      const int j = jbase + r;
-      C *o = (C *)a.out + (long)j * a.out_vs + h;
-      C res[3];
-      if (MODE == DS_HOP) {
-#pragma unroll
-        for (int i = 0; i < 3; i++) res[i] = cscale(acc[r][i], (R)0.5);
-      } else {
-        const C *p = (const C *)a.pin + (long)j * a.pin_vs + h;
-        const R d = (R)a.diag[j];
-#pragma unroll
-        for (int i = 0; i < 3; i++) {
-          C pv = p[(long)i * Vh];
-          res[i].x = fmar(d, pv.x, (R)-0.5 * acc[r][i].x);
-          res[i].y = fmar(d, pv.y, (R)-0.5 * acc[r][i].y);
-          if (MODE == DS_RESID) {
-            const double2 bv = a.b[(long)(j / a.b_group) * a.b_vs + (long)i * Vh + h];
-            res[i] = csub(mk<R>((R)bv.x, (R)bv.y), res[i]);
-          }
-          if (RED == RED_DOT) red[r] += (double)pv.x * (double)res[i].x + (double)pv.y * (double)res[i].y;
-        }
-      }
-#pragma unroll
-      for (int i = 0; i < 3; i++) {
-        o[(long)i * Vh] = res[i];
-        if (RED == RED_NORM) red[r] += (double)res[i].x * (double)res[i].x + (double)res[i].y * (double)res[i].y;
-      }
-    }
+    ds_site_out<R, NR, MODE, RED>(a, acc, act, red, jbase, h, pinb);
   }
   if (RED != RED_NONE) block_partials<NR>(red, a.red.partial, jbase, a.nvec, a.nsb, sb);
 }
@@ -690,11 +712,14 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(ds_waves
   // branch, so the loads of a whole hop are in flight together; a slot past
   // nvec reads the last vector and a stopped system is computed but not
   // stored
+  const long ish = ds_iter_shift(a);
+  const C *inb = (const C *)a.in + (a.iter_in ? ish : 0);
+  const C *pinb = (const C *)a.pin + (a.iter_pin ? ish : 0);
   const C *inr[NR];
 #pragma unroll
   for (int r = 0; r < NR; r++) {
     const int j = jbase + r < a.nvec ? jbase + r : a.nvec - 1;
-    inr[r] = (const C *)a.in + (long)j * a.in_vs;
+    inr[r] = inb + (long)j * a.in_vs;
   }
 
   for (int q = q0; any && q < q1; q++) {
@@ -734,7 +759,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(ds_waves
       if (MODE == DS_HOP) {
         res = cscale(av, (R)0.5);
       } else {
-        const C pv = ((const C *)a.pin + (long)j * a.pin_vs + h)[(long)col * Vh];
+        const C pv = (pinb + (long)j * a.pin_vs + h)[(long)col * Vh];
         const R d = (R)a.diag[j];
         res.x = fmar(d, pv.x, (R)-0.5 * av.x);
         res.y = fmar(d, pv.y, (R)-0.5 * av.y);
@@ -895,7 +920,7 @@ std::string tuning_table(Context &c) {
   std::string out = "[";
   bool first = true;
   for (auto &kv : c.ws->tuned_nr) {
-    char buf[200];
+    char buf[240];
     const unsigned long long k = kv.first;
     snprintf(buf, sizeof buf,
              "%s{\"real_bytes\":%llu,\"links\":\"%s\",\"mode\":%llu,\"reduction\":%llu,\"nvec\":%llu,"
@@ -1168,16 +1193,29 @@ __global__ __launch_bounds__(kBlock) void k_copy_f2d(double2 *x64, const float2 
 // per rhs instead of 6 + 4 x (active shifts); the shifted solutions are
 // assembled in one pass at the end.
 //
-// Per iteration k four launches (no separate finalize): A h = D_oe p;
-// B s = m^2 p - D_eo h, partials of <p,s>; C every workgroup sums B's
-// partials (fixed order) for omega and writes r^(k) = r^(k-1) + omega s into
-// basis slot k, partials of |r^(k)|^2; D every workgroup sums C's partials for
-// gamma and updates p = r^(k) + gamma p, and workgroup (0,j) advances the
-// scalar recurrences of rhs j.  Values read by every workgroup (delta, stop)
-// are double-buffered by the parity of k, so no workgroup reads a value
-// another one writes in the same launch.
+// Per iteration k three launches, with the base CG in its single-reduction
+// form (Chronopoulos and Gear, J. Comput. Appl. Math. 25 (1989) 153): the
+// operator is applied to the residual, w = A r^(k-1), and s = A p follows
+// its own recurrence, so one vector launch per iteration carries both
+// updates.  With rho_i = |r_i|^2 and mu_i = <r_i, w_i>:
+//   beta_{i-1} = rho_i / rho_{i-1},
+//   <p_i, A p_i> = mu_i - beta_{i-1} rho_i / alpha_{i-1},  alpha_i = rho_i / <p_i, A p_i>,
+//   s_i = w_i + beta_{i-1} s_{i-1},  r_{i+1} = r_i - alpha_i s_i
+// (alpha, beta are those of the textbook CG; p itself is never formed,
+// the shifted solutions come from the basis).  Launches of iteration k:
+// A h = D_oe r^(k-1); B w = m^2 r^(k-1) - D_eo h, partials of mu;
+// X every workgroup sums the partials of rho (previous X) and mu in a
+// fixed order, updates s and writes r^(k) into basis slot k with partials
+// of |r^(k)|^2, and workgroup (0,j) advances the Jegerlehner recurrences of
+// rhs j for iteration k-1 (they need beta_{k-2}, known once rho_{k-1} is).
+// The stop decision therefore lags by one launch: one extra application of
+// the operator per solve, identical recurrences and iteration counts.
+// Values read by every workgroup (rho, alpha, stop, the rho partials) are
+// double-buffered by the parity of k, so no workgroup reads a value another
+// one writes in the same launch.
 struct BmsShared {
-  double delta[2];
+  double rho[2];    // [k&1] at launch k: rho_{k-2}
+  double alpha[2];  // [k&1] at launch k: alpha_{k-2}
   int stop[2];
 };
 
@@ -1190,9 +1228,9 @@ struct BmsArgs {
   int nvec, nshift, iter_off, cap;
   const int *iterc;     // device iteration base: iteration = *iterc + iter_off
   const double2 *b;
-  void *basis;          // [rhs][cap][3][Vh]
-  void *p, *s;          // [rhs][3][Vh]
-  double *partB, *partC;
+  void *basis;          // [cap][rhs][3][Vh]: slot k of every rhs contiguous
+  void *w, *s;          // [rhs][3][Vh]: A r^(k-1) and the s = A p recurrence
+  double *partB, *partC;  // partials of mu; of rho, two buffers by parity
   int nbrB, nbrC;
   BmsShared *sh;
   MsState *ms;
@@ -1234,23 +1272,22 @@ __global__ __launch_bounds__(kBlock) void k_bms_init(BmsArgs a) {
   long h0, h1;
   site_range(Vh, bx, gridDim.x, h0, h1);
   double v[1] = {0.0};
-  C *r0 = (C *)a.basis + (long)j * a.cap * 3 * Vh;
-  C *p = (C *)a.p + (long)j * 3 * Vh;
+  C *r0 = (C *)a.basis + (long)j * 3 * Vh;
   for (long h = h0 + threadIdx.x; h < h1; h += kBlock)
     for (int i = 0; i < 3; i++) {
       const double2 bv = a.b[((long)j * 3 + i) * Vh + h];
       const C cv = mk<R>((R)bv.x, (R)bv.y);
       r0[(long)i * Vh + h] = cv;
-      p[(long)i * Vh + h] = cv;
       v[0] += (double)cv.x * (double)cv.x + (double)cv.y * (double)cv.y;
     }
-  block_partials<1>(v, a.partC, j, a.nvec, gridDim.x, bx);
+  // rho_0, read by the launch of iteration 1 from buffer 1
+  block_partials<1>(v, a.partC + (long)a.nvec * a.nbrC, j, a.nvec, gridDim.x, bx);
 }
 
-// scalar state for iteration 1 (slot 1): inverter.cc:166-198
+// scalar state before iteration 1: inverter.cc:166-198
 __global__ __launch_bounds__(kBlock) void k_bms_init_fin(BmsArgs a) {
   __shared__ double sums[kMaxRed];
-  final_sums(a.partC, a.nvec, a.nbrC, sums);
+  final_sums(a.partC + (long)a.nvec * a.nbrC, a.nvec, a.nbrC, sums);
   for (int j = threadIdx.x; j < a.nvec; j += blockDim.x) {
     MsState &s = a.ms[j];
     s.omega = 1.0;
@@ -1264,119 +1301,115 @@ __global__ __launch_bounds__(kBlock) void k_bms_init_fin(BmsArgs a) {
       s.om[k] = 0.0; s.ga[k] = 0.0;
       s.flag[k] = 1; s.flag_next[k] = 1; s.its[k] = 0;
     }
-    a.sh[j].delta[1] = sums[j];
+    a.sh[j].rho[1] = 1.0;
+    a.sh[j].alpha[1] = 1.0;
     a.sh[j].stop[1] = sums[j] == 0.0;  // zero source: x = 0
     s.done = s.stop = a.sh[j].stop[1];
   }
-  __syncthreads();
-  const int t = threadIdx.x;
-  const int n = __syncthreads_count(t < a.nvec && !a.sh[t].stop[1]);
-  if (t == 0) a.active[1] = n;
+  // the launch of iteration k counts running systems into active[(k+1)&1]
+  if (threadIdx.x == 0) a.active[0] = 0;
 }
 
-// C: r^(k) = r^(k-1) + omega s, |r^(k)|^2
+// X (iteration k >= 1, CG step i = k-1): s = w + beta s, r^(k) = r^(k-1) -
+// alpha s, partials of |r^(k)|^2; workgroup (0,j) runs the recurrences of
+// iteration k-1 (inverter.cc:224-287) and the stopping test.
 template <typename R>
-__global__ __launch_bounds__(kBlock) void k_bms_c(BmsArgs a) {
+__global__ __launch_bounds__(kBlock) void k_bms_x(BmsArgs a) {
   using C = Cx<R>;
   const unsigned bx = xcd_block(blockIdx.x, gridDim.x);
   const int j = blockIdx.y;
-  const int iter = *a.iterc + a.iter_off;
-  const int sl = iter & 1;
-  const long Vh = a.Vh;
-  if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) a.active[(iter + 1) & 1] = 0;
-  double v[1] = {0.0};
-  if (!a.sh[j].stop[sl]) {
-    const double alpha = block_sum_partials(a.partB + (long)j * a.nbrB, a.nbrB);
-    const R om = (R)(-a.sh[j].delta[sl] / alpha);
-    long h0, h1;
-    site_range(Vh, bx, gridDim.x, h0, h1);
-    const C *ro = (const C *)a.basis + ((long)j * a.cap + iter - 1) * 3 * Vh;
-    C *rn = (C *)a.basis + ((long)j * a.cap + iter) * 3 * Vh;
-    const C *sv = (const C *)a.s + (long)j * 3 * Vh;
-    for (long h = h0 + threadIdx.x; h < h1; h += kBlock)
-#pragma unroll
-      for (int i = 0; i < 3; i++) {
-        C r = ro[(long)i * Vh + h];
-        const C t = sv[(long)i * Vh + h];
-        r.x += om * t.x;
-        r.y += om * t.y;
-        rn[(long)i * Vh + h] = r;
-        v[0] += (double)r.x * (double)r.x + (double)r.y * (double)r.y;
-      }
-  }
-  block_partials<1>(v, a.partC, j, a.nvec, gridDim.x, bx);
-}
-
-// D: p = r^(k) + gamma p; workgroup (0,j) advances the recurrences of rhs j
-template <typename R>
-__global__ __launch_bounds__(kBlock) void k_bms_d(BmsArgs a) {
-  using C = Cx<R>;
-  const unsigned bx = xcd_block(blockIdx.x, gridDim.x);
-  const int j = blockIdx.y;
-  const int iter = *a.iterc + a.iter_off;
-  const int sl = iter & 1, sn = (iter + 1) & 1;
+  const int k = *a.iterc + a.iter_off;
+  const int sl = k & 1, sn = sl ^ 1;
   const long Vh = a.Vh;
   BmsShared &sh = a.sh[j];
+  // the slot the launch of iteration k+1 counts into (this launch counts into sn)
+  if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) a.active[sl] = 0;
   if (sh.stop[sl]) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
+    if (bx == 0 && threadIdx.x == 0) {
       sh.stop[sn] = 1;
-      sh.delta[sn] = sh.delta[sl];
+      sh.rho[sn] = sh.rho[sl];
+      sh.alpha[sn] = sh.alpha[sl];
     }
     return;
   }
-  const double delta = sh.delta[sl];
-  const double lambda = block_sum_partials(a.partC + (long)j * a.nbrC, a.nbrC);
-  const double gam = lambda / delta;
-  {
-    const R g = (R)gam;
+  const double rho1 = block_sum_partials(a.partC + ((long)sl * a.nvec + j) * a.nbrC, a.nbrC);  // rho_{k-1}
+  const double mu = block_sum_partials(a.partB + (long)j * a.nbrB, a.nbrB);                   // <r^(k-1), w>
+  const double rho2 = sh.rho[sl], al2 = sh.alpha[sl];                                         // rho_{k-2}, alpha_{k-2}
+  const double be = k >= 2 ? rho1 / rho2 : 0.0;
+  const double pap = k >= 2 ? mu - be * rho1 / al2 : mu;
+  const double al = rho1 / pap;
+  double v[1] = {0.0};
+  if (k < a.cap) {
+    const R b = (R)be, am = (R)(-al);
     long h0, h1;
     site_range(Vh, bx, gridDim.x, h0, h1);
-    const C *r = (const C *)a.basis + ((long)j * a.cap + iter) * 3 * Vh;
-    C *p = (C *)a.p + (long)j * 3 * Vh;
+    const C *ro = (const C *)a.basis + ((long)(k - 1) * a.nvec + j) * 3 * Vh;
+    C *rn = (C *)a.basis + ((long)k * a.nvec + j) * 3 * Vh;
+    const C *wv = (const C *)a.w + (long)j * 3 * Vh;
+    C *sv = (C *)a.s + (long)j * 3 * Vh;
     for (long h = h0 + threadIdx.x; h < h1; h += kBlock)
 #pragma unroll
       for (int i = 0; i < 3; i++) {
-        const C rv = r[(long)i * Vh + h];
-        C pv = p[(long)i * Vh + h];
-        pv.x = rv.x + g * pv.x;
-        pv.y = rv.y + g * pv.y;
-        p[(long)i * Vh + h] = pv;
+        const long e = (long)i * Vh + h;
+        const C wi = wv[e];
+        C si;
+        if (k >= 2) {
+          const C so = sv[e];
+          si.x = fmar(b, so.x, wi.x);
+          si.y = fmar(b, so.y, wi.y);
+        } else {
+          si = wi;
+        }
+        sv[e] = si;
+        C r = ro[e];
+        r.x = fmar(am, si.x, r.x);
+        r.y = fmar(am, si.y, r.y);
+        rn[e] = r;
+        v[0] += (double)r.x * (double)r.x + (double)r.y * (double)r.y;
       }
   }
-  if (blockIdx.x != 0) return;
-  // ---- scalar recurrences of inverter.cc:224-287 for rhs j
-  const double alpha = block_sum_partials(a.partB + (long)j * a.nbrB, a.nbrB);
+  block_partials<1>(v, a.partC + (long)sn * a.nvec * a.nbrC, j, a.nvec, gridDim.x, bx);
+  if (bx != 0) return;
   MsState &s = a.ms[j];
-  const double omp = s.omega, gold = s.gammag;
-  const double om = -delta / alpha;
   const int t = threadIdx.x;
-  if (t < s.nshift && s.flag[t]) {
-    const double z1 = s.z1[t], z2 = s.z2[t];
-    const double z3 = (z1 * z2 * omp) / (om * gold * (z1 - z2) + z1 * omp * (1.0 - s.shift[t] * om));
-    const double oms = om * z3 / z2;
-    const double gas = gam * z3 * oms / (z2 * om);
-    BmsHist &hh = a.hist[((long)j * a.cap + iter) * a.nshift + t];
-    hh.om = oms;
-    hh.z = z3;
-    hh.ga = gas;
-    if (sqrt(delta * z2 * z2) < s.res) {
-      s.flag[t] = 0;
-      s.its[t] = iter;
+  int stop = 0;
+  if (k >= 2) {
+    // recurrences of iteration kk = k-1: omega = -alpha_{k-2}, gamma = beta_{k-2}
+    const int kk = k - 1;
+    const double omp = s.omega, gold = s.gammag;
+    const double om = -al2, gam = be;
+    if (t < s.nshift && s.flag[t]) {
+      const double z1 = s.z1[t], z2 = s.z2[t];
+      const double z3 = (z1 * z2 * omp) / (om * gold * (z1 - z2) + z1 * omp * (1.0 - s.shift[t] * om));
+      const double oms = om * z3 / z2;
+      const double gas = gam * z3 * oms / (z2 * om);
+      BmsHist &hh = a.hist[((long)j * a.cap + kk) * a.nshift + t];
+      hh.om = oms;
+      hh.z = z3;
+      hh.ga = gas;
+      if (sqrt(rho2 * z2 * z2) < s.res) {
+        s.flag[t] = 0;
+        s.its[t] = kk;
+      }
+      s.z1[t] = z2;
+      s.z2[t] = z3;
     }
-    s.z1[t] = z2;
-    s.z2[t] = z3;
+    const int nact = __syncthreads_count(t < s.nshift && s.flag[t]);
+    if (t == 0) {
+      s.omega_prev = omp;
+      s.omega = om;
+      s.gammag = gam;
+      s.delta = rho1;
+      s.iter = kk;
+      s.nact = nact;
+      // r^(k) stays inside the basis: the last launch that writes runs at k = cap-1
+      stop = !(sqrt(rho1) > s.res && kk < s.max_iter) || nact == 0 || kk + 2 >= a.cap;
+      s.stop = s.done = stop;
+    }
   }
-  const int nact = __syncthreads_count(t < s.nshift && s.flag[t]);
   if (t == 0) {
-    s.omega_prev = omp;
-    s.omega = om;
-    s.gammag = gam;
-    s.delta = lambda;
-    s.iter = iter;
-    s.nact = nact;
-    const int stop = !(sqrt(lambda) > s.res && iter < s.max_iter) || nact == 0 || iter + 1 >= a.cap;
-    s.stop = s.done = stop;
-    sh.delta[sn] = lambda;
+    sh.rho[sn] = rho1;
+    sh.alpha[sn] = al;
     sh.stop[sn] = stop;
     if (!stop) atomicAdd(&a.active[sn], 1);
   }
@@ -1394,7 +1427,7 @@ constexpr int kTileK = 32;
 
 template <typename R, int NS, typename XO>
 __global__ __launch_bounds__(kBlock) void k_bms_assemble(const void *basis, const double *coef, const int *klen,
-                                                         XO *x, long Vh, int cap, int nshift, int s0) {
+                                                         XO *x, long Vh, int cap, int nshift, int s0, int nvec) {
   using C = Cx<R>;
   __shared__ double2 cs[kTileK][NS / 2];
   const long n3 = 3 * Vh;
@@ -1402,7 +1435,8 @@ __global__ __launch_bounds__(kBlock) void k_bms_assemble(const void *basis, cons
   const int j = blockIdx.y;
   const int kmax = klen[j];
   const int ns = nshift - s0 < NS ? nshift - s0 : NS;
-  const C *bj = (const C *)basis + (long)j * cap * n3 + (e < n3 ? e : 0);
+  const long ss = (long)nvec * n3;  // slot stride
+  const C *bj = (const C *)basis + (long)j * n3 + (e < n3 ? e : 0);
   const double *cj = coef + ((long)j * nshift + s0) * cap;
   double2 acc[NS];
 #pragma unroll
@@ -1420,7 +1454,7 @@ __global__ __launch_bounds__(kBlock) void k_bms_assemble(const void *basis, cons
     for (; u + 8 <= nk; u += 8) {
       C v[8];
 #pragma unroll
-      for (int w = 0; w < 8; w++) v[w] = bj[(long)(i0 + u + w) * n3];
+      for (int w = 0; w < 8; w++) v[w] = bj[(long)(i0 + u + w) * ss];
 #pragma unroll
       for (int w = 0; w < 8; w++) {
         const double vx = (double)v[w].x, vy = (double)v[w].y;
@@ -1435,7 +1469,7 @@ __global__ __launch_bounds__(kBlock) void k_bms_assemble(const void *basis, cons
       }
     }
     for (; u < nk; u++) {
-      const C v = bj[(long)(i0 + u) * n3];
+      const C v = bj[(long)(i0 + u) * ss];
       const double vx = (double)v.x, vy = (double)v.y;
 #pragma unroll
       for (int q2 = 0; q2 < NS / 2; q2++) {
@@ -1582,15 +1616,16 @@ static int ms_cg_basis(Context &c, const double2 *b, int nrhs, int nshift, const
   long cap = (long)std::min<size_t>(std::min<size_t>((size_t)max_iter + 2, 4096), budget / (vb * nrhs));
   if (cap < 64) return 3;
   void *basis = ws_get(c, S_BASIS, vb * nrhs * cap);
-  void *p = ws_get(c, dbl ? S_P64 : S_P32, vb * nrhs);
-  void *sv = ws_get(c, dbl ? S_S64 : S_S32, vb * nrhs);
+  void *sr = ws_get(c, dbl ? S_P64 : S_P32, vb * nrhs);   // s = A p recurrence
+  void *wv = ws_get(c, dbl ? S_S64 : S_S32, vb * nrhs);   // w = A r^(k-1)
   void *hh = ws_get(c, dbl ? S_H64 : S_H32, vb * nrhs);
   MsState *ms = (MsState *)ws_get(c, S_MS, sizeof(MsState) * nrhs);
   BmsShared *bsh = (BmsShared *)ws_get(c, S_BSH, sizeof(BmsShared) * nrhs);
   BmsHist *hist = (BmsHist *)ws_get(c, S_HIST, sizeof(BmsHist) * nrhs * cap * nshift);
   double *diag = (double *)ws_get(c, S_DIAG, sizeof(double) * 256);
-  double *partC = (double *)ws_get(c, S_PARTIAL2, sizeof(double) * nbx_of(c) * 256);
-  if (!basis || !p || !sv || !hh || !ms || !bsh || !hist || !diag || !partC) return -1;
+  // two buffers (iteration parity) of nrhs x nbr partials, nbr <= nbx_of(c)
+  double *partC = (double *)ws_get(c, S_PARTIAL2, sizeof(double) * nbx_of(c) * 2 * std::max(nrhs, 1));
+  if (!basis || !sr || !wv || !hh || !ms || !bsh || !hist || !diag || !partC) return -1;
 
   std::vector<MsState> hs(nrhs);
   for (int j = 0; j < nrhs; j++) {
@@ -1607,7 +1642,7 @@ static int ms_cg_basis(Context &c, const double2 *b, int nrhs, int nshift, const
 
   BmsArgs a = {};
   a.Vh = Vh; a.nvec = nrhs; a.nshift = nshift; a.cap = (int)cap;
-  a.b = b; a.basis = basis; a.p = p; a.s = sv;
+  a.b = b; a.basis = basis; a.w = wv; a.s = sr;
   a.partC = partC;
   a.sh = bsh; a.ms = ms; a.hist = hist;
   a.active = active_ptr(c);
@@ -1616,12 +1651,16 @@ static int ms_cg_basis(Context &c, const double2 *b, int nrhs, int nshift, const
   hipLaunchKernelGGL(k_bms_init<R>, dim3(nbr, nrhs), dim3(kBlock), 0, c.stream, a);
   hipLaunchKernelGGL(k_bms_init_fin, dim3(1), dim3(kBlock), 0, c.stream, a);
 
+  // iteration k applies the operator to basis slot k-1 (chosen on the device)
+  int *iterc = iter_ptr(c);
   DsArgs hop = {};
   hop.par_out = 1; hop.U = link_ptr<LINK>(c);
-  hop.in = p; hop.in_vs = 3 * Vh; hop.out = hh; hop.out_vs = 3 * Vh; hop.nvec = nrhs;
+  hop.in = basis; hop.in_vs = 3 * Vh; hop.out = hh; hop.out_vs = 3 * Vh; hop.nvec = nrhs;
   hop.skip_stride = sizeof(BmsShared) / sizeof(int);
+  hop.iterp = iterc; hop.iter_stride = (long)nrhs * 3 * Vh; hop.iter_in = 1;
   DsArgs mdm = hop;
-  mdm.par_out = 0; mdm.in = hh; mdm.out = sv; mdm.pin = p; mdm.pin_vs = 3 * Vh; mdm.diag = diag;
+  mdm.par_out = 0; mdm.in = hh; mdm.in_vs = 3 * Vh; mdm.out = wv; mdm.pin = basis; mdm.pin_vs = 3 * Vh;
+  mdm.diag = diag; mdm.iter_in = 0; mdm.iter_pin = 1;
   mdm.red = make_red(c, nrhs, EPI_STORE, nullptr, nullptr);
   a.partB = mdm.red.partial;
 
@@ -1629,20 +1668,20 @@ static int ms_cg_basis(Context &c, const double2 *b, int nrhs, int nshift, const
   // device (base + offset), so a chunk is a fixed launch sequence: the first
   // chunk of a solve runs eagerly (the Dslash autotuner may time candidates
   // there), later chunks replay a captured graph of it.  Iterations past the
-  // basis capacity are no-ops: k_bms_d stops a system at iteration cap-1.
-  int *iterc = iter_ptr(c);
+  // basis capacity are no-ops: k_bms_x stops a system before slot cap.
   hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, c.stream, iterc, 1);
   a.iterc = iterc;
   auto chunk_launch = [&]() {
     for (int q = 0; q < kChunk; q++) {
       hop.skip = &bsh[0].stop[(1 + q) & 1];
       mdm.skip = hop.skip;
+      hop.iter_off = q - 1;
+      mdm.iter_off = q - 1;
       dslash_dispatch<R, LINK, DS_HOP, RED_NONE>(c, hop);
       dslash_dispatch<R, LINK, DS_MDAGM, RED_DOT>(c, mdm);
       a.nbrB = mdm.red.nbr;
       a.iter_off = q;
-      hipLaunchKernelGGL(k_bms_c<R>, dim3(nbr, nrhs), dim3(kBlock), 0, c.stream, a);
-      hipLaunchKernelGGL(k_bms_d<R>, dim3(nbr, nrhs), dim3(kBlock), 0, c.stream, a);
+      hipLaunchKernelGGL(k_bms_x<R>, dim3(nbr, nrhs), dim3(kBlock), 0, c.stream, a);
     }
     hipLaunchKernelGGL(k_add_int, dim3(1), dim3(64), 0, c.stream, iterc, kChunk);
   };
@@ -1685,7 +1724,7 @@ static int ms_cg_basis(Context &c, const double2 *b, int nrhs, int nshift, const
   for (int j = 0; j < nrhs; j++) kmax = std::max(kmax, hs[j].iter);
   // basis exhausted before convergence: let the caller use the explicit algorithm
   for (int j = 0; j < nrhs; j++)
-    if (hs[j].nact > 0 && hs[j].iter + 1 >= cap && hs[j].iter < max_iter) return 3;
+    if (hs[j].nact > 0 && hs[j].iter + 2 >= cap && hs[j].iter < max_iter) return 3;
   // coefficients d_{s,i} = -z_s^(i) S_i, S_{K-1} = om^(K), S_i = om^(i+1) + ga^(i+1) S_{i+1}
   std::vector<BmsHist> hh_host((size_t)nrhs * cap * nshift);
   for (int j = 0; j < nrhs; j++)
@@ -1718,10 +1757,10 @@ static int ms_cg_basis(Context &c, const double2 *b, int nrhs, int nshift, const
   for (int s0 = 0; s0 < nshift; s0 += 16)
     if (x32)
       hipLaunchKernelGGL((k_bms_assemble<R, 16, float2>), dim3((unsigned)((3 * Vh + kBlock - 1) / kBlock), nrhs),
-                         dim3(kBlock), 0, c.stream, basis, dcoef, dklen, x32, Vh, (int)cap, nshift, s0);
+                         dim3(kBlock), 0, c.stream, basis, dcoef, dklen, x32, Vh, (int)cap, nshift, s0, nrhs);
     else
       hipLaunchKernelGGL((k_bms_assemble<R, 16, double2>), dim3((unsigned)((3 * Vh + kBlock - 1) / kBlock), nrhs),
-                         dim3(kBlock), 0, c.stream, basis, dcoef, dklen, x, Vh, (int)cap, nshift, s0);
+                         dim3(kBlock), 0, c.stream, basis, dcoef, dklen, x, Vh, (int)cap, nshift, s0, nrhs);
   if (!hip_ok(c, hipStreamSynchronize(c.stream), "basis assemble")) return -1;
   *iters_out = kmax;
   for (int j = 0; j < nrhs; j++)
