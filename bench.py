@@ -262,6 +262,18 @@ def wl_traj48c12(args):
         t64.append(-r.seconds_total)
     fp64 = float(np.mean([t for t in t64 if t > 0]))
     mixed_same = float(np.mean([-t for t in t64 if t < 0]))
+    # Dslash at this volume: one FP32 12-real half-Dslash on the 2 pseudofermions
+    # is 318 MB and one FP64 18-real on one vector 828 MB, beyond the 256 MB
+    # Infinity Cache, so even back-to-back launches stream from HBM; the cold
+    # figure evicts the caches with a 512 MB write before every launch.
+    ds = {}
+    for name, prec, nv, lb, sb in (("fp32_12real_2rhs", 1, n_pf, 48, 24), ("fp64_18real_1rhs", 0, 1, 144, 48)):
+        by = dslash_bytes(lat.Vh, nv, lb, sb)
+        ms = g.time_dslash(prec, nv, 50)
+        msc = g.time_dslash_cold(prec, nv, 20)
+        ds[name] = {"bytes": by, "ms": ms, "GBps": by / (ms * 1e-3) / 1e9,
+                    "frac_hbm": by / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                    "cold_ms": msc, "cold_GBps": by / (msc * 1e-3) / 1e9}
     g.close()
     return {"metric": "RHMC trajectory seconds, mixed vs FP64, 48^3x12", "unit": "s/traj",
             "value": float(np.mean([c["s"] for c in chain])),
@@ -272,7 +284,7 @@ def wl_traj48c12(args):
             "acceptance": float(np.mean([c["accepted"] for c in chain])),
             "max_delta_h_error_vs_fp64_1e-13": float(max(c["delta_h_err"] for c in chain)),
             "chain": chain, "fp64_s_per_traj_same_start": fp64, "mixed_s_per_traj_same_start": mixed_same,
-            "mixed_vs_fp64_speedup": fp64 / mixed_same, "spectral_bounds": [mn, mx]}
+            "mixed_vs_fp64_speedup": fp64 / mixed_same, "spectral_bounds": [mn, mx], "dslash": ds}
 
 
 def wl_multi(args):

@@ -396,11 +396,12 @@ struct HopIn {
 // backward -U_mu(x-mu)^dag psi(x-mu), link at x-mu (the other parity)
 template <typename R, int LINK, int NR>
 __device__ __forceinline__ void hop_load(const DsArgs &a, const Cx<R> *const *inr, const SiteCoord &c, int h, int k,
-                                         HopIn<R, LINK, NR> &d) {
+                                         HopIn<R, LINK, NR> &d, bool spinors = true) {
   const int mu = k >> 1;
   const bool fwd = (k & 1) == 0;
   const int hn = neighbour_i(a.L, c, mu, fwd ? +1 : -1);
   link_raw_load<R, LINK>(a, mu, fwd ? a.par_out : 1 - a.par_out, fwd ? h : hn, d.U);
+  if (!spinors) return;
   const long Vh = a.L.Vh;
   const unsigned off = (unsigned)hn * (unsigned)sizeof(Cx<R>);
 #pragma unroll
@@ -409,6 +410,33 @@ __device__ __forceinline__ void hop_load(const DsArgs &a, const Cx<R> *const *in
     d.v[r][1] = ld_off(inr[r] + Vh, off);
     d.v[r][2] = ld_off(inr[r] + 2 * Vh, off);
   }
+}
+
+// Lane i of each 16-lane row receives lane (i-1) mod 16 (DPP row_ror:1).
+__device__ __forceinline__ float row_ror1(float v) {
+  return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), 0x121, 0xF, 0xF, false));
+}
+__device__ __forceinline__ double row_ror1(double v) {
+  const long long b = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_mov_dpp((int)(b & 0xffffffffLL), 0x121, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp((int)(b >> 32), 0x121, 0xF, 0xF, false);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// With 16 half-sites per x-row (nx = 32) a wave holds whole rows, one per
+// 16-lane DPP row, and the x-1 neighbour of a site is the x+1 neighbour of
+// the site one lane to the left in the same row (cyclically, which is the
+// periodic wrap in x): the backward x hop takes its spinors from the
+// forward hop's registers instead of loading them.
+template <typename R, int LINK, int NR>
+__device__ __forceinline__ void xm_from_xp(const HopIn<R, LINK, NR> &xp, HopIn<R, LINK, NR> &xm) {
+#pragma unroll
+  for (int r = 0; r < NR; r++)
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      xm.v[r][i].x = row_ror1(xp.v[r][i].x);
+      xm.v[r][i].y = row_ror1(xp.v[r][i].y);
+    }
 }
 
 // Ends a hop: the accumulators pass through an empty asm statement (in-out
@@ -459,13 +487,14 @@ static const void *link_ptr(Context &c) {
 
 // Waves per SIMD the Dslash kernels are compiled for: the registers of two
 // hops in flight (link + NR neighbour vectors each), the NR accumulators and
-// ~32 for addresses and coordinates must fit the wave's share of the 512.
+// ~48 for addresses, coordinates and the x-row path must fit the wave's
+// share of the 512.
 template <typename R, int LINK, int NR>
 constexpr int ds_waves() {
   constexpr int rs = (int)sizeof(R) / 4;
   constexpr int link = LINK == LINK_D18 ? 36 : LINK == LINK_D12 ? 24 : 12;
   constexpr int hop = link + NR * 6 * rs;
-  constexpr int need = 2 * hop + NR * 6 * rs + 32;
+  constexpr int need = 2 * hop + NR * 6 * rs + 48;
   constexpr int w = 512 / need;
   return w < 1 ? 1 : (w > 8 ? 8 : w);
 }
@@ -549,6 +578,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(ds_waves
     const int j = jbase + r < a.nvec ? jbase + r : a.nvec - 1;
     inr[r] = inb + (long)j * a.in_vs;
   }
+  const bool xrow = L.nxh == 16;
   for (int q = q0; any && q < q1; q++) {
     const int h = (a.tiles ? a.tiles[q] : q) * kBlock + (int)threadIdx.x;
     if (h >= Vh32) continue;
@@ -564,8 +594,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(ds_waves
     hop_load<R, LINK, NR>(a, inr, c, h, 0, buf[0]);
 #pragma unroll
     for (int k = 0; k < 8; k++) {
-      if (k + 1 < 8) hop_load<R, LINK, NR>(a, inr, c, h, k + 1, buf[(k + 1) & 1]);
+      if (k + 1 < 8) hop_load<R, LINK, NR>(a, inr, c, h, k + 1, buf[(k + 1) & 1], k + 1 != 1 || !xrow);
       hop_apply<R, LINK, NR>(L, c, k, buf[k & 1], acc);
+      if (k == 0 && xrow) xm_from_xp<R, LINK, NR>(buf[0], buf[1]);
       hop_fence<R, NR>(acc);
     }
     ds_site_out<R, NR, MODE, RED>(a, acc, act, red, jbase, h, pinb);
