@@ -441,8 +441,8 @@ def main():
     live = []
     for s in range(args.steps):
         for leg in legs:
-            # the MD hop (FP32 12-real, n_pf RHS) is timed live with HIP
-            # events on the library stream during the timed mixed trajectories
+            # the MD hop (FP32 12-real, n_pf RHS) times itself on the device
+            # wall clock during the timed mixed trajectories
             g.kernel_timing(1 if leg == "mixed" else 0)
             dt, r = run(leg, 1000 + s)
             if leg == "mixed":
@@ -524,7 +524,9 @@ def main():
         "roofline": {"bound": "hbm", "kernel": "k_dslash<float,S12,HOP> on 8 rhs (MD solver half-lattice hop)",
                      "achieved": gbs_live, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs_live / HBM_PEAK_GBS,
                      "traffic": traffic, "bytes_per_launch": b32,
-                     "measured": f"HIP events around each of {n_live} launches inside the timed mixed trajectories"},
+                     "measured": f"mean kernel duration of {n_live} launches inside the timed mixed trajectories "
+                                 f"(graphs replayed), each timed on the device wall clock from the start of "
+                                 f"workgroup 0 to the end of the last workgroup"},
         "cpu_baseline": cpu,
         "dslash_autotune": g.tuning_table(),
     }

@@ -167,11 +167,15 @@ int gpu_time_dslash(rhmc_gpu_t h, int precision, int nvec, int reps, double *ms)
  * times. */
 int gpu_time_dslash_cold(rhmc_gpu_t h, int precision, int nvec, int reps, double flush_bytes, double *ms);
 
-/* Live kernel timing inside real work: while enabled, every FP32 12-real
- * half-lattice Dslash on n_pf right-hand sides (the MD solver's hop) is
- * bracketed by HIP events on the library stream.  Returns the mean duration
- * and count recorded since the last reset; enable = 1/0 then switches it on
- * or off and resets the counters, enable = -1 only reads. */
+/* Live kernel timing inside real work: while enabled, the MD solver's hop
+ * (FP32 12-real half-lattice Dslash on the n_pf pseudofermions) records its
+ * own duration on the device wall clock, from the start of workgroup 0 to
+ * the end of the last workgroup (the interval a kernel-trace profiler
+ * reports), in a slot indexed by the solver iteration, so replayed graphs
+ * are timed too.  Only iterations in which the hop did work are counted.
+ * Returns the mean duration and count since the last reset; enable = 1/0
+ * then switches timing on or off and resets, enable = -1 only reads.  No
+ * counterpart in the reference. */
 int gpu_kernel_timing(rhmc_gpu_t h, int enable, double *mean_ms, long long *count);
 
 /* Timing helper: a plain device copy of `bytes` (read + write), mean ms per

@@ -530,10 +530,10 @@ int gpu_kernel_timing(rhmc_gpu_t h, int enable, double *mean_ms, long long *coun
   if (!h) return RHMC_EINVAL;
   Context &c = h->c;
   hipSetDevice(c.device);
-  kernel_timing_harvest(c);
   if (mean_ms) *mean_ms = c.kt_count ? c.kt_sum_ms / c.kt_count : 0.0;
   if (count) *count = c.kt_count;
   if (enable >= 0) {
+    if (enable && kernel_timing_clock(c)) return RHMC_EHIP;
     c.time_kernels = enable;
     c.kt_sum_ms = 0.0;
     c.kt_count = 0;

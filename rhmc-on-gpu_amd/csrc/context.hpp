@@ -76,10 +76,11 @@ struct Context {
   // n_pf right-hand sides (the MD solver's Dslash); graphs are not replayed
   // while it is on
   int time_kernels = 0;
-  std::vector<hipEvent_t> kt_free;
-  std::vector<hipEvent_t> kt_pending;  // start/stop pairs
-  double kt_sum_ms = 0.0;
+  unsigned long long *kt_clk = nullptr; // per solver iteration: workgroup-0 start, last workgroup end
+  int kt_clk_cap = 0;
+  double kt_sum_ms = 0.0;               // kernel durations on the device wall clock
   long kt_count = 0;
+  double wall_khz = 0.0;
 
   Workspace *ws = nullptr;
   std::string last_error;
@@ -146,9 +147,12 @@ int vec_dot_diffs(Context &c, const double2 *a1, const double2 *b1, const double
 
 // Dslash tile order for the lattice of c (solver.hip); allocates c.tiles
 int build_tile_order(Context &c);
-// fold the recorded kernel-timing event pairs into kt_sum_ms / kt_count
-void kernel_timing_harvest(Context &c);
+// fold the device-clock timing slots of the first n solver iterations into
+// kt_sum_ms / kt_count and clear them; release frees the slots
+void kernel_timing_harvest(Context &c, int n);
 void kernel_timing_release(Context &c);
+// allocate and clear the slots (timing switched on)
+int kernel_timing_clock(Context &c);
 
 // host helpers
 bool hip_ok(Context &c, hipError_t e, const char *what);

@@ -36,19 +36,55 @@ __device__ __forceinline__ void shift_coord(const LatticeDims &L, int &x, int &y
 // holds the four F_mu in registers (FP64) and reads each spinor of its site
 // once.  C = double2 (FP64 solutions) or float2 (FP32 solutions of the FP32
 // MD solves).
+// One thread per site; the systems are taken in pairs with all 30 loads of
+// a pair issued before its multiply-adds (uniform base + 32-bit lane offset
+// loads), so each wave keeps two systems' operands in flight.
+template <typename C, int NQ>
+__device__ __forceinline__ void ff_systems(const C *own_v, const C *nb_v, long Vh, int j0, unsigned ho,
+                                           const unsigned (&nbo)[4], const double *coef, double sgn,
+                                           Mat<double> (&F)[4]) {
+  C own[NQ][3], w[NQ][4][3];
+#pragma unroll
+  for (int q = 0; q < NQ; q++) {
+    const long base = (long)(j0 + q) * 3 * Vh;
+#pragma unroll
+    for (int k = 0; k < 3; k++) own[q][k] = ld_off(own_v + base + (long)k * Vh, ho);
+#pragma unroll
+    for (int mu = 0; mu < 4; mu++)
+#pragma unroll
+      for (int i = 0; i < 3; i++) w[q][mu][i] = ld_off(nb_v + base + (long)i * Vh, nbo[mu]);
+  }
+#pragma unroll
+  for (int q = 0; q < NQ; q++) {
+    const double a = sgn * coef[j0 + q];
+    double2 o[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) o[k] = make_double2((double)own[q][k].x, -(double)own[q][k].y);  // conjugate
+#pragma unroll
+    for (int mu = 0; mu < 4; mu++)
+#pragma unroll
+      for (int i = 0; i < 3; i++) {
+        const double2 v = make_double2(a * (double)w[q][mu][i].x, a * (double)w[q][mu][i].y);
+#pragma unroll
+        for (int k = 0; k < 3; k++) cfma(F[mu].e[i][k], v, o[k]);
+      }
+  }
+}
+
 template <typename C>
 __global__ __launch_bounds__(kBlock) void k_fermion_force(LatticeDims L, const double2 *U, const C *s,
                                                           const C *hv, const double *coef, int nsys,
                                                           double2 *ipdot, int accumulate) {
   const long Vh = L.Vh;
   const int bx = xcd_block(blockIdx.x, gridDim.x);
-  const long h = (long)bx * kBlock + threadIdx.x;
+  const int h = bx * kBlock + (int)threadIdx.x;
   const int par = blockIdx.y;
-  if (h >= Vh) return;
-  const SiteCoord c = site_coords(L, par, h);
-  long nb[4];
+  if (h >= (int)Vh) return;
+  const SiteCoord c = site_coords_i(L, par, h);
+  unsigned nbo[4];
 #pragma unroll
-  for (int mu = 0; mu < 4; mu++) nb[mu] = neighbour(L, c, mu, +1);
+  for (int mu = 0; mu < 4; mu++) nbo[mu] = (unsigned)neighbour_i(L, c, mu, +1) * (unsigned)sizeof(C);
+  const unsigned ho = (unsigned)h * (unsigned)sizeof(C);
   Mat<double> F[4];
 #pragma unroll
   for (int mu = 0; mu < 4; mu++)
@@ -59,26 +95,9 @@ __global__ __launch_bounds__(kBlock) void k_fermion_force(LatticeDims L, const d
   const C *own_v = par == 0 ? s : hv;
   const C *nb_v = par == 0 ? hv : s;
   const double sgn = par == 0 ? 1.0 : -1.0;
-  for (int j = 0; j < nsys; j++) {
-    const double a = sgn * coef[j];
-    const long base = (long)j * 3 * Vh;
-    double2 own[3];
-#pragma unroll
-    for (int k = 0; k < 3; k++) {
-      const C o = own_v[base + (long)k * Vh + h];
-      own[k] = make_double2((double)o.x, -(double)o.y);  // conjugate
-    }
-#pragma unroll
-    for (int mu = 0; mu < 4; mu++) {
-#pragma unroll
-      for (int i = 0; i < 3; i++) {
-        const C w = nb_v[base + (long)i * Vh + nb[mu]];
-        double2 v = make_double2(a * (double)w.x, a * (double)w.y);
-#pragma unroll
-        for (int k = 0; k < 3; k++) cfma(F[mu].e[i][k], v, own[k]);
-      }
-    }
-  }
+  int j = 0;
+  for (; j + 2 <= nsys; j += 2) ff_systems<C, 2>(own_v, nb_v, Vh, j, ho, nbo, coef, sgn, F);
+  if (j < nsys) ff_systems<C, 1>(own_v, nb_v, Vh, j, ho, nbo, coef, sgn, F);
 #pragma unroll
   for (int mu = 0; mu < 4; mu++) {
     const Mat<double> Um = load_link_d18(U, Vh, mu, par, h);
@@ -136,45 +155,63 @@ int fermion_force32(Context &c, int nsys, const double *num, const float2 *x32, 
 // staples.cc:16-74, ipdot.cc:25-79:
 //   ipdot_mu(x) = TA(U_mu(x) beta/3 sum_{nu!=mu} [U_nu(x+mu) U_mu(x+nu)^+ U_nu(x)^+
 //                                              + U_nu(x+mu-nu)^+ U_mu(x-nu)^+ U_nu(x-nu)])
-__global__ __launch_bounds__(kBlock) void k_gauge_force(LatticeDims L, const double2 *U, double b3, double2 *ipdot,
+// one step along mu with the periodic wrap, no division
+__device__ __forceinline__ void step_coord(const LatticeDims &L, int &x, int &y, int &z, int &t, int mu, int d) {
+  if (mu == 0) x = d > 0 ? (x + 1 == L.nx ? 0 : x + 1) : (x == 0 ? L.nx - 1 : x - 1);
+  else if (mu == 1) y = d > 0 ? (y + 1 == L.ny ? 0 : y + 1) : (y == 0 ? L.ny - 1 : y - 1);
+  else if (mu == 2) z = d > 0 ? (z + 1 == L.nz ? 0 : z + 1) : (z == 0 ? L.nz - 1 : z - 1);
+  else t = d > 0 ? (t + 1 == L.nt ? 0 : t + 1) : (t == 0 ? L.nt - 1 : t - 1);
+}
+
+// link (x, mu) from the FP64 12-real copy, row 2 rebuilt with the link's eta
+__device__ __forceinline__ Mat<double> link12_at(const double2 *U12, const LatticeDims &L, int x, int y, int z, int t,
+                                                 int mu) {
+  return load_link_d12(U12, L.Vh, mu, (x + y + z + t) & 1, half_index_i(L, x, y, z, t),
+                       eta_phase(L, x, y, z, t, mu));
+}
+
+// staples.cc:16-74 + ipdot.cc:25-79: ipdot_mu(x) = TA(U_mu(x) beta/3 sum_nu
+// [U_nu(x+mu) U_mu(x+nu)^+ U_nu(x)^+ + U_nu(x+mu-nu)^+ U_mu(x-nu)^+ U_nu(x-nu)]).
+// Links from the FP64 12-real copy (96 instead of 144 bytes per load; each
+// thread loads 19), neighbours by compare-and-wrap in 32-bit arithmetic.
+__global__ __launch_bounds__(kBlock) void k_gauge_force(LatticeDims L, const double2 *U12, double b3, double2 *ipdot,
                                                         int accumulate) {
   const long Vh = L.Vh;
   const int bx = xcd_block(blockIdx.x, gridDim.x);
-  const long h = (long)bx * kBlock + threadIdx.x;
+  const int h = bx * kBlock + (int)threadIdx.x;
   const int par = blockIdx.y;
   const int mu = blockIdx.z;
-  if (h >= Vh) return;
-  const SiteCoord c = site_coords(L, par, h);
+  if (h >= (int)Vh) return;
+  const SiteCoord c = site_coords_i(L, par, h);
   Mat<double> st;
 #pragma unroll
   for (int i = 0; i < 3; i++)
 #pragma unroll
     for (int k = 0; k < 3; k++) st.e[i][k] = make_double2(0.0, 0.0);
+  int x1 = c.x, y1 = c.y, z1 = c.z, t1 = c.t;  // x + mu
+  step_coord(L, x1, y1, z1, t1, mu, +1);
   for (int nu = 0; nu < 4; nu++) {
     if (nu == mu) continue;
-    int x = c.x, y = c.y, z = c.z, t = c.t;
     // upper: U_nu(x+mu) U_mu(x+nu)^+ U_nu(x)^+
-    int x1 = x, y1 = y, z1 = z, t1 = t;
-    shift_coord(L, x1, y1, z1, t1, mu, +1);
-    int x2 = x, y2 = y, z2 = z, t2 = t;
-    shift_coord(L, x2, y2, z2, t2, nu, +1);
-    Mat<double> a = mmul_nd(link_at(U, L, x1, y1, z1, t1, nu), link_at(U, L, x2, y2, z2, t2, mu));
-    a = mmul_nd(a, link_at(U, L, x, y, z, t, nu));
+    int x2 = c.x, y2 = c.y, z2 = c.z, t2 = c.t;
+    step_coord(L, x2, y2, z2, t2, nu, +1);
+    Mat<double> a = mmul_nd(link12_at(U12, L, x1, y1, z1, t1, nu), link12_at(U12, L, x2, y2, z2, t2, mu));
+    a = mmul_nd(a, link12_at(U12, L, c.x, c.y, c.z, c.t, nu));
     // lower: U_nu(x+mu-nu)^+ U_mu(x-nu)^+ U_nu(x-nu)
-    int x3 = x, y3 = y, z3 = z, t3 = t;
-    shift_coord(L, x3, y3, z3, t3, nu, -1);
-    int x4 = x3, y4 = y3, z4 = z3, t4 = t3;
-    shift_coord(L, x4, y4, z4, t4, mu, +1);
-    const Mat<double> l1 = link_at(U, L, x4, y4, z4, t4, nu);
-    const Mat<double> l2 = link_at(U, L, x3, y3, z3, t3, mu);
+    int x3 = c.x, y3 = c.y, z3 = c.z, t3 = c.t;
+    step_coord(L, x3, y3, z3, t3, nu, -1);
+    int x4 = x1, y4 = y1, z4 = z1, t4 = t1;
+    step_coord(L, x4, y4, z4, t4, nu, -1);
+    const Mat<double> l1 = link12_at(U12, L, x4, y4, z4, t4, nu);
+    const Mat<double> l2 = link12_at(U12, L, x3, y3, z3, t3, mu);
     Mat<double> b = mmul_dn(l1, mdag(l2));
-    b = mmul(b, link_at(U, L, x3, y3, z3, t3, nu));
+    b = mmul(b, link12_at(U12, L, x3, y3, z3, t3, nu));
 #pragma unroll
     for (int i = 0; i < 3; i++)
 #pragma unroll
       for (int k = 0; k < 3; k++) st.e[i][k] = cadd(st.e[i][k], cadd(a.e[i][k], b.e[i][k]));
   }
-  const Mat<double> Um = load_link_d18(U, Vh, mu, par, h);
+  const Mat<double> Um = link12_at(U12, L, c.x, c.y, c.z, c.t, mu);
   Mat<double> T = mmul(Um, st);
 #pragma unroll
   for (int i = 0; i < 3; i++)
@@ -192,7 +229,7 @@ __global__ __launch_bounds__(kBlock) void k_gauge_force(LatticeDims L, const dou
 }
 
 int gauge_force(Context &c, int accumulate) {
-  hipLaunchKernelGGL(k_gauge_force, dim3(nblk(c.Vh), 2, 4), dim3(kBlock), 0, c.stream, c.L, c.U64, c.beta / 3.0,
+  hipLaunchKernelGGL(k_gauge_force, dim3(nblk(c.Vh), 2, 4), dim3(kBlock), 0, c.stream, c.L, c.U64c, c.beta / 3.0,
                      c.ipdot, accumulate);
   return hip_ok(c, hipGetLastError(), "gauge force") ? 0 : -1;
 }

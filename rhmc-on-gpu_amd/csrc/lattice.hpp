@@ -252,6 +252,13 @@ template <typename R> __device__ __forceinline__ void sunitarize(Mat<R> &m) {
 }
 
 // --------------------------------------------------------------- link I/O
+// load at a workgroup-uniform base plus a 32-bit per-lane byte offset (the
+// SGPR-base global load: one offset register serves many loads)
+template <typename T>
+__device__ __forceinline__ T ld_off(const T *base, unsigned byte_off) {
+  return *(const T *)((const char *)base + byte_off);
+}
+
 __device__ __forceinline__ Mat<double> load_link_d18(const double2 *__restrict__ U, long Vh, int mu, int par, long h) {
   Mat<double> m;
   const double2 *b = U + (long)(mu * 2 + par) * 9 * Vh + h;

@@ -326,8 +326,28 @@ struct DsArgs {
   // iteration-indexed operand: in (iter_in) and pin (iter_pin) advance by
   // (*iterp + iter_off) * iter_stride elements (a basis slot chosen on the device)
   const int *iterp; int iter_off; long iter_stride; int iter_in, iter_pin;
+  // live kernel timing (solver hop): slot (*iterp + iter_off) of clk holds
+  // the start of workgroup 0 and the latest workgroup end (device wall clock)
+  unsigned long long *clk; int clk_cap;
   RedArgs red;
 };
+
+__device__ __forceinline__ unsigned long long *clk_slot(const DsArgs &a) {
+  if (!a.clk || !a.iterp) return nullptr;
+  const int s = *a.iterp + a.iter_off;
+  return s >= 0 && s < a.clk_cap ? a.clk + 2 * (long)s : nullptr;
+}
+__device__ __forceinline__ void clk_start(const DsArgs &a) {
+  unsigned long long *t = clk_slot(a);
+  if (t && blockIdx.x == 0 && threadIdx.x == 0) t[0] = (unsigned long long)wall_clock64();
+}
+__device__ __forceinline__ void clk_end(const DsArgs &a) {
+  unsigned long long *t = clk_slot(a);
+  if (t) {
+    __syncthreads();
+    if (threadIdx.x == 0) atomicMax(&t[1], (unsigned long long)wall_clock64());
+  }
+}
 
 // element offset of the iteration-indexed operand
 __device__ __forceinline__ long ds_iter_shift(const DsArgs &a) {
@@ -343,13 +363,6 @@ struct LinkRaw {
   using T = typename std::conditional<LINK == LINK_S12, float4, double2>::type;
   T e[N];
 };
-
-// load at a workgroup-uniform base plus a 32-bit per-lane byte offset (one
-// offset register serves every load of a hop)
-template <typename T>
-__device__ __forceinline__ T ld_off(const T *base, unsigned byte_off) {
-  return *(const T *)((const char *)base + byte_off);
-}
 
 template <typename R, int LINK>
 __device__ __forceinline__ void link_raw_load(const DsArgs &a, int mu, int par, int h, LinkRaw<R, LINK> &w) {
@@ -542,6 +555,7 @@ __device__ __forceinline__ void ds_site_out(const DsArgs &a, const Cx<R> (&acc)[
 template <typename R, int LINK, int NR, int MODE, int RED>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(ds_waves<R, LINK, NR>()))) void k_dslash(DsArgs a) {
   using C = Cx<R>;
+  clk_start(a);
   const LatticeDims &L = a.L;
   const long Vh = L.Vh;
   unsigned sb, ch;
@@ -602,6 +616,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(ds_waves
     ds_site_out<R, NR, MODE, RED>(a, acc, act, red, jbase, h, pinb);
   }
   if (RED != RED_NONE) block_partials<NR>(red, a.red.partial, jbase, a.nvec, a.nsb, sb);
+  clk_end(a);
 }
 
 // Dslash traversal order.  The hardware hands consecutive workgroups to the
@@ -715,6 +730,7 @@ __device__ __forceinline__ C pick3(C a0, C a1, C a2, int k) {
 template <typename R, int LINK, int NR, int MODE, int RED>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(ds_waves<R, LINK, NR>()))) void k_dslash_ls(DsArgs a) {
   using C = Cx<R>;
+  clk_start(a);
   const LatticeDims &L = a.L;
   const long Vh = L.Vh;
   const int Vh32 = (int)Vh;
@@ -805,6 +821,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(ds_waves
     }
   }
   if (RED != RED_NONE) block_partials<NR>(red, a.red.partial, jbase, a.nvec, a.nsb, sb);
+  clk_end(a);
 }
 
 // number of right-hand sides handled per thread for a launch with nvec vectors
@@ -833,6 +850,7 @@ static inline int shape_lanes(int code) { return code >> 4; }
 
 template <typename R, int LINK, int MODE, int RED>
 static void dslash_launch(Context &c, DsArgs &a, int code) {
+#define DS_LAUNCH(KERNEL) hipLaunchKernelGGL(KERNEL, grid, dim3(kBlock), 0, c.stream, a)
   const int nr = shape_nr(code), lanes = shape_lanes(code) == 4 ? 4 : 1;
   a.L = c.L;
   a.tiles = c.tiles;
@@ -843,19 +861,20 @@ static void dslash_launch(Context &c, DsArgs &a, int code) {
   dim3 grid(a.nsb * a.nch);
   if (lanes == 4) {
     switch (nr) {
-      case 8: hipLaunchKernelGGL((k_dslash_ls<R, LINK, 8, MODE, RED>), grid, dim3(kBlock), 0, c.stream, a); break;
-      case 4: hipLaunchKernelGGL((k_dslash_ls<R, LINK, 4, MODE, RED>), grid, dim3(kBlock), 0, c.stream, a); break;
-      case 2: hipLaunchKernelGGL((k_dslash_ls<R, LINK, 2, MODE, RED>), grid, dim3(kBlock), 0, c.stream, a); break;
-      default: hipLaunchKernelGGL((k_dslash_ls<R, LINK, 1, MODE, RED>), grid, dim3(kBlock), 0, c.stream, a); break;
+      case 8: DS_LAUNCH((k_dslash_ls<R, LINK, 8, MODE, RED>)); break;
+      case 4: DS_LAUNCH((k_dslash_ls<R, LINK, 4, MODE, RED>)); break;
+      case 2: DS_LAUNCH((k_dslash_ls<R, LINK, 2, MODE, RED>)); break;
+      default: DS_LAUNCH((k_dslash_ls<R, LINK, 1, MODE, RED>)); break;
     }
     return;
   }
   switch (nr) {
-    case 8: hipLaunchKernelGGL((k_dslash<R, LINK, 8, MODE, RED>), grid, dim3(kBlock), 0, c.stream, a); break;
-    case 4: hipLaunchKernelGGL((k_dslash<R, LINK, 4, MODE, RED>), grid, dim3(kBlock), 0, c.stream, a); break;
-    case 2: hipLaunchKernelGGL((k_dslash<R, LINK, 2, MODE, RED>), grid, dim3(kBlock), 0, c.stream, a); break;
-    default: hipLaunchKernelGGL((k_dslash<R, LINK, 1, MODE, RED>), grid, dim3(kBlock), 0, c.stream, a); break;
+    case 8: DS_LAUNCH((k_dslash<R, LINK, 8, MODE, RED>)); break;
+    case 4: DS_LAUNCH((k_dslash<R, LINK, 4, MODE, RED>)); break;
+    case 2: DS_LAUNCH((k_dslash<R, LINK, 2, MODE, RED>)); break;
+    default: DS_LAUNCH((k_dslash<R, LINK, 1, MODE, RED>)); break;
   }
+#undef DS_LAUNCH
 }
 
 // Autotuner: the launch shape -- right-hand sides per thread (1, 2, 4, 8) and
@@ -906,45 +925,50 @@ static void dslash_dispatch(Context &c, DsArgs &a) {
     hipEventDestroy(e1);
     tab[key] = nr;
   }
-  if (c.time_kernels && MODE == DS_HOP && sizeof(R) == 4 && a.nvec == c.n_pf) {
-    hipEvent_t ev[2];
-    for (auto &e : ev) {
-      if (c.kt_free.empty()) {
-        hipEventCreate(&e);
-      } else {
-        e = c.kt_free.back();
-        c.kt_free.pop_back();
-      }
-    }
-    hipEventRecord(ev[0], c.stream);
-    dslash_launch<R, LINK, MODE, RED>(c, a, nr);
-    hipEventRecord(ev[1], c.stream);
-    c.kt_pending.push_back(ev[0]);
-    c.kt_pending.push_back(ev[1]);
-    return;
-  }
   dslash_launch<R, LINK, MODE, RED>(c, a, nr);
 }
 
-void kernel_timing_harvest(Context &c) {
-  if (c.kt_pending.empty()) return;
+// Fold the device-clock slots of one solve (iterations 0..n-1 of the hop
+// input) into the context's totals and clear them for the next solve.
+__global__ void k_clk_clear(unsigned long long *t, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < 2 * n) t[i] = 0ULL;
+}
+
+void kernel_timing_harvest(Context &c, int n) {
+  if (!c.kt_clk || n <= 0) return;
+  n = std::min(n, c.kt_clk_cap);
+  std::vector<unsigned long long> t(2 * (size_t)n);
+  hipMemcpyAsync(t.data(), c.kt_clk, sizeof(unsigned long long) * t.size(), hipMemcpyDeviceToHost, c.stream);
+  hipLaunchKernelGGL(k_clk_clear, dim3((2 * n + 255) / 256), dim3(256), 0, c.stream, c.kt_clk, n);
   hipStreamSynchronize(c.stream);
-  for (size_t i = 0; i + 1 < c.kt_pending.size(); i += 2) {
-    float t = 0.f;
-    if (hipEventElapsedTime(&t, c.kt_pending[i], c.kt_pending[i + 1]) == hipSuccess) {
-      c.kt_sum_ms += t;
+  for (int i = 0; i < n; i++)
+    if (t[2 * i] != 0 && t[2 * i + 1] > t[2 * i]) {
+      c.kt_sum_ms += (double)(t[2 * i + 1] - t[2 * i]) / c.wall_khz;
       c.kt_count++;
     }
-    c.kt_free.push_back(c.kt_pending[i]);
-    c.kt_free.push_back(c.kt_pending[i + 1]);
-  }
-  c.kt_pending.clear();
 }
 
 void kernel_timing_release(Context &c) {
-  kernel_timing_harvest(c);
-  for (auto e : c.kt_free) hipEventDestroy(e);
-  c.kt_free.clear();
+  if (c.kt_clk) hipFree(c.kt_clk);
+  c.kt_clk = nullptr;
+  c.kt_clk_cap = 0;
+}
+
+// device-clock slots of the live kernel timing, one per solver iteration
+// (allocated and cleared when timing is switched on)
+int kernel_timing_clock(Context &c) {
+  if (c.kt_clk) return 0;
+  int khz = 0;
+  if (!hip_ok(c, hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, c.device), "wall clock rate") ||
+      khz <= 0)
+    return -1;
+  c.wall_khz = (double)khz;
+  const int cap = 16384;
+  if (!hip_ok(c, hipMalloc((void **)&c.kt_clk, sizeof(unsigned long long) * 2 * cap), "timing slots")) return -1;
+  c.kt_clk_cap = cap;
+  hipLaunchKernelGGL(k_clk_clear, dim3((2 * cap + 255) / 256), dim3(256), 0, c.stream, c.kt_clk, cap);
+  return hip_ok(c, hipStreamSynchronize(c.stream), "timing slots") ? 0 : -1;
 }
 
 std::string tuning_table(Context &c) {
@@ -1689,9 +1713,12 @@ static int ms_cg_basis(Context &c, const double2 *b, int nrhs, int nshift, const
   hop.in = basis; hop.in_vs = 3 * Vh; hop.out = hh; hop.out_vs = 3 * Vh; hop.nvec = nrhs;
   hop.skip_stride = sizeof(BmsShared) / sizeof(int);
   hop.iterp = iterc; hop.iter_stride = (long)nrhs * 3 * Vh; hop.iter_in = 1;
+  // live timing of the MD solver's hop (FP32, all pseudofermions)
+  const bool timed = c.time_kernels && c.kt_clk && sizeof(R) == 4 && nrhs == c.n_pf;
+  if (timed) { hop.clk = c.kt_clk; hop.clk_cap = c.kt_clk_cap; }
   DsArgs mdm = hop;
   mdm.par_out = 0; mdm.in = hh; mdm.in_vs = 3 * Vh; mdm.out = wv; mdm.pin = basis; mdm.pin_vs = 3 * Vh;
-  mdm.diag = diag; mdm.iter_in = 0; mdm.iter_pin = 1;
+  mdm.diag = diag; mdm.iter_in = 0; mdm.iter_pin = 1; mdm.clk = nullptr;
   mdm.red = make_red(c, nrhs, EPI_STORE, nullptr, nullptr);
   a.partB = mdm.red.partial;
 
@@ -1720,7 +1747,7 @@ static int ms_cg_basis(Context &c, const double2 *b, int nrhs, int nshift, const
   bool converged = false;
   GraphSlot &gs = c.ws->bms_graph[sizeof(R) == 8 ? 1 : 0];
   while (true) {
-    if (k == 1 || !c.use_graphs || c.time_kernels) {
+    if (k == 1 || !c.use_graphs) {
       chunk_launch();
     } else {
       BmsGraphKey key;
@@ -1753,6 +1780,8 @@ static int ms_cg_basis(Context &c, const double2 *b, int nrhs, int nshift, const
   (void)converged;
   int kmax = 0;
   for (int j = 0; j < nrhs; j++) kmax = std::max(kmax, hs[j].iter);
+  // the hop ran with work at iterations 1..kmax+1 (input slots 0..kmax)
+  if (timed) kernel_timing_harvest(c, kmax + 1);
   // basis exhausted before convergence: let the caller use the explicit algorithm
   for (int j = 0; j < nrhs; j++)
     if (hs[j].nact > 0 && hs[j].iter + 2 >= cap && hs[j].iter < max_iter) return 3;

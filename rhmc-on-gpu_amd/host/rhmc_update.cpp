@@ -366,7 +366,6 @@ int run_trajectory(Context &c, const rhmc_traj_params &p, rhmc_traj_result &r) {
     }
   }
   if (!hip_ok(c, hipStreamSynchronize(c.stream), "trajectory")) return RHMC_EHIP;
-  kernel_timing_harvest(c);
   return RHMC_OK;
 }
 

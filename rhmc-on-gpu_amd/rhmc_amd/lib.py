@@ -314,8 +314,9 @@ class Gpu:
         return it.value, b.value, m.value, nf.value
 
     def kernel_timing(self, enable=-1):
-        """(mean ms, count) of the timed MD hop launches since the last reset;
-        enable=1/0 then switches live timing on/off and resets."""
+        """(mean kernel ms on the device wall clock, count) of the MD solver's
+        hop since the last reset; enable=1/0 then switches live timing on/off
+        and resets."""
         m, n = C.c_double(), C.c_longlong()
         self._check(self.L.gpu_kernel_timing(self.h, int(enable), C.byref(m), C.byref(n)), "gpu_kernel_timing")
         return m.value, n.value

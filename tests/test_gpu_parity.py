@@ -54,6 +54,57 @@ def test_dslash_fp64_and_fp32(dims, eps):
     g.close()
 
 
+@pytest.mark.parametrize("shape", ["1", "2", "4", "8", "1x4", "8x4"])
+def test_dslash_shapes_on_16_site_rows(shape):
+    """Every launch shape (right-hand sides per lane, lanes per site) forced
+    through RHMC_DSLASH_NR on a lattice with 16 half-sites per x-row, where
+    the one-lane kernel takes the backward x hop from the forward hop's
+    registers by a lane rotation; 5 right-hand sides, so chunks are ragged."""
+    dims = (32, 4, 4, 4)
+    say("dslash shape", shape, dims)
+    lat, U, _, _ = make_fields(dims, 13, 0.3)
+    o = Oracle(dims)
+    v = fields.gauss_spinor(lat, 6, 5)
+    ref_doe = np.stack([o.doe(U, v[k]) for k in range(5)])
+    ref_deo = np.stack([o.deo(U, v[k]) for k in range(5)])
+    os.environ["RHMC_DSLASH_NR"] = shape
+    try:
+        g = Gpu(dims, mass=0.1)
+        g.init0(U)
+        assert rel(g.dslash(1, v, 0), ref_doe) < 1e-14
+        assert rel(g.dslash(0, v, 0), ref_deo) < 1e-14
+        assert rel(g.dslash(1, v, 1), ref_doe) < 2e-6
+        assert rel(g.dslash(0, v, 1), ref_deo) < 2e-6
+        g.close()
+    finally:
+        del os.environ["RHMC_DSLASH_NR"]
+
+
+@pytest.mark.parametrize("precision", [0, 1])
+def test_multishift_on_16_site_rows(precision):
+    """The basis multi-shift solver (hop and M^dag M launches with the x-row
+    lane rotation) against the oracle on a 32 x 4^3 lattice."""
+    dims = (32, 4, 4, 4)
+    mass = 0.05
+    say("multishift 16-site rows", precision)
+    lat, U, _, phi = make_fields(dims, 23, 0.3, 2)
+    o = Oracle(dims)
+    g = Gpu(dims, mass=mass, n_pf=2, max_order=12)
+    g.init0(U)
+    g.init1(phi=phi)
+    _, b = _shifts(12, mass)
+    g.multishift(b, 1e-10, relative=True, precision=precision)
+    res = g.shifted_residuals(b)
+    bn = np.linalg.norm(phi.reshape(2, -1), axis=1)
+    assert np.all(res / bn[:, None] < 1e-10), res / bn[:, None]
+    x = g.get_fermions("smf").reshape(2, 12, lat.Vh, 3)
+    for ps in range(2):
+        xo, _ = o.multishift(U, mass, phi[ps], b, 1e-12 * bn[ps])
+        for k in range(12):
+            assert rel(x[ps, k], xo[k]) < 1e-9, (ps, k, rel(x[ps, k], xo[k]))
+    g.close()
+
+
 def test_dslash_golden_reference():
     """GPU Dslash against the reference CPU output (tests/golden)."""
     gd = dict(np.load(os.path.join(GOLD, "golden_t4.npz")))
