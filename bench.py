@@ -397,7 +397,7 @@ def main():
         reps = 50
         res = {}
         for name, prec, nv, lb, sb in (("fp32_12real_8rhs", 1, n_pf, 48, 24), ("fp64_18real_1rhs", 0, 1, 144, 48),
-                                       ("fp64_18real_8rhs", 0, n_pf, 144, 48)):
+                                       ("fp64_18real_8rhs", 0, n_pf, 144, 48), ("fp64_12real_8rhs", 2, n_pf, 96, 48)):
             ms = g.time_dslash(prec, nv, reps)
             res[name] = {"ms": ms, "GBps": dslash_bytes(lat.Vh, nv, lb, sb) / (ms * 1e-3) / 1e9,
                          "bytes": dslash_bytes(lat.Vh, nv, lb, sb)}

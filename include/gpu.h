@@ -158,7 +158,8 @@ int gpu_chiral_meas(rhmc_gpu_t h, const double *rnd_e, const double *rnd_o, unsi
 
 /* Timing helper: run `reps` device-resident half-lattice Dslash launches on
  * nvec vectors and return the mean kernel time in ms, measured with HIP
- * events on the library's stream. */
+ * events on the library's stream.  precision: 0 FP64 18-real links, 1 FP32
+ * 12-real, 2 FP64 12-real. */
 int gpu_time_dslash(rhmc_gpu_t h, int precision, int nvec, int reps, double *ms);
 
 /* The same with every timed launch preceded by a write pass over

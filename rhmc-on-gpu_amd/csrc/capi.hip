@@ -483,7 +483,8 @@ int gpu_time_dslash(rhmc_gpu_t h, int precision, int nvec, int reps, double *ms)
   Context &c = h->c;
   hipSetDevice(c.device);
   const long n = 3 * c.Vh * (long)nvec;
-  const size_t es = precision == 0 ? sizeof(double2) : sizeof(float2);
+  if (precision < 0 || precision > 2) return RHMC_EINVAL;
+  const size_t es = precision == 1 ? sizeof(float2) : sizeof(double2);
   void *a = nullptr, *b = nullptr;
   if (!hip_ok(c, hipMalloc(&a, es * n), "malloc")) return RHMC_EHIP;
   if (!hip_ok(c, hipMalloc(&b, es * n), "malloc")) { hipFree(a); return RHMC_EHIP; }
@@ -495,6 +496,9 @@ int gpu_time_dslash(rhmc_gpu_t h, int precision, int nvec, int reps, double *ms)
     const int par = k & 1;
     if (precision == 0)
       launch_hop_d(c, par, (const double2 *)((k & 1) ? b : a), (double2 *)((k & 1) ? a : b), nvec, 3 * c.Vh, 3 * c.Vh);
+    else if (precision == 2)
+      launch_hop_d12(c, par, (const double2 *)((k & 1) ? b : a), (double2 *)((k & 1) ? a : b), nvec, 3 * c.Vh,
+                     3 * c.Vh);
     else
       hop_s12(c, par, (const float2 *)((k & 1) ? b : a), (float2 *)((k & 1) ? a : b), nvec);
   };
@@ -551,7 +555,8 @@ int gpu_time_dslash_cold(rhmc_gpu_t h, int precision, int nvec, int reps, double
   Context &c = h->c;
   hipSetDevice(c.device);
   const long n = 3 * c.Vh * (long)nvec;
-  const size_t es = precision == 0 ? sizeof(double2) : sizeof(float2);
+  if (precision < 0 || precision > 2) return RHMC_EINVAL;
+  const size_t es = precision == 1 ? sizeof(float2) : sizeof(double2);
   const long nf = (long)(flush_bytes / sizeof(double4));
   void *a = nullptr, *b = nullptr, *f = nullptr;
   if (!hip_ok(c, hipMalloc(&a, es * n), "malloc")) return RHMC_EHIP;
@@ -564,6 +569,9 @@ int gpu_time_dslash_cold(rhmc_gpu_t h, int precision, int nvec, int reps, double
     const int par = k & 1;
     if (precision == 0)
       launch_hop_d(c, par, (const double2 *)((k & 1) ? b : a), (double2 *)((k & 1) ? a : b), nvec, 3 * c.Vh, 3 * c.Vh);
+    else if (precision == 2)
+      launch_hop_d12(c, par, (const double2 *)((k & 1) ? b : a), (double2 *)((k & 1) ? a : b), nvec, 3 * c.Vh,
+                     3 * c.Vh);
     else
       hop_s12(c, par, (const float2 *)((k & 1) ? b : a), (float2 *)((k & 1) ? a : b), nvec);
   };

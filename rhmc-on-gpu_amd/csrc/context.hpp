@@ -98,6 +98,9 @@ void fill_dims(Context &c, int nx, int ny, int nz, int nt, int antiperiodic);
 // out[j] = 0.5 * hop(in[j]) to parity par_out, FP64 18-real links
 void launch_hop_d(Context &c, int par_out, const double2 *in, double2 *out, int nvec, long vstride_in,
                   long vstride_out);
+// the same with the FP64 12-real links (row 2 rebuilt)
+void launch_hop_d12(Context &c, int par_out, const double2 *in, double2 *out, int nvec, long vstride_in,
+                    long vstride_out);
 // r = b - ((m^2+shift_j) p_j - D_eo D_oe p_j), norms2[j]; b index = j / b_group
 int residual_d(Context &c, const double2 *b, int b_group, const double2 *x, const double *shifts, int nvec,
                double2 *r, double *norms2);

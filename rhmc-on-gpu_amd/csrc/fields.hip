@@ -36,9 +36,10 @@ __device__ __forceinline__ void shift_coord(const LatticeDims &L, int &x, int &y
 // holds the four F_mu in registers (FP64) and reads each spinor of its site
 // once.  C = double2 (FP64 solutions) or float2 (FP32 solutions of the FP32
 // MD solves).
-// One thread per site; the systems are taken in pairs with all 30 loads of
-// a pair issued before its multiply-adds (uniform base + 32-bit lane offset
-// loads), so each wave keeps two systems' operands in flight.
+// One thread per site and system at a time: the 15 loads of a system
+// (uniform base + 32-bit lane offset) are issued before its multiply-adds.
+// Taking two systems per step keeps more loads in flight but needs ~280
+// VGPRs (one wave per SIMD) and measured slower.
 template <typename C, int NQ>
 __device__ __forceinline__ void ff_systems(const C *own_v, const C *nb_v, long Vh, int j0, unsigned ho,
                                            const unsigned (&nbo)[4], const double *coef, double sgn,
@@ -95,9 +96,7 @@ __global__ __launch_bounds__(kBlock) void k_fermion_force(LatticeDims L, const d
   const C *own_v = par == 0 ? s : hv;
   const C *nb_v = par == 0 ? hv : s;
   const double sgn = par == 0 ? 1.0 : -1.0;
-  int j = 0;
-  for (; j + 2 <= nsys; j += 2) ff_systems<C, 2>(own_v, nb_v, Vh, j, ho, nbo, coef, sgn, F);
-  if (j < nsys) ff_systems<C, 1>(own_v, nb_v, Vh, j, ho, nbo, coef, sgn, F);
+  for (int j = 0; j < nsys; j++) ff_systems<C, 1>(own_v, nb_v, Vh, j, ho, nbo, coef, sgn, F);
 #pragma unroll
   for (int mu = 0; mu < 4; mu++) {
     const Mat<double> Um = load_link_d18(U, Vh, mu, par, h);
@@ -130,7 +129,8 @@ int fermion_force(Context &c, int nsys, const double *num, const double2 *x, int
                      nsys, c.ipdot, accumulate);
   hipFreeAsync(hv, c.stream);
   hipFreeAsync(coef, c.stream);
-  return hip_ok(c, hipStreamSynchronize(c.stream), "fermion force") ? 0 : -1;
+  // stream-ordered for the caller (the staged coefficients live in hcoef)
+  return hip_ok(c, hipGetLastError(), "fermion force") ? 0 : -1;
 }
 
 int fermion_force32(Context &c, int nsys, const double *num, const float2 *x32, int accumulate) {
@@ -148,7 +148,8 @@ int fermion_force32(Context &c, int nsys, const double *num, const float2 *x32, 
                      nsys, c.ipdot, accumulate);
   hipFreeAsync(hv, c.stream);
   hipFreeAsync(coef, c.stream);
-  return hip_ok(c, hipStreamSynchronize(c.stream), "fermion force") ? 0 : -1;
+  // stream-ordered for the caller (the staged coefficients live in hcoef)
+  return hip_ok(c, hipGetLastError(), "fermion force") ? 0 : -1;
 }
 
 // ----------------------------------------------------------------- gauge force

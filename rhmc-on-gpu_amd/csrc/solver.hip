@@ -51,14 +51,23 @@ struct GraphSlot {
   unsigned char key[2048];
 };
 
+// per (rhs, iteration, shift) of the basis multi-shift solver: omega, zeta, gamma
+struct BmsHist {
+  double om, z, ga;
+};
+
 struct Workspace {
   std::map<unsigned long long, int> tuned_nr;  // Dslash RHS-per-thread choice
+  std::vector<BmsHist> h_hist;                 // basis solver host arrays (reused)
+  std::vector<double> h_coef;
+  std::vector<int> h_klen;
   GraphSlot bms_graph[2];                      // FP32 / FP64 stored-basis chunk
   void *buf[S_COUNT] = {};
   size_t cap[S_COUNT] = {};
   unsigned *counter = nullptr;
   int *h_flags = nullptr;     // pinned
   double *h_sums = nullptr;   // pinned
+  hipEvent_t poll_ev[2] = {}; // solver chunk polls, one chunk in flight ahead
 };
 
 Workspace *workspace_create(Context &c) {
@@ -67,6 +76,8 @@ Workspace *workspace_create(Context &c) {
   if (!hip_ok(c, hipMemsetAsync(w->counter, 0, 64 * sizeof(unsigned), c.stream), "counter")) return nullptr;
   if (!hip_ok(c, hipHostMalloc(&w->h_flags, 4096 * sizeof(int), hipHostMallocDefault), "hipHostMalloc")) return nullptr;
   if (!hip_ok(c, hipHostMalloc(&w->h_sums, 4096 * sizeof(double), hipHostMallocDefault), "hipHostMalloc")) return nullptr;
+  for (auto &e : w->poll_ev)
+    if (!hip_ok(c, hipEventCreateWithFlags(&e, hipEventDisableTiming), "poll event")) return nullptr;
   // reduction partials for the largest launch (kMaxRed vectors), allocated once
   const size_t nbx = (size_t)((c.Vh + kBlock - 1) / kBlock);
   w->cap[S_PARTIAL] = sizeof(double) * nbx * 256;
@@ -83,6 +94,8 @@ void workspace_destroy(Workspace *w) {
   if (w->counter) hipFree(w->counter);
   if (w->h_flags) hipHostFree(w->h_flags);
   if (w->h_sums) hipHostFree(w->h_sums);
+  for (auto e : w->poll_ev)
+    if (e) hipEventDestroy(e);
   delete w;
 }
 
@@ -1012,6 +1025,16 @@ void launch_hop_d(Context &c, int par_out, const double2 *in, double2 *out, int 
   dslash_dispatch<double, LINK_D18, DS_HOP, RED_NONE>(c, a);
 }
 
+void launch_hop_d12(Context &c, int par_out, const double2 *in, double2 *out, int nvec, long vs_in, long vs_out) {
+  DsArgs a = {};
+  a.par_out = par_out;
+  a.U = c.U64c;
+  a.in = in; a.in_vs = vs_in;
+  a.out = out; a.out_vs = vs_out;
+  a.nvec = nvec;
+  dslash_dispatch<double, LINK_D12, DS_HOP, RED_NONE>(c, a);
+}
+
 void launch_hop_s(Context &c, int par_out, const float2 *in, float2 *out, int nvec, long vs_in, long vs_out) {
   DsArgs a = {};
   a.par_out = par_out;
@@ -1274,9 +1297,6 @@ struct BmsShared {
   int stop[2];
 };
 
-struct BmsHist {
-  double om, z, ga;
-};
 
 struct BmsArgs {
   long Vh;
@@ -1482,7 +1502,7 @@ constexpr int kTileK = 32;
 
 template <typename R, int NS, typename XO>
 __global__ __launch_bounds__(kBlock) void k_bms_assemble(const void *basis, const double *coef, const int *klen,
-                                                         XO *x, long Vh, int cap, int nshift, int s0, int nvec) {
+                                                         XO *x, long Vh, int cstride, int nshift, int s0, int nvec) {
   using C = Cx<R>;
   __shared__ double2 cs[kTileK][NS / 2];
   const long n3 = 3 * Vh;
@@ -1492,7 +1512,7 @@ __global__ __launch_bounds__(kBlock) void k_bms_assemble(const void *basis, cons
   const int ns = nshift - s0 < NS ? nshift - s0 : NS;
   const long ss = (long)nvec * n3;  // slot stride
   const C *bj = (const C *)basis + (long)j * n3 + (e < n3 ? e : 0);
-  const double *cj = coef + ((long)j * nshift + s0) * cap;
+  const double *cj = coef + ((long)j * nshift + s0) * cstride;
   double2 acc[NS];
 #pragma unroll
   for (int q = 0; q < NS; q++) acc[q] = make_double2(0.0, 0.0);
@@ -1501,7 +1521,7 @@ __global__ __launch_bounds__(kBlock) void k_bms_assemble(const void *basis, cons
     __syncthreads();
     for (int idx = threadIdx.x; idx < kTileK * NS; idx += kBlock) {
       const int u = idx / NS, q = idx - u * NS;
-      ((double *)cs[u])[q] = (u < nk && q < ns) ? cj[(long)q * cap + i0 + u] : 0.0;
+      ((double *)cs[u])[q] = (u < nk && q < ns) ? cj[(long)q * cstride + i0 + u] : 0.0;
     }
     __syncthreads();
     if (e >= n3) continue;
@@ -1744,40 +1764,54 @@ static int ms_cg_basis(Context &c, const double2 *b, int nrhs, int nshift, const
     hipLaunchKernelGGL(k_add_int, dim3(1), dim3(64), 0, c.stream, iterc, kChunk);
   };
   int k = 1;
-  bool converged = false;
   GraphSlot &gs = c.ws->bms_graph[sizeof(R) == 8 ? 1 : 0];
-  while (true) {
+  // one chunk: eager for the first (autotuning), else the captured graph
+  auto launch_chunk = [&]() -> bool {
     if (k == 1 || !c.use_graphs) {
       chunk_launch();
-    } else {
-      BmsGraphKey key;
-      memset(&key, 0, sizeof(key));
-      key.a = a; key.hop = hop; key.mdm = mdm; key.nbr = nbr;
-      key.a.b = nullptr;  // the right-hand sides are read only by k_bms_init, outside the chunk
-      if (!gs.exec || memcmp(&key, &gs.key, sizeof(key)) != 0) {
-        if (gs.exec) hipGraphExecDestroy(gs.exec);
-        gs.exec = nullptr;
-        hipGraph_t g = nullptr;
-        if (!hip_ok(c, hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal), "capture")) return -1;
-        chunk_launch();
-        if (!hip_ok(c, hipStreamEndCapture(c.stream, &g), "capture end")) return -1;
-        const bool ok = hip_ok(c, hipGraphInstantiate(&gs.exec, g, nullptr, nullptr, 0), "instantiate");
-        hipGraphDestroy(g);
-        if (!ok) { gs.exec = nullptr; return -1; }
-        memcpy(&gs.key, &key, sizeof(key));
-      }
-      if (!hip_ok(c, hipGraphLaunch(gs.exec, c.stream), "graph launch")) return -1;
+      return true;
     }
+    BmsGraphKey key;
+    memset(&key, 0, sizeof(key));
+    key.a = a; key.hop = hop; key.mdm = mdm; key.nbr = nbr;
+    key.a.b = nullptr;  // the right-hand sides are read only by k_bms_init, outside the chunk
+    if (!gs.exec || memcmp(&key, &gs.key, sizeof(key)) != 0) {
+      if (gs.exec) hipGraphExecDestroy(gs.exec);
+      gs.exec = nullptr;
+      hipGraph_t g = nullptr;
+      if (!hip_ok(c, hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal), "capture")) return false;
+      chunk_launch();
+      if (!hip_ok(c, hipStreamEndCapture(c.stream, &g), "capture end")) return false;
+      const bool ok = hip_ok(c, hipGraphInstantiate(&gs.exec, g, nullptr, nullptr, 0), "instantiate");
+      hipGraphDestroy(g);
+      if (!ok) { gs.exec = nullptr; return false; }
+      memcpy(&gs.key, &key, sizeof(key));
+    }
+    return hip_ok(c, hipGraphLaunch(gs.exec, c.stream), "graph launch");
+  };
+  // The host reads each chunk's running-system count (copied behind the
+  // chunk, stream-ordered before the next chunk resets it) while the next
+  // chunk already runs: the poll's round trip overlaps a chunk instead of
+  // idling the GPU.  A chunk launched after every system has stopped does
+  // no work (stop flags are checked by every launch; nothing is written at
+  // or past the basis capacity).
+  int *hf = c.ws->h_flags;
+  hipEvent_t *pe = c.ws->poll_ev;
+  auto launch_and_poll = [&](int slot) -> bool {
+    if (!launch_chunk()) return false;
     k += kChunk;
-    int *hf = c.ws->h_flags;
-    hipMemcpyAsync(hf, a.active + (k & 1), sizeof(int), hipMemcpyDeviceToHost, c.stream);
-    hipStreamSynchronize(c.stream);
-    if (hf[0] == 0) { converged = true; break; }
-    if (k >= cap) break;
+    hipMemcpyAsync(hf + slot, a.active + (k & 1), sizeof(int), hipMemcpyDeviceToHost, c.stream);
+    return hip_ok(c, hipEventRecord(pe[slot], c.stream), "poll event");
+  };
+  if (!launch_and_poll(0)) return -1;
+  for (int n = 1;; n++) {
+    const bool more = k < cap;
+    if (more && !launch_and_poll(n & 1)) return -1;
+    if (!hip_ok(c, hipEventSynchronize(pe[(n - 1) & 1]), "poll")) return -1;
+    if (hf[(n - 1) & 1] == 0 || !more) break;
   }
   hipMemcpyAsync(hs.data(), ms, sizeof(MsState) * nrhs, hipMemcpyDeviceToHost, c.stream);
   if (!hip_ok(c, hipStreamSynchronize(c.stream), "basis multishift")) return -1;
-  (void)converged;
   int kmax = 0;
   for (int j = 0; j < nrhs; j++) kmax = std::max(kmax, hs[j].iter);
   // the hop ran with work at iterations 1..kmax+1 (input slots 0..kmax)
@@ -1786,26 +1820,32 @@ static int ms_cg_basis(Context &c, const double2 *b, int nrhs, int nshift, const
   for (int j = 0; j < nrhs; j++)
     if (hs[j].nact > 0 && hs[j].iter + 2 >= cap && hs[j].iter < max_iter) return 3;
   // coefficients d_{s,i} = -z_s^(i) S_i, S_{K-1} = om^(K), S_i = om^(i+1) + ga^(i+1) S_{i+1}
-  std::vector<BmsHist> hh_host((size_t)nrhs * cap * nshift);
+  // (host arrays sized by the iterations run, kept in the workspace)
+  const size_t hst = (size_t)kmax + 2;  // history slots per rhs on the host
+  const size_t cst = (size_t)kmax + 1;  // coefficients per (rhs, shift): K <= kmax
+  std::vector<BmsHist> &hh_host = c.ws->h_hist;
+  hh_host.resize((size_t)nrhs * hst * nshift);
   for (int j = 0; j < nrhs; j++)
-    hipMemcpyAsync(hh_host.data() + (size_t)j * cap * nshift, hist + (size_t)j * cap * nshift,
+    hipMemcpyAsync(hh_host.data() + (size_t)j * hst * nshift, hist + (size_t)j * cap * nshift,
                    sizeof(BmsHist) * (size_t)(kmax + 1) * nshift, hipMemcpyDeviceToHost, c.stream);
   if (!hip_ok(c, hipStreamSynchronize(c.stream), "basis history")) return -1;
-  std::vector<double> coef((size_t)nrhs * nshift * cap, 0.0);
+  std::vector<double> &coef = c.ws->h_coef;
+  coef.assign((size_t)nrhs * nshift * cst, 0.0);
   for (int j = 0; j < nrhs; j++)
     for (int sft = 0; sft < nshift; sft++) {
       const int K = hs[j].flag[sft] ? hs[j].iter : hs[j].its[sft];
       double S = 0.0;
-      double *cf = coef.data() + ((size_t)j * nshift + sft) * cap;
+      double *cf = coef.data() + ((size_t)j * nshift + sft) * cst;
       for (int i = K - 1; i >= 0; i--) {
-        const BmsHist &n1 = hh_host[((size_t)j * cap + i + 1) * nshift + sft];
+        const BmsHist &n1 = hh_host[((size_t)j * hst + i + 1) * nshift + sft];
         S = n1.om + n1.ga * S;
-        const double z = i == 0 ? 1.0 : hh_host[((size_t)j * cap + i) * nshift + sft].z;
+        const double z = i == 0 ? 1.0 : hh_host[((size_t)j * hst + i) * nshift + sft].z;
         cf[i] = -z * S;
       }
     }
   // per-rhs basis length = largest K over its shifts
-  std::vector<int> klen(nrhs, 0);
+  std::vector<int> &klen = c.ws->h_klen;
+  klen.assign(nrhs, 0);
   for (int j = 0; j < nrhs; j++)
     for (int sft = 0; sft < nshift; sft++)
       klen[j] = std::max(klen[j], hs[j].flag[sft] ? hs[j].iter : hs[j].its[sft]);
@@ -1817,11 +1857,12 @@ static int ms_cg_basis(Context &c, const double2 *b, int nrhs, int nshift, const
   for (int s0 = 0; s0 < nshift; s0 += 16)
     if (x32)
       hipLaunchKernelGGL((k_bms_assemble<R, 16, float2>), dim3((unsigned)((3 * Vh + kBlock - 1) / kBlock), nrhs),
-                         dim3(kBlock), 0, c.stream, basis, dcoef, dklen, x32, Vh, (int)cap, nshift, s0, nrhs);
+                         dim3(kBlock), 0, c.stream, basis, dcoef, dklen, x32, Vh, (int)cst, nshift, s0, nrhs);
     else
       hipLaunchKernelGGL((k_bms_assemble<R, 16, double2>), dim3((unsigned)((3 * Vh + kBlock - 1) / kBlock), nrhs),
-                         dim3(kBlock), 0, c.stream, basis, dcoef, dklen, x, Vh, (int)cap, nshift, s0, nrhs);
-  if (!hip_ok(c, hipStreamSynchronize(c.stream), "basis assemble")) return -1;
+                         dim3(kBlock), 0, c.stream, basis, dcoef, dklen, x, Vh, (int)cst, nshift, s0, nrhs);
+  // stream-ordered for the caller; the host arrays stay alive in the workspace
+  if (!hip_ok(c, hipGetLastError(), "basis assemble")) return -1;
   *iters_out = kmax;
   for (int j = 0; j < nrhs; j++)
     if (hs[j].iter >= max_iter && hs[j].nact > 0) {
