@@ -40,7 +40,7 @@ enum Slot {
   S_CR64, S_CX32, S_CR32, S_CP32, S_CS32, S_CH32,
   S_PW1, S_PW2, S_PW3,
   S_FH64,
-  S_PARTIAL2, S_BASIS, S_BSH, S_HIST, S_COEF,
+  S_PARTIAL2, S_BASIS, S_BSH, S_HIST, S_COEF, S_TUNE,
   S_COUNT
 };
 
@@ -323,6 +323,16 @@ enum DsMode { DS_HOP = 0, DS_MDAGM = 1, DS_RESID = 2 };
 enum DsRed { RED_NONE = 0, RED_DOT = 1, RED_NORM = 2 };
 enum LinkFmt { LINK_D18 = 0, LINK_S12 = 1, LINK_D12 = 2 };
 
+// scalars of one system of the stored-basis multi-shift CG shared between
+// its launches (the algorithm is described at ms_cg_basis)
+struct BmsShared {
+  double rho[2];    // [k&1] at launch k: rho_{k-2}
+  double alpha[2];  // [k&1] at launch k: alpha_{k-2}
+  int stop[2];
+  double rho1;      // rho_{k-1}, written by the hop launch of iteration k
+  double beta;      // beta_{k-2} = rho_{k-1} / rho_{k-2} (0 at k = 1)
+};
+
 struct DsArgs {
   LatticeDims L;
   int par_out;
@@ -342,8 +352,33 @@ struct DsArgs {
   // live kernel timing (solver hop): slot (*iterp + iter_off) of clk holds
   // the start of workgroup 0 and the latest workgroup end (device wall clock)
   unsigned long long *clk; int clk_cap;
+  // stored-basis CG: the hop launch (bms_part set) forms rho1/beta of every
+  // running system from the previous r update's partials (bms_nbr per
+  // system, two buffers by iteration parity); the M^dag M launch (bms set)
+  // then stores s = (m^2 r - D_eo h) + beta s into out in place
+  BmsShared *bms; const double *bms_part; int bms_nbr;
   RedArgs red;
 };
+
+// rho_{k-1} (fixed-order sum of its partials) and beta of every running
+// system; one workgroup, one wave per system
+__device__ __forceinline__ void bms_beta(const DsArgs &a) {
+  const int k = *a.iterp + a.iter_off + 1;
+  const int sl = k & 1;
+  const double *part = a.bms_part + (long)sl * a.nvec * a.bms_nbr;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  for (int j = w; j < a.nvec; j += kBlock / 64) {
+    BmsShared &sh = a.bms[j];
+    if (sh.stop[sl]) continue;
+    double s = 0.0;
+    for (int b = lane; b < a.bms_nbr; b += 64) s += part[(long)j * a.bms_nbr + b];
+    s = wave_sum(s);
+    if (lane == 0) {
+      sh.rho1 = s;
+      sh.beta = k >= 2 ? s / sh.rho[sl] : 0.0;
+    }
+  }
+}
 
 __device__ __forceinline__ unsigned long long *clk_slot(const DsArgs &a) {
   if (!a.clk || !a.iterp) return nullptr;
@@ -556,6 +591,16 @@ __device__ __forceinline__ void ds_site_out(const DsArgs &a, const Cx<R> (&acc)[
         }
         if (RED == RED_DOT) red[r] += (double)pv.x * (double)res[i].x + (double)pv.y * (double)res[i].y;
       }
+      if (MODE == DS_MDAGM && a.bms) {
+        const R be = (R)a.bms[j].beta;
+        if (be != (R)0)
+#pragma unroll
+          for (int i = 0; i < 3; i++) {
+            const C so = o[(long)i * Vh];
+            res[i].x = fmar(be, so.x, res[i].x);
+            res[i].y = fmar(be, so.y, res[i].y);
+          }
+      }
     }
 #pragma unroll
     for (int i = 0; i < 3; i++) {
@@ -569,6 +614,7 @@ template <typename R, int LINK, int NR, int MODE, int RED>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(ds_waves<R, LINK, NR>()))) void k_dslash(DsArgs a) {
   using C = Cx<R>;
   clk_start(a);
+  if (MODE == DS_HOP && a.bms_part && blockIdx.x == 0) bms_beta(a);
   const LatticeDims &L = a.L;
   const long Vh = L.Vh;
   unsigned sb, ch;
@@ -744,6 +790,7 @@ template <typename R, int LINK, int NR, int MODE, int RED>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(ds_waves<R, LINK, NR>()))) void k_dslash_ls(DsArgs a) {
   using C = Cx<R>;
   clk_start(a);
+  if (MODE == DS_HOP && a.bms_part && blockIdx.x == 0) bms_beta(a);
   const LatticeDims &L = a.L;
   const long Vh = L.Vh;
   const int Vh32 = (int)Vh;
@@ -828,6 +875,14 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(ds_waves
           res = csub(mk<R>((R)bv.x, (R)bv.y), res);
         }
         if (RED == RED_DOT) red[r] += (double)pv.x * (double)res.x + (double)pv.y * (double)res.y;
+        if (MODE == DS_MDAGM && a.bms) {
+          const R be = (R)a.bms[j].beta;
+          if (be != (R)0) {
+            const C so = ((const C *)a.out + (long)j * a.out_vs + h)[(long)col * Vh];
+            res.x = fmar(be, so.x, res.x);
+            res.y = fmar(be, so.y, res.y);
+          }
+        }
       }
       ((C *)a.out + (long)j * a.out_vs + h)[(long)col * Vh] = res;
       if (RED == RED_NORM) red[r] += (double)res.x * (double)res.x + (double)res.y * (double)res.y;
@@ -895,8 +950,10 @@ static void dslash_launch(Context &c, DsArgs &a, int code) {
 // registers against waves in flight (a single 8-RHS thread per site leaves a
 // 32^3x8 half lattice at ~2 waves per SIMD).  On the first launch of each
 // (precision, links, mode, reduction, nvec) every candidate is timed in place
-// with HIP events (these launches are idempotent) and the fastest is kept for
-// the lifetime of the context.  RHMC_DSLASH_NR ("4" or "4x4") overrides it.
+// with HIP events and the fastest is kept for the lifetime of the context.
+// The launches are idempotent, except the stored-basis solver's M^dag M,
+// which updates s in place: its candidates write w into a scratch vector
+// instead.  RHMC_DSLASH_NR ("4" or "4x4") overrides it.
 template <typename R, int LINK, int MODE, int RED>
 static void dslash_dispatch(Context &c, DsArgs &a) {
   const unsigned long long key = ((unsigned long long)sizeof(R) << 40) | ((unsigned long long)LINK << 36) |
@@ -919,12 +976,17 @@ static void dslash_dispatch(Context &c, DsArgs &a) {
     hipEventCreate(&e1);
     float best = 1e30f;
     nr = 16 + pick_nr(a.nvec);
-    for (int lanes = 1; lanes <= 4; lanes *= 4)
+    DsArgs t = a;
+    if (MODE == DS_MDAGM && a.bms) {
+      t.bms = nullptr;
+      t.out = ws_get(c, S_TUNE, sizeof(Cx<R>) * (size_t)a.out_vs * a.nvec);
+    }
+    for (int lanes = 1; lanes <= 4 && t.out; lanes *= 4)
       for (int v = 1; v <= 8 && v <= std::max(1, a.nvec); v *= 2) {
         const int cand = lanes * 16 + v;
-        dslash_launch<R, LINK, MODE, RED>(c, a, cand);
+        dslash_launch<R, LINK, MODE, RED>(c, t, cand);
         hipEventRecord(e0, c.stream);
-        for (int k = 0; k < 3; k++) dslash_launch<R, LINK, MODE, RED>(c, a, cand);
+        for (int k = 0; k < 3; k++) dslash_launch<R, LINK, MODE, RED>(c, t, cand);
         hipEventRecord(e1, c.stream);
         hipEventSynchronize(e1);
         float t = 0.f;
@@ -936,6 +998,11 @@ static void dslash_dispatch(Context &c, DsArgs &a) {
       }
     hipEventDestroy(e0);
     hipEventDestroy(e1);
+    if (c.ws->buf[S_TUNE]) {  // every candidate has finished (event sync above)
+      hipFree(c.ws->buf[S_TUNE]);
+      c.ws->buf[S_TUNE] = nullptr;
+      c.ws->cap[S_TUNE] = 0;
+    }
     tab[key] = nr;
   }
   dslash_launch<R, LINK, MODE, RED>(c, a, nr);
@@ -1267,36 +1334,35 @@ __global__ __launch_bounds__(kBlock) void k_copy_f2d(double2 *x64, const float2 
 //   x_s = sum_{i<K_s} d_{s,i} r^(i),
 // with d_{s,i} from the same Jegerlehner scalar recurrences.  Keeping the
 // basis resident in HBM (one vector per iteration and right-hand side) makes
-// the per-iteration traffic independent of the number of shifts: 6 vectors
-// per rhs instead of 6 + 4 x (active shifts); the shifted solutions are
-// assembled in one pass at the end.
+// the per-iteration traffic independent of the number of shifts (the
+// explicit-shift form adds 4 vectors per active shift); the shifted
+// solutions are assembled in one pass at the end.
 //
 // Per iteration k three launches, with the base CG in its single-reduction
 // form (Chronopoulos and Gear, J. Comput. Appl. Math. 25 (1989) 153): the
 // operator is applied to the residual, w = A r^(k-1), and s = A p follows
-// its own recurrence, so one vector launch per iteration carries both
-// updates.  With rho_i = |r_i|^2 and mu_i = <r_i, w_i>:
+// its own recurrence, so p is never formed.  With rho_i = |r_i|^2 and
+// mu_i = <r_i, w_i>:
 //   beta_{i-1} = rho_i / rho_{i-1},
 //   <p_i, A p_i> = mu_i - beta_{i-1} rho_i / alpha_{i-1},  alpha_i = rho_i / <p_i, A p_i>,
 //   s_i = w_i + beta_{i-1} s_{i-1},  r_{i+1} = r_i - alpha_i s_i
-// (alpha, beta are those of the textbook CG; p itself is never formed,
-// the shifted solutions come from the basis).  Launches of iteration k:
-// A h = D_oe r^(k-1); B w = m^2 r^(k-1) - D_eo h, partials of mu;
-// X every workgroup sums the partials of rho (previous X) and mu in a
-// fixed order, updates s and writes r^(k) into basis slot k with partials
-// of |r^(k)|^2, and workgroup (0,j) advances the Jegerlehner recurrences of
-// rhs j for iteration k-1 (they need beta_{k-2}, known once rho_{k-1} is).
-// The stop decision therefore lags by one launch: one extra application of
-// the operator per solve, identical recurrences and iteration counts.
-// Values read by every workgroup (rho, alpha, stop, the rho partials) are
-// double-buffered by the parity of k, so no workgroup reads a value another
-// one writes in the same launch.
-struct BmsShared {
-  double rho[2];    // [k&1] at launch k: rho_{k-2}
-  double alpha[2];  // [k&1] at launch k: alpha_{k-2}
-  int stop[2];
-};
-
+// (alpha, beta are those of the textbook CG; the shifted solutions come
+// from the basis).  Launches of iteration k:
+// A h = D_oe r^(k-1); its first workgroup also sums the partials of
+//   rho_{k-1} (previous X) in a fixed order and stores rho_{k-1} and beta;
+// B w = m^2 r^(k-1) - D_eo h, partials of mu, and s = w + beta s in place
+//   (w itself is never stored);
+// X every workgroup sums the partials of mu in a fixed order, writes
+//   r^(k) = r^(k-1) - alpha s into basis slot k with partials of |r^(k)|^2,
+//   and workgroup (0,j) advances the Jegerlehner recurrences of rhs j for
+//   iteration k-1 (they need beta_{k-2}, known once rho_{k-1} is).
+// Vector traffic per iteration and rhs: 2 (A) + 4 (B) + 3 (X) passes,
+// independent of the number of shifts.  The stop decision lags by one
+// launch: one extra application of the operator per solve, identical
+// recurrences and iteration counts.  Values read by every workgroup of X
+// (rho_{k-2}, alpha, stop, the rho partials) are double-buffered by the
+// parity of k, so no workgroup reads a value another one writes in the same
+// launch.
 
 struct BmsArgs {
   long Vh;
@@ -1304,7 +1370,7 @@ struct BmsArgs {
   const int *iterc;     // device iteration base: iteration = *iterc + iter_off
   const double2 *b;
   void *basis;          // [cap][rhs][3][Vh]: slot k of every rhs contiguous
-  void *w, *s;          // [rhs][3][Vh]: A r^(k-1) and the s = A p recurrence
+  void *s;              // [rhs][3][Vh]: the s = A p recurrence
   double *partB, *partC;  // partials of mu; of rho, two buffers by parity
   int nbrB, nbrC;
   BmsShared *sh;
@@ -1385,9 +1451,10 @@ __global__ __launch_bounds__(kBlock) void k_bms_init_fin(BmsArgs a) {
   if (threadIdx.x == 0) a.active[0] = 0;
 }
 
-// X (iteration k >= 1, CG step i = k-1): s = w + beta s, r^(k) = r^(k-1) -
-// alpha s, partials of |r^(k)|^2; workgroup (0,j) runs the recurrences of
-// iteration k-1 (inverter.cc:224-287) and the stopping test.
+// X (iteration k >= 1, CG step i = k-1): r^(k) = r^(k-1) - alpha s (s was
+// updated by the M^dag M launch), partials of |r^(k)|^2; workgroup (0,j)
+// runs the recurrences of iteration k-1 (inverter.cc:224-287) and the
+// stopping test.
 template <typename R>
 __global__ __launch_bounds__(kBlock) void k_bms_x(BmsArgs a) {
   using C = Cx<R>;
@@ -1407,35 +1474,24 @@ __global__ __launch_bounds__(kBlock) void k_bms_x(BmsArgs a) {
     }
     return;
   }
-  const double rho1 = block_sum_partials(a.partC + ((long)sl * a.nvec + j) * a.nbrC, a.nbrC);  // rho_{k-1}
-  const double mu = block_sum_partials(a.partB + (long)j * a.nbrB, a.nbrB);                   // <r^(k-1), w>
-  const double rho2 = sh.rho[sl], al2 = sh.alpha[sl];                                         // rho_{k-2}, alpha_{k-2}
-  const double be = k >= 2 ? rho1 / rho2 : 0.0;
+  const double mu = block_sum_partials(a.partB + (long)j * a.nbrB, a.nbrB);  // <r^(k-1), w>
+  const double rho1 = sh.rho1, be = sh.beta;                                 // rho_{k-1}, beta_{k-2}
+  const double rho2 = sh.rho[sl], al2 = sh.alpha[sl];                        // rho_{k-2}, alpha_{k-2}
   const double pap = k >= 2 ? mu - be * rho1 / al2 : mu;
   const double al = rho1 / pap;
   double v[1] = {0.0};
   if (k < a.cap) {
-    const R b = (R)be, am = (R)(-al);
+    const R am = (R)(-al);
     long h0, h1;
     site_range(Vh, bx, gridDim.x, h0, h1);
     const C *ro = (const C *)a.basis + ((long)(k - 1) * a.nvec + j) * 3 * Vh;
     C *rn = (C *)a.basis + ((long)k * a.nvec + j) * 3 * Vh;
-    const C *wv = (const C *)a.w + (long)j * 3 * Vh;
-    C *sv = (C *)a.s + (long)j * 3 * Vh;
+    const C *sv = (const C *)a.s + (long)j * 3 * Vh;
     for (long h = h0 + threadIdx.x; h < h1; h += kBlock)
 #pragma unroll
       for (int i = 0; i < 3; i++) {
         const long e = (long)i * Vh + h;
-        const C wi = wv[e];
-        C si;
-        if (k >= 2) {
-          const C so = sv[e];
-          si.x = fmar(b, so.x, wi.x);
-          si.y = fmar(b, so.y, wi.y);
-        } else {
-          si = wi;
-        }
-        sv[e] = si;
+        const C si = sv[e];
         C r = ro[e];
         r.x = fmar(am, si.x, r.x);
         r.y = fmar(am, si.y, r.y);
@@ -1692,7 +1748,6 @@ static int ms_cg_basis(Context &c, const double2 *b, int nrhs, int nshift, const
   if (cap < 64) return 3;
   void *basis = ws_get(c, S_BASIS, vb * nrhs * cap);
   void *sr = ws_get(c, dbl ? S_P64 : S_P32, vb * nrhs);   // s = A p recurrence
-  void *wv = ws_get(c, dbl ? S_S64 : S_S32, vb * nrhs);   // w = A r^(k-1)
   void *hh = ws_get(c, dbl ? S_H64 : S_H32, vb * nrhs);
   MsState *ms = (MsState *)ws_get(c, S_MS, sizeof(MsState) * nrhs);
   BmsShared *bsh = (BmsShared *)ws_get(c, S_BSH, sizeof(BmsShared) * nrhs);
@@ -1700,7 +1755,7 @@ static int ms_cg_basis(Context &c, const double2 *b, int nrhs, int nshift, const
   double *diag = (double *)ws_get(c, S_DIAG, sizeof(double) * 256);
   // two buffers (iteration parity) of nrhs x nbr partials, nbr <= nbx_of(c)
   double *partC = (double *)ws_get(c, S_PARTIAL2, sizeof(double) * nbx_of(c) * 2 * std::max(nrhs, 1));
-  if (!basis || !sr || !wv || !hh || !ms || !bsh || !hist || !diag || !partC) return -1;
+  if (!basis || !sr || !hh || !ms || !bsh || !hist || !diag || !partC) return -1;
 
   std::vector<MsState> hs(nrhs);
   for (int j = 0; j < nrhs; j++) {
@@ -1717,7 +1772,7 @@ static int ms_cg_basis(Context &c, const double2 *b, int nrhs, int nshift, const
 
   BmsArgs a = {};
   a.Vh = Vh; a.nvec = nrhs; a.nshift = nshift; a.cap = (int)cap;
-  a.b = b; a.basis = basis; a.w = wv; a.s = sr;
+  a.b = b; a.basis = basis; a.s = sr;
   a.partC = partC;
   a.sh = bsh; a.ms = ms; a.hist = hist;
   a.active = active_ptr(c);
@@ -1733,12 +1788,13 @@ static int ms_cg_basis(Context &c, const double2 *b, int nrhs, int nshift, const
   hop.in = basis; hop.in_vs = 3 * Vh; hop.out = hh; hop.out_vs = 3 * Vh; hop.nvec = nrhs;
   hop.skip_stride = sizeof(BmsShared) / sizeof(int);
   hop.iterp = iterc; hop.iter_stride = (long)nrhs * 3 * Vh; hop.iter_in = 1;
+  hop.bms = bsh; hop.bms_part = partC; hop.bms_nbr = (int)nbr;
   // live timing of the MD solver's hop (FP32, all pseudofermions)
   const bool timed = c.time_kernels && c.kt_clk && sizeof(R) == 4 && nrhs == c.n_pf;
   if (timed) { hop.clk = c.kt_clk; hop.clk_cap = c.kt_clk_cap; }
   DsArgs mdm = hop;
-  mdm.par_out = 0; mdm.in = hh; mdm.in_vs = 3 * Vh; mdm.out = wv; mdm.pin = basis; mdm.pin_vs = 3 * Vh;
-  mdm.diag = diag; mdm.iter_in = 0; mdm.iter_pin = 1; mdm.clk = nullptr;
+  mdm.par_out = 0; mdm.in = hh; mdm.in_vs = 3 * Vh; mdm.out = sr; mdm.pin = basis; mdm.pin_vs = 3 * Vh;
+  mdm.diag = diag; mdm.iter_in = 0; mdm.iter_pin = 1; mdm.clk = nullptr; mdm.bms_part = nullptr;
   mdm.red = make_red(c, nrhs, EPI_STORE, nullptr, nullptr);
   a.partB = mdm.red.partial;
 
