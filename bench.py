@@ -525,8 +525,10 @@ def main():
                      "achieved": gbs_live, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs_live / HBM_PEAK_GBS,
                      "traffic": traffic, "bytes_per_launch": b32,
                      "measured": f"mean kernel duration of {n_live} launches inside the timed mixed trajectories "
-                                 f"(graphs replayed), each timed on the device wall clock from the start of "
-                                 f"workgroup 0 to the end of the last workgroup"},
+                                 f"(graphs replayed), each timed on the device wall clock from the end of the "
+                                 f"launch before it (the residual update) to the end of its last workgroup, "
+                                 f"the per-dispatch interval of a kernel trace; the first hop of each "
+                                 f"8-iteration chunk is not counted"},
         "cpu_baseline": cpu,
         "dslash_autotune": g.tuning_table(),
     }

@@ -169,11 +169,14 @@ int gpu_time_dslash(rhmc_gpu_t h, int precision, int nvec, int reps, double *ms)
 int gpu_time_dslash_cold(rhmc_gpu_t h, int precision, int nvec, int reps, double flush_bytes, double *ms);
 
 /* Live kernel timing inside real work: while enabled, the MD solver's hop
- * (FP32 12-real half-lattice Dslash on the n_pf pseudofermions) records its
- * own duration on the device wall clock, from the start of workgroup 0 to
- * the end of the last workgroup (the interval a kernel-trace profiler
- * reports), in a slot indexed by the solver iteration, so replayed graphs
- * are timed too.  Only iterations in which the hop did work are counted.
+ * (FP32 12-real half-lattice Dslash on the n_pf pseudofermions) is timed on
+ * the device wall clock, from the end of the launch before it (the residual
+ * update, which stamps its own end) to the end of its last workgroup: the
+ * per-dispatch interval a kernel-trace profiler reports (dispatch, execution
+ * and end of kernel).  The stamps sit in slots indexed by the solver
+ * iteration, so replayed graphs are timed too.  Only iterations in which the
+ * hop did work are counted, and not the first hop of each 8-iteration chunk
+ * (it follows the chunk's bookkeeping launches instead).
  * Returns the mean duration and count since the last reset; enable = 1/0
  * then switches timing on or off and resets, enable = -1 only reads.  No
  * counterpart in the reference. */

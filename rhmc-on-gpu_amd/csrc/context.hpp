@@ -72,12 +72,14 @@ struct Context {
   // replay the stored-basis solver's iteration chunks as captured HIP graphs
   int use_graphs = 1;
 
-  // live kernel timing: HIP events around every FP32 multi-RHS hop with
-  // n_pf right-hand sides (the MD solver's Dslash); graphs are not replayed
-  // while it is on
+  // live kernel timing of every FP32 hop with n_pf right-hand sides in the
+  // stored-basis solver (the MD solver's Dslash), on the device wall clock;
+  // graphs stay on
   int time_kernels = 0;
-  unsigned long long *kt_clk = nullptr; // per solver iteration: workgroup-0 start, last workgroup end
+  unsigned long long *kt_clk = nullptr; // per solver iteration: end of the launch before the hop, end of the hop
   int kt_clk_cap = 0;
+  unsigned long long *kt_wg = nullptr;  // per-workgroup end stamps of the timed launches
+  size_t kt_wg_cap = 0;
   double kt_sum_ms = 0.0;               // kernel durations on the device wall clock
   long kt_count = 0;
   double wall_khz = 0.0;
@@ -156,6 +158,8 @@ void kernel_timing_harvest(Context &c, int n);
 void kernel_timing_release(Context &c);
 // allocate and clear the slots (timing switched on)
 int kernel_timing_clock(Context &c);
+// per-workgroup end stamps, at least n
+int kernel_timing_stamps(Context &c, size_t n);
 
 // host helpers
 bool hip_ok(Context &c, hipError_t e, const char *what);

@@ -60,6 +60,7 @@ struct Workspace {
   std::map<unsigned long long, int> tuned_nr;  // Dslash RHS-per-thread choice
   std::vector<BmsHist> h_hist;                 // basis solver host arrays (reused)
   std::vector<double> h_coef;
+  std::vector<float> h_coef32;
   std::vector<int> h_klen;
   GraphSlot bms_graph[2];                      // FP32 / FP64 stored-basis chunk
   void *buf[S_COUNT] = {};
@@ -349,9 +350,15 @@ struct DsArgs {
   // iteration-indexed operand: in (iter_in) and pin (iter_pin) advance by
   // (*iterp + iter_off) * iter_stride elements (a basis slot chosen on the device)
   const int *iterp; int iter_off; long iter_stride; int iter_in, iter_pin;
-  // live kernel timing (solver hop): slot (*iterp + iter_off) of clk holds
-  // the start of workgroup 0 and the latest workgroup end (device wall clock)
+  // live kernel timing (solver hop, device wall clock): slot s = (*iterp +
+  // iter_off) of clk holds the end of the launch before the hop and the end
+  // of the hop.  A timed launch stores each workgroup's end in clk_wg (plain
+  // stores, no contention); the first workgroup of the launch after it takes
+  // the maximum of those clk_nprev stamps (clk_prev) into field clk_field of
+  // slot s.
   unsigned long long *clk; int clk_cap;
+  unsigned long long *clk_wg;
+  const unsigned long long *clk_prev; int clk_nprev, clk_field;
   // stored-basis CG: the hop launch (bms_part set) forms rho1/beta of every
   // running system from the previous r update's partials (bms_nbr per
   // system, two buffers by iteration parity); the M^dag M launch (bms set)
@@ -385,16 +392,33 @@ __device__ __forceinline__ unsigned long long *clk_slot(const DsArgs &a) {
   const int s = *a.iterp + a.iter_off;
   return s >= 0 && s < a.clk_cap ? a.clk + 2 * (long)s : nullptr;
 }
-__device__ __forceinline__ void clk_start(const DsArgs &a) {
-  unsigned long long *t = clk_slot(a);
-  if (t && blockIdx.x == 0 && threadIdx.x == 0) t[0] = (unsigned long long)wall_clock64();
-}
-__device__ __forceinline__ void clk_end(const DsArgs &a) {
-  unsigned long long *t = clk_slot(a);
-  if (t) {
-    __syncthreads();
-    if (threadIdx.x == 0) atomicMax(&t[1], (unsigned long long)wall_clock64());
+// workgroup maximum of n end stamps, stored by thread 0 into *dst
+__device__ __forceinline__ void clk_max(const unsigned long long *p, int n, unsigned long long *dst) {
+  __shared__ unsigned long long sm[kBlock / 64];
+  unsigned long long m = 0;
+  for (int i = threadIdx.x; i < n; i += kBlock) m = p[i] > m ? p[i] : m;
+  for (int o = 32; o > 0; o >>= 1) {
+    const unsigned long long q = __shfl_xor(m, o);
+    m = q > m ? q : m;
   }
+  if ((threadIdx.x & 63) == 0) sm[threadIdx.x >> 6] = m;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int k = 0; k < kBlock / 64; k++) m = sm[k] > m ? sm[k] : m;
+    *dst = m;
+  }
+}
+// first workgroup: the previous launch's end into this launch's slot
+__device__ __forceinline__ void clk_begin(const DsArgs &a) {
+  if (!a.clk_prev || blockIdx.x != 0) return;
+  unsigned long long *t = clk_slot(a);
+  if (t) clk_max(a.clk_prev, a.clk_nprev, t + a.clk_field);
+}
+// every workgroup: its end stamp
+__device__ __forceinline__ void clk_end(const DsArgs &a) {
+  if (!a.clk_wg) return;
+  __syncthreads();
+  if (threadIdx.x == 0) a.clk_wg[blockIdx.x] = (unsigned long long)wall_clock64();
 }
 
 // element offset of the iteration-indexed operand
@@ -613,8 +637,8 @@ __device__ __forceinline__ void ds_site_out(const DsArgs &a, const Cx<R> (&acc)[
 template <typename R, int LINK, int NR, int MODE, int RED>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(ds_waves<R, LINK, NR>()))) void k_dslash(DsArgs a) {
   using C = Cx<R>;
-  clk_start(a);
   if (MODE == DS_HOP && a.bms_part && blockIdx.x == 0) bms_beta(a);
+  clk_begin(a);
   const LatticeDims &L = a.L;
   const long Vh = L.Vh;
   unsigned sb, ch;
@@ -789,8 +813,8 @@ __device__ __forceinline__ C pick3(C a0, C a1, C a2, int k) {
 template <typename R, int LINK, int NR, int MODE, int RED>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(ds_waves<R, LINK, NR>()))) void k_dslash_ls(DsArgs a) {
   using C = Cx<R>;
-  clk_start(a);
   if (MODE == DS_HOP && a.bms_part && blockIdx.x == 0) bms_beta(a);
+  clk_begin(a);
   const LatticeDims &L = a.L;
   const long Vh = L.Vh;
   const int Vh32 = (int)Vh;
@@ -1008,6 +1032,8 @@ static void dslash_dispatch(Context &c, DsArgs &a) {
   dslash_launch<R, LINK, MODE, RED>(c, a, nr);
 }
 
+constexpr int kChunk = 8;  // stored-basis solver iterations between host polls
+
 // Fold the device-clock slots of one solve (iterations 0..n-1 of the hop
 // input) into the context's totals and clear them for the next solve.
 __global__ void k_clk_clear(unsigned long long *t, int n) {
@@ -1022,8 +1048,14 @@ void kernel_timing_harvest(Context &c, int n) {
   hipMemcpyAsync(t.data(), c.kt_clk, sizeof(unsigned long long) * t.size(), hipMemcpyDeviceToHost, c.stream);
   hipLaunchKernelGGL(k_clk_clear, dim3((2 * n + 255) / 256), dim3(256), 0, c.stream, c.kt_clk, n);
   hipStreamSynchronize(c.stream);
-  for (int i = 0; i < n; i++)
-    if (t[2 * i] != 0 && t[2 * i + 1] > t[2 * i]) {
+  // The interval of slot i runs from the end of the r update that wrote basis
+  // slot i to the end of the hop that reads it: what a per-dispatch trace
+  // records for the hop (dispatch, execution, end of kernel), as the hop
+  // directly follows that launch.  The first hop of a chunk follows the
+  // chunk's bookkeeping launches and the host poll instead, so slots that
+  // are multiples of the chunk length are left out.
+  for (int i = 1; i < n; i++)
+    if (i % kChunk != 0 && t[2 * i] != 0 && t[2 * i + 1] > t[2 * i]) {
       c.kt_sum_ms += (double)(t[2 * i + 1] - t[2 * i]) / c.wall_khz;
       c.kt_count++;
     }
@@ -1031,8 +1063,25 @@ void kernel_timing_harvest(Context &c, int n) {
 
 void kernel_timing_release(Context &c) {
   if (c.kt_clk) hipFree(c.kt_clk);
+  if (c.kt_wg) hipFree(c.kt_wg);
   c.kt_clk = nullptr;
+  c.kt_wg = nullptr;
   c.kt_clk_cap = 0;
+  c.kt_wg_cap = 0;
+}
+
+// per-workgroup end stamps of the timed launches (grown on demand)
+int kernel_timing_stamps(Context &c, size_t n) {
+  if (c.kt_wg_cap >= n) return 0;
+  if (c.kt_wg) {
+    hipStreamSynchronize(c.stream);
+    hipFree(c.kt_wg);
+  }
+  c.kt_wg = nullptr;
+  c.kt_wg_cap = 0;
+  if (!hip_ok(c, hipMalloc((void **)&c.kt_wg, sizeof(unsigned long long) * n), "timing stamps")) return -1;
+  c.kt_wg_cap = n;
+  return 0;
 }
 
 // device-clock slots of the live kernel timing, one per solver iteration
@@ -1377,9 +1426,11 @@ struct BmsArgs {
   MsState *ms;
   BmsHist *hist;        // [rhs][cap][nshift]
   int *active;          // [2]
+  // live timing: each workgroup stores its end stamp in clk_wg (read by the
+  // hop that follows)
+  unsigned long long *clk_wg;
 };
 
-constexpr int kChunk = 8;  // solver iterations between host polls
 
 struct BmsGraphKey {
   BmsArgs a;
@@ -1456,7 +1507,7 @@ __global__ __launch_bounds__(kBlock) void k_bms_init_fin(BmsArgs a) {
 // runs the recurrences of iteration k-1 (inverter.cc:224-287) and the
 // stopping test.
 template <typename R>
-__global__ __launch_bounds__(kBlock) void k_bms_x(BmsArgs a) {
+__device__ __forceinline__ void bms_x_body(const BmsArgs &a) {
   using C = Cx<R>;
   const unsigned bx = xcd_block(blockIdx.x, gridDim.x);
   const int j = blockIdx.y;
@@ -1546,6 +1597,15 @@ __global__ __launch_bounds__(kBlock) void k_bms_x(BmsArgs a) {
   }
 }
 
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_bms_x(BmsArgs a) {
+  bms_x_body<R>(a);  // returns are uniform per workgroup
+  if (a.clk_wg) {
+    __syncthreads();
+    if (threadIdx.x == 0) a.clk_wg[blockIdx.y * gridDim.x + blockIdx.x] = (unsigned long long)wall_clock64();
+  }
+}
+
 // x[j][s] (FP64) = sum_{i<K} coef[j][s][i] basis[j][i] for NS shifts per
 // pass; klen[j] = basis vectors of rhs j (slots beyond were never written).
 // A tall-skinny product (3 Vh complex rows x K) . (K x NS real): one thread
@@ -1621,6 +1681,81 @@ __global__ __launch_bounds__(kBlock) void k_bms_assemble(const void *basis, cons
       v.y = acc[q].y;
       x[((long)j * nshift + s0 + q) * n3 + e] = v;
     }
+}
+
+// FP32 basis, FP32 accumulation on the matrix cores (the FP32 solver, whose
+// solutions carry FP32 accuracy anyway).  The 3Vh complex rows are 6Vh real
+// rows, so x = basis^T . coef is a real (rows x K) . (K x 16 shifts) product,
+// run as v_mfma_f32_16x16x4f32 on 16 rows x 4 basis slots x 16 shifts.  The
+// FP64 form above spends its time in FP64 FMAs (16 flop per basis byte);
+// here the product runs at the matrix-core rate and the basis stream bounds
+// the kernel.
+// A wave owns 32 consecutive complex elements.  Lane l (col = l%16, kq =
+// l/16) loads elements 2col and 2col+1 of slot i0+kq as one float4: its four
+// words are the A operands of four products (product c takes logical row m
+// as element 2m + c/2, component c%2; A[m][k] sits in lane m + 16k), and the
+// coefficient of shift col at slot i0+kq is the B operand (B[k][n] in lane n
+// + 16k).  Accumulator c holds D[m][n] in lane n + 16(m/4), register m%4, so
+// lane l ends with shift col of elements 8kq .. 8kq+7 and stores them as four
+// 16-byte words.  coef: FP32 [rhs][shift group][kp slots][16], zero beyond
+// each shift's length and for absent shifts; slots at or beyond klen[j] are
+// never read from the basis (they may hold stale values).
+constexpr int kMxUnroll = 4;  // K steps (4 slots each) per loop trip: kp is a multiple of 16
+
+template <typename XO>
+__global__ __launch_bounds__(kBlock) void k_bms_assemble_mx(const float2 *basis, const float *coef, const int *klen,
+                                                            XO *x, long Vh, int kp, int nshift, int nvec) {
+  typedef float f4 __attribute__((ext_vector_type(4)));
+  const long n3 = 3 * Vh;
+  const int lane = threadIdx.x & 63;
+  const int col = lane & 15, kq = lane >> 4;
+  const int j = blockIdx.y, g = blockIdx.z;
+  const long e0 = ((long)xcd_block(blockIdx.x, gridDim.x) * (kBlock / 64) + (threadIdx.x >> 6)) * 32;
+  const int kmax = klen[j];
+  const long ea = e0 + 2 * col;
+  const bool in = ea < n3;  // n3 is even: ea and ea+1 are both inside or both outside
+  const long ss = (long)nvec * n3;
+  const float2 *bp = basis + (long)j * n3 + (in ? ea : 0);
+  const float *cp = coef + (long)(j * (int)gridDim.z + g) * kp * 16 + lane;
+  f4 acc[4];
+#pragma unroll
+  for (int c = 0; c < 4; c++) acc[c] = (f4){0.f, 0.f, 0.f, 0.f};
+  for (int i0 = 0; i0 < kmax; i0 += 4 * kMxUnroll) {
+    float4 v[kMxUnroll];
+    float b[kMxUnroll];
+#pragma unroll
+    for (int u = 0; u < kMxUnroll; u++) {
+      const int i = i0 + 4 * u + kq;
+      v[u] = in && i < kmax ? *(const float4 *)(bp + (long)i * ss) : make_float4(0.f, 0.f, 0.f, 0.f);
+      b[u] = cp[(long)(i0 + 4 * u) * 16];
+    }
+#pragma unroll
+    for (int u = 0; u < kMxUnroll; u++) {
+      acc[0] = __builtin_amdgcn_mfma_f32_16x16x4f32(v[u].x, b[u], acc[0], 0, 0, 0);
+      acc[1] = __builtin_amdgcn_mfma_f32_16x16x4f32(v[u].y, b[u], acc[1], 0, 0, 0);
+      acc[2] = __builtin_amdgcn_mfma_f32_16x16x4f32(v[u].z, b[u], acc[2], 0, 0, 0);
+      acc[3] = __builtin_amdgcn_mfma_f32_16x16x4f32(v[u].w, b[u], acc[3], 0, 0, 0);
+    }
+  }
+  const int q = g * 16 + col;
+  if (q >= nshift) return;
+  XO *xq = x + ((long)j * nshift + q) * n3;
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    const long e = e0 + 8 * kq + 2 * r;
+    if (e < n3) {
+      if constexpr (sizeof(XO) == 8) {
+        // (j nshift + q) n3 + e is even: one 16-byte store
+        *(float4 *)(xq + e) = make_float4(acc[0][r], acc[1][r], acc[2][r], acc[3][r]);
+      } else {
+        XO w0, w1;
+        w0.x = acc[0][r]; w0.y = acc[1][r];
+        w1.x = acc[2][r]; w1.y = acc[3][r];
+        xq[e] = w0;
+        xq[e + 1] = w1;
+      }
+    }
+  }
 }
 
 // single-value kernels: set / advance the device iteration base (kernels,
@@ -1734,7 +1869,8 @@ static int ms_cg(Context &c, const double2 *b, int nrhs, int nshift, const doubl
 // caller then runs the explicit-shift algorithm).
 template <typename R, int LINK>
 static int ms_cg_basis(Context &c, const double2 *b, int nrhs, int nshift, const double *shifts,
-                       const double *res_abs, int max_iter, double2 *x, int *iters_out, float2 *x32 = nullptr) {
+                       const double *res_abs, int max_iter, double2 *x, int *iters_out, float2 *x32 = nullptr,
+                       bool asm_fp32 = false) {
   using C = Cx<R>;
   const long Vh = c.Vh;
   const size_t vb = sizeof(C) * 3 * Vh;
@@ -1791,10 +1927,22 @@ static int ms_cg_basis(Context &c, const double2 *b, int nrhs, int nshift, const
   hop.bms = bsh; hop.bms_part = partC; hop.bms_nbr = (int)nbr;
   // live timing of the MD solver's hop (FP32, all pseudofermions)
   const bool timed = c.time_kernels && c.kt_clk && sizeof(R) == 4 && nrhs == c.n_pf;
-  if (timed) { hop.clk = c.kt_clk; hop.clk_cap = c.kt_clk_cap; }
+  if (timed) {
+    // end stamps of the r update (nbr x nrhs workgroups) and of the hop (at
+    // most 4 nbx x nrhs workgroups)
+    const size_t need = (size_t)nbr * nrhs + 4 * (size_t)nbx_of(c) * nrhs;
+    if (kernel_timing_stamps(c, need)) return -1;
+    hop.clk = c.kt_clk; hop.clk_cap = c.kt_clk_cap;
+    a.clk_wg = c.kt_wg;
+    hop.clk_prev = c.kt_wg; hop.clk_nprev = (int)nbr * nrhs; hop.clk_field = 0;
+    hop.clk_wg = c.kt_wg + (size_t)nbr * nrhs;
+  }
   DsArgs mdm = hop;
   mdm.par_out = 0; mdm.in = hh; mdm.in_vs = 3 * Vh; mdm.out = sr; mdm.pin = basis; mdm.pin_vs = 3 * Vh;
-  mdm.diag = diag; mdm.iter_in = 0; mdm.iter_pin = 1; mdm.clk = nullptr; mdm.bms_part = nullptr;
+  mdm.diag = diag; mdm.iter_in = 0; mdm.iter_pin = 1; mdm.bms_part = nullptr;
+  // M^dag M reads the hop's end stamps (field 1); it stamps nothing itself
+  mdm.clk_wg = nullptr;
+  mdm.clk_prev = hop.clk_wg; mdm.clk_field = 1;
   mdm.red = make_red(c, nrhs, EPI_STORE, nullptr, nullptr);
   a.partB = mdm.red.partial;
 
@@ -1812,6 +1960,7 @@ static int ms_cg_basis(Context &c, const double2 *b, int nrhs, int nshift, const
       hop.iter_off = q - 1;
       mdm.iter_off = q - 1;
       dslash_dispatch<R, LINK, DS_HOP, RED_NONE>(c, hop);
+      mdm.clk_nprev = (int)(hop.nsb * hop.nch);  // the hop's grid
       dslash_dispatch<R, LINK, DS_MDAGM, RED_DOT>(c, mdm);
       a.nbrB = mdm.red.nbr;
       a.iter_off = q;
@@ -1905,18 +2054,45 @@ static int ms_cg_basis(Context &c, const double2 *b, int nrhs, int nshift, const
   for (int j = 0; j < nrhs; j++)
     for (int sft = 0; sft < nshift; sft++)
       klen[j] = std::max(klen[j], hs[j].flag[sft] ? hs[j].iter : hs[j].its[sft]);
-  double *dcoef = (double *)ws_get(c, S_COEF, sizeof(double) * coef.size() + sizeof(int) * 256);
-  if (!dcoef) return -1;
-  int *dklen = (int *)(dcoef + coef.size());
-  hipMemcpyAsync(dcoef, coef.data(), sizeof(double) * coef.size(), hipMemcpyHostToDevice, c.stream);
-  hipMemcpyAsync(dklen, klen.data(), sizeof(int) * nrhs, hipMemcpyHostToDevice, c.stream);
-  for (int s0 = 0; s0 < nshift; s0 += 16)
+  if (asm_fp32 && sizeof(R) == 4) {
+    // matrix-core assembly: FP32 coefficients [rhs][shift group][kp][16]
+    const int ng = (nshift + 15) / 16;
+    const size_t kp = (cst + 15) / 16 * 16;
+    std::vector<float> &cf = c.ws->h_coef32;
+    cf.assign((size_t)nrhs * ng * kp * 16, 0.f);
+    for (int j = 0; j < nrhs; j++)
+      for (int sft = 0; sft < nshift; sft++) {
+        const double *src = coef.data() + ((size_t)j * nshift + sft) * cst;
+        float *dst = cf.data() + ((size_t)j * ng + sft / 16) * kp * 16 + sft % 16;
+        for (size_t i = 0; i < cst; i++) dst[i * 16] = (float)src[i];
+      }
+    float *dcf = (float *)ws_get(c, S_COEF, sizeof(float) * cf.size() + sizeof(int) * 256);
+    if (!dcf) return -1;
+    int *dkl = (int *)(dcf + cf.size());
+    hipMemcpyAsync(dcf, cf.data(), sizeof(float) * cf.size(), hipMemcpyHostToDevice, c.stream);
+    hipMemcpyAsync(dkl, klen.data(), sizeof(int) * nrhs, hipMemcpyHostToDevice, c.stream);
+    const dim3 grid((unsigned)((3 * Vh + 32 * (kBlock / 64) - 1) / (32 * (kBlock / 64))), nrhs, ng);
     if (x32)
-      hipLaunchKernelGGL((k_bms_assemble<R, 16, float2>), dim3((unsigned)((3 * Vh + kBlock - 1) / kBlock), nrhs),
-                         dim3(kBlock), 0, c.stream, basis, dcoef, dklen, x32, Vh, (int)cst, nshift, s0, nrhs);
+      hipLaunchKernelGGL(k_bms_assemble_mx<float2>, grid, dim3(kBlock), 0, c.stream, (const float2 *)basis, dcf,
+                         dkl, x32, Vh, (int)kp, nshift, nrhs);
     else
-      hipLaunchKernelGGL((k_bms_assemble<R, 16, double2>), dim3((unsigned)((3 * Vh + kBlock - 1) / kBlock), nrhs),
-                         dim3(kBlock), 0, c.stream, basis, dcoef, dklen, x, Vh, (int)cst, nshift, s0, nrhs);
+      hipLaunchKernelGGL(k_bms_assemble_mx<double2>, grid, dim3(kBlock), 0, c.stream, (const float2 *)basis, dcf,
+                         dkl, x, Vh, (int)kp, nshift, nrhs);
+  } else {
+    double *dcoef = (double *)ws_get(c, S_COEF, sizeof(double) * coef.size() + sizeof(int) * 256);
+    if (!dcoef) return -1;
+    int *dklen = (int *)(dcoef + coef.size());
+    hipMemcpyAsync(dcoef, coef.data(), sizeof(double) * coef.size(), hipMemcpyHostToDevice, c.stream);
+    hipMemcpyAsync(dklen, klen.data(), sizeof(int) * nrhs, hipMemcpyHostToDevice, c.stream);
+    const dim3 grid((unsigned)((3 * Vh + kBlock - 1) / kBlock), nrhs);
+    for (int s0 = 0; s0 < nshift; s0 += 16)
+      if (x32)
+        hipLaunchKernelGGL((k_bms_assemble<R, 16, float2>), grid, dim3(kBlock), 0, c.stream, basis, dcoef, dklen,
+                           x32, Vh, (int)cst, nshift, s0, nrhs);
+      else
+        hipLaunchKernelGGL((k_bms_assemble<R, 16, double2>), grid, dim3(kBlock), 0, c.stream, basis, dcoef, dklen,
+                           x, Vh, (int)cst, nshift, s0, nrhs);
+  }
   // stream-ordered for the caller; the host arrays stay alive in the workspace
   if (!hip_ok(c, hipGetLastError(), "basis assemble")) return -1;
   *iters_out = kmax;
@@ -2052,7 +2228,7 @@ int multishift_solve(Context &c, const double2 *b, int nrhs, int nshift, const d
   const long n = 3 * Vh * (long)nrhs * nshift;
   int rc = explicit_shifts ? 3
                            : ms_cg_basis<float, LINK_S12>(c, b, nrhs, nshift, shifts, res32.data(), max_iter, x, &it32,
-                                                          x32);
+                                                          x32, prec == PREC_FP32);
   if (rc == 3) {
     float2 *xs = x32 ? x32 : (float2 *)ws_get(c, S_X32, sizeof(float2) * 3 * Vh * nrhs * nshift);
     if (!xs) return -1;
