@@ -585,8 +585,9 @@ constexpr int ds_waves() {
 }
 
 // Output of one site for the chunk's right-hand sides: the hop result (times
-// 1/2), or m^2 + shift times the input minus it (M^dag M), or the residual
-// b - M^dag M x, with the dot/norm partials of the reducing modes.
+// 1/2), or m^2 + shift times the input minus it (M^dag M; the stored-basis
+// solver then adds beta times the old s), or the residual b - M^dag M x,
+// with the dot/norm partials of the reducing modes.
 template <typename R, int NR, int MODE, int RED>
 __device__ __forceinline__ void ds_site_out(const DsArgs &a, const Cx<R> (&acc)[NR][3], const bool (&act)[NR],
                                             double (&red)[NR], int jbase, int h, const Cx<R> *pinb) {
@@ -602,11 +603,11 @@ __device__ __forceinline__ void ds_site_out(const DsArgs &a, const Cx<R> (&acc)[
 #pragma unroll
       for (int i = 0; i < 3; i++) res[i] = cscale(acc[r][i], (R)0.5);
     } else {
-      const C *p = pinb + (long)j * a.pin_vs + h;
       const R d = (R)a.diag[j];
+      const C *p = pinb + (long)j * a.pin_vs + h;
 #pragma unroll
       for (int i = 0; i < 3; i++) {
-        C pv = p[(long)i * Vh];
+        const C pv = p[(long)i * Vh];
         res[i].x = fmar(d, pv.x, (R)-0.5 * acc[r][i].x);
         res[i].y = fmar(d, pv.y, (R)-0.5 * acc[r][i].y);
         if (MODE == DS_RESID) {
@@ -1535,20 +1536,49 @@ __device__ __forceinline__ void bms_x_body(const BmsArgs &a) {
     const R am = (R)(-al);
     long h0, h1;
     site_range(Vh, bx, gridDim.x, h0, h1);
-    const C *ro = (const C *)a.basis + ((long)(k - 1) * a.nvec + j) * 3 * Vh;
-    C *rn = (C *)a.basis + ((long)k * a.nvec + j) * 3 * Vh;
-    const C *sv = (const C *)a.s + (long)j * 3 * Vh;
-    for (long h = h0 + threadIdx.x; h < h1; h += kBlock)
+    const C *__restrict__ ro = (const C *)a.basis + ((long)(k - 1) * a.nvec + j) * 3 * Vh;
+    C *__restrict__ rn = (C *)a.basis + ((long)k * a.nvec + j) * 3 * Vh;
+    const C *__restrict__ sv = (const C *)a.s + (long)j * 3 * Vh;
+    // two sites per trip with all twelve loads issued before the first use
+    // (basis slots k-1 and k never overlap)
+    long h = h0 + threadIdx.x;
+    for (; h + kBlock < h1; h += 2 * kBlock) {
+      C sa[2][3], ra[2][3];
+#pragma unroll
+      for (int u = 0; u < 2; u++)
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+          const long e = (long)i * Vh + h + u * kBlock;
+          sa[u][i] = sv[e];
+          ra[u][i] = ro[e];
+        }
+#pragma unroll
+      for (int u = 0; u < 2; u++)
+#pragma unroll
+        for (int i = 0; i < 3; i++) {
+          C r = ra[u][i];
+          r.x = fmar(am, sa[u][i].x, r.x);
+          r.y = fmar(am, sa[u][i].y, r.y);
+          rn[(long)i * Vh + h + u * kBlock] = r;
+          v[0] += (double)r.x * (double)r.x + (double)r.y * (double)r.y;
+        }
+    }
+    if (h < h1) {
+      C sa[3], ra[3];
 #pragma unroll
       for (int i = 0; i < 3; i++) {
-        const long e = (long)i * Vh + h;
-        const C si = sv[e];
-        C r = ro[e];
-        r.x = fmar(am, si.x, r.x);
-        r.y = fmar(am, si.y, r.y);
-        rn[e] = r;
+        sa[i] = sv[(long)i * Vh + h];
+        ra[i] = ro[(long)i * Vh + h];
+      }
+#pragma unroll
+      for (int i = 0; i < 3; i++) {
+        C r = ra[i];
+        r.x = fmar(am, sa[i].x, r.x);
+        r.y = fmar(am, sa[i].y, r.y);
+        rn[(long)i * Vh + h] = r;
         v[0] += (double)r.x * (double)r.x + (double)r.y * (double)r.y;
       }
+    }
   }
   block_partials<1>(v, a.partC + (long)sn * a.nvec * a.nbrC, j, a.nvec, gridDim.x, bx);
   if (bx != 0) return;
