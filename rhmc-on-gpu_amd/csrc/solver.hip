@@ -140,9 +140,17 @@ __device__ __forceinline__ void site_range(long Vh, unsigned bx, unsigned nbr, l
 // 1-D grid of nsb site blocks x nch vector chunks, flattened so that the
 // workgroups dispatched consecutively to one XCD (hardware ids b, b+8, ...)
 // cover the same site block for every chunk: the shared links then stay in
-// that XCD's L2.  Each XCD owns a contiguous slab of site blocks.
+// that XCD's L2.  Each XCD owns a contiguous slab of site blocks.  With 8
+// chunks or more (the force's hop over every pole and pseudofermion) the
+// chunks' vectors no longer fit the caches together, and the grid runs
+// chunk by chunk instead, each chunk over the XCDs' slabs (128 right-hand
+// sides at 32^3x8: 343 -> 315 us; with 4 chunks of 2 it costs 16 us).
 __device__ __forceinline__ void xcd_split(unsigned b, unsigned nsb, unsigned nch, unsigned &sb, unsigned &ch) {
-  if (nsb % 8 == 0) {
+  if (nsb % 8 == 0 && nch >= 8) {
+    const unsigned xcd = b % 8, l = b / 8, per = nsb / 8;
+    sb = xcd * per + l % per;
+    ch = l / per;
+  } else if (nsb % 8 == 0) {
     const unsigned xcd = b % 8, l = b / 8;
     sb = xcd * (nsb / 8) + l / nch;
     ch = l % nch;

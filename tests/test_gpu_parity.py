@@ -80,6 +80,27 @@ def test_dslash_shapes_on_16_site_rows(shape):
         del os.environ["RHMC_DSLASH_NR"]
 
 
+@pytest.mark.parametrize("shape", ["1", "1x4"])
+def test_dslash_chunk_by_chunk_order(shape):
+    """12 right-hand sides at one per lane: 12 chunks, which the grid runs
+    chunk by chunk (8 or more chunks) over 8^4's 8 (or 32) site blocks."""
+    dims = (8, 8, 8, 8)
+    say("dslash chunk order", shape)
+    lat, U, _, _ = make_fields(dims, 17, 0.3)
+    o = Oracle(dims)
+    v = fields.gauss_spinor(lat, 8, 12)
+    ref_doe = np.stack([o.doe(U, v[k]) for k in range(12)])
+    os.environ["RHMC_DSLASH_NR"] = shape
+    try:
+        g = Gpu(dims, mass=0.1)
+        g.init0(U)
+        assert rel(g.dslash(1, v, 0), ref_doe) < 1e-14
+        assert rel(g.dslash(1, v, 1), ref_doe) < 2e-6
+        g.close()
+    finally:
+        del os.environ["RHMC_DSLASH_NR"]
+
+
 @pytest.mark.parametrize("precision", [0, 1])
 def test_multishift_on_16_site_rows(precision):
     """The basis multi-shift solver (hop and M^dag M launches with the x-row
@@ -350,6 +371,37 @@ def test_trajectory_fp32_md_against_reference(name):
     assert rel(P, gd["traj_mom"]) < 1e-4
     assert abs(r.delta_h - float(gd["traj_dh"])) < 1e-4 * max(1.0, abs(float(gd["traj_dh"])))
     g.close()
+
+
+def test_live_kernel_timing():
+    """Live timing of the MD hop: switched on around a mixed trajectory it
+    counts the FP32 hops that did work (not the first of each 8-iteration
+    chunk) with a per-launch duration between 1 us and 10 ms; switched off,
+    a second trajectory adds nothing.  Timing does not change the result
+    beyond FP32 rounding (each context autotunes its own launch shapes, and
+    the shape sets the reduction order)."""
+    gd = dict(np.load(os.path.join(GOLD, "golden_t4.npz")))
+    dims = tuple(gd["dims"])
+    say("live kernel timing")
+    kw = _traj_args(gd)
+    kw["res_md"] = 1e-9
+    out = []
+    for timed in (1, 0):
+        g = Gpu(dims, mass=float(gd["mass"]), beta=float(gd["beta"]), n_pf=int(gd["no_ps"]), max_order=8)
+        g.init0(gd["links"])
+        g.init1(momenta=gd["mom"])
+        g.put_fermions("phi", gd["phi"])
+        g.kernel_timing(timed)
+        r = g.trajectory(precision=1, fp32_threshold=1e-10, **kw)
+        mean_ms, n = g.kernel_timing(-1)
+        say("timed", timed, "mean ms", mean_ms, "launches", n, "dH", r.delta_h)
+        if timed:
+            assert n > 0 and 1e-3 < mean_ms < 10.0, (mean_ms, n)
+        else:
+            assert n == 0
+        out.append(r.delta_h)
+        g.close()
+    assert abs(out[0] - out[1]) < 1e-4 * max(1.0, abs(out[1]))
 
 
 def test_multi_trajectory_matches_single():
