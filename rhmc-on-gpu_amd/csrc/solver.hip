@@ -376,22 +376,42 @@ struct DsArgs {
 };
 
 // rho_{k-1} (fixed-order sum of its partials) and beta of every running
-// system; one workgroup, one wave per system
+// system, by one workgroup: the partials of up to 8 systems are loaded
+// together (one round trip), summed per lane in index order, then over the
+// wave and over the four waves in order.
 __device__ __forceinline__ void bms_beta(const DsArgs &a) {
   const int k = *a.iterp + a.iter_off + 1;
   const int sl = k & 1;
-  const double *part = a.bms_part + (long)sl * a.nvec * a.bms_nbr;
+  const int nbr = a.bms_nbr;
+  const double *part = a.bms_part + (long)sl * a.nvec * nbr;
+  __shared__ double sm[kBlock / 64][8];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  for (int j = w; j < a.nvec; j += kBlock / 64) {
-    BmsShared &sh = a.bms[j];
-    if (sh.stop[sl]) continue;
-    double s = 0.0;
-    for (int b = lane; b < a.bms_nbr; b += 64) s += part[(long)j * a.bms_nbr + b];
-    s = wave_sum(s);
-    if (lane == 0) {
-      sh.rho1 = s;
-      sh.beta = k >= 2 ? s / sh.rho[sl] : 0.0;
+  for (int j0 = 0; j0 < a.nvec; j0 += 8) {
+    double v[8];
+#pragma unroll
+    for (int r = 0; r < 8; r++) v[r] = 0.0;
+    for (int b = threadIdx.x; b < nbr; b += kBlock)
+#pragma unroll
+      for (int r = 0; r < 8; r++)
+        if (j0 + r < a.nvec) v[r] += part[(long)(j0 + r) * nbr + b];
+#pragma unroll
+    for (int r = 0; r < 8; r++) {
+      const double t = wave_sum(v[r]);
+      if (lane == 0) sm[w][r] = t;
     }
+    __syncthreads();
+    const int j = j0 + (int)threadIdx.x;
+    if (threadIdx.x < 8 && j < a.nvec) {
+      double rho = 0.0;
+#pragma unroll
+      for (int q = 0; q < kBlock / 64; q++) rho += sm[q][threadIdx.x];
+      BmsShared &sh = a.bms[j];
+      if (!sh.stop[sl]) {
+        sh.rho1 = rho;
+        sh.beta = k >= 2 ? rho / sh.rho[sl] : 0.0;
+      }
+    }
+    __syncthreads();
   }
 }
 
