@@ -45,21 +45,37 @@ def dslash_bytes(Vh, nrhs, link_b, spin_b):
     return Vh * (8 * link_b + 2 * nrhs * spin_b)
 
 
-def dslash_traffic(table, Vh, nrhs):
-    """HBM bytes per launch of the autotuned FP32 12-real hop on nrhs vectors:
-    FETCH_SIZE + WRITE_SIZE of the same kernel and grid from the rocprofv3
-    counter runs (profiles/dslash_32c8_counters.json, each counter scaled by
-    the copy probe), or None when that shape was not profiled."""
+def mdagm_update_bytes(Vh, nrhs, link_b, spin_b):
+    """Algorithmic bytes of the stored-basis solver's M^dag M launch: links,
+    the hop result h, r^(k-1) at the site, s read and written, r^(k) written:
+    N_out (8 L_b + 5 N_rhs S_b)."""
+    return Vh * (8 * link_b + 5 * nrhs * spin_b)
+
+
+def dslash_traffic(table, Vh, nrhs, mode=0, red=0):
+    """HBM bytes per launch of the autotuned FP32 12-real Dslash-family kernel
+    (mode, reduction) on nrhs vectors: FETCH_SIZE + WRITE_SIZE of the same
+    kernel and grid from the rocprofv3 counter runs
+    (profiles/dslash_32c8_counters.json, each counter scaled by the copy
+    probe), or None when that shape was not profiled."""
     path = os.path.join(ROOT, "profiles", "dslash_32c8_counters.json")
     if not os.path.exists(path):
         return None
-    ent = [t for t in table if t["real_bytes"] == 4 and t["links"] == "fp32_12" and t["mode"] == 0
-           and t["reduction"] == 0 and t["nvec"] == nrhs]
+    ent = [t for t in table if t["real_bytes"] == 4 and t["links"] == "fp32_12" and t["mode"] == mode
+           and t["reduction"] == red and t["nvec"] == nrhs]
     if not ent:
         return None
     nr, lanes = ent[0]["rhs_per_thread"], ent[0]["lanes_per_site"]
-    name = f"void rhmc::k_dslash{'_ls' if lanes == 4 else ''}<float, 1, {nr}, 0, 0>(rhmc::DsArgs)"
-    grid = Vh * lanes * ((nrhs + nr - 1) // nr)
+    name = f"void rhmc::k_dslash{'_ls' if lanes == 4 else ''}<float, 1, {nr}, {mode}, {red}>(rhmc::DsArgs)"
+    nch = (nrhs + nr - 1) // nr
+    if mode == 0:
+        grid = Vh * lanes * nch  # one workgroup per tile (kBlock threads each)
+    else:
+        # reducing modes: reduce_blocks(nch) workgroups per chunk (kBlock threads)
+        nbx = (Vh + 255) // 256
+        want = max(64, (2048 + nch - 1) // nch)
+        want = (want + 7) // 8 * 8
+        grid = min(want, nbx) * 256 * nch
     with open(path) as fh:
         k = json.load(fh)["kernels"].get(f"{name}|{grid}")
     return None if k is None else k["read_bytes"] + k["write_bytes"]
@@ -404,6 +420,14 @@ def main():
         cp = 1 << 30
         ms = g.time_copy(cp, 20)
         res["copy_1GiB"] = {"ms": ms, "GBps": 2 * cp / (ms * 1e-3) / 1e9, "bytes": 2 * cp}
+        # one FP32 multi-shift solve on n_pf right-hand sides: the MD solver's
+        # own launches (the hop with its |h|^2 partials, M^dag M with the CG
+        # updates, the matrix-core assembly) for the counter runs
+        g.init1(phi=fields.gauss_spinor(lat, 7, n_pf))
+        shifts = np.geomspace(1e-4, 1.0, order) * 4.0
+        t0 = time.perf_counter()
+        it = g.multishift(shifts, 1e-6, relative=True, precision=2)
+        res["solve_fp32_8rhs_16shifts"] = {"iterations": it, "s": time.perf_counter() - t0}
         res["autotune"] = g.tuning_table()
         print(json.dumps(res))
         g.close()
@@ -473,10 +497,18 @@ def main():
     # by a 512 MB write pass before each (HBM-resident operands)
     ms32c = g.time_dslash_cold(1, n_pf, 50)
     ms64c = g.time_dslash_cold(0, 1, 50)
-    n_live = sum(n for _, n in live)
-    live_ms = sum(m * n for m, n in live) / n_live if n_live else ms32
+    # live: the hop and the M^dag M launch (with the s and r updates) of the
+    # MD solver, timed inside the trajectories
+    n_live = sum(h[1] for h, _ in live)
+    live_ms = sum(h[0] * h[1] for h, _ in live) / n_live if n_live else ms32
     gbs_live = b32 / (live_ms * 1e-3) / 1e9
-    traffic = dslash_traffic(g.tuning_table(), lat.Vh, n_pf)
+    n_mdm = sum(m[1] for _, m in live)
+    mdm_ms = sum(m[0] * m[1] for _, m in live) / n_mdm if n_mdm else float("nan")
+    b_mdm = mdagm_update_bytes(lat.Vh, n_pf, 48, 24)
+    gbs_mdm = b_mdm / (mdm_ms * 1e-3) / 1e9
+    table = g.tuning_table()
+    traffic_hop = dslash_traffic(table, lat.Vh, n_pf, mode=0, red=2)
+    traffic_mdm = dslash_traffic(table, lat.Vh, n_pf, mode=3, red=2)
 
     cpu = None
     if not args.no_cpu and iters_d:
@@ -520,15 +552,19 @@ def main():
                    "frac_hbm_fp32": gbs32 / HBM_PEAK_GBS, "frac_hbm_fp64": gbs64 / HBM_PEAK_GBS,
                    "cold_fp32_12real_8rhs_GBps": b32 / (ms32c * 1e-3) / 1e9,
                    "cold_fp64_18real_1rhs_GBps": b64 / (ms64c * 1e-3) / 1e9,
-                   "live_fp32_12real_8rhs_ms": live_ms, "live_launches": n_live},
-        "roofline": {"bound": "hbm", "kernel": "k_dslash<float,S12,HOP> on 8 rhs (MD solver half-lattice hop)",
-                     "achieved": gbs_live, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs_live / HBM_PEAK_GBS,
-                     "traffic": traffic, "bytes_per_launch": b32,
-                     "measured": f"mean kernel duration of {n_live} launches inside the timed mixed trajectories "
+                   "live_fp32_12real_8rhs_ms": live_ms, "live_launches": n_live,
+                   "live_fp32_12real_8rhs_GBps": gbs_live, "live_frac_hbm": gbs_live / HBM_PEAK_GBS,
+                   "live_traffic": traffic_hop},
+        # the dominant kernel of the trajectory: the MD solver's M^dag M launch
+        # with the CG's s and r updates
+        "roofline": {"bound": "hbm",
+                     "kernel": "k_dslash<float,S12,BMS> on 8 rhs (MD solver M^dag M with the CG s and r updates)",
+                     "achieved": gbs_mdm, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs_mdm / HBM_PEAK_GBS,
+                     "traffic": traffic_mdm, "bytes_per_launch": b_mdm,
+                     "measured": f"mean kernel duration of {n_mdm} launches inside the timed mixed trajectories "
                                  f"(graphs replayed), each timed on the device wall clock from the end of the "
-                                 f"launch before it (the residual update) to the end of its last workgroup, "
-                                 f"the per-dispatch interval of a kernel trace; the first hop of each "
-                                 f"8-iteration chunk is not counted"},
+                                 f"hop before it to the end of its last workgroup, the per-dispatch interval "
+                                 f"of a kernel trace"},
         "cpu_baseline": cpu,
         "dslash_autotune": g.tuning_table(),
     }

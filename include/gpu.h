@@ -168,18 +168,20 @@ int gpu_time_dslash(rhmc_gpu_t h, int precision, int nvec, int reps, double *ms)
  * times. */
 int gpu_time_dslash_cold(rhmc_gpu_t h, int precision, int nvec, int reps, double flush_bytes, double *ms);
 
-/* Live kernel timing inside real work: while enabled, the MD solver's hop
- * (FP32 12-real half-lattice Dslash on the n_pf pseudofermions) is timed on
- * the device wall clock, from the end of the launch before it (the residual
- * update, which stamps its own end) to the end of its last workgroup: the
- * per-dispatch interval a kernel-trace profiler reports (dispatch, execution
- * and end of kernel).  The stamps sit in slots indexed by the solver
- * iteration, so replayed graphs are timed too.  Only iterations in which the
- * hop did work are counted, and not the first hop of each 8-iteration chunk
- * (it follows the chunk's bookkeeping launches instead).
- * Returns the mean duration and count since the last reset; enable = 1/0
- * then switches timing on or off and resets, enable = -1 only reads.  No
- * counterpart in the reference. */
+/* Live kernel timing inside real work: while enabled, the two launches of
+ * each iteration of the MD solver (FP32 12-real links, the n_pf
+ * pseudofermions) are timed on the device wall clock: [0] the half-lattice
+ * hop D_oe, [1] the M^dag M launch after it (with the CG's s and r
+ * updates).  Each interval runs from the end of the launch before to the end
+ * of the timed launch's last workgroup, the per-dispatch interval a
+ * kernel-trace profiler reports (dispatch, execution and end of kernel);
+ * every workgroup stamps its own end.  The stamps sit in slots indexed by
+ * the solver iteration, so replayed graphs are timed too.  Only iterations
+ * that did work are counted, and not the first hop of each 8-iteration
+ * chunk (it follows the chunk's bookkeeping launches instead).
+ * mean_ms[2] and count[2] receive the mean durations and counts since the
+ * last reset; enable = 1/0 then switches timing on or off and resets,
+ * enable = -1 only reads.  No counterpart in the reference. */
 int gpu_kernel_timing(rhmc_gpu_t h, int enable, double *mean_ms, long long *count);
 
 /* Timing helper: a plain device copy of `bytes` (read + write), mean ms per

@@ -534,13 +534,19 @@ int gpu_kernel_timing(rhmc_gpu_t h, int enable, double *mean_ms, long long *coun
   if (!h) return RHMC_EINVAL;
   Context &c = h->c;
   hipSetDevice(c.device);
-  if (mean_ms) *mean_ms = c.kt_count ? c.kt_sum_ms / c.kt_count : 0.0;
-  if (count) *count = c.kt_count;
+  if (mean_ms) {
+    mean_ms[0] = c.kt_count ? c.kt_sum_ms / c.kt_count : 0.0;
+    mean_ms[1] = c.kt_count2 ? c.kt_sum2_ms / c.kt_count2 : 0.0;
+  }
+  if (count) {
+    count[0] = c.kt_count;
+    count[1] = c.kt_count2;
+  }
   if (enable >= 0) {
     if (enable && kernel_timing_clock(c)) return RHMC_EHIP;
     c.time_kernels = enable;
-    c.kt_sum_ms = 0.0;
-    c.kt_count = 0;
+    c.kt_sum_ms = c.kt_sum2_ms = 0.0;
+    c.kt_count = c.kt_count2 = 0;
   }
   return RHMC_OK;
 }

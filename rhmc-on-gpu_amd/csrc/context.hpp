@@ -80,8 +80,10 @@ struct Context {
   int kt_clk_cap = 0;
   unsigned long long *kt_wg = nullptr;  // per-workgroup end stamps of the timed launches
   size_t kt_wg_cap = 0;
-  double kt_sum_ms = 0.0;               // kernel durations on the device wall clock
+  double kt_sum_ms = 0.0;               // hop durations on the device wall clock
   long kt_count = 0;
+  double kt_sum2_ms = 0.0;              // durations of the M^dag M launch after it
+  long kt_count2 = 0;
   double wall_khz = 0.0;
 
   Workspace *ws = nullptr;
@@ -153,7 +155,8 @@ int vec_dot_diffs(Context &c, const double2 *a1, const double2 *b1, const double
 // Dslash tile order for the lattice of c (solver.hip); allocates c.tiles
 int build_tile_order(Context &c);
 // fold the device-clock timing slots of the first n solver iterations into
-// kt_sum_ms / kt_count and clear them; release frees the slots
+// kt_sum_ms / kt_count (hop) and kt_sum2_ms / kt_count2 (M^dag M) and clear
+// them; release frees the slots
 void kernel_timing_harvest(Context &c, int n);
 void kernel_timing_release(Context &c);
 // allocate and clear the slots (timing switched on)

@@ -40,7 +40,7 @@ enum Slot {
   S_CR64, S_CX32, S_CR32, S_CP32, S_CS32, S_CH32,
   S_PW1, S_PW2, S_PW3,
   S_FH64,
-  S_PARTIAL2, S_BASIS, S_BSH, S_HIST, S_COEF, S_TUNE,
+  S_PARTIAL2, S_BASIS, S_BSH, S_HIST, S_COEF, S_TUNE, S_HPART,
   S_COUNT
 };
 
@@ -327,19 +327,36 @@ __global__ __launch_bounds__(kBlock) void k_finalize(RedArgs ra, int nvec, int *
   }
 }
 
+// fixed-order workgroup sum of n partials
+__device__ __forceinline__ double block_sum_partials(const double *p, int n) {
+  __shared__ double sh[kBlock / 64];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  double v = 0.0;
+  for (int i = threadIdx.x; i < n; i += kBlock) v += p[i];
+  v = wave_sum(v);
+  if (lane == 0) sh[w] = v;
+  __syncthreads();
+  double s = 0.0;
+#pragma unroll
+  for (int k = 0; k < kBlock / 64; k++) s += sh[k];
+  __syncthreads();
+  return s;
+}
+
 // ------------------------------------------------------------------ Dslash
-enum DsMode { DS_HOP = 0, DS_MDAGM = 1, DS_RESID = 2 };
+// DS_BMS: the stored-basis solver's M^dag M with the s and r updates
+enum DsMode { DS_HOP = 0, DS_MDAGM = 1, DS_RESID = 2, DS_BMS = 3 };
 enum DsRed { RED_NONE = 0, RED_DOT = 1, RED_NORM = 2 };
 enum LinkFmt { LINK_D18 = 0, LINK_S12 = 1, LINK_D12 = 2 };
 
 // scalars of one system of the stored-basis multi-shift CG shared between
 // its launches (the algorithm is described at ms_cg_basis)
 struct BmsShared {
-  double rho[2];    // [k&1] at launch k: rho_{k-2}
-  double alpha[2];  // [k&1] at launch k: alpha_{k-2}
-  int stop[2];
-  double rho1;      // rho_{k-1}, written by the hop launch of iteration k
-  double beta;      // beta_{k-2} = rho_{k-1} / rho_{k-2} (0 at k = 1)
+  double rho[2];    // [k&1] at the hop of iteration k: rho_{k-2}
+  double alpha[2];  // [k&1] at iteration k: the alpha of iteration k-1
+  int stop[2];      // [k&1]: system stopped before iteration k
+  double rho1;      // rho_{k-1}, written by the hop of iteration k (rho_0 by the set-up)
+  double beta;      // rho_{k-1} / rho_{k-2} (0 at k = 1), written with rho1
 };
 
 struct DsArgs {
@@ -367,52 +384,138 @@ struct DsArgs {
   unsigned long long *clk; int clk_cap;
   unsigned long long *clk_wg;
   const unsigned long long *clk_prev; int clk_nprev, clk_field;
-  // stored-basis CG: the hop launch (bms_part set) forms rho1/beta of every
-  // running system from the previous r update's partials (bms_nbr per
-  // system, two buffers by iteration parity); the M^dag M launch (bms set)
-  // then stores s = (m^2 r - D_eo h) + beta s into out in place
+  // stored-basis CG (iteration k = *iterp + iter_off + 1): the hop (bms_ms
+  // set) has workgroup j < nvec advance system j's scalars from the rho
+  // partials of the previous M^dag M launch (bms_nbr per system); the
+  // DS_BMS launch forms alpha from the hop's |h|^2 partials (bms_hnbr per
+  // system), updates s (out) in place and writes r^(k) into the basis slot
+  // after pin's, below bms_cap, with |r^(k)|^2 partials in red
   BmsShared *bms; const double *bms_part; int bms_nbr;
+  MsState *bms_ms; BmsHist *bms_hist; int *bms_active; int bms_cap, bms_nshift;
+  const double *bms_hpart; int bms_hnbr;
   RedArgs red;
 };
 
-// rho_{k-1} (fixed-order sum of its partials) and beta of every running
-// system, by one workgroup: the partials of up to 8 systems are loaded
-// together (one round trip), summed per lane in index order, then over the
-// wave and over the four waves in order.
-__device__ __forceinline__ void bms_beta(const DsArgs &a) {
+// Hop of iteration k, workgroup j (< nvec): system j's scalars.  rho_{k-1}
+// is the fixed-order sum of the previous M^dag M launch's partials (at k = 1
+// the set-up's rho_0), beta = rho_{k-1} / rho_{k-2}; then the Jegerlehner
+// recurrences of iteration k-1 (inverter.cc:224-287, they need alpha and
+// beta of the base CG's step k-2) and the stopping test, whose result
+// applies from iteration k+1: the stop decision lags one iteration.  The
+// workgroup of system 0 also clears the running-system count of iteration k
+// (slot k&1) while the others add to that of k+1.  Values the other
+// launches of iteration k read (stop, rho, alpha) are double-buffered by
+// the parity of k.
+__device__ __forceinline__ void bms_step(const DsArgs &a, int j) {
   const int k = *a.iterp + a.iter_off + 1;
-  const int sl = k & 1;
-  const int nbr = a.bms_nbr;
-  const double *part = a.bms_part + (long)sl * a.nvec * nbr;
-  __shared__ double sm[kBlock / 64][8];
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  for (int j0 = 0; j0 < a.nvec; j0 += 8) {
-    double v[8];
-#pragma unroll
-    for (int r = 0; r < 8; r++) v[r] = 0.0;
-    for (int b = threadIdx.x; b < nbr; b += kBlock)
-#pragma unroll
-      for (int r = 0; r < 8; r++)
-        if (j0 + r < a.nvec) v[r] += part[(long)(j0 + r) * nbr + b];
-#pragma unroll
-    for (int r = 0; r < 8; r++) {
-      const double t = wave_sum(v[r]);
-      if (lane == 0) sm[w][r] = t;
+  const int sl = k & 1, sn = sl ^ 1;
+  const int t = threadIdx.x;
+  BmsShared &sh = a.bms[j];
+  if (j == 0 && t == 0) a.bms_active[sl] = 0;
+  if (sh.stop[sl]) {
+    if (t == 0) {
+      sh.stop[sn] = 1;
+      sh.rho[sn] = sh.rho[sl];
+      sh.alpha[sn] = sh.alpha[sl];
     }
-    __syncthreads();
-    const int j = j0 + (int)threadIdx.x;
-    if (threadIdx.x < 8 && j < a.nvec) {
-      double rho = 0.0;
-#pragma unroll
-      for (int q = 0; q < kBlock / 64; q++) rho += sm[q][threadIdx.x];
-      BmsShared &sh = a.bms[j];
-      if (!sh.stop[sl]) {
-        sh.rho1 = rho;
-        sh.beta = k >= 2 ? rho / sh.rho[sl] : 0.0;
-      }
-    }
-    __syncthreads();
+    return;
   }
+  const double rho1 = k == 1 ? sh.rho1 : block_sum_partials(a.bms_part + (long)j * a.bms_nbr, a.bms_nbr);
+  const double rho2 = sh.rho[sl], al2 = sh.alpha[sl];  // rho_{k-2}, alpha of iteration k-1
+  const double be = k >= 2 ? rho1 / rho2 : 0.0;
+  MsState &s = a.bms_ms[j];
+  int stop = 0;
+  if (k >= 2) {
+    // recurrences of iteration kk = k-1: omega = -alpha, gamma = beta
+    const int kk = k - 1;
+    const double omp = s.omega, gold = s.gammag;
+    const double om = -al2, gam = be;
+    if (t < s.nshift && s.flag[t]) {
+      const double z1 = s.z1[t], z2 = s.z2[t];
+      const double z3 = (z1 * z2 * omp) / (om * gold * (z1 - z2) + z1 * omp * (1.0 - s.shift[t] * om));
+      const double oms = om * z3 / z2;
+      const double gas = gam * z3 * oms / (z2 * om);
+      BmsHist &hh = a.bms_hist[((long)j * a.bms_cap + kk) * a.bms_nshift + t];
+      hh.om = oms;
+      hh.z = z3;
+      hh.ga = gas;
+      if (sqrt(rho2 * z2 * z2) < s.res) {
+        s.flag[t] = 0;
+        s.its[t] = kk;
+      }
+      s.z1[t] = z2;
+      s.z2[t] = z3;
+    }
+    const int nact = __syncthreads_count(t < s.nshift && s.flag[t]);
+    if (t == 0) {
+      s.omega_prev = omp;
+      s.omega = om;
+      s.gammag = gam;
+      s.delta = rho1;
+      s.iter = kk;
+      s.nact = nact;
+      // r^(k) stays inside the basis: the last M^dag M launch that writes
+      // runs at k = cap-1
+      stop = !(sqrt(rho1) > s.res && kk < s.max_iter) || nact == 0 || kk + 2 >= a.bms_cap;
+      s.stop = s.done = stop;
+    }
+  }
+  if (t == 0) {
+    sh.rho1 = rho1;
+    sh.beta = be;
+    sh.rho[sn] = rho1;
+    sh.stop[sn] = stop;
+    if (!stop) atomicAdd(&a.bms_active[sn], 1);
+  }
+}
+
+// M^dag M of iteration k (DS_BMS), every workgroup: alpha and beta of its NR
+// systems.  mu = <r, M^dag M r> = diag rho_{k-1} + |D_oe r|^2 (the staggered
+// D is anti-hermitian, D_eo^+ = -D_oe), from the hop's |h|^2 partials summed
+// in a fixed order, so every workgroup forms the same value; <p, A p> = mu -
+// beta rho_{k-1} / alpha_prev (single-reduction CG), alpha = rho_{k-1} /
+// <p, A p>, into ab (workgroup memory: [0] alpha, [1] beta, read by the
+// epilogue).  The first site block's workgroup of each chunk stores alpha
+// for iteration k+1.  A system that is stopped is not used (its output is
+// not stored).
+template <typename R, int NR>
+__device__ __forceinline__ void bms_alpha(const DsArgs &a, int jbase, unsigned sb, R (&ab)[2][NR]) {
+  const int k = *a.iterp + a.iter_off + 1;
+  const int sl = k & 1, sn = sl ^ 1;
+  const int n = a.bms_hnbr;
+  __shared__ double sm[kBlock / 64][NR];
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+  double v[NR];
+#pragma unroll
+  for (int r = 0; r < NR; r++) {
+    v[r] = 0.0;
+    const int j = jbase + r < a.nvec ? jbase + r : a.nvec - 1;
+    for (int b = threadIdx.x; b < n; b += kBlock) v[r] += a.bms_hpart[(long)j * n + b];
+  }
+#pragma unroll
+  for (int r = 0; r < NR; r++) {
+    const double q = wave_sum(v[r]);
+    if (lane == 0) sm[w][r] = q;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int r = 0; r < NR; r++) {
+    const int j = jbase + r < a.nvec ? jbase + r : a.nvec - 1;
+    double h2 = 0.0;
+#pragma unroll
+    for (int q = 0; q < kBlock / 64; q++) h2 += sm[q][r];
+    const BmsShared &sh = a.bms[j];
+    const double rho1 = sh.rho1, b = sh.beta;
+    const double mu = a.diag[j] * rho1 + h2;
+    const double pap = k >= 2 ? mu - b * rho1 / sh.alpha[sl] : mu;
+    const double alpha = rho1 / pap;
+    if (threadIdx.x == 0) {
+      ab[0][r] = (R)alpha;
+      ab[1][r] = (R)b;
+      if (sb == 0 && jbase + r < a.nvec && !sh.stop[sl]) a.bms[j].alpha[sn] = alpha;
+    }
+  }
+  __syncthreads();
 }
 
 __device__ __forceinline__ unsigned long long *clk_slot(const DsArgs &a) {
@@ -613,12 +716,15 @@ constexpr int ds_waves() {
 }
 
 // Output of one site for the chunk's right-hand sides: the hop result (times
-// 1/2), or m^2 + shift times the input minus it (M^dag M; the stored-basis
-// solver then adds beta times the old s), or the residual b - M^dag M x,
-// with the dot/norm partials of the reducing modes.
+// 1/2), or m^2 + shift times the input minus it (M^dag M), or the residual
+// b - M^dag M x, with the dot/norm partials of the reducing modes.  DS_BMS
+// (the stored-basis solver): w = m^2 r - D_eo h, s = w + beta s in place,
+// r^(k) = r - alpha s into the next basis slot (when rk_ok), |r^(k)|^2
+// partials.
 template <typename R, int NR, int MODE, int RED>
 __device__ __forceinline__ void ds_site_out(const DsArgs &a, const Cx<R> (&acc)[NR][3], const bool (&act)[NR],
-                                            double (&red)[NR], int jbase, int h, const Cx<R> *pinb) {
+                                            double (&red)[NR], int jbase, int h, const Cx<R> *pinb,
+                                            const R (&ab)[2][NR], bool rk_ok) {
   using C = Cx<R>;
   const long Vh = a.L.Vh;
 #pragma unroll
@@ -633,6 +739,7 @@ __device__ __forceinline__ void ds_site_out(const DsArgs &a, const Cx<R> (&acc)[
     } else {
       const R d = (R)a.diag[j];
       const C *p = pinb + (long)j * a.pin_vs + h;
+      C *rk = (C *)(pinb + a.iter_stride) + (long)j * a.pin_vs + h;
 #pragma unroll
       for (int i = 0; i < 3; i++) {
         const C pv = p[(long)i * Vh];
@@ -643,22 +750,29 @@ __device__ __forceinline__ void ds_site_out(const DsArgs &a, const Cx<R> (&acc)[
           res[i] = csub(mk<R>((R)bv.x, (R)bv.y), res[i]);
         }
         if (RED == RED_DOT) red[r] += (double)pv.x * (double)res[i].x + (double)pv.y * (double)res[i].y;
-      }
-      if (MODE == DS_MDAGM && a.bms) {
-        const R be = (R)a.bms[j].beta;
-        if (be != (R)0)
-#pragma unroll
-          for (int i = 0; i < 3; i++) {
+        if (MODE == DS_BMS) {
+          const R be = ab[1][r], al = ab[0][r];
+          if (be != (R)0) {
             const C so = o[(long)i * Vh];
             res[i].x = fmar(be, so.x, res[i].x);
             res[i].y = fmar(be, so.y, res[i].y);
           }
+          C rn;
+          rn.x = fmar(-al, res[i].x, pv.x);
+          rn.y = fmar(-al, res[i].y, pv.y);
+          if (rk_ok) rk[(long)i * Vh] = rn;
+          red[r] += (double)rn.x * (double)rn.x + (double)rn.y * (double)rn.y;
+        }
       }
     }
 #pragma unroll
-    for (int i = 0; i < 3; i++) {
-      o[(long)i * Vh] = res[i];
-      if (RED == RED_NORM) red[r] += (double)res[i].x * (double)res[i].x + (double)res[i].y * (double)res[i].y;
+    for (int i = 0; i < 3; i++) o[(long)i * Vh] = res[i];
+    if (RED == RED_NORM && MODE != DS_BMS) {
+      // the site's |v|^2 in the working precision, accumulated in FP64
+      R n2 = (R)0;
+#pragma unroll
+      for (int i = 0; i < 3; i++) n2 = fmar(res[i].x, res[i].x, fmar(res[i].y, res[i].y, n2));
+      red[r] += (double)n2;
     }
   }
 }
@@ -666,13 +780,16 @@ __device__ __forceinline__ void ds_site_out(const DsArgs &a, const Cx<R> (&acc)[
 template <typename R, int LINK, int NR, int MODE, int RED>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(ds_waves<R, LINK, NR>()))) void k_dslash(DsArgs a) {
   using C = Cx<R>;
-  if (MODE == DS_HOP && a.bms_part && blockIdx.x == 0) bms_beta(a);
+  if (MODE == DS_HOP && a.bms_ms && (int)blockIdx.x < a.nvec) bms_step(a, blockIdx.x);
   clk_begin(a);
   const LatticeDims &L = a.L;
   const long Vh = L.Vh;
   unsigned sb, ch;
   xcd_split(blockIdx.x, a.nsb, a.nch, sb, ch);
   const int jbase = ch * NR;
+  __shared__ R bab[2][NR];
+  if (MODE == DS_BMS) bms_alpha<R, NR>(a, jbase, sb, bab);
+  const bool rk_ok = MODE == DS_BMS && *a.iterp + a.iter_off + 1 < a.bms_cap;
   // this workgroup's run of tiles in the traversal order
   const int per = (a.ntiles + (int)a.nsb - 1) / (int)a.nsb;
   const int q0 = (int)sb * per;
@@ -705,7 +822,10 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(ds_waves
     inr[r] = inb + (long)j * a.in_vs;
   }
   const bool xrow = L.nxh == 16;
-  for (int q = q0; any && q < q1; q++) {
+  // the hop's grid has one workgroup per tile: a single pass, so that the
+  // reduction's accumulators are not carried through a loop around the hops
+  const int qend = MODE == DS_HOP ? (q0 + 1 < q1 ? q0 + 1 : q1) : q1;
+  for (int q = q0; any && q < qend; q++) {
     const int h = (a.tiles ? a.tiles[q] : q) * kBlock + (int)threadIdx.x;
     if (h >= Vh32) continue;
     const SiteCoord c = site_coords_i(L, a.par_out, h);
@@ -725,7 +845,7 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(ds_waves
       if (k == 0 && xrow) xm_from_xp<R, LINK, NR>(buf[0], buf[1]);
       hop_fence<R, NR>(acc);
     }
-    ds_site_out<R, NR, MODE, RED>(a, acc, act, red, jbase, h, pinb);
+    ds_site_out<R, NR, MODE, RED>(a, acc, act, red, jbase, h, pinb, bab, rk_ok);
   }
   if (RED != RED_NONE) block_partials<NR>(red, a.red.partial, jbase, a.nvec, a.nsb, sb);
   clk_end(a);
@@ -842,7 +962,7 @@ __device__ __forceinline__ C pick3(C a0, C a1, C a2, int k) {
 template <typename R, int LINK, int NR, int MODE, int RED>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(ds_waves<R, LINK, NR>()))) void k_dslash_ls(DsArgs a) {
   using C = Cx<R>;
-  if (MODE == DS_HOP && a.bms_part && blockIdx.x == 0) bms_beta(a);
+  if (MODE == DS_HOP && a.bms_ms && (int)blockIdx.x < a.nvec) bms_step(a, blockIdx.x);
   clk_begin(a);
   const LatticeDims &L = a.L;
   const long Vh = L.Vh;
@@ -850,6 +970,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(ds_waves
   unsigned sb, ch;
   xcd_split(blockIdx.x, a.nsb, a.nch, sb, ch);
   const int jbase = ch * NR;
+  __shared__ R bab[2][NR];
+  if (MODE == DS_BMS) bms_alpha<R, NR>(a, jbase, sb, bab);
+  const bool rk_ok = MODE == DS_BMS && *a.iterp + a.iter_off + 1 < a.bms_cap;
   const int nunits = a.ntiles * (kBlock / kLsSites);
   const int per = (nunits + (int)a.nsb - 1) / (int)a.nsb;
   const int q0 = (int)sb * per;
@@ -882,7 +1005,9 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(ds_waves
     inr[r] = inb + (long)j * a.in_vs;
   }
 
-  for (int q = q0; any && q < q1; q++) {
+  // one workgroup per tile quarter for the hop: a single pass (as k_dslash)
+  const int qend = MODE == DS_HOP ? (q0 + 1 < q1 ? q0 + 1 : q1) : q1;
+  for (int q = q0; any && q < qend; q++) {
     const int tile = a.tiles ? a.tiles[q >> 2] : (q >> 2);
     const int h = tile * kBlock + (q & 3) * kLsSites + slot;
     const bool valid = h < Vh32;
@@ -928,17 +1053,22 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(ds_waves
           res = csub(mk<R>((R)bv.x, (R)bv.y), res);
         }
         if (RED == RED_DOT) red[r] += (double)pv.x * (double)res.x + (double)pv.y * (double)res.y;
-        if (MODE == DS_MDAGM && a.bms) {
-          const R be = (R)a.bms[j].beta;
+        if (MODE == DS_BMS) {
+          const R be = bab[1][r], al = bab[0][r];
           if (be != (R)0) {
             const C so = ((const C *)a.out + (long)j * a.out_vs + h)[(long)col * Vh];
             res.x = fmar(be, so.x, res.x);
             res.y = fmar(be, so.y, res.y);
           }
+          C rn;
+          rn.x = fmar(-al, res.x, pv.x);
+          rn.y = fmar(-al, res.y, pv.y);
+          if (rk_ok) ((C *)(pinb + a.iter_stride) + (long)j * a.pin_vs + h)[(long)col * Vh] = rn;
+          red[r] += (double)rn.x * (double)rn.x + (double)rn.y * (double)rn.y;
         }
       }
       ((C *)a.out + (long)j * a.out_vs + h)[(long)col * Vh] = res;
-      if (RED == RED_NORM) red[r] += (double)res.x * (double)res.x + (double)res.y * (double)res.y;
+      if (RED == RED_NORM && MODE != DS_BMS) red[r] += (double)res.x * (double)res.x + (double)res.y * (double)res.y;
     }
   }
   if (RED != RED_NONE) block_partials<NR>(red, a.red.partial, jbase, a.nvec, a.nsb, sb);
@@ -977,7 +1107,9 @@ static void dslash_launch(Context &c, DsArgs &a, int code) {
   a.tiles = c.tiles;
   a.ntiles = c.ntiles;
   a.nch = (unsigned)((a.nvec + nr - 1) / nr);
-  a.nsb = RED == RED_NONE ? nbx_of(c) * lanes : reduce_blocks(c, a.nch);
+  // one workgroup per tile for the hop (its reduction, when any, then has a
+  // partial per tile); fewer, longer workgroups for the other reducing modes
+  a.nsb = RED == RED_NONE || MODE == DS_HOP ? nbx_of(c) * lanes : reduce_blocks(c, a.nch);
   a.red.nbr = (int)a.nsb;
   dim3 grid(a.nsb * a.nch);
   if (lanes == 4) {
@@ -1004,9 +1136,11 @@ static void dslash_launch(Context &c, DsArgs &a, int code) {
 // 32^3x8 half lattice at ~2 waves per SIMD).  On the first launch of each
 // (precision, links, mode, reduction, nvec) every candidate is timed in place
 // with HIP events and the fastest is kept for the lifetime of the context.
-// The launches are idempotent, except the stored-basis solver's M^dag M,
-// which updates s in place: its candidates write w into a scratch vector
-// instead.  RHMC_DSLASH_NR ("4" or "4x4") overrides it.
+// The launches are idempotent, except two of the stored-basis solver: its
+// hop advances the systems' scalar state (the candidates skip that step)
+// and its M^dag M updates s in place (the candidates write s into a scratch
+// vector; the r^(k) they write is overwritten by the final launch).
+// RHMC_DSLASH_NR ("4" or "4x4") overrides it.
 template <typename R, int LINK, int MODE, int RED>
 static void dslash_dispatch(Context &c, DsArgs &a) {
   const unsigned long long key = ((unsigned long long)sizeof(R) << 40) | ((unsigned long long)LINK << 36) |
@@ -1030,10 +1164,8 @@ static void dslash_dispatch(Context &c, DsArgs &a) {
     float best = 1e30f;
     nr = 16 + pick_nr(a.nvec);
     DsArgs t = a;
-    if (MODE == DS_MDAGM && a.bms) {
-      t.bms = nullptr;
-      t.out = ws_get(c, S_TUNE, sizeof(Cx<R>) * (size_t)a.out_vs * a.nvec);
-    }
+    t.bms_ms = nullptr;  // the hop's scalar step runs once, in the final launch
+    if (MODE == DS_BMS) t.out = ws_get(c, S_TUNE, sizeof(Cx<R>) * (size_t)a.out_vs * a.nvec);
     for (int lanes = 1; lanes <= 4 && t.out; lanes *= 4)
       for (int v = 1; v <= 8 && v <= std::max(1, a.nvec); v *= 2) {
         const int cand = lanes * 16 + v;
@@ -1077,16 +1209,24 @@ void kernel_timing_harvest(Context &c, int n) {
   hipMemcpyAsync(t.data(), c.kt_clk, sizeof(unsigned long long) * t.size(), hipMemcpyDeviceToHost, c.stream);
   hipLaunchKernelGGL(k_clk_clear, dim3((2 * n + 255) / 256), dim3(256), 0, c.stream, c.kt_clk, n);
   hipStreamSynchronize(c.stream);
-  // The interval of slot i runs from the end of the r update that wrote basis
-  // slot i to the end of the hop that reads it: what a per-dispatch trace
-  // records for the hop (dispatch, execution, end of kernel), as the hop
-  // directly follows that launch.  The first hop of a chunk follows the
-  // chunk's bookkeeping launches and the host poll instead, so slots that
-  // are multiples of the chunk length are left out.
+  // Slot i holds the end of the M^dag M launch of iteration i (the launch
+  // before the hop of iteration i+1, stored by that hop) and the end of that
+  // hop (stored by the M^dag M launch after it).  The hop's interval, end of
+  // M^dag M(i) to end of hop(i+1), is what a per-dispatch trace records for
+  // the hop (dispatch, execution, end of kernel); the first hop of a chunk
+  // follows the chunk's bookkeeping launches and the host poll instead, so
+  // slots that are multiples of the chunk length are left out.  The M^dag M
+  // launch of iteration i+1 directly follows that hop: its interval runs to
+  // the end stored in slot i+1.
   for (int i = 1; i < n; i++)
     if (i % kChunk != 0 && t[2 * i] != 0 && t[2 * i + 1] > t[2 * i]) {
       c.kt_sum_ms += (double)(t[2 * i + 1] - t[2 * i]) / c.wall_khz;
       c.kt_count++;
+    }
+  for (int i = 0; i + 1 < n; i++)
+    if (t[2 * i + 1] != 0 && t[2 * i + 2] > t[2 * i + 1]) {
+      c.kt_sum2_ms += (double)(t[2 * i + 2] - t[2 * i + 1]) / c.wall_khz;
+      c.kt_count2++;
     }
 }
 
@@ -1416,50 +1556,46 @@ __global__ __launch_bounds__(kBlock) void k_copy_f2d(double2 *x64, const float2 
 // explicit-shift form adds 4 vectors per active shift); the shifted
 // solutions are assembled in one pass at the end.
 //
-// Per iteration k three launches, with the base CG in its single-reduction
+// Per iteration k two launches, with the base CG in its single-reduction
 // form (Chronopoulos and Gear, J. Comput. Appl. Math. 25 (1989) 153): the
 // operator is applied to the residual, w = A r^(k-1), and s = A p follows
 // its own recurrence, so p is never formed.  With rho_i = |r_i|^2 and
-// mu_i = <r_i, w_i>:
+// mu_i = <r_i, A r_i>:
 //   beta_{i-1} = rho_i / rho_{i-1},
 //   <p_i, A p_i> = mu_i - beta_{i-1} rho_i / alpha_{i-1},  alpha_i = rho_i / <p_i, A p_i>,
 //   s_i = w_i + beta_{i-1} s_{i-1},  r_{i+1} = r_i - alpha_i s_i
 // (alpha, beta are those of the textbook CG; the shifted solutions come
-// from the basis).  Launches of iteration k:
-// A h = D_oe r^(k-1); its first workgroup also sums the partials of
-//   rho_{k-1} (previous X) in a fixed order and stores rho_{k-1} and beta;
-// B w = m^2 r^(k-1) - D_eo h, partials of mu, and s = w + beta s in place
-//   (w itself is never stored);
-// X every workgroup sums the partials of mu in a fixed order, writes
-//   r^(k) = r^(k-1) - alpha s into basis slot k with partials of |r^(k)|^2,
-//   and workgroup (0,j) advances the Jegerlehner recurrences of rhs j for
-//   iteration k-1 (they need beta_{k-2}, known once rho_{k-1} is).
-// Vector traffic per iteration and rhs: 2 (A) + 4 (B) + 3 (X) passes,
-// independent of the number of shifts.  The stop decision lags by one
-// launch: one extra application of the operator per solve, identical
-// recurrences and iteration counts.  Values read by every workgroup of X
-// (rho_{k-2}, alpha, stop, the rho partials) are double-buffered by the
-// parity of k, so no workgroup reads a value another one writes in the same
-// launch.
+// from the basis).  With A = m^2 - D_eo D_oe and the staggered D
+// anti-hermitian, mu = m^2 rho + |D_oe r|^2, which the hop can reduce, so
+// alpha is known when M^dag M runs and the residual update joins it.
+// Launches of iteration k:
+// A h = D_oe r^(k-1) with |h|^2 partials; workgroup j < nrhs also sums the
+//   partials of rho_{k-1} in a fixed order, forms beta, advances the
+//   Jegerlehner recurrences of rhs j for iteration k-1 and the stopping
+//   test (bms_step);
+// B every workgroup forms alpha from the |h|^2 partials (bms_alpha), then
+//   w = m^2 r^(k-1) - D_eo h, s = w + beta s in place, r^(k) = r^(k-1) -
+//   alpha s into basis slot k, with partials of |r^(k)|^2 (w is never
+//   stored).
+// Vector traffic per iteration and rhs: 2 (A) + 5 (B) passes, independent
+// of the number of shifts.  The stop decision lags by one iteration: one
+// extra application of the operator per solve, identical recurrences and
+// iteration counts.  Scalars read by every workgroup of a launch (rho_{k-2},
+// alpha, stop) are double-buffered by the parity of k, so no workgroup
+// reads a value another one writes in the same launch.
 
+// set-up launches of the stored-basis solver
 struct BmsArgs {
   long Vh;
-  int nvec, nshift, iter_off, cap;
-  const int *iterc;     // device iteration base: iteration = *iterc + iter_off
+  int nvec, nshift, cap;
   const double2 *b;
   void *basis;          // [cap][rhs][3][Vh]: slot k of every rhs contiguous
-  void *s;              // [rhs][3][Vh]: the s = A p recurrence
-  double *partB, *partC;  // partials of mu; of rho, two buffers by parity
-  int nbrB, nbrC;
+  double *partC;        // partials of rho_0
+  int nbrC;
   BmsShared *sh;
   MsState *ms;
-  BmsHist *hist;        // [rhs][cap][nshift]
   int *active;          // [2]
-  // live timing: each workgroup stores its end stamp in clk_wg (read by the
-  // hop that follows)
-  unsigned long long *clk_wg;
 };
-
 
 struct BmsGraphKey {
   BmsArgs a;
@@ -1467,22 +1603,6 @@ struct BmsGraphKey {
   unsigned nbr;
 };
 static_assert(sizeof(BmsGraphKey) <= sizeof(GraphSlot::key), "graph key");
-
-// fixed-order workgroup sum of n partials
-__device__ __forceinline__ double block_sum_partials(const double *p, int n) {
-  __shared__ double sh[kBlock / 64];
-  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  double v = 0.0;
-  for (int i = threadIdx.x; i < n; i += kBlock) v += p[i];
-  v = wave_sum(v);
-  if (lane == 0) sh[w] = v;
-  __syncthreads();
-  double s = 0.0;
-#pragma unroll
-  for (int k = 0; k < kBlock / 64; k++) s += sh[k];
-  __syncthreads();
-  return s;
-}
 
 template <typename R>
 __global__ __launch_bounds__(kBlock) void k_bms_init(BmsArgs a) {
@@ -1501,14 +1621,13 @@ __global__ __launch_bounds__(kBlock) void k_bms_init(BmsArgs a) {
       r0[(long)i * Vh + h] = cv;
       v[0] += (double)cv.x * (double)cv.x + (double)cv.y * (double)cv.y;
     }
-  // rho_0, read by the launch of iteration 1 from buffer 1
-  block_partials<1>(v, a.partC + (long)a.nvec * a.nbrC, j, a.nvec, gridDim.x, bx);
+  block_partials<1>(v, a.partC, j, a.nvec, gridDim.x, bx);  // rho_0
 }
 
 // scalar state before iteration 1: inverter.cc:166-198
 __global__ __launch_bounds__(kBlock) void k_bms_init_fin(BmsArgs a) {
   __shared__ double sums[kMaxRed];
-  final_sums(a.partC + (long)a.nvec * a.nbrC, a.nvec, a.nbrC, sums);
+  final_sums(a.partC, a.nvec, a.nbrC, sums);
   for (int j = threadIdx.x; j < a.nvec; j += blockDim.x) {
     MsState &s = a.ms[j];
     s.omega = 1.0;
@@ -1524,144 +1643,12 @@ __global__ __launch_bounds__(kBlock) void k_bms_init_fin(BmsArgs a) {
     }
     a.sh[j].rho[1] = 1.0;
     a.sh[j].alpha[1] = 1.0;
+    a.sh[j].rho1 = sums[j];            // rho_0 for the hop of iteration 1
     a.sh[j].stop[1] = sums[j] == 0.0;  // zero source: x = 0
     s.done = s.stop = a.sh[j].stop[1];
   }
-  // the launch of iteration k counts running systems into active[(k+1)&1]
+  // the hop of iteration k counts running systems into active[(k+1)&1]
   if (threadIdx.x == 0) a.active[0] = 0;
-}
-
-// X (iteration k >= 1, CG step i = k-1): r^(k) = r^(k-1) - alpha s (s was
-// updated by the M^dag M launch), partials of |r^(k)|^2; workgroup (0,j)
-// runs the recurrences of iteration k-1 (inverter.cc:224-287) and the
-// stopping test.
-template <typename R>
-__device__ __forceinline__ void bms_x_body(const BmsArgs &a) {
-  using C = Cx<R>;
-  const unsigned bx = xcd_block(blockIdx.x, gridDim.x);
-  const int j = blockIdx.y;
-  const int k = *a.iterc + a.iter_off;
-  const int sl = k & 1, sn = sl ^ 1;
-  const long Vh = a.Vh;
-  BmsShared &sh = a.sh[j];
-  // the slot the launch of iteration k+1 counts into (this launch counts into sn)
-  if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) a.active[sl] = 0;
-  if (sh.stop[sl]) {
-    if (bx == 0 && threadIdx.x == 0) {
-      sh.stop[sn] = 1;
-      sh.rho[sn] = sh.rho[sl];
-      sh.alpha[sn] = sh.alpha[sl];
-    }
-    return;
-  }
-  const double mu = block_sum_partials(a.partB + (long)j * a.nbrB, a.nbrB);  // <r^(k-1), w>
-  const double rho1 = sh.rho1, be = sh.beta;                                 // rho_{k-1}, beta_{k-2}
-  const double rho2 = sh.rho[sl], al2 = sh.alpha[sl];                        // rho_{k-2}, alpha_{k-2}
-  const double pap = k >= 2 ? mu - be * rho1 / al2 : mu;
-  const double al = rho1 / pap;
-  double v[1] = {0.0};
-  if (k < a.cap) {
-    const R am = (R)(-al);
-    long h0, h1;
-    site_range(Vh, bx, gridDim.x, h0, h1);
-    const C *__restrict__ ro = (const C *)a.basis + ((long)(k - 1) * a.nvec + j) * 3 * Vh;
-    C *__restrict__ rn = (C *)a.basis + ((long)k * a.nvec + j) * 3 * Vh;
-    const C *__restrict__ sv = (const C *)a.s + (long)j * 3 * Vh;
-    // two sites per trip with all twelve loads issued before the first use
-    // (basis slots k-1 and k never overlap)
-    long h = h0 + threadIdx.x;
-    for (; h + kBlock < h1; h += 2 * kBlock) {
-      C sa[2][3], ra[2][3];
-#pragma unroll
-      for (int u = 0; u < 2; u++)
-#pragma unroll
-        for (int i = 0; i < 3; i++) {
-          const long e = (long)i * Vh + h + u * kBlock;
-          sa[u][i] = sv[e];
-          ra[u][i] = ro[e];
-        }
-#pragma unroll
-      for (int u = 0; u < 2; u++)
-#pragma unroll
-        for (int i = 0; i < 3; i++) {
-          C r = ra[u][i];
-          r.x = fmar(am, sa[u][i].x, r.x);
-          r.y = fmar(am, sa[u][i].y, r.y);
-          rn[(long)i * Vh + h + u * kBlock] = r;
-          v[0] += (double)r.x * (double)r.x + (double)r.y * (double)r.y;
-        }
-    }
-    if (h < h1) {
-      C sa[3], ra[3];
-#pragma unroll
-      for (int i = 0; i < 3; i++) {
-        sa[i] = sv[(long)i * Vh + h];
-        ra[i] = ro[(long)i * Vh + h];
-      }
-#pragma unroll
-      for (int i = 0; i < 3; i++) {
-        C r = ra[i];
-        r.x = fmar(am, sa[i].x, r.x);
-        r.y = fmar(am, sa[i].y, r.y);
-        rn[(long)i * Vh + h] = r;
-        v[0] += (double)r.x * (double)r.x + (double)r.y * (double)r.y;
-      }
-    }
-  }
-  block_partials<1>(v, a.partC + (long)sn * a.nvec * a.nbrC, j, a.nvec, gridDim.x, bx);
-  if (bx != 0) return;
-  MsState &s = a.ms[j];
-  const int t = threadIdx.x;
-  int stop = 0;
-  if (k >= 2) {
-    // recurrences of iteration kk = k-1: omega = -alpha_{k-2}, gamma = beta_{k-2}
-    const int kk = k - 1;
-    const double omp = s.omega, gold = s.gammag;
-    const double om = -al2, gam = be;
-    if (t < s.nshift && s.flag[t]) {
-      const double z1 = s.z1[t], z2 = s.z2[t];
-      const double z3 = (z1 * z2 * omp) / (om * gold * (z1 - z2) + z1 * omp * (1.0 - s.shift[t] * om));
-      const double oms = om * z3 / z2;
-      const double gas = gam * z3 * oms / (z2 * om);
-      BmsHist &hh = a.hist[((long)j * a.cap + kk) * a.nshift + t];
-      hh.om = oms;
-      hh.z = z3;
-      hh.ga = gas;
-      if (sqrt(rho2 * z2 * z2) < s.res) {
-        s.flag[t] = 0;
-        s.its[t] = kk;
-      }
-      s.z1[t] = z2;
-      s.z2[t] = z3;
-    }
-    const int nact = __syncthreads_count(t < s.nshift && s.flag[t]);
-    if (t == 0) {
-      s.omega_prev = omp;
-      s.omega = om;
-      s.gammag = gam;
-      s.delta = rho1;
-      s.iter = kk;
-      s.nact = nact;
-      // r^(k) stays inside the basis: the last launch that writes runs at k = cap-1
-      stop = !(sqrt(rho1) > s.res && kk < s.max_iter) || nact == 0 || kk + 2 >= a.cap;
-      s.stop = s.done = stop;
-    }
-  }
-  if (t == 0) {
-    sh.rho[sn] = rho1;
-    sh.alpha[sn] = al;
-    sh.stop[sn] = stop;
-    if (!stop) atomicAdd(&a.active[sn], 1);
-  }
-}
-
-template <typename R>
-__global__ __launch_bounds__(kBlock) void k_bms_x(BmsArgs a) {
-  bms_x_body<R>(a);  // returns are uniform per workgroup
-  if (a.clk_wg) {
-    __syncthreads();
-    if (threadIdx.x == 0) a.clk_wg[blockIdx.y * gridDim.x + blockIdx.x] = (unsigned long long)wall_clock64();
-  }
 }
 
 // x[j][s] (FP64) = sum_{i<K} coef[j][s][i] basis[j][i] for NS shifts per
@@ -1947,9 +1934,11 @@ static int ms_cg_basis(Context &c, const double2 *b, int nrhs, int nshift, const
   BmsShared *bsh = (BmsShared *)ws_get(c, S_BSH, sizeof(BmsShared) * nrhs);
   BmsHist *hist = (BmsHist *)ws_get(c, S_HIST, sizeof(BmsHist) * nrhs * cap * nshift);
   double *diag = (double *)ws_get(c, S_DIAG, sizeof(double) * 256);
-  // two buffers (iteration parity) of nrhs x nbr partials, nbr <= nbx_of(c)
-  double *partC = (double *)ws_get(c, S_PARTIAL2, sizeof(double) * nbx_of(c) * 2 * std::max(nrhs, 1));
-  if (!basis || !sr || !hh || !ms || !bsh || !hist || !diag || !partC) return -1;
+  // partials of |r|^2 (M^dag M: at most nbx per rhs) and of |h|^2 (hop: one
+  // per tile, at most 4 nbx per rhs with four lanes per site)
+  double *partC = (double *)ws_get(c, S_PARTIAL2, sizeof(double) * nbx_of(c) * std::max(nrhs, 1));
+  double *partH = (double *)ws_get(c, S_HPART, sizeof(double) * 4 * nbx_of(c) * std::max(nrhs, 1));
+  if (!basis || !sr || !hh || !ms || !bsh || !hist || !diag || !partC || !partH) return -1;
 
   std::vector<MsState> hs(nrhs);
   for (int j = 0; j < nrhs; j++) {
@@ -1966,9 +1955,9 @@ static int ms_cg_basis(Context &c, const double2 *b, int nrhs, int nshift, const
 
   BmsArgs a = {};
   a.Vh = Vh; a.nvec = nrhs; a.nshift = nshift; a.cap = (int)cap;
-  a.b = b; a.basis = basis; a.s = sr;
+  a.b = b; a.basis = basis;
   a.partC = partC;
-  a.sh = bsh; a.ms = ms; a.hist = hist;
+  a.sh = bsh; a.ms = ms;
   a.active = active_ptr(c);
   const unsigned nbr = reduce_blocks(c, nrhs);
   a.nbrC = (int)nbr;
@@ -1982,47 +1971,49 @@ static int ms_cg_basis(Context &c, const double2 *b, int nrhs, int nshift, const
   hop.in = basis; hop.in_vs = 3 * Vh; hop.out = hh; hop.out_vs = 3 * Vh; hop.nvec = nrhs;
   hop.skip_stride = sizeof(BmsShared) / sizeof(int);
   hop.iterp = iterc; hop.iter_stride = (long)nrhs * 3 * Vh; hop.iter_in = 1;
-  hop.bms = bsh; hop.bms_part = partC; hop.bms_nbr = (int)nbr;
+  hop.bms = bsh; hop.bms_part = partC; hop.bms_nbr = 0;  // set from M^dag M's grid (unused at k = 1)
+  hop.bms_ms = ms; hop.bms_hist = hist; hop.bms_active = active_ptr(c);
+  hop.bms_cap = (int)cap; hop.bms_nshift = nshift;
+  hop.red = make_red(c, nrhs, EPI_STORE, nullptr, nullptr);
+  hop.red.partial = partH;
   // live timing of the MD solver's hop (FP32, all pseudofermions)
   const bool timed = c.time_kernels && c.kt_clk && sizeof(R) == 4 && nrhs == c.n_pf;
-  if (timed) {
-    // end stamps of the r update (nbr x nrhs workgroups) and of the hop (at
-    // most 4 nbx x nrhs workgroups)
-    const size_t need = (size_t)nbr * nrhs + 4 * (size_t)nbx_of(c) * nrhs;
-    if (kernel_timing_stamps(c, need)) return -1;
-    hop.clk = c.kt_clk; hop.clk_cap = c.kt_clk_cap;
-    a.clk_wg = c.kt_wg;
-    hop.clk_prev = c.kt_wg; hop.clk_nprev = (int)nbr * nrhs; hop.clk_field = 0;
-    hop.clk_wg = c.kt_wg + (size_t)nbr * nrhs;
-  }
   DsArgs mdm = hop;
   mdm.par_out = 0; mdm.in = hh; mdm.in_vs = 3 * Vh; mdm.out = sr; mdm.pin = basis; mdm.pin_vs = 3 * Vh;
-  mdm.diag = diag; mdm.iter_in = 0; mdm.iter_pin = 1; mdm.bms_part = nullptr;
-  // M^dag M reads the hop's end stamps (field 1); it stamps nothing itself
-  mdm.clk_wg = nullptr;
-  mdm.clk_prev = hop.clk_wg; mdm.clk_field = 1;
-  mdm.red = make_red(c, nrhs, EPI_STORE, nullptr, nullptr);
-  a.partB = mdm.red.partial;
+  mdm.diag = diag; mdm.iter_in = 0; mdm.iter_pin = 1;
+  mdm.bms_ms = nullptr; mdm.bms_part = nullptr; mdm.bms_hpart = partH;
+  mdm.red.partial = partC;
+  if (timed) {
+    // end stamps of M^dag M (at most nbx x nrhs workgroups) and of the hop
+    // (at most 4 nbx x nrhs): each launch reads the other's
+    const size_t nm = (size_t)nbx_of(c) * nrhs;
+    if (kernel_timing_stamps(c, nm + 4 * nm)) return -1;
+    hop.clk = mdm.clk = c.kt_clk;
+    hop.clk_cap = mdm.clk_cap = c.kt_clk_cap;
+    mdm.clk_wg = c.kt_wg;
+    hop.clk_wg = c.kt_wg + nm;
+    hop.clk_prev = mdm.clk_wg; hop.clk_field = 0; hop.clk_nprev = 0;
+    mdm.clk_prev = hop.clk_wg; mdm.clk_field = 1;
+  }
 
   // Iterations run in chunks of kChunk with the iteration number read on the
   // device (base + offset), so a chunk is a fixed launch sequence: the first
   // chunk of a solve runs eagerly (the Dslash autotuner may time candidates
   // there), later chunks replay a captured graph of it.  Iterations past the
-  // basis capacity are no-ops: k_bms_x stops a system before slot cap.
+  // basis capacity are no-ops: the hop stops a system before slot cap.
   hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, c.stream, iterc, 1);
-  a.iterc = iterc;
   auto chunk_launch = [&]() {
     for (int q = 0; q < kChunk; q++) {
       hop.skip = &bsh[0].stop[(1 + q) & 1];
       mdm.skip = hop.skip;
       hop.iter_off = q - 1;
       mdm.iter_off = q - 1;
-      dslash_dispatch<R, LINK, DS_HOP, RED_NONE>(c, hop);
-      mdm.clk_nprev = (int)(hop.nsb * hop.nch);  // the hop's grid
-      dslash_dispatch<R, LINK, DS_MDAGM, RED_DOT>(c, mdm);
-      a.nbrB = mdm.red.nbr;
-      a.iter_off = q;
-      hipLaunchKernelGGL(k_bms_x<R>, dim3(nbr, nrhs), dim3(kBlock), 0, c.stream, a);
+      dslash_dispatch<R, LINK, DS_HOP, RED_NORM>(c, hop);
+      mdm.bms_hnbr = hop.red.nbr;
+      mdm.clk_nprev = (int)(hop.nsb * hop.nch);
+      dslash_dispatch<R, LINK, DS_BMS, RED_NORM>(c, mdm);
+      hop.bms_nbr = mdm.red.nbr;
+      hop.clk_nprev = (int)(mdm.nsb * mdm.nch);
     }
     hipLaunchKernelGGL(k_add_int, dim3(1), dim3(64), 0, c.stream, iterc, kChunk);
   };
@@ -2077,7 +2068,8 @@ static int ms_cg_basis(Context &c, const double2 *b, int nrhs, int nshift, const
   if (!hip_ok(c, hipStreamSynchronize(c.stream), "basis multishift")) return -1;
   int kmax = 0;
   for (int j = 0; j < nrhs; j++) kmax = std::max(kmax, hs[j].iter);
-  // the hop ran with work at iterations 1..kmax+1 (input slots 0..kmax)
+  // the hop ran with work at iterations 1..kmax+1 (input slots 0..kmax);
+  // slot kmax+1 holds the end of M^dag M(kmax+1) when a hop followed it
   if (timed) kernel_timing_harvest(c, kmax + 1);
   // basis exhausted before convergence: let the caller use the explicit algorithm
   for (int j = 0; j < nrhs; j++)

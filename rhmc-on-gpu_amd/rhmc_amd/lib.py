@@ -314,12 +314,13 @@ class Gpu:
         return it.value, b.value, m.value, nf.value
 
     def kernel_timing(self, enable=-1):
-        """(mean kernel ms on the device wall clock, count) of the MD solver's
-        hop since the last reset; enable=1/0 then switches live timing on/off
-        and resets."""
-        m, n = C.c_double(), C.c_longlong()
-        self._check(self.L.gpu_kernel_timing(self.h, int(enable), C.byref(m), C.byref(n)), "gpu_kernel_timing")
-        return m.value, n.value
+        """((hop mean ms, count), (M^dag M mean ms, count)) of the MD solver's
+        two launches per iteration on the device wall clock since the last
+        reset; enable=1/0 then switches live timing on/off and resets."""
+        m = (C.c_double * 2)()
+        n = (C.c_longlong * 2)()
+        self._check(self.L.gpu_kernel_timing(self.h, int(enable), m, n), "gpu_kernel_timing")
+        return (m[0], n[0]), (m[1], n[1])
 
     def tuning_table(self):
         import json

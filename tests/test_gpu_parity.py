@@ -374,9 +374,10 @@ def test_trajectory_fp32_md_against_reference(name):
 
 
 def test_live_kernel_timing():
-    """Live timing of the MD hop: switched on around a mixed trajectory it
-    counts the FP32 hops that did work (not the first of each 8-iteration
-    chunk) with a per-launch duration between 1 us and 10 ms; switched off,
+    """Live timing of the MD solver's two launches: switched on around a
+    mixed trajectory it counts the FP32 hops that did work (not the first of
+    each 8-iteration chunk) and the M^dag M launches after them, each with a
+    per-launch duration between 1 us and 10 ms; switched off,
     a second trajectory adds nothing.  Timing does not change the result
     beyond FP32 rounding (each context autotunes its own launch shapes, and
     the shape sets the reduction order)."""
@@ -393,12 +394,13 @@ def test_live_kernel_timing():
         g.put_fermions("phi", gd["phi"])
         g.kernel_timing(timed)
         r = g.trajectory(precision=1, fp32_threshold=1e-10, **kw)
-        mean_ms, n = g.kernel_timing(-1)
-        say("timed", timed, "mean ms", mean_ms, "launches", n, "dH", r.delta_h)
+        (hop_ms, n_hop), (mdm_ms, n_mdm) = g.kernel_timing(-1)
+        say("timed", timed, "hop ms", hop_ms, n_hop, "M^dag M ms", mdm_ms, n_mdm, "dH", r.delta_h)
         if timed:
-            assert n > 0 and 1e-3 < mean_ms < 10.0, (mean_ms, n)
+            assert n_hop > 0 and 1e-3 < hop_ms < 10.0, (hop_ms, n_hop)
+            assert n_mdm >= n_hop and 1e-3 < mdm_ms < 10.0, (mdm_ms, n_mdm)
         else:
-            assert n == 0
+            assert n_hop == 0 and n_mdm == 0
         out.append(r.delta_h)
         g.close()
     assert abs(out[0] - out[1]) < 1e-4 * max(1.0, abs(out[1]))
