@@ -396,7 +396,7 @@ struct DsArgs {
   RedArgs red;
 };
 
-// Hop of iteration k, workgroup j (< nvec): system j's scalars.  rho_{k-1}
+// Hop of iteration k, workgroup j mod the grid: system j's scalars.  rho_{k-1}
 // is the fixed-order sum of the previous M^dag M launch's partials (at k = 1
 // the set-up's rho_0), beta = rho_{k-1} / rho_{k-2}; then the Jegerlehner
 // recurrences of iteration k-1 (inverter.cc:224-287, they need alpha and
@@ -780,7 +780,8 @@ __device__ __forceinline__ void ds_site_out(const DsArgs &a, const Cx<R> (&acc)[
 template <typename R, int LINK, int NR, int MODE, int RED>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(ds_waves<R, LINK, NR>()))) void k_dslash(DsArgs a) {
   using C = Cx<R>;
-  if (MODE == DS_HOP && a.bms_ms && (int)blockIdx.x < a.nvec) bms_step(a, blockIdx.x);
+  if (MODE == DS_HOP && a.bms_ms)
+    for (int j = blockIdx.x; j < a.nvec; j += gridDim.x) bms_step(a, j);  // a small grid takes several
   clk_begin(a);
   const LatticeDims &L = a.L;
   const long Vh = L.Vh;
@@ -962,7 +963,8 @@ __device__ __forceinline__ C pick3(C a0, C a1, C a2, int k) {
 template <typename R, int LINK, int NR, int MODE, int RED>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(ds_waves<R, LINK, NR>()))) void k_dslash_ls(DsArgs a) {
   using C = Cx<R>;
-  if (MODE == DS_HOP && a.bms_ms && (int)blockIdx.x < a.nvec) bms_step(a, blockIdx.x);
+  if (MODE == DS_HOP && a.bms_ms)
+    for (int j = blockIdx.x; j < a.nvec; j += gridDim.x) bms_step(a, j);  // a small grid takes several
   clk_begin(a);
   const LatticeDims &L = a.L;
   const long Vh = L.Vh;
