@@ -18,6 +18,9 @@ Timed paths, each from the same start configuration and seed:
               search directions: the reference's solve layout.
 Delta H of every mixed trajectory is recomputed on the same endpoints with
 FP64 solves to 1e-13 (untimed rerun); the difference is the Delta H error.
+roofline: the trajectory's dominant kernel, the MD solver's M^dag M launch
+with the CG's s and r updates, timed live inside the mixed trajectories (the
+hop before it is reported in dslash.live_*).
 
 Prints ONE JSON line.  --gpus above 1 is refused (one-GPU build).
 """
