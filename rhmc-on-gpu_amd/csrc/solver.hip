@@ -777,6 +777,54 @@ __device__ __forceinline__ void ds_site_out(const DsArgs &a, const Cx<R> (&acc)[
   }
 }
 
+// DS_BMS output of one site: all of the site's loads (r^(k-1) and the old s
+// of every right-hand side of the chunk) are issued before the first store,
+// so they overlap instead of waiting on one another; a slot past nvec reads
+// the last vector, and beta = 0 (the first iteration) takes w without the
+// old s, whose contents are then undefined.
+template <typename R, int NR>
+__device__ __forceinline__ void ds_site_out_bms(const DsArgs &a, const Cx<R> (&acc)[NR][3], const bool (&act)[NR],
+                                                double (&red)[NR], int jbase, int h, const Cx<R> *pinb,
+                                                const R (&ab)[2][NR], bool rk_ok) {
+  using C = Cx<R>;
+  const long Vh = a.L.Vh;
+  C pv[NR][3], so[NR][3];
+#pragma unroll
+  for (int r = 0; r < NR; r++) {
+    const int j = jbase + r < a.nvec ? jbase + r : a.nvec - 1;
+    const C *p = pinb + (long)j * a.pin_vs + h;
+    const C *o = (const C *)a.out + (long)j * a.out_vs + h;
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      pv[r][i] = p[(long)i * Vh];
+      so[r][i] = o[(long)i * Vh];
+    }
+  }
+#pragma unroll
+  for (int r = 0; r < NR; r++) {
+    if (!act[r]) continue;
+    const int j = jbase + r;
+    C *o = (C *)a.out + (long)j * a.out_vs + h;
+    C *rk = (C *)(pinb + a.iter_stride) + (long)j * a.pin_vs + h;
+    const R d = (R)a.diag[j];
+    const R be = ab[1][r], al = ab[0][r];
+    const bool use_s = be != (R)0;
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      C w, sv, rn;
+      w.x = fmar(d, pv[r][i].x, (R)-0.5 * acc[r][i].x);
+      w.y = fmar(d, pv[r][i].y, (R)-0.5 * acc[r][i].y);
+      sv.x = use_s ? fmar(be, so[r][i].x, w.x) : w.x;
+      sv.y = use_s ? fmar(be, so[r][i].y, w.y) : w.y;
+      rn.x = fmar(-al, sv.x, pv[r][i].x);
+      rn.y = fmar(-al, sv.y, pv[r][i].y);
+      o[(long)i * Vh] = sv;
+      if (rk_ok) rk[(long)i * Vh] = rn;
+      red[r] += (double)rn.x * (double)rn.x + (double)rn.y * (double)rn.y;
+    }
+  }
+}
+
 template <typename R, int LINK, int NR, int MODE, int RED>
 __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(ds_waves<R, LINK, NR>()))) void k_dslash(DsArgs a) {
   using C = Cx<R>;
@@ -846,7 +894,8 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(ds_waves
       if (k == 0 && xrow) xm_from_xp<R, LINK, NR>(buf[0], buf[1]);
       hop_fence<R, NR>(acc);
     }
-    ds_site_out<R, NR, MODE, RED>(a, acc, act, red, jbase, h, pinb, bab, rk_ok);
+    if constexpr (MODE == DS_BMS) ds_site_out_bms<R, NR>(a, acc, act, red, jbase, h, pinb, bab, rk_ok);
+    else ds_site_out<R, NR, MODE, RED>(a, acc, act, red, jbase, h, pinb, bab, rk_ok);
   }
   if (RED != RED_NONE) block_partials<NR>(red, a.red.partial, jbase, a.nvec, a.nsb, sb);
   clk_end(a);
