@@ -328,12 +328,31 @@ __global__ __launch_bounds__(kBlock) void k_finalize(RedArgs ra, int nvec, int *
 }
 
 // fixed-order workgroup sum of n partials
+// per-lane sums over b = threadIdx.x + m kBlock (b < n) of NV arrays, in
+// order of m, with the loads of four strides of every array in flight
+// together (adding the 0 of a stride past n changes nothing)
+template <int NV>
+__device__ __forceinline__ void lane_sums(const double *const (&p)[NV], int n, double (&v)[NV]) {
+  for (int b0 = threadIdx.x; b0 < n; b0 += 4 * kBlock) {
+    double x[4][NV];
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+#pragma unroll
+      for (int r = 0; r < NV; r++) x[u][r] = b0 + u * kBlock < n ? p[r][b0 + u * kBlock] : 0.0;
+#pragma unroll
+    for (int u = 0; u < 4; u++)
+#pragma unroll
+      for (int r = 0; r < NV; r++) v[r] += x[u][r];
+  }
+}
+
 __device__ __forceinline__ double block_sum_partials(const double *p, int n) {
   __shared__ double sh[kBlock / 64];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-  double v = 0.0;
-  for (int i = threadIdx.x; i < n; i += kBlock) v += p[i];
-  v = wave_sum(v);
+  const double *pp[1] = {p};
+  double vv[1] = {0.0};
+  lane_sums<1>(pp, n, vv);
+  double v = wave_sum(vv[0]);
   if (lane == 0) sh[w] = v;
   __syncthreads();
   double s = 0.0;
@@ -486,12 +505,14 @@ __device__ __forceinline__ void bms_alpha(const DsArgs &a, int jbase, unsigned s
   __shared__ double sm[kBlock / 64][NR];
   const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
   double v[NR];
+  const double *pr[NR];
 #pragma unroll
   for (int r = 0; r < NR; r++) {
     v[r] = 0.0;
     const int j = jbase + r < a.nvec ? jbase + r : a.nvec - 1;
-    for (int b = threadIdx.x; b < n; b += kBlock) v[r] += a.bms_hpart[(long)j * n + b];
+    pr[r] = a.bms_hpart + (long)j * n;
   }
+  lane_sums<NR>(pr, n, v);
 #pragma unroll
   for (int r = 0; r < NR; r++) {
     const double q = wave_sum(v[r]);
