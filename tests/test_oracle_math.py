@@ -86,6 +86,26 @@ def test_antihermiticity_and_gauge_covariance():
     assert np.linalg.norm(out - ref) < 1e-12 * np.linalg.norm(ref)
 
 
+def test_mu_from_the_hop_norm():
+    """The stored-basis solver forms mu = <r, M^dag M r> as m^2 |r|^2 +
+    |D_oe r|^2 (from the norm of the hop it has just computed) instead of a
+    dot product after M^dag M.  With D_eo^+ = -D_oe the two agree, including
+    the antiperiodic temporal boundary (nt = 4) and random links."""
+    dims = (4, 4, 2, 4)
+    lat = fields.Lattice(*dims)
+    o = Oracle(dims)
+    U = fields.random_links(lat, 13, 0.7)
+    mass = 0.1
+    for seed in (3, 4):
+        r = fields.gauss_spinor(lat, seed)[0]
+        h = o.doe(U, r)
+        w = mass * mass * r - o.deo(U, h)
+        mu_dot = np.vdot(r, w)
+        mu_hop = mass * mass * np.vdot(r, r).real + np.vdot(h, h).real
+        assert abs(mu_dot.imag) < 1e-12 * abs(mu_dot.real)
+        assert abs(mu_dot.real - mu_hop) < 1e-13 * mu_hop
+
+
 def test_multishift_against_dense_solve():
     dims = (4, 4, 4, 4)
     mass = 0.2
