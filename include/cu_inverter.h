@@ -46,6 +46,12 @@ int cu_shifted_residuals(rhmc_gpu_t h, int order, const double *shifts, double *
  * refinement cycles. */
 int cu_last_solve_stats(rhmc_gpu_t h, int *iters_fp32, int *iters_refine, int *cycles);
 
+/* Solver bookkeeping, no counterpart in the reference: continuations of the
+ * last solve past a full stored basis, solver-chunk graphs captured on this
+ * handle so far, and the largest true residual / target of the last solve
+ * whose true residuals were checked (FP64 stored-basis and mixed solves). */
+int cu_solver_counters(rhmc_gpu_t h, int *restarts, int *graph_captures, double *true_res_max);
+
 #ifdef __cplusplus
 }
 #endif

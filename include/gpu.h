@@ -38,8 +38,11 @@ typedef struct {
   int device;            /* HIP device ordinal                                */
 } rhmc_gpu_params;
 
-/* Status codes */
-enum { RHMC_OK = 0, RHMC_EINVAL = 1, RHMC_EMAXCG = 2, RHMC_EHIP = -1 };
+/* Status codes.  RHMC_EMAXCG: max_cg exhausted before convergence (the
+ * reference writes a warning and carries on, inverter.cc:290-296, or exits,
+ * cu_inverter.cc:62-69); RHMC_ENONFINITE: a solver reduction produced NaN or
+ * Inf (non-finite links or sources), the solve stops at once. */
+enum { RHMC_OK = 0, RHMC_EINVAL = 1, RHMC_EMAXCG = 2, RHMC_ENONFINITE = 3, RHMC_EHIP = -1 };
 
 /* Context lifetime (the reference's implicit global state; cuda_init.cu:5-59). */
 int gpu_create(const rhmc_gpu_params *p, rhmc_gpu_t *out);
@@ -251,6 +254,16 @@ typedef struct {
 } rhmc_traj_result;
 
 int gpu_trajectory(rhmc_gpu_t h, const rhmc_traj_params *p, rhmc_traj_result *r);
+
+/* lib/Update/RevUpdate/revupdate.cc:19-97 (rev_update, the reference's
+ * REVERSIBILITY_TEST): fields (draw_fields), bounds and heatbath as in
+ * gpu_trajectory, then the MD of update_nometro, momenta reversed, the MD
+ * again.  delta_conf = sqrt(sum |U - U_0|^2 / (V 4 8)) and delta_mom =
+ * sqrt(sum |P + P_0|^2 / (V 4 8)) (revupdate.cc:86-92; zero for an exactly
+ * reversible integration).  The links and momenta are left at the end of the
+ * second MD; no Metropolis step. */
+int gpu_rev_update(rhmc_gpu_t h, const rhmc_traj_params *p, double *delta_conf, double *delta_mom,
+                   rhmc_traj_result *r);
 
 /* Several independent lattices (distinct handles, any sizes), one trajectory
  * each, run concurrently on the device: independent Markov chains as in the

@@ -25,6 +25,7 @@ int upload_spinors(Context &c, const double *host, double2 *dst, int n);
 int download_spinors(Context &c, const double2 *src, double *host, int n);
 int hop_s12(Context &c, int par_out, const float2 *in, float2 *out, int nvec);
 int run_trajectory(Context &c, const rhmc_traj_params &p, rhmc_traj_result &r);
+int run_rev_update(Context &c, const rhmc_traj_params &p, rhmc_traj_result &r, double *d_conf, double *d_mom);
 std::string tuning_table(Context &c);
 int time_copy(Context &c, size_t bytes, int reps, double *ms);
 
@@ -349,7 +350,13 @@ int gpu_sol_sum(rhmc_gpu_t h, int num_shifts, double a0, const double *num) {
   return sol_sum(c, c.mf, c.smf, num_shifts, a0, num) ? RHMC_EHIP : RHMC_OK;
 }
 
-static int solve_status(int rc) { return rc == 0 ? RHMC_OK : rc == 1 ? RHMC_EINVAL : rc == 2 ? RHMC_EMAXCG : RHMC_EHIP; }
+static int solve_status(int rc) {
+  return rc == 0 ? RHMC_OK
+         : rc == 1 ? RHMC_EINVAL
+         : rc == kStatusMaxCg ? RHMC_EMAXCG
+         : rc == kStatusNonFinite ? RHMC_ENONFINITE
+                                 : RHMC_EHIP;
+}
 
 int gpu_fermionforce(rhmc_gpu_t h, double res, int relative, int order, const double *shifts, const double *num,
                      int precision, int max_cg, int pole_chunk, int *iterations) {
@@ -629,7 +636,7 @@ int gpu_shifted_inverter(rhmc_gpu_t h, double res, const double *shifts, int num
                             c.smf + (long)psferm * num_shifts * sp, &it);
   c.smf_order = num_shifts;
   if (ncount) *ncount = it;
-  return rc == 0 ? RHMC_OK : (rc == 2 ? RHMC_EMAXCG : RHMC_EHIP);
+  return solve_status(rc);
 }
 
 int cu_shifted_residuals(rhmc_gpu_t h, int order, const double *shifts, double *out) {
@@ -656,6 +663,14 @@ int cu_last_solve_stats(rhmc_gpu_t h, int *i32, int *iref, int *cyc) {
   return RHMC_OK;
 }
 
+int cu_solver_counters(rhmc_gpu_t h, int *restarts, int *graph_captures, double *true_res_max) {
+  if (!h) return RHMC_EINVAL;
+  if (restarts) *restarts = h->c.last_restarts;
+  if (graph_captures) *graph_captures = h->c.graph_captures;
+  if (true_res_max) *true_res_max = h->c.last_true_res_max;
+  return RHMC_OK;
+}
+
 int gpu_time_copy(rhmc_gpu_t h, double bytes, int reps, double *ms) {
   if (!h || !ms || bytes < 1024 || reps < 1) return RHMC_EINVAL;
   hipSetDevice(h->c.device);
@@ -675,6 +690,13 @@ int gpu_trajectory(rhmc_gpu_t h, const rhmc_traj_params *p, rhmc_traj_result *r)
   if (!h || !p || !r) return RHMC_EINVAL;
   hipSetDevice(h->c.device);
   return run_trajectory(h->c, *p, *r);
+}
+
+int gpu_rev_update(rhmc_gpu_t h, const rhmc_traj_params *p, double *delta_conf, double *delta_mom,
+                   rhmc_traj_result *r) {
+  if (!h || !p || !r || !delta_conf || !delta_mom) return RHMC_EINVAL;
+  hipSetDevice(h->c.device);
+  return run_rev_update(h->c, *p, *r, delta_conf, delta_mom);
 }
 
 int gpu_multi_trajectory(rhmc_gpu_t *hs, int n, const rhmc_traj_params *ps, rhmc_traj_result *rs, int *status) {

@@ -20,6 +20,7 @@ enum Prec { PREC_FP64 = 0, PREC_MIXED = 1, PREC_FP32 = 2 };
 struct MsState {
   double delta, omega, omega_prev, gammag, res;
   int iter, max_iter, done, stop, nshift, nact;
+  int nonfinite;  // a reduction or recurrence produced NaN/Inf (the solve stops)
   double shift[kMaxShifts], z1[kMaxShifts], z2[kMaxShifts], z3[kMaxShifts];
   double om[kMaxShifts], ga[kMaxShifts];
   int flag[kMaxShifts], flag_next[kMaxShifts], its[kMaxShifts];
@@ -29,7 +30,12 @@ struct MsState {
 struct CgState {
   double delta, alpha, beta, tol2, shift;
   int iter, max_iter, done, stop;
+  int nonfinite;  // a reduction produced NaN/Inf (the system stops)
 };
+
+// status codes of the solver entry points (0 = converged)
+constexpr int kStatusMaxCg = 2;      // max_cg exhausted (RHMC_EMAXCG)
+constexpr int kStatusNonFinite = 3;  // NaN/Inf in a solver reduction (RHMC_ENONFINITE)
 
 struct Workspace;
 
@@ -91,7 +97,9 @@ struct Context {
 
   // statistics of the last solve
   int last_iters = 0, last_refine_cycles = 0, last_iters_fp32 = 0, last_iters_refine = 0;
-  double last_true_res_max = 0.0;
+  double last_true_res_max = 0.0;  // largest |b - (MdagM+s) x| / target of the last checked solve
+  int last_restarts = 0;           // continuations past a full stored basis in the last solve
+  int graph_captures = 0;          // solver chunk graphs captured (instantiated) so far
 };
 
 // ------------------------------------------------------------------ launches
@@ -133,6 +141,11 @@ int fermion_force32(Context &c, int nsys, const double *num, const float2 *x32, 
 int gauge_force(Context &c, int accumulate);
 int exp_update(Context &c, double step);
 int momenta_update(Context &c, double step);
+// P <- s P
+int momenta_scale(Context &c, double s);
+// sum of l2norm2(a - b) over the FP64 18-real links (kind 0) or of
+// l2norm2(a + b) over hermitian algebra fields (kind 1), FP64
+int pair_norm2(Context &c, const double2 *a, const double2 *b, int kind, double *out);
 int sol_sum(Context &c, double2 *io, const double2 *x, int nshift, double a0, const double *num);
 int action_local(Context &c, const double2 *phi, const double2 *chi, int with_fermions, int init);
 int action_sum(Context &c, double *out);
@@ -163,6 +176,12 @@ void kernel_timing_release(Context &c);
 int kernel_timing_clock(Context &c);
 // per-workgroup end stamps, at least n
 int kernel_timing_stamps(Context &c, size_t n);
+
+// persistent workspace buffers of the gauge-field kernels (grown on demand,
+// freed with the context): the force's hop outputs and coefficients, the
+// rational sums' coefficients
+enum ScratchUse { SCR_FORCE_HOP = 0, SCR_FORCE_COEF, SCR_SOLSUM_COEF, SCR_COUNT };
+void *scratch_get(Context &c, int use, size_t bytes);
 
 // host helpers
 bool hip_ok(Context &c, hipError_t e, const char *what);

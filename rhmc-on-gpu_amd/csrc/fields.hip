@@ -121,18 +121,16 @@ int fermion_force(Context &c, int nsys, const double *num, const double2 *x, int
   static thread_local std::vector<double> hcoef;
   const long Vh = c.Vh;
   size_t need = sizeof(double2) * 3 * Vh * (size_t)nsys;
-  // h_j = D_oe s_j for all systems (one multi-RHS hop sharing link loads)
-  double2 *hv = nullptr;
-  double *coef = nullptr;
-  if (!hip_ok(c, hipMallocAsync((void **)&hv, need, c.stream), "force buffer")) return -1;
-  if (!hip_ok(c, hipMallocAsync((void **)&coef, sizeof(double) * nsys, c.stream), "force coef")) return -1;
+  // h_j = D_oe s_j for all systems (one multi-RHS hop sharing link loads),
+  // into workspace buffers kept for the context's lifetime
+  double2 *hv = (double2 *)scratch_get(c, SCR_FORCE_HOP, need);
+  double *coef = (double *)scratch_get(c, SCR_FORCE_COEF, sizeof(double) * 256);
+  if (!hv || !coef || nsys > 256) return -1;
   hcoef.assign(num, num + nsys);
   hipMemcpyAsync(coef, hcoef.data(), sizeof(double) * nsys, hipMemcpyHostToDevice, c.stream);
   launch_hop_d(c, 1, x, hv, nsys, 3 * Vh, 3 * Vh);
   hipLaunchKernelGGL(k_fermion_force<double2>, dim3(nblk(Vh), 2), dim3(kBlock), 0, c.stream, c.L, c.U64, x, hv, coef,
                      nsys, c.ipdot, accumulate);
-  hipFreeAsync(hv, c.stream);
-  hipFreeAsync(coef, c.stream);
   // stream-ordered for the caller (the staged coefficients live in hcoef)
   return hip_ok(c, hipGetLastError(), "fermion force") ? 0 : -1;
 }
@@ -140,18 +138,14 @@ int fermion_force(Context &c, int nsys, const double *num, const double2 *x, int
 int fermion_force32(Context &c, int nsys, const double *num, const float2 *x32, int accumulate) {
   static thread_local std::vector<double> hcoef;
   const long Vh = c.Vh;
-  float2 *hv = nullptr;
-  double *coef = nullptr;
-  if (!hip_ok(c, hipMallocAsync((void **)&hv, sizeof(float2) * 3 * Vh * (size_t)nsys, c.stream), "force buffer"))
-    return -1;
-  if (!hip_ok(c, hipMallocAsync((void **)&coef, sizeof(double) * nsys, c.stream), "force coef")) return -1;
+  float2 *hv = (float2 *)scratch_get(c, SCR_FORCE_HOP, sizeof(float2) * 3 * Vh * (size_t)nsys);
+  double *coef = (double *)scratch_get(c, SCR_FORCE_COEF, sizeof(double) * 256);
+  if (!hv || !coef || nsys > 256) return -1;
   hcoef.assign(num, num + nsys);
   hipMemcpyAsync(coef, hcoef.data(), sizeof(double) * nsys, hipMemcpyHostToDevice, c.stream);
   launch_hop_s(c, 1, x32, hv, nsys, 3 * Vh, 3 * Vh);
   hipLaunchKernelGGL(k_fermion_force<float2>, dim3(nblk(Vh), 2), dim3(kBlock), 0, c.stream, c.L, c.U64, x32, hv, coef,
                      nsys, c.ipdot, accumulate);
-  hipFreeAsync(hv, c.stream);
-  hipFreeAsync(coef, c.stream);
   // stream-ordered for the caller (the staged coefficients live in hcoef)
   return hip_ok(c, hipGetLastError(), "fermion force") ? 0 : -1;
 }
@@ -334,6 +328,20 @@ int momenta_update(Context &c, double step) {
   return hip_ok(c, hipGetLastError(), "momenta update") ? 0 : -1;
 }
 
+// P <- s P (revupdate.cc:53, reverse_momenta with s = -1)
+__global__ __launch_bounds__(kBlock) void k_scale_inplace(double2 *v, double s, long n) {
+  const long i = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const double2 p = v[i];
+  v[i] = make_double2(s * p.x, s * p.y);
+}
+
+int momenta_scale(Context &c, double s) {
+  const long n = 32 * c.Vh;
+  hipLaunchKernelGGL(k_scale_inplace, dim3(nblk(n)), dim3(kBlock), 0, c.stream, c.P, s, n);
+  return hip_ok(c, hipGetLastError(), "momenta scale") ? 0 : -1;
+}
+
 // ------------------------------------------------------------ solution sums
 // rationalapprox.cc:269-369 / cuda_sol_sum: io[ps] = a0 io[ps] + sum_k a_k x[ps][k]
 __global__ __launch_bounds__(kBlock) void k_sol_sum(double2 *io, const double2 *x, int nshift, double a0,
@@ -353,13 +361,12 @@ __global__ __launch_bounds__(kBlock) void k_sol_sum(double2 *io, const double2 *
 }
 
 int sol_sum(Context &c, double2 *io, const double2 *x, int nshift, double a0, const double *num) {
-  double *d = nullptr;
-  if (!hip_ok(c, hipMallocAsync((void **)&d, sizeof(double) * nshift, c.stream), "sol_sum")) return -1;
+  double *d = (double *)scratch_get(c, SCR_SOLSUM_COEF, sizeof(double) * kMaxShifts);
+  if (!d || nshift > kMaxShifts) return -1;
   std::vector<double> h(num, num + nshift);
   hipMemcpyAsync(d, h.data(), sizeof(double) * nshift, hipMemcpyHostToDevice, c.stream);
   const long n3 = 3 * c.Vh;
   hipLaunchKernelGGL(k_sol_sum, dim3(nblk(n3), c.n_pf), dim3(kBlock), 0, c.stream, io, x, nshift, a0, d, n3);
-  hipFreeAsync(d, c.stream);
   return hip_ok(c, hipStreamSynchronize(c.stream), "sol_sum") ? 0 : -1;
 }
 
@@ -720,6 +727,76 @@ __global__ __launch_bounds__(kBlock) void k_gauss_spinor(double2 *out, long Vh, 
 int gauss_spinor(Context &c, double2 *out, int n, unsigned long long seed, unsigned long long stream) {
   hipLaunchKernelGGL(k_gauss_spinor, dim3(nblk(c.Vh), n), dim3(kBlock), 0, c.stream, out, c.Vh, seed, stream);
   return hip_ok(c, hipGetLastError(), "gauss spinor") ? 0 : -1;
+}
+
+// ------------------------------------------------ reversibility distances
+// revupdate.cc:66-92: sum over links of l2norm2(U - U_save) (kind 0: FP64
+// 18-real links, n = 72 Vh complex entries) or of l2norm2(P + P_save) (kind
+// 1: hermitian algebra fields stored as P01, P02, P12, (P00, P11), n = 32 Vh;
+// the full 3x3 norm counts each off-diagonal entry twice and P22 = -(P00 +
+// P11)).  One FP64 partial per workgroup, summed in a fixed order by one
+// workgroup.
+__global__ __launch_bounds__(kBlock) void k_pair_norm2(const double2 *a, const double2 *b, long Vh, int kind,
+                                                       double *partial) {
+  __shared__ double sh[kBlock / 64];
+  const long stride = (long)gridDim.x * kBlock;
+  double v = 0.0;
+  if (kind == 0) {
+    const long n = 72 * Vh;
+    for (long i = (long)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+      const double2 x = a[i], y = b[i];
+      const double dx = x.x - y.x, dy = x.y - y.y;
+      v += dx * dx + dy * dy;
+    }
+  } else {
+    const long n = 8 * Vh;  // (mu, parity, h)
+    for (long i = (long)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
+      const long blk = i / Vh, h = i - blk * Vh;
+      double2 d[4];
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        const double2 x = a[(blk * 4 + k) * Vh + h], y = b[(blk * 4 + k) * Vh + h];
+        d[k] = make_double2(x.x + y.x, x.y + y.y);
+      }
+      const double p22 = d[3].x + d[3].y;
+      v += 2.0 * (d[0].x * d[0].x + d[0].y * d[0].y + d[1].x * d[1].x + d[1].y * d[1].y + d[2].x * d[2].x +
+                  d[2].y * d[2].y) +
+           d[3].x * d[3].x + d[3].y * d[3].y + p22 * p22;
+    }
+  }
+  v = wave_sum(v);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int k = 0; k < kBlock / 64; k++) s += sh[k];
+    partial[blockIdx.x] = s;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_sum_partials(const double *partial, int n, double *out) {
+  __shared__ double sh[kBlock / 64];
+  double v = 0.0;
+  for (int i = threadIdx.x; i < n; i += kBlock) v += partial[i];
+  v = wave_sum(v);
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = v;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double s = 0.0;
+    for (int k = 0; k < kBlock / 64; k++) s += sh[k];
+    *out = s;
+  }
+}
+
+int pair_norm2(Context &c, const double2 *a, const double2 *b, int kind, double *out) {
+  const int nb = 1024;
+  double *part = nullptr;
+  if (!hip_ok(c, hipMallocAsync((void **)&part, sizeof(double) * (nb + 1), c.stream), "pair norm")) return -1;
+  hipLaunchKernelGGL(k_pair_norm2, dim3(nb), dim3(kBlock), 0, c.stream, a, b, c.Vh, kind, part);
+  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(kBlock), 0, c.stream, part, nb, part + nb);
+  hipMemcpyAsync(out, part + nb, sizeof(double), hipMemcpyDeviceToHost, c.stream);
+  hipFreeAsync(part, c.stream);
+  return hip_ok(c, hipStreamSynchronize(c.stream), "pair norm") ? 0 : -1;
 }
 
 // ------------------------------------------------------- copy probe

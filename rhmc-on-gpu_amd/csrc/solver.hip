@@ -41,6 +41,8 @@ enum Slot {
   S_PW1, S_PW2, S_PW3,
   S_FH64,
   S_PARTIAL2, S_BASIS, S_BSH, S_HIST, S_COEF, S_TUNE, S_HPART,
+  S_FORCE_H, S_FORCE_COEF, S_SOLSUM_COEF,
+  S_RS_SRC, S_RS_Y, S_RS_IDX,
   S_COUNT
 };
 
@@ -63,6 +65,7 @@ struct Workspace {
   std::vector<float> h_coef32;
   std::vector<int> h_klen;
   GraphSlot bms_graph[2];                      // FP32 / FP64 stored-basis chunk
+  long basis_kmax[2] = {0, 0};                 // most basis iterations a FP32 / FP64 solve needed
   void *buf[S_COUNT] = {};
   size_t cap[S_COUNT] = {};
   unsigned *counter = nullptr;
@@ -114,6 +117,13 @@ static void *ws_get(Context &c, int slot, size_t bytes) {
   w->buf[slot] = p;
   w->cap[slot] = bytes;
   return p;
+}
+
+// workspace buffers of the gauge-field side (fields.hip), kept for the
+// context's lifetime instead of an allocation per MD force
+void *scratch_get(Context &c, int use, size_t bytes) {
+  static const int slot[SCR_COUNT] = {S_FORCE_H, S_FORCE_COEF, S_SOLSUM_COEF};
+  return use >= 0 && use < SCR_COUNT ? ws_get(c, slot[use], bytes) : nullptr;
 }
 
 // ------------------------------------------------------------- reductions
@@ -281,6 +291,7 @@ __device__ void ms_epilogue(int epi, MsState *st, const double *sums, int nvec) 
     s.delta = sums[j];
     s.iter++;
     s.stop = !(sqrt(sums[j]) > s.res && s.iter < s.max_iter) || nact == 0;
+    if (!isfinite(sums[j]) || !isfinite(s.omega)) s.nonfinite = s.stop = 1;
   }
 }
 
@@ -293,6 +304,7 @@ __device__ void cg_epilogue(int epi, CgState *st, const double *sums, int nvec) 
       s.iter = 0;
       s.done = (v <= s.tol2) ? 1 : 0;
       s.stop = s.done;
+      if (!isfinite(v)) s.nonfinite = s.done = s.stop = 1;
     } else if (epi == EPI_CG_B) {
       if (s.stop) s.done = 1;
       if (s.done) continue;
@@ -303,6 +315,7 @@ __device__ void cg_epilogue(int epi, CgState *st, const double *sums, int nvec) 
       s.delta = v;
       s.iter++;
       s.stop = (v <= s.tol2) || s.iter >= s.max_iter;
+      if (!isfinite(v) || !isfinite(s.alpha)) s.nonfinite = s.stop = 1;
     }
   }
 }
@@ -476,6 +489,8 @@ __device__ __forceinline__ void bms_step(const DsArgs &a, int j) {
       // r^(k) stays inside the basis: the last M^dag M launch that writes
       // runs at k = cap-1
       stop = !(sqrt(rho1) > s.res && kk < s.max_iter) || nact == 0 || kk + 2 >= a.bms_cap;
+      // NaN/Inf in the reductions or the shifted recurrences: stop and report
+      if (!isfinite(rho1) || !isfinite(be) || !isfinite(om)) s.nonfinite = stop = 1;
       s.stop = s.done = stop;
     }
   }
@@ -1974,32 +1989,67 @@ static int ms_cg(Context &c, const double2 *b, int nrhs, int nshift, const doubl
   for (int j = 0; j < nrhs; j++) maxit = std::max(maxit, hs[j].iter);
   *iters_out = maxit;
   for (int j = 0; j < nrhs; j++)
+    if (hs[j].nonfinite) {
+      c.last_error = "multishift: non-finite value in the solver reductions";
+      return kStatusNonFinite;
+    }
+  for (int j = 0; j < nrhs; j++)
     if (hs[j].iter >= max_iter && hs[j].nact > 0) {
       c.last_error = "multishift: max_cg reached";
-      return 2;
+      return kStatusMaxCg;
     }
   return 0;
 }
 
+// Basis capacity policy.  The slots a solve needs are not known in advance,
+// so the capacity starts at max(kMinSlots, twice the most iterations a solve
+// of this precision has needed, the slots already allocated) and is bounded
+// by half of the device memory not held elsewhere (a 32^3x8 FP64 slot of 8
+// right-hand sides is 25 MB).  A solve that fills it is continued from its
+// last residual (ms_basis_solve) instead of being restarted from zero.
+constexpr long kMinSlots = 128;
+constexpr long kMaxSlots = 16384;
+constexpr int kBasisFull = 100;     // not even kMinBasis slots fit: the explicit-shift solver runs
+constexpr int kBasisRestart = 101;  // the basis filled: x holds the solutions so far (see BasisRestart)
+constexpr long kMinBasis = 16;
+
+// Systems still running when the basis filled.  Their shifted residuals
+// are collinear with the base residual, r_s^(K) = zeta_s^(K) r^(K)
+// (inverter.cc:224-284), so the remaining equations (MdagM + s) e_s = r_s
+// are one multi-shift solve with source r^(K) (basis slot K), whose
+// solutions are added with the factors zeta_s.
+struct BasisRestart {
+  const void *basis = nullptr;  // [cap][nrhs][3][Vh] of the call that filled
+  int nrhs = 0;
+  std::vector<int> rhs;         // systems to continue (indices into that call's right-hand sides)
+  std::vector<int> slot;        // their K
+  std::vector<double> zeta;     // [rhs.size()][nshift]: zeta_s^(K) of the running shifts, 0 for frozen ones
+};
+
 // multi-shift CG with the stored residual basis; x (FP64) receives
-// [nrhs][nshift] solutions.  Returns 3 when the basis does not fit (the
-// caller then runs the explicit-shift algorithm).
+// [nrhs][nshift] solutions.  Returns kBasisFull when not even kMinBasis
+// slots fit (the caller then runs the explicit-shift algorithm) and
+// kBasisRestart, with rs filled, when systems were still running when the
+// basis filled.
 template <typename R, int LINK>
 static int ms_cg_basis(Context &c, const double2 *b, int nrhs, int nshift, const double *shifts,
-                       const double *res_abs, int max_iter, double2 *x, int *iters_out, float2 *x32 = nullptr,
-                       bool asm_fp32 = false) {
+                       const double *res_abs, int max_iter, double2 *x, int *iters_out, float2 *x32,
+                       bool asm_fp32, BasisRestart *rs) {
   using C = Cx<R>;
   const long Vh = c.Vh;
   const size_t vb = sizeof(C) * 3 * Vh;
   const bool dbl = sizeof(R) == 8;
-  // basis capacity: at most max_iter+1 slots and a share of free HBM
   size_t fr = 0, tot = 0;
   hipMemGetInfo(&fr, &tot);
+  const size_t slot_bytes = vb * nrhs;
   const size_t have = c.ws->cap[S_BASIS];
-  const size_t budget = std::min<size_t>((size_t)24 << 30, (size_t)((fr + have) * 0.45));
-  long cap = (long)std::min<size_t>(std::min<size_t>((size_t)max_iter + 2, 4096), budget / (vb * nrhs));
-  if (cap < 64) return 3;
-  void *basis = ws_get(c, S_BASIS, vb * nrhs * cap);
+  const long budget = (long)((fr + have) / 2 / slot_bytes);
+  long want = std::max<long>(kMinSlots, 2L * c.ws->basis_kmax[dbl ? 1 : 0]);
+  want = std::max<long>(want, (long)(have / slot_bytes));
+  long cap = std::min<long>(std::min<long>((long)max_iter + 2, kMaxSlots), std::min<long>(budget, want));
+  if (const char *e = getenv("RHMC_BASIS_CAP")) cap = std::min<long>(cap, std::max(atol(e), kMinBasis));
+  if (cap < kMinBasis) return kBasisFull;
+  void *basis = ws_get(c, S_BASIS, slot_bytes * cap);
   void *sr = ws_get(c, dbl ? S_P64 : S_P32, vb * nrhs);   // s = A p recurrence
   void *hh = ws_get(c, dbl ? S_H64 : S_H32, vb * nrhs);
   MsState *ms = (MsState *)ws_get(c, S_MS, sizeof(MsState) * nrhs);
@@ -2102,8 +2152,15 @@ static int ms_cg_basis(Context &c, const double2 *b, int nrhs, int nshift, const
     key.a = a; key.hop = hop; key.mdm = mdm; key.nbr = nbr;
     key.a.b = nullptr;  // the right-hand sides are read only by k_bms_init, outside the chunk
     if (!gs.exec || memcmp(&key, &gs.key, sizeof(key)) != 0) {
-      if (gs.exec) hipGraphExecDestroy(gs.exec);
+      // An executable graph is destroyed only with nothing left in flight
+      // on the stream: no earlier launch of it (from this solve's previous
+      // chunk or an earlier solve) can still be pending when it goes.
+      if (gs.exec) {
+        if (!hip_ok(c, hipStreamSynchronize(c.stream), "graph retire")) return false;
+        hipGraphExecDestroy(gs.exec);
+      }
       gs.exec = nullptr;
+      c.graph_captures++;
       hipGraph_t g = nullptr;
       if (!hip_ok(c, hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal), "capture")) return false;
       chunk_launch();
@@ -2140,12 +2197,39 @@ static int ms_cg_basis(Context &c, const double2 *b, int nrhs, int nshift, const
   if (!hip_ok(c, hipStreamSynchronize(c.stream), "basis multishift")) return -1;
   int kmax = 0;
   for (int j = 0; j < nrhs; j++) kmax = std::max(kmax, hs[j].iter);
+  long &kseen = c.ws->basis_kmax[dbl ? 1 : 0];
+  kseen = std::max<long>(kseen, kmax);
   // the hop ran with work at iterations 1..kmax+1 (input slots 0..kmax);
   // slot kmax+1 holds the end of M^dag M(kmax+1) when a hop followed it
   if (timed) kernel_timing_harvest(c, kmax + 1);
-  // basis exhausted before convergence: let the caller use the explicit algorithm
   for (int j = 0; j < nrhs; j++)
-    if (hs[j].nact > 0 && hs[j].iter + 2 >= cap && hs[j].iter < max_iter) return 3;
+    if (hs[j].nonfinite) {
+      c.last_error = "multishift: non-finite value in the solver reductions";
+      return kStatusNonFinite;
+    }
+  // basis filled before convergence: the solutions so far are assembled
+  // below and the caller continues from r^(K)
+  if (rs) {
+    rs->rhs.clear();
+    rs->slot.clear();
+    rs->zeta.clear();
+    rs->basis = basis;
+    rs->nrhs = nrhs;
+  }
+  bool restart = false;
+  for (int j = 0; j < nrhs; j++)
+    if (hs[j].nact > 0 && hs[j].iter + 2 >= cap && hs[j].iter < max_iter) {
+      restart = true;
+      if (rs) {
+        rs->rhs.push_back(j);
+        rs->slot.push_back(hs[j].iter);
+        for (int sft = 0; sft < nshift; sft++) rs->zeta.push_back(hs[j].flag[sft] ? hs[j].z2[sft] : 0.0);
+      }
+    }
+  if (restart && !rs) {
+    c.last_error = "multishift: stored basis filled";
+    return kStatusMaxCg;
+  }
   // coefficients d_{s,i} = -z_s^(i) S_i, S_{K-1} = om^(K), S_i = om^(i+1) + ga^(i+1) S_{i+1}
   // (host arrays sized by the iterations run, kept in the workspace)
   const size_t hst = (size_t)kmax + 2;  // history slots per rhs on the host
@@ -2218,12 +2302,103 @@ static int ms_cg_basis(Context &c, const double2 *b, int nrhs, int nshift, const
   // stream-ordered for the caller; the host arrays stay alive in the workspace
   if (!hip_ok(c, hipGetLastError(), "basis assemble")) return -1;
   *iters_out = kmax;
+  if (restart) return kBasisRestart;
   for (int j = 0; j < nrhs; j++)
     if (hs[j].iter >= max_iter && hs[j].nact > 0) {
       c.last_error = "multishift: max_cg reached";
-      return 2;
+      return kStatusMaxCg;
     }
   return 0;
+}
+
+// r^(K) of a continued system (basis slot K, precision R) as an FP64 source
+template <typename R>
+__global__ __launch_bounds__(kBlock) void k_slot_to_src(const Cx<R> *v, double2 *out, long n) {
+  const long i = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) out[i] = make_double2((double)v[i].x, (double)v[i].y);
+}
+
+// x[orig[q]][s] += scale[q][s] y[q][s] for the continued systems
+template <typename XO>
+__global__ __launch_bounds__(kBlock) void k_add_scaled(XO *x, const XO *y, const int *orig, const double *scale,
+                                                       int nshift, long n3) {
+  const long e = (long)blockIdx.x * kBlock + threadIdx.x;
+  const int qs = blockIdx.y, q = qs / nshift, s = qs - q * nshift;
+  const double f = scale[qs];
+  if (e >= n3 || f == 0.0) return;
+  const XO v = y[(long)qs * n3 + e];
+  XO &o = x[((long)orig[q] * nshift + s) * n3 + e];
+  o.x += f * v.x;
+  o.y += f * v.y;
+}
+
+// Stored-basis multi-shift solve continued past a full basis: each
+// continuation solves (MdagM + s) y_s = r^(K) for the systems still running,
+// with the target divided by the largest |zeta_s| (so every shift reaches
+// res_abs), and adds zeta_s y_s to their solutions; factors compound over
+// continuations.  Returns 0, kStatusMaxCg, kStatusNonFinite, kBasisFull (the
+// first call did not fit) or -1.
+template <typename R, int LINK>
+static int ms_basis_solve(Context &c, const double2 *b, int nrhs, int nshift, const double *shifts,
+                          const double *res_abs, int max_iter, double2 *x, int *iters_out, float2 *x32 = nullptr,
+                          bool asm_fp32 = false) {
+  using C = Cx<R>;
+  BasisRestart rs;
+  int it = 0;
+  int rc = ms_cg_basis<R, LINK>(c, b, nrhs, nshift, shifts, res_abs, max_iter, x, &it, x32, asm_fp32, &rs);
+  int total = it;
+  std::vector<int> orig(nrhs);
+  for (int j = 0; j < nrhs; j++) orig[j] = j;
+  std::vector<double> scale((size_t)nrhs * nshift, 1.0);
+  const long n3 = 3 * c.Vh;
+  while (rc == kBasisRestart) {
+    c.last_restarts++;
+    const int n2 = (int)rs.rhs.size();
+    std::vector<int> orig2(n2);
+    std::vector<double> scale2((size_t)n2 * nshift), res2(n2);
+    for (int q = 0; q < n2; q++) {
+      const int j = rs.rhs[q];
+      orig2[q] = orig[j];
+      double zmax = 0.0;
+      for (int s = 0; s < nshift; s++) {
+        const double f = scale[(size_t)j * nshift + s] * rs.zeta[(size_t)q * nshift + s];
+        scale2[(size_t)q * nshift + s] = f;
+        zmax = std::max(zmax, std::fabs(f));
+      }
+      res2[q] = zmax > 0.0 ? res_abs[orig2[q]] / zmax : res_abs[orig2[q]];
+    }
+    double2 *src = (double2 *)ws_get(c, S_RS_SRC, sizeof(double2) * n3 * n2);
+    const size_t yb = (x32 ? sizeof(float2) : sizeof(double2)) * n3 * (size_t)n2 * nshift;
+    void *y = ws_get(c, S_RS_Y, yb);
+    int *dorig = (int *)ws_get(c, S_RS_IDX, sizeof(int) * 256 + sizeof(double) * 256 * kMaxShifts);
+    if (!src || !y || !dorig || n2 * nshift > 256 * kMaxShifts) return -1;
+    double *dscale = (double *)(dorig + 256);
+    for (int q = 0; q < n2; q++) {
+      const C *v = (const C *)rs.basis + ((size_t)rs.slot[q] * rs.nrhs + rs.rhs[q]) * n3;
+      hipLaunchKernelGGL(k_slot_to_src<R>, dim3((unsigned)((n3 + kBlock - 1) / kBlock)), dim3(kBlock), 0, c.stream,
+                         v, src + (size_t)q * n3, n3);
+    }
+    hipMemcpyAsync(dorig, orig2.data(), sizeof(int) * n2, hipMemcpyHostToDevice, c.stream);
+    hipMemcpyAsync(dscale, scale2.data(), sizeof(double) * scale2.size(), hipMemcpyHostToDevice, c.stream);
+    it = 0;
+    rc = ms_cg_basis<R, LINK>(c, src, n2, nshift, shifts, res2.data(), std::max(1, max_iter - total),
+                              x32 ? nullptr : (double2 *)y, &it, x32 ? (float2 *)y : nullptr, asm_fp32, &rs);
+    total += it;
+    if (rc < 0 || rc == kStatusNonFinite || rc == kBasisFull) return rc == kBasisFull ? -1 : rc;
+    const dim3 grid((unsigned)((n3 + kBlock - 1) / kBlock), (unsigned)(n2 * nshift));
+    if (x32)
+      hipLaunchKernelGGL(k_add_scaled<float2>, grid, dim3(kBlock), 0, c.stream, x32, (const float2 *)y, dorig,
+                         dscale, nshift, n3);
+    else
+      hipLaunchKernelGGL(k_add_scaled<double2>, grid, dim3(kBlock), 0, c.stream, x, (const double2 *)y, dorig,
+                         dscale, nshift, n3);
+    orig = orig2;
+    scale = scale2;
+    if (rc == kBasisRestart && total >= max_iter) rc = kStatusMaxCg;
+  }
+  *iters_out = total;
+  if (!hip_ok(c, hipGetLastError(), "basis continuation")) return -1;
+  return rc;
 }
 
 // batched CG on (MdagM + shift_j) e_j = r_j (FP32 iterations, FP32 links);
@@ -2287,6 +2462,11 @@ static int block_cg32(Context &c, const double2 *r64, int nsys, const double *sh
   int maxit = 0;
   for (int j = 0; j < nsys; j++) maxit = std::max(maxit, hs[j].iter);
   *iters_out = maxit;
+  for (int j = 0; j < nsys; j++)
+    if (hs[j].nonfinite) {
+      c.last_error = "defect correction: non-finite value in the solver reductions";
+      return kStatusNonFinite;
+    }
   return 0;
 }
 
@@ -2316,6 +2496,63 @@ int residual_d(Context &c, const double2 *b, int b_group, const double2 *x, cons
   return hip_ok(c, hipStreamSynchronize(c.stream), "residual") ? 0 : -1;
 }
 
+// FP64 defect correction of the shifted solutions x ([nrhs][nshift], FP64):
+// every system's true residual r = b - (MdagM + s) x is computed in FP64 with
+// the FP64 links; the systems above their target solve (MdagM + s) e = r
+// as one batched FP32 CG (to ~1e-5 of |r|, never below half the target) and
+// x += e, until every system meets its target (at most 8 cycles).  ref_iters
+// receives the FP32 iterations spent; returns kStatusMaxCg when the targets
+// are not reached, kStatusNonFinite on NaN/Inf.
+static int refine_fp64(Context &c, const double2 *b, int nrhs, int nshift, const double *shifts,
+                       const double *res_abs, int max_iter, double2 *x, int *ref_iters) {
+  const long Vh = c.Vh;
+  const int nsys = nrhs * nshift;
+  const long n = 3 * Vh * (long)nsys;
+  std::vector<double> sh(nsys), nr2(nsys), tol(nsys);
+  for (int j = 0; j < nsys; j++) sh[j] = shifts[j % nshift];
+  double2 *r64 = (double2 *)ws_get(c, S_CR64, sizeof(double2) * 3 * Vh * nsys);
+  if (!r64) return -1;
+  float2 *e32 = nullptr;
+  int total_ref = 0;
+  double worst = 0.0;
+  *ref_iters = 0;
+  for (int cyc = 0;; cyc++) {
+    if (residual_d(c, b, nshift, x, sh.data(), nsys, r64, nr2.data())) return -1;
+    bool ok = true;
+    worst = 0.0;
+    for (int j = 0; j < nsys; j++) {
+      if (!std::isfinite(nr2[j])) {
+        c.last_error = "multishift: non-finite true residual";
+        return kStatusNonFinite;
+      }
+      const double rn = sqrt(nr2[j]);
+      const double target = res_abs[j / nshift];
+      worst = std::max(worst, rn / target);
+      // inner FP32 solve: reduce this residual by ~1e-5, never below 0.5*target;
+      // systems already converged get a huge tolerance (done at the first test)
+      tol[j] = rn <= target ? 1e150 : std::max(0.5 * target, 1e-5 * rn);
+      if (rn > target) ok = false;
+    }
+    c.last_true_res_max = worst;
+    if (ok || cyc == 8) break;
+    if (!e32 && !(e32 = (float2 *)ws_get(c, S_CX32, sizeof(float2) * 3 * Vh * nsys))) return -1;
+    int itc = 0;
+    const int rc = block_cg32(c, r64, nsys, sh.data(), tol.data(), max_iter, e32, &itc);
+    if (rc) return rc;
+    total_ref += itc;
+    hipLaunchKernelGGL(k_accum_f2d, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, c.stream, x,
+                       e32, n);
+    c.last_refine_cycles = cyc + 1;
+  }
+  *ref_iters = total_ref;
+  if (!hip_ok(c, hipStreamSynchronize(c.stream), "defect correction")) return -1;
+  if (worst > 1.0) {
+    c.last_error = "multishift: defect correction did not reach the target";
+    return kStatusMaxCg;
+  }
+  return 0;
+}
+
 int multishift_solve(Context &c, const double2 *b, int nrhs, int nshift, const double *shifts, double res, int rel,
                      int prec, int max_iter, double2 *x, int *iters, float2 *x32) {
   if (x32 && prec != PREC_FP32) {
@@ -2333,12 +2570,27 @@ int multishift_solve(Context &c, const double2 *b, int nrhs, int nshift, const d
   for (int j = 0; j < nrhs; j++) res_abs[j] = rel ? res * sqrt(bn2[j]) : res;
   c.last_refine_cycles = 0;
   c.last_iters_refine = 0;
+  c.last_restarts = 0;
+  c.last_true_res_max = 0.0;
   const bool explicit_shifts = c.ms_algorithm == 1 || getenv("RHMC_MS_EXPLICIT") != nullptr;
   if (prec == PREC_FP64) {
-    int rc = explicit_shifts ? 3 : ms_cg_basis<double, LINK_D12>(c, b, nrhs, nshift, shifts, res_abs.data(), max_iter, x, iters);
-    if (rc == 3) rc = ms_cg<double, LINK_D12>(c, b, nrhs, nshift, shifts, res_abs.data(), max_iter, x, iters);
+    int rc = explicit_shifts ? kBasisFull
+                             : ms_basis_solve<double, LINK_D12>(c, b, nrhs, nshift, shifts, res_abs.data(), max_iter,
+                                                                x, iters);
+    const bool basis = rc != kBasisFull;
+    if (rc == kBasisFull) rc = ms_cg<double, LINK_D12>(c, b, nrhs, nshift, shifts, res_abs.data(), max_iter, x, iters);
     c.last_iters = *iters;
     c.last_iters_fp32 = 0;
+    if (rc != 0 || !basis) return rc;
+    // The stored-basis solver stops on its recursive residual, which in
+    // the single-reduction form can drift above the true one: every
+    // shifted system's FP64 true residual is checked and, where it is above
+    // the target, reduced by defect correction.
+    int ref = 0;
+    rc = refine_fp64(c, b, nrhs, nshift, shifts, res_abs.data(), max_iter, x, &ref);
+    c.last_iters_refine = ref;
+    c.last_iters = *iters + ref;
+    *iters = c.last_iters;
     return rc;
   }
   // mixed: FP32 multi-shift to ~1e-6 relative, then FP64 defect correction;
@@ -2348,18 +2600,18 @@ int multishift_solve(Context &c, const double2 *b, int nrhs, int nshift, const d
   for (int j = 0; j < nrhs; j++) res32[j] = std::max(res_abs[j], rel32 * sqrt(bn2[j]));
   int it32 = 0;
   const long n = 3 * Vh * (long)nrhs * nshift;
-  int rc = explicit_shifts ? 3
-                           : ms_cg_basis<float, LINK_S12>(c, b, nrhs, nshift, shifts, res32.data(), max_iter, x, &it32,
-                                                          x32, prec == PREC_FP32);
-  if (rc == 3) {
+  int rc = explicit_shifts ? kBasisFull
+                           : ms_basis_solve<float, LINK_S12>(c, b, nrhs, nshift, shifts, res32.data(), max_iter, x,
+                                                             &it32, x32, prec == PREC_FP32);
+  if (rc == kBasisFull) {
     float2 *xs = x32 ? x32 : (float2 *)ws_get(c, S_X32, sizeof(float2) * 3 * Vh * nrhs * nshift);
     if (!xs) return -1;
     rc = ms_cg<float, LINK_S12>(c, b, nrhs, nshift, shifts, res32.data(), max_iter, xs, &it32);
-    if (rc < 0) return rc;
+    if (rc < 0 || rc == kStatusNonFinite) return rc;
     if (!x32)
       hipLaunchKernelGGL(k_copy_f2d, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, c.stream, x, xs, n);
   }
-  if (rc < 0) return rc;
+  if (rc < 0 || rc == kStatusNonFinite) return rc;
   if (prec == PREC_FP32) {
     if (!hip_ok(c, hipStreamSynchronize(c.stream), "fp32 solve")) return -1;
     c.last_iters_fp32 = it32;
@@ -2367,48 +2619,13 @@ int multishift_solve(Context &c, const double2 *b, int nrhs, int nshift, const d
     *iters = it32;
     return rc;
   }
-  const int nsys = nrhs * nshift;
-  std::vector<double> sh(nsys), nr2(nsys), tol(nsys);
-  for (int j = 0; j < nsys; j++) sh[j] = shifts[j % nshift];
-  double2 *r64 = (double2 *)ws_get(c, S_CR64, sizeof(double2) * 3 * Vh * nsys);
-  float2 *e32 = (float2 *)ws_get(c, S_CX32, sizeof(float2) * 3 * Vh * nsys);
-  if (!r64 || !e32) return -1;
-  int total_ref = 0;
-  double worst = 0.0;
-  for (int cyc = 0; cyc < 8; cyc++) {
-    if (residual_d(c, b, nshift, x, sh.data(), nsys, r64, nr2.data())) return -1;
-    bool ok = true;
-    worst = 0.0;
-    for (int j = 0; j < nsys; j++) {
-      const double rn = sqrt(nr2[j]);
-      const double target = res_abs[j / nshift];
-      worst = std::max(worst, rn / target);
-      // inner FP32 solve: reduce this residual by ~1e-5, never below 0.5*target
-      tol[j] = rn <= target ? 1e300 : std::max(0.5 * target, 1e-5 * rn);
-      if (rn > target) ok = false;
-    }
-    c.last_true_res_max = worst;
-    if (ok) break;
-    // systems already converged get tol=inf -> done at init
-    for (int j = 0; j < nsys; j++)
-      if (tol[j] > 1e299) tol[j] = 1e150;
-    int itc = 0;
-    if (block_cg32(c, r64, nsys, sh.data(), tol.data(), max_iter, e32, &itc)) return -1;
-    total_ref += itc;
-    hipLaunchKernelGGL(k_accum_f2d, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, c.stream, x,
-                       e32, n);
-    c.last_refine_cycles = cyc + 1;
-  }
-  if (!hip_ok(c, hipStreamSynchronize(c.stream), "mixed solve")) return -1;
+  int ref = 0;
+  const int rr = refine_fp64(c, b, nrhs, nshift, shifts, res_abs.data(), max_iter, x, &ref);
   c.last_iters_fp32 = it32;
-  c.last_iters_refine = total_ref;
-  c.last_iters = it32 + total_ref;
+  c.last_iters_refine = ref;
+  c.last_iters = it32 + ref;
   *iters = c.last_iters;
-  if (worst > 1.0) {
-    c.last_error = "mixed multishift: refinement did not reach the target";
-    return 2;
-  }
-  return rc;
+  return rr ? rr : rc;
 }
 
 // ---------------------------------------------------------- norms / scaling

@@ -253,12 +253,22 @@ struct Traj {
     P(-eps, f, true);
     Q(0.5 * eps);
   }
+  // update_nometro's MD (update.cc:16-48): the configured integrator
+  void run_md() {
+    if (p.integrator == RHMC_INT_OMELYAN) omelyan();
+    else if (p.integrator == RHMC_INT_MULTISTEP_2MN) ms2mn();
+    else if (p.integrator == RHMC_INT_MULTISTEP_4MN) ms4mn();
+    else leapfrog();
+  }
+  // solver status -> ABI status
+  int md_status() const {
+    if (status == 0) return RHMC_OK;
+    if (status < 0) return RHMC_EHIP;
+    return status == 2 ? RHMC_EMAXCG : status;
+  }
 };
 
-}  // namespace
-
-int run_trajectory(Context &c, const rhmc_traj_params &p, rhmc_traj_result &r) {
-  memset(&r, 0, sizeof(r));
+int check_params(Context &c, const rhmc_traj_params &p) {
   if (p.no_md < 1 || p.integrator < 0 || p.integrator > RHMC_INT_MULTISTEP_4MN ||
       ((p.integrator == RHMC_INT_MULTISTEP_2MN || p.integrator == RHMC_INT_MULTISTEP_4MN) && p.gauge_scale < 1)) {
     c.last_error = "trajectory: bad MD parameters";
@@ -269,16 +279,21 @@ int run_trajectory(Context &c, const rhmc_traj_params &p, rhmc_traj_result &r) {
     c.last_error = "trajectory: bad rational orders";
     return RHMC_EINVAL;
   }
-  const auto t0 = clk::now();
-  Traj T(c, p, r);
+  return RHMC_OK;
+}
+
+// Fields, spectral bounds and the heatbath of update.cc:64-79 /
+// revupdate.cc:31-47: momenta and phi drawn on the device (draw_fields), the
+// bounds (fixed or power method), chi = r_first(MdagM) phi and the rescaled
+// MD approximation in T.md.
+int prologue(Context &c, const rhmc_traj_params &p, Traj &T, double &mn, double &mx) {
   const long sp = 3 * c.Vh;
-  // Metropolis backup of the links (Conf::save / copy_saved)
-  hipMemcpyAsync(c.U64_save, c.U64, sizeof(double2) * 72 * c.Vh, hipMemcpyDeviceToDevice, c.stream);
   if (p.draw_fields) {
     gauss_momenta(c, p.seed, 0);
     gauss_spinor(c, c.phi0, c.n_pf, p.seed, 1);
   }
-  double mn = p.eig_min, mx = p.eig_max;
+  mn = p.eig_min;
+  mx = p.eig_max;
   if (!p.pure_gauge && p.bounds_mode == 1) {
     int loops;
     gauss_spinor(c, c.phinew, 1, p.seed, 2);
@@ -286,29 +301,81 @@ int run_trajectory(Context &c, const rhmc_traj_params &p, rhmc_traj_result &r) {
     gauss_spinor(c, c.phinew, 1, p.seed, 3);
     if (power_min(c, c.phinew, p.power_tol, mx, &mn, &loops)) return RHMC_EHIP;
   }
-  r.eig_min = mn;
-  r.eig_max = mx;
-  const double e_first = (double)p.n_flavours / 8.0 / (double)c.n_pf;
-  const double e_md = -(double)p.n_flavours / 4.0 / (double)c.n_pf;
+  T.r.eig_min = mn;
+  T.r.eig_max = mx;
   if (!p.pure_gauge) {
+    const double e_first = (double)p.n_flavours / 8.0 / (double)c.n_pf;
+    const double e_md = -(double)p.n_flavours / 4.0 / (double)c.n_pf;
     // first_inv_approx_calc: chi = r_first(MdagM) phi
     Coeffs cf = rescale(p.order_first, p.first_a0, p.first_a, p.first_b, mn, mx, e_first);
-    if (T.solve(c.phi0, cf, p.res_metro) && T.status > 0) return RHMC_EMAXCG;
+    if (T.solve(c.phi0, cf, p.res_metro)) return T.md_status();
     hipMemcpyAsync(c.chi, c.phi0, sizeof(double2) * sp * c.n_pf, hipMemcpyDeviceToDevice, c.stream);
     if (sol_sum(c, c.chi, c.smf, p.order_first, cf.a0, cf.a.data())) return RHMC_EHIP;
     T.md = rescale(p.order_md, p.md_a0, p.md_a, p.md_b, mn, mx, e_md);
   }
+  return RHMC_OK;
+}
+
+}  // namespace
+
+// revupdate.cc:19-97 (rev_update): heatbath, MD, momenta reversed, MD again;
+// distances per degree of freedom |U - U_0| and |P + P_0| with the
+// reference's normalisation sqrt(sum) * sqrt(1/(V 4 8)).  The links and
+// momenta are left at the end of the second MD (as the reference leaves them).
+int run_rev_update(Context &c, const rhmc_traj_params &p, rhmc_traj_result &r, double *d_conf, double *d_mom) {
+  memset(&r, 0, sizeof(r));
+  if (const int e = check_params(c, p)) return e;
+  const auto t0 = clk::now();
+  Traj T(c, p, r);
+  double mn, mx;
+  if (const int e = prologue(c, p, T, mn, mx)) return e;
+  double2 *P0 = nullptr;
+  if (!hip_ok(c, hipMalloc((void **)&P0, sizeof(double2) * 32 * c.Vh), "saved momenta")) return RHMC_EHIP;
+  // save_momenta / the links at the start (Conf::save)
+  hipMemcpyAsync(P0, c.P, sizeof(double2) * 32 * c.Vh, hipMemcpyDeviceToDevice, c.stream);
+  hipMemcpyAsync(c.U64_save, c.U64, sizeof(double2) * 72 * c.Vh, hipMemcpyDeviceToDevice, c.stream);
+  T.run_md();
+  int rc = T.md_status();
+  if (rc == RHMC_OK) {
+    momenta_scale(c, -1.0);  // reverse_momenta: P <- -P
+    T.run_md();
+    rc = T.md_status();
+  }
+  double s1 = 0.0, s2 = 0.0;
+  if (rc == RHMC_OK && (pair_norm2(c, c.U64, c.U64_save, 0, &s1) || pair_norm2(c, c.P, P0, 1, &s2)))
+    rc = RHMC_EHIP;
+  hipStreamSynchronize(c.stream);
+  hipFree(P0);
+  if (rc != RHMC_OK) return rc;
+  const double norm = std::sqrt(1.0 / (double)c.V * 0.25 * 0.125);
+  *d_conf = std::sqrt(s1) * norm;
+  *d_mom = std::sqrt(s2) * norm;
+  r.seconds_total = std::chrono::duration<double>(clk::now() - t0).count();
+  r.seconds_solver = T.t_solve;
+  r.seconds_force = T.t_force;
+  r.seconds_update = T.t_update;
+  return RHMC_OK;
+}
+
+int run_trajectory(Context &c, const rhmc_traj_params &p, rhmc_traj_result &r) {
+  memset(&r, 0, sizeof(r));
+  if (const int e = check_params(c, p)) return e;
+  const auto t0 = clk::now();
+  Traj T(c, p, r);
+  const long sp = 3 * c.Vh;
+  // Metropolis backup of the links (Conf::save / copy_saved)
+  hipMemcpyAsync(c.U64_save, c.U64, sizeof(double2) * 72 * c.Vh, hipMemcpyDeviceToDevice, c.stream);
+  double mn, mx;
+  if (const int e = prologue(c, p, T, mn, mx)) return e;
+  const double e_first = (double)p.n_flavours / 8.0 / (double)c.n_pf;
+  const double e_md = -(double)p.n_flavours / 4.0 / (double)c.n_pf;
   action_local(c, c.phi0, c.chi, !p.pure_gauge, 0);
-  if (p.integrator == RHMC_INT_OMELYAN) T.omelyan();
-  else if (p.integrator == RHMC_INT_MULTISTEP_2MN) T.ms2mn();
-  else if (p.integrator == RHMC_INT_MULTISTEP_4MN) T.ms4mn();
-  else T.leapfrog();
-  if (T.status > 0) return RHMC_EMAXCG;
-  if (T.status < 0) return RHMC_EHIP;
+  T.run_md();
+  if (T.status) return T.md_status();
   if (!p.pure_gauge) {
     // last_inv_approx_calc: phi_new = r_last(MdagM) chi
     Coeffs cl = rescale(p.order_last, p.last_a0, p.last_a, p.last_b, mn, mx, e_md);
-    if (T.solve(c.chi, cl, p.res_metro)) return T.status > 0 ? RHMC_EMAXCG : RHMC_EHIP;
+    if (T.solve(c.chi, cl, p.res_metro)) return T.md_status();
     hipMemcpyAsync(c.phinew, c.chi, sizeof(double2) * sp * c.n_pf, hipMemcpyDeviceToDevice, c.stream);
     if (sol_sum(c, c.phinew, c.smf, p.order_last, cl.a0, cl.a.data())) return RHMC_EHIP;
   }

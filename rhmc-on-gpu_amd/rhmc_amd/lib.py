@@ -21,7 +21,14 @@ _lib = None
 
 
 class RhmcError(RuntimeError):
-    pass
+    """A nonzero status of the C ABI; .rc holds it (RHMC_* codes)."""
+
+    def __init__(self, msg, rc=None):
+        super().__init__(msg)
+        self.rc = rc
+
+
+RHMC_OK, RHMC_EINVAL, RHMC_EMAXCG, RHMC_ENONFINITE, RHMC_EHIP = 0, 1, 2, 3, -1
 
 
 def load():
@@ -119,12 +126,14 @@ def _declare(L):
         "gpu_dslash": (I, [VP, I, I, I, VP, VP]),
         "gpu_time_dslash": (I, [VP, I, I, I, PD]),
         "gpu_trajectory": (I, [VP, C.POINTER(TrajParams), C.POINTER(TrajResult)]),
+        "gpu_rev_update": (I, [VP, C.POINTER(TrajParams), PD, PD, C.POINTER(TrajResult)]),
         "gpu_tuning_table": (I, [VP, C.c_char_p, I]),
         "gpu_time_copy": (I, [VP, D, I, PD]),
         "cu_multips_shifted_invert": (I, [VP, D, I, I, VP, I, I, PI]),
         "gpu_shifted_inverter": (I, [VP, D, VP, I, I, I, PI]),
         "cu_shifted_residuals": (I, [VP, I, VP, VP]),
         "cu_last_solve_stats": (I, [VP, PI, PI, PI]),
+        "cu_solver_counters": (I, [VP, PI, PI, PD]),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)
@@ -241,7 +250,7 @@ class Gpu:
 
     def _check(self, rc, what):
         if rc:
-            raise RhmcError(f"{what}: rc={rc} {self._err()}")
+            raise RhmcError(f"{what}: rc={rc} {self._err()}", rc)
 
     # -- transfers
     def init0(self, links):
@@ -358,6 +367,14 @@ class Gpu:
         self.L.cu_last_solve_stats(self.h, C.byref(a), C.byref(b), C.byref(c))
         return a.value, b.value, c.value
 
+    def solver_counters(self):
+        """(continuations past a full basis in the last solve, solver-chunk
+        graphs captured so far, largest true residual / target of the last
+        checked solve)."""
+        a, b, c = C.c_int(), C.c_int(), C.c_double()
+        self.L.cu_solver_counters(self.h, C.byref(a), C.byref(b), C.byref(c))
+        return a.value, b.value, c.value
+
     def fermion_force(self, numerators, accumulate=False):
         num = np.ascontiguousarray(numerators, np.float64)
         self._check(self.L.gpu_fermion_force(self.h, len(num), _ptr(num), int(accumulate)), "gpu_fermion_force")
@@ -427,3 +444,14 @@ class Gpu:
         r = TrajResult()
         self._check(self.L.gpu_trajectory(self.h, C.byref(t), C.byref(r)), "gpu_trajectory")
         return r
+
+    def rev_update(self, **kw):
+        """Reversibility check (gpu_rev_update, revupdate.cc:19-97): heatbath,
+        MD, momenta reversed, MD again; returns (delta_conf, delta_mom,
+        result) with the reference's per-degree-of-freedom normalisation."""
+        t, keep = traj_params(**kw)
+        r = TrajResult()
+        dc, dm = C.c_double(), C.c_double()
+        self._check(self.L.gpu_rev_update(self.h, C.byref(t), C.byref(dc), C.byref(dm), C.byref(r)),
+                    "gpu_rev_update")
+        return dc.value, dm.value, r

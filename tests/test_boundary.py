@@ -83,7 +83,7 @@ def test_every_wrapper_converts_its_arguments():
         lambda: g.chiral_meas(2.0, 1e-10, sp[0], sp[0]), lambda: g.chiral_meas(2.0, 1e-10),
         lambda: g.write_conf(tempfile.gettempdir() + "/never_written", 5.5, 0.1, 2, 0),
         lambda: g.read_conf(tempfile.gettempdir() + "/never_read"), lambda: g.trajectory(**traj),
-        lambda: multi_trajectory([g], [traj]),
+        lambda: multi_trajectory([g], [traj]), lambda: g.rev_update(**traj), lambda: g.solver_counters(),
     ]
     for k, call in enumerate(calls):
         try:

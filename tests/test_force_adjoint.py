@@ -23,12 +23,12 @@ def say(*a):
     print(*a, flush=True)
 
 
-def _setup():
-    lat = fields.Lattice(*DIMS)
+def _setup(dims=DIMS):
+    lat = fields.Lattice(*dims)
     U = fields.random_links(lat, 31, 0.3)
     phi = fields.gauss_spinor(lat, 32, 1)
     _, md, _ = rhmc_approximations(2, 1, 12, 12, 12)
-    g = Gpu(DIMS, mass=MASS, beta=5.5, n_pf=1, max_order=12)
+    g = Gpu(dims, mass=MASS, beta=5.5, n_pf=1, max_order=12)
     g.init0(U)
     mx = g.find_max(1e-6, seed=3)
     mn = max(g.find_min(mx, 1e-6, seed=4), MASS * MASS)
@@ -37,9 +37,12 @@ def _setup():
 
 
 @pytest.mark.gpu
-def test_force_adjoint_against_finite_differences_and_oracle():
-    say("force adjoint", DIMS)
-    lat, U, phi, g, a0, a, b = _setup()
+@pytest.mark.parametrize("dims", [DIMS, (24, 24, 24, 24)])
+def test_force_adjoint_against_finite_differences_and_oracle(dims):
+    """At 12^4 and at 24^4 (the C3 configuration: 64 central differences at
+    delta = 1e-5 and the oracle's force to 1e-12)."""
+    say("force adjoint", dims)
+    lat, U, phi, g, a0, a, b = _setup(dims)
     model = PfModel(g, U, phi, a0, a, b, res=1e-14)
     omega = torch.zeros((4 * lat.V, 8), dtype=torch.float64, requires_grad=True)
     S = PseudofermionAction.apply(omega, model)
@@ -67,7 +70,7 @@ def test_force_adjoint_against_finite_differences_and_oracle():
     say("finite differences: worst relative deviation", worst, "noise floor", noise)
 
     # the oracle's FP64 force on the same inputs
-    o = Oracle(DIMS)
+    o = Oracle(dims)
     ip_ref, _ = o.full_force(U, 5.5, MASS, phi, a0, a, b, 1e-14 * np.linalg.norm(phi), with_gauge=False)
     ip = model.force()
     err = np.linalg.norm(ip - ip_ref) / np.linalg.norm(ip_ref)

@@ -125,6 +125,34 @@ def test_multishift_against_dense_solve():
     assert np.allclose(M, mass ** 2 * np.eye(M.shape[0]) - Deo @ Doe, atol=1e-13)
 
 
+@pytest.mark.parametrize("seed,eps", [(4, None), (8, 0.5)])
+def test_power_method_against_dense_spectrum(seed, eps):
+    """orc_findmaxeig / orc_findmineig (findminmax.cc:12-137) converge to the
+    extreme eigenvalues of the dense M^dag M on 4^4: the largest by power
+    iterations on M^dag M, the smallest as max - (largest eigenvalue of
+    max - M^dag M).  A tight tolerance on the relative change takes them to
+    the eigenvalues; the reference's own 1e-5 leaves them within a few
+    1e-4 (the stopping rule measures the change per step, not the error),
+    and from below for the largest, from above for the smallest (Rayleigh
+    bounds of a hermitian operator)."""
+    dims = (4, 4, 4, 4)
+    mass = 0.1
+    lat = fields.Lattice(*dims)
+    o = Oracle(dims)
+    U = fields.random_links(lat, seed, eps)
+    ev = np.linalg.eigvalsh(dense_mdagm(o, U, mass))
+    start = fields.gauss_spinor(lat, seed + 20)[0]
+    mx, n_mx = o.findmaxeig(U, mass, start, 1e-14)
+    mn, n_mn = o.findmineig(U, mass, start, mx, 1e-14)
+    assert n_mx > 10 and n_mn > 10
+    assert abs(mx - ev[-1]) < 1e-9 * ev[-1], (mx, ev[-1])
+    assert abs(mn - ev[0]) < 1e-9 * ev[-1], (mn, ev[0])
+    mx5, _ = o.findmaxeig(U, mass, start, 1e-5)
+    mn5, _ = o.findmineig(U, mass, start, mx5, 1e-5)
+    assert ev[-1] * (1 - 2e-3) < mx5 <= ev[-1] * (1 + 1e-12)
+    assert ev[0] - 1e-12 <= mn5 + (ev[-1] - mx5) and mn5 < ev[0] + 2e-3 * ev[-1]
+
+
 def test_su3_exp_and_projection():
     rng = np.random.default_rng(7)
     lat = fields.Lattice(2, 2, 2, 2)
