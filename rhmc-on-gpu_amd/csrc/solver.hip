@@ -7,15 +7,20 @@
 // link load; neighbours are computed from the site index.
 //
 // Multi-shift CG: Jegerlehner recurrences of
-// lib/Action/Inverter/inverter.cc:142-339.  Per iteration four launches:
-//   A  h = D_oe p                                   (hop, even -> odd)
-//   B  s = m^2 p - D_eo h, alpha = Re<p,s>          (+ zeta/omega recurrences)
-//   C  r += omega s, lambda = |r|^2                 (+ gamma, shift freezing)
-//   D  p = r + gamma p; x_s -= omega_s p_s; p_s = zeta_s r + gamma_s p_s
-// Reductions are FP64, wave64 shuffles + one partial per workgroup; the last
-// workgroup to finish (device-scope counter) sums the partials in a fixed
-// order and advances the scalar recurrences on the device, so the host only
-// polls a done flag every few iterations.
+// lib/Action/Inverter/inverter.cc:142-339, in two organisations:
+// * stored residual basis (ms_cg_basis, the default): the base CG in its
+//   single-reduction form, two launches per iteration -- the hop
+//   h = D_oe r^(k-1) with |h|^2 partials (its first workgroups advance the
+//   scalar recurrences), then M^dag M with the s and r^(k) updates and
+//   |r^(k)|^2 partials; the shifted solutions are assembled from the basis
+//   at the end; a full basis is continued from its last residual
+//   (ms_basis_solve);
+// * explicit shifts (ms_cg, the reference's organisation): hop, M^dag M with
+//   <p, A p>, r update with |r|^2, p and per-shift x_s / p_s updates.
+// Reductions are FP64: wave64 shuffles and one partial per workgroup, summed
+// in a fixed order by the next launch (or a one-workgroup finalize), so no
+// device-scope fence is needed and the host only polls a running-system
+// count every few iterations.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
