@@ -64,6 +64,10 @@ def dslash_traffic(table, Vh, nrhs, mode=0, red=0):
     path = os.path.join(ROOT, "profiles", "dslash_32c8_counters.json")
     if not os.path.exists(path):
         return None
+    with open(path) as fh:
+        prof = json.load(fh)
+    if prof.get("source_sha256") != kernel_source_hash():
+        return None  # profiled on other kernel sources: not this build's traffic
     ent = [t for t in table if t["real_bytes"] == 4 and t["links"] == "fp32_12" and t["mode"] == mode
            and t["reduction"] == red and t["nvec"] == nrhs]
     if not ent:
@@ -79,9 +83,23 @@ def dslash_traffic(table, Vh, nrhs, mode=0, red=0):
         want = max(64, (2048 + nch - 1) // nch)
         want = (want + 7) // 8 * 8
         grid = min(want, nbx) * 256 * nch
-    with open(path) as fh:
-        k = json.load(fh)["kernels"].get(f"{name}|{grid}")
+    k = prof["kernels"].get(f"{name}|{grid}")
     return None if k is None else k["read_bytes"] + k["write_bytes"]
+
+
+KERNEL_SOURCES = ("rhmc-on-gpu_amd/csrc/solver.hip", "rhmc-on-gpu_amd/csrc/lattice.hpp",
+                  "rhmc-on-gpu_amd/csrc/context.hpp")
+
+
+def kernel_source_hash():
+    """SHA-256 over the Dslash kernel sources; a counter profile records the
+    hash of the sources it was taken on and is used only when they match."""
+    import hashlib
+    h = hashlib.sha256()
+    for f in KERNEL_SOURCES:
+        with open(os.path.join(ROOT, f), "rb") as fh:
+            h.update(fh.read())
+    return h.hexdigest()
 
 
 def cpu_baseline(dims, mass, beta, n_pf, shifts, md_a, chi, U, iters_total, n_force, n_q):
@@ -391,6 +409,8 @@ def main():
     ap.add_argument("--res-metro", type=float, default=1e-10)
     ap.add_argument("--metro-refined", action="store_true",
                     help="also time mixed with FP32 + FP64 defect-correction heatbath/Metropolis solves")
+    ap.add_argument("--no-power-bounds", action="store_true",
+                    help="skip the leg that measures the spectral bounds in every trajectory")
     ap.add_argument("--dslash-only", action="store_true",
                     help="only the Dslash and copy-probe timings (for counter runs)")
     ap.add_argument("--workload", default="traj32c8", choices=sorted(WORKLOADS),
@@ -442,14 +462,18 @@ def main():
     traj_kw = dict(n_flavours=2, no_md=args.no_md, integrator="omelyan", res_md=args.res_md,
                    res_metro=args.res_metro, relative=True, bounds=(mn, mx), draw_fields=True,
                    metropolis=False, fp32_threshold=1e-7, **co)
+    # mixed_power_bounds: the mixed leg with the spectral bounds measured by
+    # power iterations at the start of every trajectory (the reference's
+    # findminmax at each rescaling, rationalapprox.cc:145, tolerance 1e-5)
     modes = {"mixed": dict(precision=1), "fp64": dict(precision=0),
              "fp64_plain": dict(precision=0, per_pf_solves=True),
-             "mixed_metro_refined": dict(precision=1, metro_precision=1)}
+             "mixed_metro_refined": dict(precision=1, metro_precision=1),
+             "mixed_power_bounds": dict(precision=1, bounds=None, power_tol=1e-5)}
 
     def run(mode, seed, verify=False):
         g.init0(U0)
         t0 = time.perf_counter()
-        r = g.trajectory(seed=seed, verify_fp64=verify, verify_res=1e-13, **modes[mode], **traj_kw)
+        r = g.trajectory(seed=seed, verify_fp64=verify, verify_res=1e-13, **dict(traj_kw, **modes[mode]))
         dt = time.perf_counter() - t0
         print(f"[bench] trajectory {mode}{' +verify' if verify else ''} seed={seed} {dt:.3f}s "
               f"dH={r.delta_h:.6e} dH64={r.delta_h_fp64:.6e} iters={r.cg_iterations} solves={r.n_solves} "
@@ -457,7 +481,8 @@ def main():
         return dt, r
 
     legs = ["mixed"] + ([] if args.no_fp64 else ["fp64", "fp64_plain"]) + \
-        (["mixed_metro_refined"] if args.metro_refined else [])
+        (["mixed_metro_refined"] if args.metro_refined else []) + \
+        ([] if args.no_power_bounds else ["mixed_power_bounds"])
     for w in range(args.warmup):
         for leg in legs:
             run(leg, 100 + w)
@@ -466,6 +491,7 @@ def main():
     iters = {leg: [] for leg in legs}
     dh_err = {leg: [] for leg in legs if leg.startswith("mixed")}
     live = []
+    pbounds = []
     for s in range(args.steps):
         for leg in legs:
             # the MD hop (FP32 12-real, n_pf RHS) times itself on the device
@@ -475,6 +501,8 @@ def main():
             if leg == "mixed":
                 live.append(g.kernel_timing(0))
             times[leg].append(dt)
+            if leg == "mixed_power_bounds":
+                pbounds.append([r.eig_min, r.eig_max])
             dh[leg].append(r.delta_h)
             iters[leg].append(r.cg_iterations)
             if leg in dh_err:
@@ -550,6 +578,7 @@ def main():
         "delta_h_error_vs_fp64_1e-13": dh_err,
         "cg_iterations": iters,
         "spectral_bounds": [mn, mx],
+        "power_bounds_per_trajectory": pbounds,
         "dslash": {"fp32_12real_8rhs_ms": ms32, "fp32_12real_8rhs_GBps": gbs32,
                    "fp64_18real_1rhs_ms": ms64, "fp64_18real_1rhs_GBps": gbs64,
                    "frac_hbm_fp32": gbs32 / HBM_PEAK_GBS, "frac_hbm_fp64": gbs64 / HBM_PEAK_GBS,

@@ -26,6 +26,9 @@ int download_spinors(Context &c, const double2 *src, double *host, int n);
 int hop_s12(Context &c, int par_out, const float2 *in, float2 *out, int nvec);
 int run_trajectory(Context &c, const rhmc_traj_params &p, rhmc_traj_result &r);
 int run_rev_update(Context &c, const rhmc_traj_params &p, rhmc_traj_result &r, double *d_conf, double *d_mom);
+bool trajectories_lockstep(const std::vector<const rhmc_traj_params *> &ps);
+int run_trajectory_group(std::vector<Context *> &cs, std::vector<const rhmc_traj_params *> &ps,
+                         std::vector<rhmc_traj_result *> &rs, std::vector<int> &status);
 std::string tuning_table(Context &c);
 int time_copy(Context &c, size_t bytes, int reps, double *ms);
 
@@ -706,8 +709,31 @@ int gpu_multi_trajectory(rhmc_gpu_t *hs, int n, const rhmc_traj_params *ps, rhmc
     for (int k = 0; k < i; k++)
       if (hs[i] == hs[k]) return RHMC_EINVAL;
   }
-  // one host thread per lattice, each sequencing its own stream; the device
-  // runs the lattices' kernels (and replayed solver graphs) concurrently
+  // Lattices with the same integrator, MD steps, precision dispatch and
+  // residuals (each with its own extents, pseudofermions, rational
+  // functions, bounds and seed) advance in lockstep on one device: every
+  // solver step is one launch over all of their tiles.  Others: one host
+  // thread per lattice, each sequencing its own stream.
+  std::vector<const rhmc_traj_params *> pv(n);
+  bool same_device = true;
+  for (int i = 0; i < n; i++) {
+    pv[i] = &ps[i];
+    same_device = same_device && hs[i]->c.device == hs[0]->c.device;
+  }
+  if (same_device && trajectories_lockstep(pv)) {
+    hipSetDevice(hs[0]->c.device);
+    std::vector<Context *> cs(n);
+    std::vector<rhmc_traj_result *> rv(n);
+    for (int i = 0; i < n; i++) {
+      cs[i] = &hs[i]->c;
+      rv[i] = &rs[i];
+    }
+    std::vector<int> st;
+    const int rc = run_trajectory_group(cs, pv, rv, st);
+    if (status)
+      for (int i = 0; i < n; i++) status[i] = st[i];
+    return rc;
+  }
   std::vector<int> rc(n, RHMC_OK);
   std::vector<std::thread> th;
   th.reserve(n);

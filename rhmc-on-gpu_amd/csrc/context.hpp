@@ -125,6 +125,20 @@ int residual_d(Context &c, const double2 *b, int b_group, const double2 *x, cons
 // x32 (PREC_FP32 only): FP32 solutions written there instead of x.
 int multishift_solve(Context &c, const double2 *b, int nrhs, int nshift, const double *shifts, double res,
                      int rel, int prec, int max_iter, double2 *x, int *iters, float2 *x32 = nullptr);
+// one multishift_solve per lattice, run in lockstep on one stream (every
+// context's stream must be the same); rc and iters per lattice in the specs,
+// the first failure (or 0) returned
+struct MsSpec {
+  const double2 *b;
+  int nrhs, nshift;
+  const double *shifts;
+  double res;
+  int rel, prec, max_iter;
+  double2 *x;
+  float2 *x32;
+  int iters, rc;
+};
+int multishift_solve_group(std::vector<Context *> &cs, std::vector<MsSpec> &sp);
 // out[j] = 0.5 * hop(in[j]) to parity par_out, FP32 12-real links
 void launch_hop_s(Context &c, int par_out, const float2 *in, float2 *out, int nvec, long vstride_in,
                   long vstride_out);

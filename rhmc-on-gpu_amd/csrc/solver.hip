@@ -47,7 +47,7 @@ enum Slot {
   S_FH64,
   S_PARTIAL2, S_BASIS, S_BSH, S_HIST, S_COEF, S_TUNE, S_HPART,
   S_FORCE_H, S_FORCE_COEF, S_SOLSUM_COEF,
-  S_RS_SRC, S_RS_Y, S_RS_IDX,
+  S_RS_SRC, S_RS_Y, S_RS_IDX, S_GROUP_TAB,
   S_COUNT
 };
 
@@ -55,7 +55,8 @@ enum Slot {
 // captured with
 struct GraphSlot {
   hipGraphExec_t exec = nullptr;
-  unsigned char key[2048];
+  size_t key_bytes = 0;
+  unsigned char key[4096];
 };
 
 // per (rhs, iteration, shift) of the basis multi-shift solver: omega, zeta, gamma
@@ -63,13 +64,21 @@ struct BmsHist {
   double om, z, ga;
 };
 
+struct DsArgs;
+
 struct Workspace {
-  std::map<unsigned long long, int> tuned_nr;  // Dslash RHS-per-thread choice
+  std::map<unsigned long long, int> tuned_nr;  // Dslash launch shape per (precision, links, mode, red, nvec)
+  std::map<unsigned long long, double> tuned_ms;  // the chosen shape's time per launch when timed here
+  std::string tune_path;                       // tuning cache file ("" = none)
+  std::string arch;                            // GPU architecture, e.g. gfx950
+  bool tune_init = false;                      // cache path from RHMC_TUNE_CACHE looked up and loaded
   std::vector<BmsHist> h_hist;                 // basis solver host arrays (reused)
   std::vector<double> h_coef;
   std::vector<float> h_coef32;
   std::vector<int> h_klen;
   GraphSlot bms_graph[2];                      // FP32 / FP64 stored-basis chunk
+  GraphSlot group_graph[2];                    // the same for a group of lattices led by this context
+  std::vector<DsArgs> group_tab;               // argument tables of the group's chunk (host copy)
   long basis_kmax[2] = {0, 0};                 // most basis iterations a FP32 / FP64 solve needed
   void *buf[S_COUNT] = {};
   size_t cap[S_COUNT] = {};
@@ -81,6 +90,12 @@ struct Workspace {
 
 Workspace *workspace_create(Context &c) {
   Workspace *w = new Workspace();
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, c.device) == hipSuccess) {
+    w->arch = prop.gcnArchName;
+    const size_t colon = w->arch.find(':');
+    if (colon != std::string::npos) w->arch.resize(colon);
+  }
   if (!hip_ok(c, hipMalloc(&w->counter, 64 * sizeof(unsigned)), "hipMalloc counter")) return nullptr;
   if (!hip_ok(c, hipMemsetAsync(w->counter, 0, 64 * sizeof(unsigned), c.stream), "counter")) return nullptr;
   if (!hip_ok(c, hipHostMalloc(&w->h_flags, 4096 * sizeof(int), hipHostMallocDefault), "hipHostMalloc")) return nullptr;
@@ -97,6 +112,8 @@ Workspace *workspace_create(Context &c) {
 void workspace_destroy(Workspace *w) {
   if (!w) return;
   for (auto &g : w->bms_graph)
+    if (g.exec) hipGraphExecDestroy(g.exec);
+  for (auto &g : w->group_graph)
     if (g.exec) hipGraphExecDestroy(g.exec);
   for (int i = 0; i < S_COUNT; i++)
     if (w->buf[i]) hipFree(w->buf[i]);
@@ -581,16 +598,16 @@ __device__ __forceinline__ void clk_max(const unsigned long long *p, int n, unsi
   }
 }
 // first workgroup: the previous launch's end into this launch's slot
-__device__ __forceinline__ void clk_begin(const DsArgs &a) {
-  if (!a.clk_prev || blockIdx.x != 0) return;
+__device__ __forceinline__ void clk_begin(const DsArgs &a, unsigned bid) {
+  if (!a.clk_prev || bid != 0) return;
   unsigned long long *t = clk_slot(a);
   if (t) clk_max(a.clk_prev, a.clk_nprev, t + a.clk_field);
 }
 // every workgroup: its end stamp
-__device__ __forceinline__ void clk_end(const DsArgs &a) {
+__device__ __forceinline__ void clk_end(const DsArgs &a, unsigned bid) {
   if (!a.clk_wg) return;
   __syncthreads();
-  if (threadIdx.x == 0) a.clk_wg[blockIdx.x] = (unsigned long long)wall_clock64();
+  if (threadIdx.x == 0) a.clk_wg[bid] = (unsigned long long)wall_clock64();
 }
 
 // element offset of the iteration-indexed operand
@@ -866,16 +883,18 @@ __device__ __forceinline__ void ds_site_out_bms(const DsArgs &a, const Cx<R> (&a
   }
 }
 
+// One workgroup (bid of nblk) of the one-lane Dslash: the body of k_dslash
+// and of the multi-lattice k_dslash_group.
 template <typename R, int LINK, int NR, int MODE, int RED>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(ds_waves<R, LINK, NR>()))) void k_dslash(DsArgs a) {
+__device__ __forceinline__ void ds_body(const DsArgs &a, unsigned bid, unsigned nblk) {
   using C = Cx<R>;
   if (MODE == DS_HOP && a.bms_ms)
-    for (int j = blockIdx.x; j < a.nvec; j += gridDim.x) bms_step(a, j);  // a small grid takes several
-  clk_begin(a);
+    for (int j = bid; j < a.nvec; j += nblk) bms_step(a, j);  // a small grid takes several
+  clk_begin(a, bid);
   const LatticeDims &L = a.L;
   const long Vh = L.Vh;
   unsigned sb, ch;
-  xcd_split(blockIdx.x, a.nsb, a.nch, sb, ch);
+  xcd_split(bid, a.nsb, a.nch, sb, ch);
   const int jbase = ch * NR;
   __shared__ R bab[2][NR];
   if (MODE == DS_BMS) bms_alpha<R, NR>(a, jbase, sb, bab);
@@ -939,7 +958,12 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(ds_waves
     else ds_site_out<R, NR, MODE, RED>(a, acc, act, red, jbase, h, pinb, bab, rk_ok);
   }
   if (RED != RED_NONE) block_partials<NR>(red, a.red.partial, jbase, a.nvec, a.nsb, sb);
-  clk_end(a);
+  clk_end(a, bid);
+}
+
+template <typename R, int LINK, int NR, int MODE, int RED>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(ds_waves<R, LINK, NR>()))) void k_dslash(DsArgs a) {
+  ds_body<R, LINK, NR, MODE, RED>(a, blockIdx.x, gridDim.x);
 }
 
 // Dslash traversal order.  The hardware hands consecutive workgroups to the
@@ -1050,17 +1074,19 @@ __device__ __forceinline__ C pick3(C a0, C a1, C a2, int k) {
   return r;
 }
 
+// one workgroup (bid of nblk) of the lane-split Dslash (k_dslash_ls and the
+// multi-lattice k_dslash_ls_group)
 template <typename R, int LINK, int NR, int MODE, int RED>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(ds_waves<R, LINK, NR>()))) void k_dslash_ls(DsArgs a) {
+__device__ __forceinline__ void ds_ls_body(const DsArgs &a, unsigned bid, unsigned nblk) {
   using C = Cx<R>;
   if (MODE == DS_HOP && a.bms_ms)
-    for (int j = blockIdx.x; j < a.nvec; j += gridDim.x) bms_step(a, j);  // a small grid takes several
-  clk_begin(a);
+    for (int j = bid; j < a.nvec; j += nblk) bms_step(a, j);  // a small grid takes several
+  clk_begin(a, bid);
   const LatticeDims &L = a.L;
   const long Vh = L.Vh;
   const int Vh32 = (int)Vh;
   unsigned sb, ch;
-  xcd_split(blockIdx.x, a.nsb, a.nch, sb, ch);
+  xcd_split(bid, a.nsb, a.nch, sb, ch);
   const int jbase = ch * NR;
   __shared__ R bab[2][NR];
   if (MODE == DS_BMS) bms_alpha<R, NR>(a, jbase, sb, bab);
@@ -1164,7 +1190,49 @@ __global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(ds_waves
     }
   }
   if (RED != RED_NONE) block_partials<NR>(red, a.red.partial, jbase, a.nvec, a.nsb, sb);
-  clk_end(a);
+  clk_end(a, bid);
+}
+
+template <typename R, int LINK, int NR, int MODE, int RED>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(ds_waves<R, LINK, NR>()))) void k_dslash_ls(DsArgs a) {
+  ds_ls_body<R, LINK, NR, MODE, RED>(a, blockIdx.x, gridDim.x);
+}
+
+// ---- several lattices in one launch.  The grid is the concatenation of
+// every lattice's own grid, each padded to a multiple of 8 workgroups so that
+// a lattice's local workgroup index keeps the XCD mapping of xcd_split
+// (the hardware deals workgroup ids to the 8 XCDs in turn); workgroup b runs
+// lattice l with start[l] <= b < start[l+1] on its arguments tab[l] (read
+// from device memory, uniform over the workgroup).  Padding workgroups exit.
+constexpr int kMaxGroup = 16;
+struct GroupArgs {
+  const DsArgs *tab;
+  unsigned start[kMaxGroup + 1];
+  int n;
+};
+
+__device__ __forceinline__ int group_member(const GroupArgs &g, unsigned b) {
+  int l = 0;
+  while (l + 1 < g.n && b >= g.start[l + 1]) l++;
+  return l;
+}
+
+template <typename R, int LINK, int NR, int MODE, int RED>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(ds_waves<R, LINK, NR>()))) void k_dslash_group(GroupArgs g) {
+  const int l = group_member(g, blockIdx.x);
+  const DsArgs &a = g.tab[l];
+  const unsigned bid = blockIdx.x - g.start[l], nblk = a.nsb * a.nch;
+  if (bid >= nblk) return;
+  ds_body<R, LINK, NR, MODE, RED>(a, bid, nblk);
+}
+
+template <typename R, int LINK, int NR, int MODE, int RED>
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(ds_waves<R, LINK, NR>()))) void k_dslash_ls_group(GroupArgs g) {
+  const int l = group_member(g, blockIdx.x);
+  const DsArgs &a = g.tab[l];
+  const unsigned bid = blockIdx.x - g.start[l], nblk = a.nsb * a.nch;
+  if (bid >= nblk) return;
+  ds_ls_body<R, LINK, NR, MODE, RED>(a, bid, nblk);
 }
 
 // number of right-hand sides handled per thread for a launch with nvec vectors
@@ -1187,22 +1255,166 @@ static unsigned reduce_blocks(Context &c, int ny) {
   return want < nbx ? want : nbx;
 }
 
-// launch shape code: rhs per thread (1,2,4,8) + 16 * lanes per site (1 or 4)
+// Launch shape code:
+//   bits 0-3   right-hand sides per lane (1, 2, 4, 8)
+//   bits 4-7   lanes per site (1, or 4 with the lane-split kernel)
+//   bit  8     tile order: 0 the XCD sub-block order, 1 lattice order
+//   bits 9-11  reducing M^dag M launches: log2 of the kBlock-site tiles per
+//              workgroup (0: reduce_blocks' default of one tile per
+//              workgroup down to 2048 workgroups overall)
 static inline int shape_nr(int code) { return code & 15; }
-static inline int shape_lanes(int code) { return code >> 4; }
+static inline int shape_lanes(int code) { return (code >> 4) & 15; }
+static inline int shape_identity(int code) { return (code >> 8) & 1; }
+static inline int shape_tpw_log2(int code) { return (code >> 9) & 7; }
+static inline int make_shape(int nr, int lanes, int identity, int tpw_log2) {
+  return nr | (lanes << 4) | (identity << 8) | (tpw_log2 << 9);
+}
+
+static unsigned long long tune_key(int real_bytes, int link, int mode, int red, int nvec) {
+  return ((unsigned long long)real_bytes << 40) | ((unsigned long long)link << 36) |
+         ((unsigned long long)mode << 32) | ((unsigned long long)red << 28) | (unsigned long long)nvec;
+}
+
+// ---- tuning cache file: a JSON object with one entry per line,
+//   {"arch": "gfx950", "dims": [nx, ny, nz, nt], "real_bytes": 4, "links": 1,
+//    "mode": 3, "reduction": 2, "nvec": 8, "shape": 530, ...readable fields...}
+// keyed by (arch, dims, real_bytes, links, mode, reduction, nvec).
+static bool json_num(const std::string &s, const char *field, double &v) {
+  const std::string f = std::string("\"") + field + "\":";
+  const size_t p = s.find(f);
+  if (p == std::string::npos) return false;
+  return sscanf(s.c_str() + p + f.size(), " %lf", &v) == 1;
+}
+static std::string json_str(const std::string &s, const char *field) {
+  const std::string f = std::string("\"") + field + "\": \"";
+  const size_t p = s.find(f);
+  if (p == std::string::npos) return "";
+  const size_t e = s.find('"', p + f.size());
+  return e == std::string::npos ? "" : s.substr(p + f.size(), e - p - f.size());
+}
+static std::string dims_json(const LatticeDims &L) {
+  char b[96];
+  snprintf(b, sizeof b, "[%d, %d, %d, %d]", L.nx, L.ny, L.nz, L.nt);
+  return b;
+}
+static std::string tune_entry_json(Context &c, unsigned long long k, int code, double ms) {
+  char b[512];
+  snprintf(b, sizeof b,
+           "{\"arch\": \"%s\", \"dims\": %s, \"real_bytes\": %llu, \"links\": %llu, \"mode\": %llu, "
+           "\"reduction\": %llu, \"nvec\": %llu, \"shape\": %d, \"rhs_per_lane\": %d, \"lanes_per_site\": %d, "
+           "\"tile_order\": \"%s\", \"tiles_per_workgroup\": %d, \"ms\": %.6g}",
+           c.ws->arch.c_str(), dims_json(c.L).c_str(), k >> 40, (k >> 36) & 15, (k >> 32) & 15, (k >> 28) & 15,
+           k & 0xfffffff, code, shape_nr(code), shape_lanes(code), shape_identity(code) ? "lattice" : "xcd",
+           1 << shape_tpw_log2(code), ms);
+  return b;
+}
+static bool entry_key(const std::string &line, std::string &arch, std::string &dims, unsigned long long &k,
+                      int &code) {
+  double rb, li, mo, re, nv, sh;
+  if (!json_num(line, "real_bytes", rb) || !json_num(line, "links", li) || !json_num(line, "mode", mo) ||
+      !json_num(line, "reduction", re) || !json_num(line, "nvec", nv) || !json_num(line, "shape", sh))
+    return false;
+  arch = json_str(line, "arch");
+  const size_t p = line.find("\"dims\": [");
+  const size_t e = p == std::string::npos ? p : line.find(']', p);
+  dims = p == std::string::npos || e == std::string::npos ? "" : line.substr(p + 8, e - p - 7);
+  k = tune_key((int)rb, (int)li, (int)mo, (int)re, (int)nv);
+  code = (int)sh;
+  return true;
+}
+static std::vector<std::string> read_lines(const std::string &path) {
+  std::vector<std::string> out;
+  FILE *f = fopen(path.c_str(), "r");
+  if (!f) return out;
+  char buf[1024];
+  while (fgets(buf, sizeof buf, f)) out.emplace_back(buf);
+  fclose(f);
+  return out;
+}
+
+// shapes stored for this context's GPU architecture and lattice extents
+void tune_cache_load(Context &c) {
+  Workspace *w = c.ws;
+  if (w->tune_path.empty()) return;
+  const std::string mydims = dims_json(c.L);
+  for (const std::string &line : read_lines(w->tune_path)) {
+    std::string arch, dims;
+    unsigned long long k;
+    int code;
+    if (!entry_key(line, arch, dims, k, code) || arch != w->arch || dims != mydims) continue;
+    if (shape_nr(code) < 1 || shape_nr(code) > 8 || (shape_lanes(code) != 1 && shape_lanes(code) != 4)) continue;
+    if (shape_identity(code) == 0 && !c.tiles) continue;
+    w->tuned_nr.emplace(k, code);
+  }
+}
+
+// this context's tuned shapes merged into the cache file (entries of other
+// architectures or extents kept), written to path.tmp and renamed
+void tune_cache_store(Context &c) {
+  Workspace *w = c.ws;
+  if (w->tune_path.empty()) return;
+  const std::string mydims = dims_json(c.L);
+  std::vector<std::string> keep;
+  for (const std::string &line : read_lines(w->tune_path)) {
+    std::string arch, dims;
+    unsigned long long k;
+    int code;
+    if (!entry_key(line, arch, dims, k, code)) continue;
+    if (arch == w->arch && dims == mydims && w->tuned_nr.count(k)) continue;
+    std::string l = line;
+    while (!l.empty() && (l.back() == '\n' || l.back() == ',' || l.back() == ' ')) l.pop_back();
+    keep.push_back(l);
+  }
+  for (auto &kv : w->tuned_nr) {
+    auto m = w->tuned_ms.find(kv.first);
+    keep.push_back(tune_entry_json(c, kv.first, kv.second, m == w->tuned_ms.end() ? 0.0 : m->second));
+  }
+  const std::string tmp = w->tune_path + ".tmp";
+  FILE *f = fopen(tmp.c_str(), "w");
+  if (!f) return;
+  fprintf(f, "{\"rhmc_tuning_cache\": 1, \"entries\": [\n");
+  for (size_t i = 0; i < keep.size(); i++) fprintf(f, "%s%s\n", keep[i].c_str(), i + 1 < keep.size() ? "," : "");
+  fprintf(f, "]}\n");
+  fclose(f);
+  rename(tmp.c_str(), w->tune_path.c_str());
+}
+
+int tune_cache_set(Context &c, const char *path) {
+  c.ws->tune_path = path ? path : "";
+  tune_cache_load(c);
+  return 0;
+}
+
+// workgroups per vector chunk of a reducing launch with 2^k tiles per
+// workgroup (k = 0: reduce_blocks), a multiple of 8 so that the XCD
+// mapping of xcd_split holds
+static unsigned reduce_blocks_tpw(Context &c, int nch, int k) {
+  if (k == 0) return reduce_blocks(c, nch);
+  const unsigned nbx = nbx_of(c);
+  unsigned n = (nbx + (1u << k) - 1) >> k;
+  n = (n + 7) / 8 * 8;
+  return n < nbx ? n : nbx;
+}
+
+// the lattice and grid fields of a launch with shape code
+template <int MODE, int RED>
+static void dslash_prep(Context &c, DsArgs &a, int code) {
+  const int nr = shape_nr(code), lanes = shape_lanes(code) == 4 ? 4 : 1;
+  a.L = c.L;
+  a.tiles = shape_identity(code) ? nullptr : c.tiles;
+  a.ntiles = c.ntiles;
+  a.nch = (unsigned)((a.nvec + nr - 1) / nr);
+  // one workgroup per tile for the hop (its reduction, when any, then has a
+  // partial per tile); fewer, longer workgroups for the other reducing modes
+  a.nsb = RED == RED_NONE || MODE == DS_HOP ? nbx_of(c) * lanes : reduce_blocks_tpw(c, a.nch, shape_tpw_log2(code));
+  a.red.nbr = (int)a.nsb;
+}
 
 template <typename R, int LINK, int MODE, int RED>
 static void dslash_launch(Context &c, DsArgs &a, int code) {
 #define DS_LAUNCH(KERNEL) hipLaunchKernelGGL(KERNEL, grid, dim3(kBlock), 0, c.stream, a)
   const int nr = shape_nr(code), lanes = shape_lanes(code) == 4 ? 4 : 1;
-  a.L = c.L;
-  a.tiles = c.tiles;
-  a.ntiles = c.ntiles;
-  a.nch = (unsigned)((a.nvec + nr - 1) / nr);
-  // one workgroup per tile for the hop (its reduction, when any, then has a
-  // partial per tile); fewer, longer workgroups for the other reducing modes
-  a.nsb = RED == RED_NONE || MODE == DS_HOP ? nbx_of(c) * lanes : reduce_blocks(c, a.nch);
-  a.red.nbr = (int)a.nsb;
+  dslash_prep<MODE, RED>(c, a, code);
   dim3 grid(a.nsb * a.nch);
   if (lanes == 4) {
     switch (nr) {
@@ -1222,57 +1434,71 @@ static void dslash_launch(Context &c, DsArgs &a, int code) {
 #undef DS_LAUNCH
 }
 
-// Autotuner: the launch shape -- right-hand sides per thread (1, 2, 4, 8) and
-// lanes per site (1, or 4 with the lane-split kernel) -- trades link reuse in
-// registers against waves in flight (a single 8-RHS thread per site leaves a
-// 32^3x8 half lattice at ~2 waves per SIMD).  On the first launch of each
-// (precision, links, mode, reduction, nvec) every candidate is timed in place
-// with HIP events and the fastest is kept for the lifetime of the context.
-// The launches are idempotent, except two of the stored-basis solver: its
-// hop advances the systems' scalar state (the candidates skip that step)
-// and its M^dag M updates s in place (the candidates write s into a scratch
-// vector; the r^(k) they write is overwritten by the final launch).
-// RHMC_DSLASH_NR ("4" or "4x4") overrides it.
+// Autotuner.  The launch shape trades link reuse in registers against waves
+// in flight and cache locality:
+// * right-hand sides per lane (1, 2, 4, 8) and lanes per site (1, or 4 with
+//   the lane-split kernel): a single 8-RHS lane per site leaves a 32^3x8
+//   half lattice at ~2 waves per SIMD;
+// * tile order: the XCD sub-block order (build_tile_order) or lattice order;
+// * for the reducing M^dag M launches, the tiles per workgroup (1, 2, 4:
+//   fewer, longer workgroups and fewer reduction partials).
+// On the first launch of each (precision, links, mode, reduction, nvec)
+// every candidate is timed in place with HIP events and the fastest is kept
+// for the lifetime of the context -- and, when a tuning cache file is set
+// (RHMC_TUNE_CACHE or gpu_tune_cache), stored there under (kernel, lattice
+// extents, precision, nvec, GPU architecture) and reused by later contexts
+// and processes without timing.  The launches are idempotent, except two of
+// the stored-basis solver: its hop advances the systems' scalar state (the
+// candidates skip that step) and its M^dag M updates s in place (the
+// candidates write s into a scratch vector; the r^(k) they write is
+// overwritten by the final launch).  RHMC_DSLASH_NR ("4" or "4x4") forces
+// the right-hand sides per lane and lanes per site.
 template <typename R, int LINK, int MODE, int RED>
 static void dslash_dispatch(Context &c, DsArgs &a) {
-  const unsigned long long key = ((unsigned long long)sizeof(R) << 40) | ((unsigned long long)LINK << 36) |
-                                 ((unsigned long long)MODE << 32) | ((unsigned long long)RED << 28) |
-                                 (unsigned long long)a.nvec;
+  const unsigned long long key = tune_key(sizeof(R), LINK, MODE, RED, a.nvec);
+  if (!c.ws->tune_init) {
+    c.ws->tune_init = true;
+    if (const char *p = getenv("RHMC_TUNE_CACHE"))
+      if (c.ws->tune_path.empty()) tune_cache_set(c, p);
+  }
   auto &tab = c.ws->tuned_nr;
   auto it = tab.find(key);
-  int nr;
+  int code;
   if (it != tab.end()) {
-    nr = it->second;
+    code = it->second;
   } else if (const char *env = getenv("RHMC_DSLASH_NR")) {
     // "<rhs per thread>" or "<rhs per thread>x4" for four lanes per site
     int v = std::max(1, std::min(8, atoi(env)));
     v = v >= 8 ? 8 : v >= 4 ? 4 : v >= 2 ? 2 : 1;
-    nr = (strchr(env, 'x') ? 64 : 16) + v;
-    tab[key] = nr;
+    code = make_shape(v, strchr(env, 'x') ? 4 : 1, 0, 0);
+    tab[key] = code;
   } else {
     hipEvent_t e0, e1;
     hipEventCreate(&e0);
     hipEventCreate(&e1);
     float best = 1e30f;
-    nr = 16 + pick_nr(a.nvec);
+    code = make_shape(pick_nr(a.nvec), 1, 0, 0);
     DsArgs t = a;
     t.bms_ms = nullptr;  // the hop's scalar step runs once, in the final launch
     if (MODE == DS_BMS) t.out = ws_get(c, S_TUNE, sizeof(Cx<R>) * (size_t)a.out_vs * a.nvec);
+    const bool reducing = RED != RED_NONE && MODE != DS_HOP;
     for (int lanes = 1; lanes <= 4 && t.out; lanes *= 4)
-      for (int v = 1; v <= 8 && v <= std::max(1, a.nvec); v *= 2) {
-        const int cand = lanes * 16 + v;
-        dslash_launch<R, LINK, MODE, RED>(c, t, cand);
-        hipEventRecord(e0, c.stream);
-        for (int k = 0; k < 3; k++) dslash_launch<R, LINK, MODE, RED>(c, t, cand);
-        hipEventRecord(e1, c.stream);
-        hipEventSynchronize(e1);
-        float t = 0.f;
-        hipEventElapsedTime(&t, e0, e1);
-        if (t < best) {
-          best = t;
-          nr = cand;
-        }
-      }
+      for (int v = 1; v <= 8 && v <= std::max(1, a.nvec); v *= 2)
+        for (int ident = c.tiles ? 0 : 1; ident <= 1; ident++)
+          for (int tpw = 0; tpw <= (reducing ? 2 : 0); tpw++) {
+            const int cand = make_shape(v, lanes, ident, tpw);
+            dslash_launch<R, LINK, MODE, RED>(c, t, cand);
+            hipEventRecord(e0, c.stream);
+            for (int k = 0; k < 3; k++) dslash_launch<R, LINK, MODE, RED>(c, t, cand);
+            hipEventRecord(e1, c.stream);
+            hipEventSynchronize(e1);
+            float ms = 0.f;
+            hipEventElapsedTime(&ms, e0, e1);
+            if (ms < best) {
+              best = ms;
+              code = cand;
+            }
+          }
     hipEventDestroy(e0);
     hipEventDestroy(e1);
     if (c.ws->buf[S_TUNE]) {  // every candidate has finished (event sync above)
@@ -1280,9 +1506,11 @@ static void dslash_dispatch(Context &c, DsArgs &a) {
       c.ws->buf[S_TUNE] = nullptr;
       c.ws->cap[S_TUNE] = 0;
     }
-    tab[key] = nr;
+    tab[key] = code;
+    c.ws->tuned_ms[key] = best / 3.0;
+    tune_cache_store(c);
   }
-  dslash_launch<R, LINK, MODE, RED>(c, a, nr);
+  dslash_launch<R, LINK, MODE, RED>(c, a, code);
 }
 
 constexpr int kChunk = 8;  // stored-basis solver iterations between host polls
@@ -1365,15 +1593,15 @@ std::string tuning_table(Context &c) {
   std::string out = "[";
   bool first = true;
   for (auto &kv : c.ws->tuned_nr) {
-    char buf[240];
+    char buf[320];
     const unsigned long long k = kv.first;
     snprintf(buf, sizeof buf,
              "%s{\"real_bytes\":%llu,\"links\":\"%s\",\"mode\":%llu,\"reduction\":%llu,\"nvec\":%llu,"
-             "\"rhs_per_thread\":%d,\"lanes_per_site\":%d}",
+             "\"rhs_per_thread\":%d,\"lanes_per_site\":%d,\"tile_order\":\"%s\",\"tiles_per_workgroup\":%d}",
              first ? "" : ",", k >> 40,
              ((k >> 36) & 15) == LINK_S12 ? "fp32_12" : ((k >> 36) & 15) == LINK_D12 ? "fp64_12" : "fp64_18",
-             (k >> 32) & 15, (k >> 28) & 15,
-             k & 0xfffffff, shape_nr(kv.second), shape_lanes(kv.second));
+             (k >> 32) & 15, (k >> 28) & 15, k & 0xfffffff, shape_nr(kv.second), shape_lanes(kv.second),
+             shape_identity(kv.second) ? "lattice" : "xcd", 1 << shape_tpw_log2(kv.second));
     out += buf;
     first = false;
   }
@@ -2031,15 +2259,42 @@ struct BasisRestart {
   std::vector<double> zeta;     // [rhs.size()][nshift]: zeta_s^(K) of the running shifts, 0 for frozen ones
 };
 
+// The first pass of a stored-basis solve when it was already run (a
+// multi-lattice solve): its status, continuation state and iterations.
+struct FirstPass {
+  int rc = 0, iters = 0;
+  BasisRestart rs;
+};
+
 // multi-shift CG with the stored residual basis; x (FP64) receives
 // [nrhs][nshift] solutions.  Returns kBasisFull when not even kMinBasis
 // slots fit (the caller then runs the explicit-shift algorithm) and
 // kBasisRestart, with rs filled, when systems were still running when the
 // basis filled.
+// One stored-basis solve on one lattice, split into phases so that several
+// lattices can run their iterations in lockstep (bms_group_iterate): set-up
+// (buffers, the right-hand sides into basis slot 0, the scalar state), the
+// iterations (chunks of kChunk, replayed as HIP graphs) and the assembly.
+struct BasisRun {
+  int nrhs = 0, nshift = 0, max_iter = 0;
+  long cap = 0;
+  void *basis = nullptr, *sr = nullptr, *hh = nullptr;
+  MsState *ms = nullptr;
+  BmsShared *bsh = nullptr;
+  BmsHist *hist = nullptr;
+  double *diag = nullptr, *partC = nullptr, *partH = nullptr;
+  std::vector<MsState> hs;
+  BmsArgs a = {};
+  DsArgs hop = {}, mdm = {};
+  unsigned nbr = 0;
+  bool timed = false;
+};
+
+// buffers, scalar state and the first launches; the iteration counter
+// iterc (device) is read by every launch of the iterations
 template <typename R, int LINK>
-static int ms_cg_basis(Context &c, const double2 *b, int nrhs, int nshift, const double *shifts,
-                       const double *res_abs, int max_iter, double2 *x, int *iters_out, float2 *x32,
-                       bool asm_fp32, BasisRestart *rs) {
+static int bms_setup(Context &c, BasisRun &br, const double2 *b, int nrhs, int nshift, const double *shifts,
+                     const double *res_abs, int max_iter, int *iterc) {
   using C = Cx<R>;
   const long Vh = c.Vh;
   const size_t vb = sizeof(C) * 3 * Vh;
@@ -2054,63 +2309,70 @@ static int ms_cg_basis(Context &c, const double2 *b, int nrhs, int nshift, const
   long cap = std::min<long>(std::min<long>((long)max_iter + 2, kMaxSlots), std::min<long>(budget, want));
   if (const char *e = getenv("RHMC_BASIS_CAP")) cap = std::min<long>(cap, std::max(atol(e), kMinBasis));
   if (cap < kMinBasis) return kBasisFull;
-  void *basis = ws_get(c, S_BASIS, slot_bytes * cap);
-  void *sr = ws_get(c, dbl ? S_P64 : S_P32, vb * nrhs);   // s = A p recurrence
-  void *hh = ws_get(c, dbl ? S_H64 : S_H32, vb * nrhs);
-  MsState *ms = (MsState *)ws_get(c, S_MS, sizeof(MsState) * nrhs);
-  BmsShared *bsh = (BmsShared *)ws_get(c, S_BSH, sizeof(BmsShared) * nrhs);
-  BmsHist *hist = (BmsHist *)ws_get(c, S_HIST, sizeof(BmsHist) * nrhs * cap * nshift);
-  double *diag = (double *)ws_get(c, S_DIAG, sizeof(double) * 256);
+  br.nrhs = nrhs;
+  br.nshift = nshift;
+  br.max_iter = max_iter;
+  br.cap = cap;
+  br.basis = ws_get(c, S_BASIS, slot_bytes * cap);
+  br.sr = ws_get(c, dbl ? S_P64 : S_P32, vb * nrhs);  // s = A p recurrence
+  br.hh = ws_get(c, dbl ? S_H64 : S_H32, vb * nrhs);
+  br.ms = (MsState *)ws_get(c, S_MS, sizeof(MsState) * nrhs);
+  br.bsh = (BmsShared *)ws_get(c, S_BSH, sizeof(BmsShared) * nrhs);
+  br.hist = (BmsHist *)ws_get(c, S_HIST, sizeof(BmsHist) * nrhs * cap * nshift);
+  br.diag = (double *)ws_get(c, S_DIAG, sizeof(double) * 256);
   // partials of |r|^2 (M^dag M: at most nbx per rhs) and of |h|^2 (hop: one
   // per tile, at most 4 nbx per rhs with four lanes per site)
-  double *partC = (double *)ws_get(c, S_PARTIAL2, sizeof(double) * nbx_of(c) * std::max(nrhs, 1));
-  double *partH = (double *)ws_get(c, S_HPART, sizeof(double) * 4 * nbx_of(c) * std::max(nrhs, 1));
-  if (!basis || !sr || !hh || !ms || !bsh || !hist || !diag || !partC || !partH) return -1;
+  br.partC = (double *)ws_get(c, S_PARTIAL2, sizeof(double) * nbx_of(c) * std::max(nrhs, 1));
+  br.partH = (double *)ws_get(c, S_HPART, sizeof(double) * 4 * nbx_of(c) * std::max(nrhs, 1));
+  if (!br.basis || !br.sr || !br.hh || !br.ms || !br.bsh || !br.hist || !br.diag || !br.partC || !br.partH)
+    return -1;
 
-  std::vector<MsState> hs(nrhs);
+  br.hs.assign(nrhs, MsState());
   for (int j = 0; j < nrhs; j++) {
-    MsState &st = hs[j];
+    MsState &st = br.hs[j];
     memset(&st, 0, sizeof(st));
     st.nshift = nshift;
     st.res = res_abs[j];
     st.max_iter = max_iter;
     for (int k = 0; k < nshift; k++) st.shift[k] = shifts[k];
   }
-  hipMemcpyAsync(ms, hs.data(), sizeof(MsState) * nrhs, hipMemcpyHostToDevice, c.stream);
+  hipMemcpyAsync(br.ms, br.hs.data(), sizeof(MsState) * nrhs, hipMemcpyHostToDevice, c.stream);
   std::vector<double> hd(nrhs, c.mass * c.mass);
-  hipMemcpyAsync(diag, hd.data(), sizeof(double) * nrhs, hipMemcpyHostToDevice, c.stream);
+  hipMemcpyAsync(br.diag, hd.data(), sizeof(double) * nrhs, hipMemcpyHostToDevice, c.stream);
 
-  BmsArgs a = {};
+  BmsArgs &a = br.a;
+  a = BmsArgs{};
   a.Vh = Vh; a.nvec = nrhs; a.nshift = nshift; a.cap = (int)cap;
-  a.b = b; a.basis = basis;
-  a.partC = partC;
-  a.sh = bsh; a.ms = ms;
+  a.b = b; a.basis = br.basis;
+  a.partC = br.partC;
+  a.sh = br.bsh; a.ms = br.ms;
   a.active = active_ptr(c);
-  const unsigned nbr = reduce_blocks(c, nrhs);
-  a.nbrC = (int)nbr;
-  hipLaunchKernelGGL(k_bms_init<R>, dim3(nbr, nrhs), dim3(kBlock), 0, c.stream, a);
+  br.nbr = reduce_blocks(c, nrhs);
+  a.nbrC = (int)br.nbr;
+  hipLaunchKernelGGL(k_bms_init<R>, dim3(br.nbr, nrhs), dim3(kBlock), 0, c.stream, a);
   hipLaunchKernelGGL(k_bms_init_fin, dim3(1), dim3(kBlock), 0, c.stream, a);
 
   // iteration k applies the operator to basis slot k-1 (chosen on the device)
-  int *iterc = iter_ptr(c);
-  DsArgs hop = {};
+  DsArgs &hop = br.hop;
+  hop = DsArgs{};
   hop.par_out = 1; hop.U = link_ptr<LINK>(c);
-  hop.in = basis; hop.in_vs = 3 * Vh; hop.out = hh; hop.out_vs = 3 * Vh; hop.nvec = nrhs;
+  hop.in = br.basis; hop.in_vs = 3 * Vh; hop.out = br.hh; hop.out_vs = 3 * Vh; hop.nvec = nrhs;
   hop.skip_stride = sizeof(BmsShared) / sizeof(int);
   hop.iterp = iterc; hop.iter_stride = (long)nrhs * 3 * Vh; hop.iter_in = 1;
-  hop.bms = bsh; hop.bms_part = partC; hop.bms_nbr = 0;  // set from M^dag M's grid (unused at k = 1)
-  hop.bms_ms = ms; hop.bms_hist = hist; hop.bms_active = active_ptr(c);
+  hop.bms = br.bsh; hop.bms_part = br.partC; hop.bms_nbr = 0;  // set from M^dag M's grid (unused at k = 1)
+  hop.bms_ms = br.ms; hop.bms_hist = br.hist; hop.bms_active = active_ptr(c);
   hop.bms_cap = (int)cap; hop.bms_nshift = nshift;
   hop.red = make_red(c, nrhs, EPI_STORE, nullptr, nullptr);
-  hop.red.partial = partH;
+  hop.red.partial = br.partH;
   // live timing of the MD solver's hop (FP32, all pseudofermions)
-  const bool timed = c.time_kernels && c.kt_clk && sizeof(R) == 4 && nrhs == c.n_pf;
-  DsArgs mdm = hop;
-  mdm.par_out = 0; mdm.in = hh; mdm.in_vs = 3 * Vh; mdm.out = sr; mdm.pin = basis; mdm.pin_vs = 3 * Vh;
-  mdm.diag = diag; mdm.iter_in = 0; mdm.iter_pin = 1;
-  mdm.bms_ms = nullptr; mdm.bms_part = nullptr; mdm.bms_hpart = partH;
-  mdm.red.partial = partC;
-  if (timed) {
+  br.timed = c.time_kernels && c.kt_clk && sizeof(R) == 4 && nrhs == c.n_pf && iterc == iter_ptr(c);
+  DsArgs &mdm = br.mdm;
+  mdm = hop;
+  mdm.par_out = 0; mdm.in = br.hh; mdm.in_vs = 3 * Vh; mdm.out = br.sr; mdm.pin = br.basis; mdm.pin_vs = 3 * Vh;
+  mdm.diag = br.diag; mdm.iter_in = 0; mdm.iter_pin = 1;
+  mdm.bms_ms = nullptr; mdm.bms_part = nullptr; mdm.bms_hpart = br.partH;
+  mdm.red.partial = br.partC;
+  if (br.timed) {
     // end stamps of M^dag M (at most nbx x nrhs workgroups) and of the hop
     // (at most 4 nbx x nrhs): each launch reads the other's
     const size_t nm = (size_t)nbx_of(c) * nrhs;
@@ -2122,83 +2384,111 @@ static int ms_cg_basis(Context &c, const double2 *b, int nrhs, int nshift, const
     hop.clk_prev = mdm.clk_wg; hop.clk_field = 0; hop.clk_nprev = 0;
     mdm.clk_prev = hop.clk_wg; mdm.clk_field = 1;
   }
+  return 0;
+}
 
-  // Iterations run in chunks of kChunk with the iteration number read on the
-  // device (base + offset), so a chunk is a fixed launch sequence: the first
-  // chunk of a solve runs eagerly (the Dslash autotuner may time candidates
-  // there), later chunks replay a captured graph of it.  Iterations past the
-  // basis capacity are no-ops: the hop stops a system before slot cap.
-  hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, c.stream, iterc, 1);
-  auto chunk_launch = [&]() {
-    for (int q = 0; q < kChunk; q++) {
-      hop.skip = &bsh[0].stop[(1 + q) & 1];
-      mdm.skip = hop.skip;
-      hop.iter_off = q - 1;
-      mdm.iter_off = q - 1;
-      dslash_dispatch<R, LINK, DS_HOP, RED_NORM>(c, hop);
-      mdm.bms_hnbr = hop.red.nbr;
-      mdm.clk_nprev = (int)(hop.nsb * hop.nch);
-      dslash_dispatch<R, LINK, DS_BMS, RED_NORM>(c, mdm);
-      hop.bms_nbr = mdm.red.nbr;
-      hop.clk_nprev = (int)(mdm.nsb * mdm.nch);
+// A chunk is a fixed launch sequence: the first chunk of a solve runs
+// eagerly (the Dslash autotuner may time candidates there), later chunks
+// replay a captured graph of it, re-captured when a launch argument (key)
+// changes.  An executable graph is destroyed only with nothing left in
+// flight on the stream, so no earlier launch of it can still be pending.
+template <typename Launch>
+static bool chunk_run(Context &c, GraphSlot &gs, bool eager, const void *key, size_t key_bytes, Launch &&launch) {
+  if (eager || !c.use_graphs) {
+    launch();
+    return true;
+  }
+  if (key_bytes > sizeof(gs.key)) {
+    c.last_error = "solver graph key too large";
+    return false;
+  }
+  if (!gs.exec || gs.key_bytes != key_bytes || memcmp(key, gs.key, key_bytes) != 0) {
+    if (gs.exec) {
+      if (!hip_ok(c, hipStreamSynchronize(c.stream), "graph retire")) return false;
+      hipGraphExecDestroy(gs.exec);
     }
-    hipLaunchKernelGGL(k_add_int, dim3(1), dim3(64), 0, c.stream, iterc, kChunk);
-  };
-  int k = 1;
-  GraphSlot &gs = c.ws->bms_graph[sizeof(R) == 8 ? 1 : 0];
-  // one chunk: eager for the first (autotuning), else the captured graph
-  auto launch_chunk = [&]() -> bool {
-    if (k == 1 || !c.use_graphs) {
-      chunk_launch();
-      return true;
-    }
-    BmsGraphKey key;
-    memset(&key, 0, sizeof(key));
-    key.a = a; key.hop = hop; key.mdm = mdm; key.nbr = nbr;
-    key.a.b = nullptr;  // the right-hand sides are read only by k_bms_init, outside the chunk
-    if (!gs.exec || memcmp(&key, &gs.key, sizeof(key)) != 0) {
-      // An executable graph is destroyed only with nothing left in flight
-      // on the stream: no earlier launch of it (from this solve's previous
-      // chunk or an earlier solve) can still be pending when it goes.
-      if (gs.exec) {
-        if (!hip_ok(c, hipStreamSynchronize(c.stream), "graph retire")) return false;
-        hipGraphExecDestroy(gs.exec);
-      }
+    gs.exec = nullptr;
+    c.graph_captures++;
+    hipGraph_t g = nullptr;
+    if (!hip_ok(c, hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal), "capture")) return false;
+    launch();
+    if (!hip_ok(c, hipStreamEndCapture(c.stream, &g), "capture end")) return false;
+    const bool ok = hip_ok(c, hipGraphInstantiate(&gs.exec, g, nullptr, nullptr, 0), "instantiate");
+    hipGraphDestroy(g);
+    if (!ok) {
       gs.exec = nullptr;
-      c.graph_captures++;
-      hipGraph_t g = nullptr;
-      if (!hip_ok(c, hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal), "capture")) return false;
-      chunk_launch();
-      if (!hip_ok(c, hipStreamEndCapture(c.stream, &g), "capture end")) return false;
-      const bool ok = hip_ok(c, hipGraphInstantiate(&gs.exec, g, nullptr, nullptr, 0), "instantiate");
-      hipGraphDestroy(g);
-      if (!ok) { gs.exec = nullptr; return false; }
-      memcpy(&gs.key, &key, sizeof(key));
+      return false;
     }
-    return hip_ok(c, hipGraphLaunch(gs.exec, c.stream), "graph launch");
-  };
-  // The host reads each chunk's running-system count (copied behind the
-  // chunk, stream-ordered before the next chunk resets it) while the next
-  // chunk already runs: the poll's round trip overlaps a chunk instead of
-  // idling the GPU.  A chunk launched after every system has stopped does
-  // no work (stop flags are checked by every launch; nothing is written at
-  // or past the basis capacity).
+    memcpy(gs.key, key, key_bytes);
+    gs.key_bytes = key_bytes;
+  }
+  return hip_ok(c, hipGraphLaunch(gs.exec, c.stream), "graph launch");
+}
+
+// Chunks until every system of every run has stopped or the iteration
+// count passes the largest capacity.  The host reads each chunk's
+// running-system counts (copied behind the chunk, stream-ordered before the
+// next chunk resets them) while the next chunk already runs, so the poll's
+// round trip overlaps a chunk instead of idling the GPU.  A chunk launched
+// after a system has stopped does no work for it (stop flags are checked by
+// every launch; nothing is written at or past a basis capacity).
+template <typename Chunk>
+static int bms_iterate(Context &c, int *iterc, const std::vector<const int *> &active, long cap_max, Chunk &&chunk) {
+  hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, c.stream, iterc, 1);
+  const int n = (int)active.size();
+  if (2 * n > 4096) return -1;
   int *hf = c.ws->h_flags;
   hipEvent_t *pe = c.ws->poll_ev;
+  int k = 1;
   auto launch_and_poll = [&](int slot) -> bool {
-    if (!launch_chunk()) return false;
+    if (!chunk(k == 1)) return false;
     k += kChunk;
-    hipMemcpyAsync(hf + slot, a.active + (k & 1), sizeof(int), hipMemcpyDeviceToHost, c.stream);
+    for (int l = 0; l < n; l++)
+      hipMemcpyAsync(hf + slot * n + l, active[l] + (k & 1), sizeof(int), hipMemcpyDeviceToHost, c.stream);
     return hip_ok(c, hipEventRecord(pe[slot], c.stream), "poll event");
   };
   if (!launch_and_poll(0)) return -1;
-  for (int n = 1;; n++) {
-    const bool more = k < cap;
-    if (more && !launch_and_poll(n & 1)) return -1;
-    if (!hip_ok(c, hipEventSynchronize(pe[(n - 1) & 1]), "poll")) return -1;
-    if (hf[(n - 1) & 1] == 0 || !more) break;
+  for (int it = 1;; it++) {
+    const bool more = k < cap_max;
+    if (more && !launch_and_poll(it & 1)) return -1;
+    if (!hip_ok(c, hipEventSynchronize(pe[(it - 1) & 1]), "poll")) return -1;
+    int running = 0;
+    for (int l = 0; l < n; l++) running += hf[((it - 1) & 1) * n + l];
+    if (running == 0 || !more) break;
   }
-  hipMemcpyAsync(hs.data(), ms, sizeof(MsState) * nrhs, hipMemcpyDeviceToHost, c.stream);
+  return 0;
+}
+
+// the chunk of one lattice: kChunk iterations of the two launches
+template <typename R, int LINK>
+static void bms_chunk(Context &c, BasisRun &br, int *iterc) {
+  DsArgs &hop = br.hop, &mdm = br.mdm;
+  for (int q = 0; q < kChunk; q++) {
+    hop.skip = &br.bsh[0].stop[(1 + q) & 1];
+    mdm.skip = hop.skip;
+    hop.iter_off = q - 1;
+    mdm.iter_off = q - 1;
+    dslash_dispatch<R, LINK, DS_HOP, RED_NORM>(c, hop);
+    mdm.bms_hnbr = hop.red.nbr;
+    mdm.clk_nprev = (int)(hop.nsb * hop.nch);
+    dslash_dispatch<R, LINK, DS_BMS, RED_NORM>(c, mdm);
+    hop.bms_nbr = mdm.red.nbr;
+    hop.clk_nprev = (int)(mdm.nsb * mdm.nch);
+  }
+  hipLaunchKernelGGL(k_add_int, dim3(1), dim3(64), 0, c.stream, iterc, kChunk);
+}
+
+// host state after the iterations, continuation bookkeeping, coefficients
+// and the assembly of the shifted solutions
+template <typename R, int LINK>
+static int bms_finish(Context &c, BasisRun &br, double2 *x, int *iters_out, float2 *x32, bool asm_fp32,
+                      BasisRestart *rs) {
+  const long Vh = c.Vh;
+  const bool dbl = sizeof(R) == 8;
+  const int nrhs = br.nrhs, nshift = br.nshift;
+  const long cap = br.cap;
+  std::vector<MsState> &hs = br.hs;
+  hipMemcpyAsync(hs.data(), br.ms, sizeof(MsState) * nrhs, hipMemcpyDeviceToHost, c.stream);
   if (!hip_ok(c, hipStreamSynchronize(c.stream), "basis multishift")) return -1;
   int kmax = 0;
   for (int j = 0; j < nrhs; j++) kmax = std::max(kmax, hs[j].iter);
@@ -2206,7 +2496,7 @@ static int ms_cg_basis(Context &c, const double2 *b, int nrhs, int nshift, const
   kseen = std::max<long>(kseen, kmax);
   // the hop ran with work at iterations 1..kmax+1 (input slots 0..kmax);
   // slot kmax+1 holds the end of M^dag M(kmax+1) when a hop followed it
-  if (timed) kernel_timing_harvest(c, kmax + 1);
+  if (br.timed) kernel_timing_harvest(c, kmax + 1);
   for (int j = 0; j < nrhs; j++)
     if (hs[j].nonfinite) {
       c.last_error = "multishift: non-finite value in the solver reductions";
@@ -2218,12 +2508,12 @@ static int ms_cg_basis(Context &c, const double2 *b, int nrhs, int nshift, const
     rs->rhs.clear();
     rs->slot.clear();
     rs->zeta.clear();
-    rs->basis = basis;
+    rs->basis = br.basis;
     rs->nrhs = nrhs;
   }
   bool restart = false;
   for (int j = 0; j < nrhs; j++)
-    if (hs[j].nact > 0 && hs[j].iter + 2 >= cap && hs[j].iter < max_iter) {
+    if (hs[j].nact > 0 && hs[j].iter + 2 >= cap && hs[j].iter < br.max_iter) {
       restart = true;
       if (rs) {
         rs->rhs.push_back(j);
@@ -2242,7 +2532,7 @@ static int ms_cg_basis(Context &c, const double2 *b, int nrhs, int nshift, const
   std::vector<BmsHist> &hh_host = c.ws->h_hist;
   hh_host.resize((size_t)nrhs * hst * nshift);
   for (int j = 0; j < nrhs; j++)
-    hipMemcpyAsync(hh_host.data() + (size_t)j * hst * nshift, hist + (size_t)j * cap * nshift,
+    hipMemcpyAsync(hh_host.data() + (size_t)j * hst * nshift, br.hist + (size_t)j * cap * nshift,
                    sizeof(BmsHist) * (size_t)(kmax + 1) * nshift, hipMemcpyDeviceToHost, c.stream);
   if (!hip_ok(c, hipStreamSynchronize(c.stream), "basis history")) return -1;
   std::vector<double> &coef = c.ws->h_coef;
@@ -2284,10 +2574,10 @@ static int ms_cg_basis(Context &c, const double2 *b, int nrhs, int nshift, const
     hipMemcpyAsync(dkl, klen.data(), sizeof(int) * nrhs, hipMemcpyHostToDevice, c.stream);
     const dim3 grid((unsigned)((3 * Vh + 32 * (kBlock / 64) - 1) / (32 * (kBlock / 64))), nrhs, ng);
     if (x32)
-      hipLaunchKernelGGL(k_bms_assemble_mx<float2>, grid, dim3(kBlock), 0, c.stream, (const float2 *)basis, dcf,
+      hipLaunchKernelGGL(k_bms_assemble_mx<float2>, grid, dim3(kBlock), 0, c.stream, (const float2 *)br.basis, dcf,
                          dkl, x32, Vh, (int)kp, nshift, nrhs);
     else
-      hipLaunchKernelGGL(k_bms_assemble_mx<double2>, grid, dim3(kBlock), 0, c.stream, (const float2 *)basis, dcf,
+      hipLaunchKernelGGL(k_bms_assemble_mx<double2>, grid, dim3(kBlock), 0, c.stream, (const float2 *)br.basis, dcf,
                          dkl, x, Vh, (int)kp, nshift, nrhs);
   } else {
     double *dcoef = (double *)ws_get(c, S_COEF, sizeof(double) * coef.size() + sizeof(int) * 256);
@@ -2298,10 +2588,10 @@ static int ms_cg_basis(Context &c, const double2 *b, int nrhs, int nshift, const
     const dim3 grid((unsigned)((3 * Vh + kBlock - 1) / kBlock), nrhs);
     for (int s0 = 0; s0 < nshift; s0 += 16)
       if (x32)
-        hipLaunchKernelGGL((k_bms_assemble<R, 16, float2>), grid, dim3(kBlock), 0, c.stream, basis, dcoef, dklen,
+        hipLaunchKernelGGL((k_bms_assemble<R, 16, float2>), grid, dim3(kBlock), 0, c.stream, br.basis, dcoef, dklen,
                            x32, Vh, (int)cst, nshift, s0, nrhs);
       else
-        hipLaunchKernelGGL((k_bms_assemble<R, 16, double2>), grid, dim3(kBlock), 0, c.stream, basis, dcoef, dklen,
+        hipLaunchKernelGGL((k_bms_assemble<R, 16, double2>), grid, dim3(kBlock), 0, c.stream, br.basis, dcoef, dklen,
                            x, Vh, (int)cst, nshift, s0, nrhs);
   }
   // stream-ordered for the caller; the host arrays stay alive in the workspace
@@ -2309,10 +2599,175 @@ static int ms_cg_basis(Context &c, const double2 *b, int nrhs, int nshift, const
   *iters_out = kmax;
   if (restart) return kBasisRestart;
   for (int j = 0; j < nrhs; j++)
-    if (hs[j].iter >= max_iter && hs[j].nact > 0) {
+    if (hs[j].iter >= br.max_iter && hs[j].nact > 0) {
       c.last_error = "multishift: max_cg reached";
       return kStatusMaxCg;
     }
+  return 0;
+}
+
+// multi-shift CG with the stored residual basis on one lattice; x (FP64)
+// receives [nrhs][nshift] solutions (or x32 in FP32).  Returns kBasisFull
+// when not even kMinBasis slots fit (the caller then runs the
+// explicit-shift algorithm) and kBasisRestart, with rs filled, when systems
+// were still running when the basis filled.
+template <typename R, int LINK>
+static int ms_cg_basis(Context &c, const double2 *b, int nrhs, int nshift, const double *shifts,
+                       const double *res_abs, int max_iter, double2 *x, int *iters_out, float2 *x32,
+                       bool asm_fp32, BasisRestart *rs) {
+  BasisRun br;
+  int *iterc = iter_ptr(c);
+  const int rc = bms_setup<R, LINK>(c, br, b, nrhs, nshift, shifts, res_abs, max_iter, iterc);
+  if (rc) return rc;
+  GraphSlot &gs = c.ws->bms_graph[sizeof(R) == 8 ? 1 : 0];
+  BmsGraphKey key;
+  auto chunk = [&](bool eager) -> bool {
+    memset(&key, 0, sizeof(key));
+    key.a = br.a; key.hop = br.hop; key.mdm = br.mdm; key.nbr = br.nbr;
+    key.a.b = nullptr;  // the right-hand sides are read only by k_bms_init, outside the chunk
+    return chunk_run(c, gs, eager, &key, sizeof(key), [&]() { bms_chunk<R, LINK>(c, br, iterc); });
+  };
+  const std::vector<const int *> active = {br.a.active};
+  if (bms_iterate(c, iterc, active, br.cap, chunk)) return -1;
+  return bms_finish<R, LINK>(c, br, x, iters_out, x32, asm_fp32, rs);
+}
+
+// ---- several lattices in lockstep
+// One solve per lattice (each its own right-hand sides, shifts and
+// targets), all advanced together: each solver step is one hop launch and
+// one M^dag M launch covering every lattice's tiles (k_dslash_group, a
+// per-lattice argument table in device memory), chunks of kChunk steps are
+// captured as one HIP graph, and the host polls every lattice's running
+// count per chunk.  Lattices whose systems have stopped cost only their
+// workgroups' early exit.  Every context must be on one stream (the
+// caller's group stream).  The launch shape is fixed (no timing): right-hand
+// sides per lane from the smallest number of systems, four lanes per site
+// while the lattices together have fewer than 2048 tiles (small lattices
+// need the extra waves), else one.
+struct GroupJob {
+  const double2 *b;
+  int nrhs, nshift;
+  const double *shifts;
+  const double *res_abs;
+  int max_iter;
+  double2 *x;
+  float2 *x32;
+  bool asm_fp32;
+  FirstPass out;  // status, continuation state, iterations of this lattice
+};
+
+static unsigned long long fnv1a(const void *p, size_t n, unsigned long long h = 1469598103934665603ULL) {
+  const unsigned char *b = (const unsigned char *)p;
+  for (size_t i = 0; i < n; i++) h = (h ^ b[i]) * 1099511628211ULL;
+  return h;
+}
+
+template <typename R, int LINK, int MODE, int RED>
+static void dslash_group_launch(hipStream_t st, const DsArgs *dtab, const DsArgs *htab, int n, int code) {
+  GroupArgs g;
+  g.tab = dtab;
+  g.n = n;
+  unsigned s = 0;
+  for (int l = 0; l < n; l++) {
+    g.start[l] = s;
+    s += (htab[l].nsb * htab[l].nch + 7) / 8 * 8;
+  }
+  g.start[n] = s;
+  const dim3 grid(s);
+  const int nr = shape_nr(code);
+#define DSG(KERNEL) hipLaunchKernelGGL(KERNEL, grid, dim3(kBlock), 0, st, g)
+  if (shape_lanes(code) == 4) {
+    switch (nr) {
+      case 8: DSG((k_dslash_ls_group<R, LINK, 8, MODE, RED>)); break;
+      case 4: DSG((k_dslash_ls_group<R, LINK, 4, MODE, RED>)); break;
+      case 2: DSG((k_dslash_ls_group<R, LINK, 2, MODE, RED>)); break;
+      default: DSG((k_dslash_ls_group<R, LINK, 1, MODE, RED>)); break;
+    }
+  } else {
+    switch (nr) {
+      case 8: DSG((k_dslash_group<R, LINK, 8, MODE, RED>)); break;
+      case 4: DSG((k_dslash_group<R, LINK, 4, MODE, RED>)); break;
+      case 2: DSG((k_dslash_group<R, LINK, 2, MODE, RED>)); break;
+      default: DSG((k_dslash_group<R, LINK, 1, MODE, RED>)); break;
+    }
+  }
+#undef DSG
+}
+
+// Returns kBasisFull when some lattice's basis does not fit (nothing has run
+// then beyond set-up; the caller solves the lattices one by one), -1 on a
+// HIP error, else 0 with each job's first-pass status in jobs[l].out.
+template <typename R, int LINK>
+static int ms_cg_basis_group(std::vector<Context *> &cs, std::vector<GroupJob> &jobs) {
+  const int n = (int)cs.size();
+  if (n < 1 || n > kMaxGroup) return kBasisFull;
+  Context &c0 = *cs[0];
+  int *iterc = iter_ptr(c0);
+  std::vector<BasisRun> runs(n);
+  int nmin = 1 << 30;
+  long cap_max = 0, tiles = 0;
+  for (int l = 0; l < n; l++) {
+    GroupJob &j = jobs[l];
+    const int rc = bms_setup<R, LINK>(*cs[l], runs[l], j.b, j.nrhs, j.nshift, j.shifts, j.res_abs, j.max_iter, iterc);
+    if (rc) return rc;
+    BasisRun &br = runs[l];
+    br.timed = false;
+    for (DsArgs *d : {&br.hop, &br.mdm}) {
+      d->clk = nullptr;
+      d->clk_wg = nullptr;
+      d->clk_prev = nullptr;
+    }
+    nmin = std::min(nmin, j.nrhs);
+    cap_max = std::max(cap_max, br.cap);
+    tiles += nbx_of(*cs[l]);
+  }
+  const int lanes = tiles < 2048 ? 4 : 1;
+  const int code = make_shape(pick_nr(nmin), lanes, 0, 0);
+  // argument tables of the kChunk steps: [q][hop, M^dag M][lattice]
+  std::vector<DsArgs> &htab = c0.ws->group_tab;
+  htab.assign((size_t)kChunk * 2 * n, DsArgs{});
+  for (int q = 0; q < kChunk; q++)
+    for (int l = 0; l < n; l++) {
+      BasisRun &br = runs[l];
+      DsArgs hop = br.hop, mdm = br.mdm;
+      const int cl = cs[l]->tiles ? code : make_shape(pick_nr(nmin), lanes, 1, 0);
+      hop.skip = &br.bsh[0].stop[(1 + q) & 1];
+      mdm.skip = hop.skip;
+      hop.iter_off = q - 1;
+      mdm.iter_off = q - 1;
+      dslash_prep<DS_HOP, RED_NORM>(*cs[l], hop, cl);
+      dslash_prep<DS_BMS, RED_NORM>(*cs[l], mdm, cl);
+      mdm.bms_hnbr = hop.red.nbr;
+      hop.bms_nbr = mdm.red.nbr;  // the hop of step k reads the |r|^2 partials of M^dag M at k-1
+      htab[((size_t)q * 2 + 0) * n + l] = hop;
+      htab[((size_t)q * 2 + 1) * n + l] = mdm;
+    }
+  DsArgs *dtab = (DsArgs *)ws_get(c0, S_GROUP_TAB, sizeof(DsArgs) * htab.size());
+  if (!dtab) return -1;
+  hipMemcpyAsync(dtab, htab.data(), sizeof(DsArgs) * htab.size(), hipMemcpyHostToDevice, c0.stream);
+  GraphSlot &gs = c0.ws->group_graph[sizeof(R) == 8 ? 1 : 0];
+  unsigned long long key[3];
+  key[0] = fnv1a(htab.data(), sizeof(DsArgs) * htab.size());
+  key[1] = (unsigned long long)(size_t)dtab;
+  key[2] = ((unsigned long long)n << 32) | (unsigned)code;
+  auto launch = [&]() {
+    for (int q = 0; q < kChunk; q++) {
+      const size_t h0 = ((size_t)q * 2 + 0) * n, m0 = ((size_t)q * 2 + 1) * n;
+      dslash_group_launch<R, LINK, DS_HOP, RED_NORM>(c0.stream, dtab + h0, htab.data() + h0, n, code);
+      dslash_group_launch<R, LINK, DS_BMS, RED_NORM>(c0.stream, dtab + m0, htab.data() + m0, n, code);
+    }
+    hipLaunchKernelGGL(k_add_int, dim3(1), dim3(64), 0, c0.stream, iterc, kChunk);
+  };
+  auto chunk = [&](bool) -> bool { return chunk_run(c0, gs, false, key, sizeof(key), launch); };
+  std::vector<const int *> active(n);
+  for (int l = 0; l < n; l++) active[l] = runs[l].a.active;
+  if (bms_iterate(c0, iterc, active, cap_max, chunk)) return -1;
+  for (int l = 0; l < n; l++) {
+    GroupJob &j = jobs[l];
+    j.out.iters = 0;
+    j.out.rc = bms_finish<R, LINK>(*cs[l], runs[l], j.x, &j.out.iters, j.x32, j.asm_fp32, &j.out.rs);
+    if (j.out.rc < 0) return -1;
+  }
   return 0;
 }
 
@@ -2343,14 +2798,22 @@ __global__ __launch_bounds__(kBlock) void k_add_scaled(XO *x, const XO *y, const
 // res_abs), and adds zeta_s y_s to their solutions; factors compound over
 // continuations.  Returns 0, kStatusMaxCg, kStatusNonFinite, kBasisFull (the
 // first call did not fit) or -1.
+// first: when not null, the first pass was already run (multi-lattice solve).
 template <typename R, int LINK>
 static int ms_basis_solve(Context &c, const double2 *b, int nrhs, int nshift, const double *shifts,
                           const double *res_abs, int max_iter, double2 *x, int *iters_out, float2 *x32 = nullptr,
-                          bool asm_fp32 = false) {
+                          bool asm_fp32 = false, FirstPass *first = nullptr) {
   using C = Cx<R>;
   BasisRestart rs;
   int it = 0;
-  int rc = ms_cg_basis<R, LINK>(c, b, nrhs, nshift, shifts, res_abs, max_iter, x, &it, x32, asm_fp32, &rs);
+  int rc;
+  if (first) {
+    rc = first->rc;
+    it = first->iters;
+    rs = first->rs;
+  } else {
+    rc = ms_cg_basis<R, LINK>(c, b, nrhs, nshift, shifts, res_abs, max_iter, x, &it, x32, asm_fp32, &rs);
+  }
   int total = it;
   std::vector<int> orig(nrhs);
   for (int j = 0; j < nrhs; j++) orig[j] = j;
@@ -2558,8 +3021,10 @@ static int refine_fp64(Context &c, const double2 *b, int nrhs, int nshift, const
   return 0;
 }
 
-int multishift_solve(Context &c, const double2 *b, int nrhs, int nshift, const double *shifts, double res, int rel,
-                     int prec, int max_iter, double2 *x, int *iters, float2 *x32) {
+// first: the stored-basis first pass already run by a multi-lattice solve
+// (for the precision's basis solver), or null
+static int ms_solve(Context &c, const double2 *b, int nrhs, int nshift, const double *shifts, double res, int rel,
+                    int prec, int max_iter, double2 *x, int *iters, float2 *x32, FirstPass *first) {
   if (x32 && prec != PREC_FP32) {
     c.last_error = "multishift: FP32 solutions only from the FP32 solver";
     return 1;
@@ -2581,7 +3046,7 @@ int multishift_solve(Context &c, const double2 *b, int nrhs, int nshift, const d
   if (prec == PREC_FP64) {
     int rc = explicit_shifts ? kBasisFull
                              : ms_basis_solve<double, LINK_D12>(c, b, nrhs, nshift, shifts, res_abs.data(), max_iter,
-                                                                x, iters);
+                                                                x, iters, nullptr, false, first);
     const bool basis = rc != kBasisFull;
     if (rc == kBasisFull) rc = ms_cg<double, LINK_D12>(c, b, nrhs, nshift, shifts, res_abs.data(), max_iter, x, iters);
     c.last_iters = *iters;
@@ -2607,7 +3072,7 @@ int multishift_solve(Context &c, const double2 *b, int nrhs, int nshift, const d
   const long n = 3 * Vh * (long)nrhs * nshift;
   int rc = explicit_shifts ? kBasisFull
                            : ms_basis_solve<float, LINK_S12>(c, b, nrhs, nshift, shifts, res32.data(), max_iter, x,
-                                                             &it32, x32, prec == PREC_FP32);
+                                                             &it32, x32, prec == PREC_FP32, first);
   if (rc == kBasisFull) {
     float2 *xs = x32 ? x32 : (float2 *)ws_get(c, S_X32, sizeof(float2) * 3 * Vh * nrhs * nshift);
     if (!xs) return -1;
@@ -2631,6 +3096,64 @@ int multishift_solve(Context &c, const double2 *b, int nrhs, int nshift, const d
   c.last_iters = it32 + ref;
   *iters = c.last_iters;
   return rr ? rr : rc;
+}
+
+int multishift_solve(Context &c, const double2 *b, int nrhs, int nshift, const double *shifts, double res, int rel,
+                     int prec, int max_iter, double2 *x, int *iters, float2 *x32) {
+  return ms_solve(c, b, nrhs, nshift, shifts, res, rel, prec, max_iter, x, iters, x32, nullptr);
+}
+
+// Several lattices' solves of one precision in lockstep (ms_cg_basis_group
+// for the stored-basis pass), then each lattice's continuation and FP64
+// defect correction as in multishift_solve.  Falls back to one lattice
+// after another when the precisions differ, the explicit-shift solver is
+// selected, or a basis does not fit.  All contexts on one stream.
+int multishift_solve_group(std::vector<Context *> &cs, std::vector<MsSpec> &sp) {
+  const int n = (int)cs.size();
+  bool group = n > 1 && n <= kMaxGroup && getenv("RHMC_MS_EXPLICIT") == nullptr;
+  for (int l = 0; l < n && group; l++) {
+    const MsSpec &q = sp[l];
+    group = cs[l]->ms_algorithm == 0 && q.prec == sp[0].prec && !(q.x32 && q.prec != PREC_FP32) &&
+            q.nshift >= 1 && q.nshift <= kMaxShifts && q.nrhs >= 1 && q.nrhs * q.nshift <= kMaxRed;
+  }
+  std::vector<FirstPass> first(n);
+  bool have_first = false;
+  if (group) {
+    std::vector<GroupJob> jobs(n);
+    std::vector<std::vector<double>> res_abs(n);
+    const int prec = sp[0].prec;
+    for (int l = 0; l < n; l++) {
+      Context &c = *cs[l];
+      MsSpec &q = sp[l];
+      std::vector<double> bn2(q.nrhs);
+      if (vec_norms2(c, q.b, q.nrhs, 3 * c.Vh, bn2.data())) return -1;
+      res_abs[l].resize(q.nrhs);
+      // the basis pass's target (ms_solve: FP32 stage of the mixed solver at 2e-6 |b|)
+      for (int j = 0; j < q.nrhs; j++) {
+        const double ra = q.rel ? q.res * sqrt(bn2[j]) : q.res;
+        res_abs[l][j] = prec == PREC_MIXED ? std::max(ra, 2e-6 * sqrt(bn2[j])) : ra;
+      }
+      jobs[l] = GroupJob{q.b, q.nrhs, q.nshift, q.shifts, res_abs[l].data(), q.max_iter, q.x,
+                         prec == PREC_FP32 ? q.x32 : nullptr, prec == PREC_FP32, FirstPass()};
+    }
+    const int rc = prec == PREC_FP64 ? ms_cg_basis_group<double, LINK_D12>(cs, jobs)
+                                     : ms_cg_basis_group<float, LINK_S12>(cs, jobs);
+    if (rc < 0) return rc;
+    if (rc == 0) {
+      have_first = true;
+      for (int l = 0; l < n; l++) first[l] = jobs[l].out;
+    }
+  }
+  int worst = 0;
+  for (int l = 0; l < n; l++) {
+    MsSpec &q = sp[l];
+    q.iters = 0;
+    q.rc = ms_solve(*cs[l], q.b, q.nrhs, q.nshift, q.shifts, q.res, q.rel, q.prec, q.max_iter, q.x, &q.iters, q.x32,
+                    have_first ? &first[l] : nullptr);
+    if (q.rc < 0) return -1;
+    if (q.rc && !worst) worst = q.rc;
+  }
+  return worst;
 }
 
 // ---------------------------------------------------------- norms / scaling
