@@ -525,7 +525,7 @@ def main():
     gbs32 = b32 / (ms32 * 1e-3) / 1e9
     gbs64 = b64 / (ms64 * 1e-3) / 1e9
     # the same launches with the operands evicted from L2 and Infinity Cache
-    # by a 512 MB write pass before each (HBM-resident operands)
+    # by a 512 MB read pass before each (HBM-resident operands)
     ms32c = g.time_dslash_cold(1, n_pf, 50)
     ms64c = g.time_dslash_cold(0, 1, 50)
     # live: the hop and the M^dag M launch (with the s and r updates) of the

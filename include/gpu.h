@@ -165,9 +165,10 @@ int gpu_chiral_meas(rhmc_gpu_t h, const double *rnd_e, const double *rnd_o, unsi
  * 12-real, 2 FP64 12-real. */
 int gpu_time_dslash(rhmc_gpu_t h, int precision, int nvec, int reps, double *ms);
 
-/* The same with every timed launch preceded by a write pass over
- * flush_bytes (<= 8e9) of other device memory, so the operands come from HBM
- * rather than the L2s or the Infinity Cache; mean of the per-launch event
+/* The same with every timed launch preceded by a read pass over flush_bytes
+ * (<= 8e9) of other device memory (written once beforehand), so the operands
+ * come from HBM rather than the L2s or the Infinity Cache and no dirty lines
+ * are left for the timed launch to write back; mean of the per-launch event
  * times. */
 int gpu_time_dslash_cold(rhmc_gpu_t h, int precision, int nvec, int reps, double flush_bytes, double *ms);
 

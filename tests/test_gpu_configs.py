@@ -73,7 +73,8 @@ def test_c1_golden_on_device():
     assert rel(g.dslash(1, gd["dslash_in"][0], 0)[0], gd["doe"]) < 1e-14
     assert rel(g.dslash(0, gd["dslash_in"][1], 0)[0], gd["deo"]) < 1e-14
     assert rel(g.dslash(1, gd["dslash_in"][0], 1)[0], gd["doe"]) < 2e-6
-    g.init1(phi=phi)
+    # the reference's multi-shift golden solves for the chi draw (seed 606)
+    g.init1(phi=fields.gauss_spinor(lat, 606, 1))
     g.multishift(gd["ms_shifts"], float(gd["ms_res"]), relative=False, precision=0)
     x = g.get_fermions("smf").reshape(gd["ms_x"].shape)
     assert rel(x, gd["ms_x"]) < 1e-10
@@ -517,8 +518,10 @@ def test_basis_continuation_with_small_capacity(precision, monkeypatch):
 
 def test_graph_capture_once_per_solve_shape():
     """Solver chunks replayed as HIP graphs: repeated solves with the same
-    arguments capture one graph per precision and reuse it (recapture only
-    when a launch argument changes)."""
+    arguments reuse the captured graph.  The second solve may capture once
+    more, because the basis capacity adapts to the iterations the first one
+    needed (a launch argument); from then on no solve recaptures, and every
+    solve returns the same solutions bit for bit."""
     dims, mass = (8, 8, 8, 4), 0.05
     lat = fields.Lattice(*dims)
     U = fields.random_links(lat, 8, 0.3)
@@ -526,16 +529,15 @@ def test_graph_capture_once_per_solve_shape():
     b = np.geomspace(1e-4, 1.0, 8)
     g = Gpu(dims, mass=mass, n_pf=2, max_order=8)
     g.init0(U)
-    g.init1(phi=phi)
-    g.multishift(b, 1e-10, relative=True, precision=0)
-    n0 = g.solver_counters()[1]
-    x0 = g.get_fermions("smf")
-    for _ in range(3):
+    counts, sols = [], []
+    for _ in range(5):
         g.init1(phi=phi)
         g.multishift(b, 1e-10, relative=True, precision=0)
-    n1 = g.solver_counters()[1]
-    x1 = g.get_fermions("smf")
+        counts.append(g.solver_counters()[1])
+        sols.append(g.get_fermions("smf"))
     g.close()
-    say("graph captures", n0, n1)
-    assert n0 <= 1 and n1 == n0
-    assert np.array_equal(x0, x1)
+    say("graph captures after each solve", counts)
+    assert counts[0] <= 1 and counts[1] <= counts[0] + 1
+    assert counts[4] == counts[1]
+    for x in sols[1:]:
+        assert np.array_equal(x, sols[0])

@@ -439,9 +439,11 @@ def test_live_kernel_timing():
 
 
 def test_multi_trajectory_matches_single():
-    """Independent lattices of different sizes run concurrently
-    (gpu_multi_trajectory) give the same trajectories as one at a time, and
-    each matches the reference's update()."""
+    """Independent lattices run in lockstep (gpu_multi_trajectory) each
+    match the reference's update(), and the trajectory the same lattice runs
+    alone from the same start to Delta H 1e-9 and links 1e-10 (the lockstep
+    launch has its own shape, so the FP64 rounding of its sums differs from
+    the single-lattice launch's in the last bits)."""
     from rhmc_amd import multi_trajectory
     names = ["t4", "t4o", "t4m"]
     gds = [dict(np.load(os.path.join(GOLD, f"golden_{n}.npz"))) for n in names]
@@ -465,7 +467,9 @@ def test_multi_trajectory_matches_single():
         g.init1(momenta=gd["mom"])
         g.put_fermions("phi", gd["phi"])
         r1 = g.trajectory(**_traj_args(gd))
-        assert r1.delta_h == r.delta_h
+        U1, _, _ = g.end(links=True, ipdot=False, momenta=False)
+        assert abs(r1.delta_h - r.delta_h) < 1e-9
+        assert rel(U1, U) < 1e-10
         g.close()
 
 
