@@ -98,6 +98,20 @@ int gpu_gauge_deriv(rhmc_gpu_t h, int ipdot_init);
  * them to float, fermionforce.cc:42-43).  accumulate=1 adds to ipdot. */
 int gpu_fermion_force(rhmc_gpu_t h, int num_shifts, const double *numerators, int accumulate);
 
+/* No counterpart in the reference: gpu_fermion_force for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Same arguments and result; it
+ * calls no hipMalloc/hipFree and creates nothing.  Its scratch (the
+ * coefficients and the hop results D_oe s_j) is one fixed 64 MiB device
+ * array per GPU, held until the process ends; the systems are processed in
+ * groups that fit it, each group's force added to the previous groups'.
+ * Limit: V/2 <= 1,398,058 sites (one system, 48 V/2 bytes, must fit);
+ * larger lattices get RHMC_EINVAL.  The scratch is shared by every call on
+ * that GPU and keeps the last call's contents: calls that overlap on one GPU
+ * (several handles on different streams) race.  The Dslash launch shape is
+ * the one already tuned for this handle, else the default (no autotuning). */
+int gpu_fermion_force_noalloc(rhmc_gpu_t h, int num_shifts, const double *numerators, int accumulate);
+
 /* lib/Cuda/cuda_sol_sum.h:4 cuda_sol_sum (rationalapprox.cc:288,340):
  * mf <- a0*mf + sum_k num[k]*smf[k] for every pseudofermion. */
 int gpu_sol_sum(rhmc_gpu_t h, int num_shifts, double mult_const, const double *numerators);

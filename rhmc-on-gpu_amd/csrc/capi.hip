@@ -131,6 +131,10 @@ int gpu_create(const rhmc_gpu_params *p, rhmc_gpu_t *out) {
   }
   c.ws = workspace_create(c);
   if (!c.ws || build_tile_order(c)) { gpu_destroy(h); return RHMC_EHIP; }
+  // the fixed scratch of the allocation-free entry points: its address on
+  // this device, looked up once per device with the device current (on
+  // failure those entry points report it; nothing else depends on it)
+  noalloc_scratch_init(c);
   c.use_graphs = getenv("RHMC_NO_GRAPHS") == nullptr;
   *out = h;
   return RHMC_OK;
@@ -343,6 +347,18 @@ int gpu_fermion_force(rhmc_gpu_t h, int num_shifts, const double *num, int accum
   for (int ps = 0; ps < c.n_pf; ps++)
     for (int k = 0; k < num_shifts; k++) coef[(size_t)ps * num_shifts + k] = num[k];
   return fermion_force(c, c.n_pf * num_shifts, coef.data(), c.smf, accumulate) ? RHMC_EHIP : RHMC_OK;
+}
+
+int gpu_fermion_force_noalloc(rhmc_gpu_t h, int num_shifts, const double *num, int accumulate) {
+  if (!h || !num) return RHMC_EINVAL;
+  Context &c = h->c;
+  if ((size_t)sizeof(double2) * 3 * (size_t)c.Vh > kNoAllocScratchBytes - 256 * sizeof(double))
+    return fail(h, "gpu_fermion_force_noalloc: one system (48 V/2 bytes) exceeds the 64 MiB fixed scratch");
+  if (num_shifts != c.smf_order || num_shifts < 1)
+    return fail(h, "gpu_fermion_force_noalloc: order differs from the last solve");
+  hipSetDevice(c.device);
+  const int rc = fermion_force_noalloc(c, num_shifts, num, accumulate);
+  return rc == 0 ? RHMC_OK : rc == -2 ? RHMC_EINVAL : RHMC_EHIP;
 }
 
 int gpu_sol_sum(rhmc_gpu_t h, int num_shifts, double a0, const double *num) {

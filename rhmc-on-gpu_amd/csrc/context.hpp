@@ -64,6 +64,7 @@ struct Context {
   double *action = nullptr;  // [V] per-site action differences
   float2 *smf32 = nullptr;   // FP32 MD solutions [n_pf][max_order][3][Vh] (allocated on first use)
   int smf_order = 0;         // poles held in smf
+  void *noalloc_scratch = nullptr;  // this device's fixed scratch (fields.hip), or null
 
   // Dslash traversal order: kBlock-site tiles listed so that each eighth of
   // the list (one XCD's share) is a compact sub-block of the lattice
@@ -139,6 +140,10 @@ struct MsSpec {
   int iters, rc;
 };
 int multishift_solve_group(std::vector<Context *> &cs, std::vector<MsSpec> &sp);
+// the FP64 18-real hop with the tuned (or default) shape, never timing
+// candidates: no event, file or allocation
+void launch_hop_d_fixed(Context &c, int par_out, const double2 *in, double2 *out, int nvec, long vstride_in,
+                        long vstride_out);
 // out[j] = 0.5 * hop(in[j]) to parity par_out, FP32 12-real links
 void launch_hop_s(Context &c, int par_out, const float2 *in, float2 *out, int nvec, long vstride_in,
                   long vstride_out);
@@ -152,6 +157,15 @@ int fermion_force(Context &c, int nsys_per_pf, const double *num, const double2 
 // the same from FP32 solutions: FP32 hop on 12-real links, FP32 loads, FP64
 // accumulation and projection
 int fermion_force32(Context &c, int nsys, const double *num, const float2 *x32, int accumulate);
+// The allocation-free force: as fermion_force, with the per-system hop
+// results and the coefficients in the library's fixed device scratch
+// (kNoAllocScratchBytes per GPU), the systems taken in groups that fit it.
+// Returns -2 when one system does not fit (c.last_error says why).
+constexpr size_t kNoAllocScratchBytes = 64ull << 20;
+int fermion_force_noalloc(Context &c, int order, const double *num, int accumulate);
+// once per device, with the device current: the device address of the
+// fixed scratch (kept per device ordinal); sets c.noalloc_scratch
+int noalloc_scratch_init(Context &c);
 int gauge_force(Context &c, int accumulate);
 int exp_update(Context &c, double step);
 int momenta_update(Context &c, double step);

@@ -4,8 +4,10 @@
 // conversions of the C ABI.  All arithmetic here is FP64.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <mutex>
 #include <vector>
 
 #include "context.hpp"
@@ -148,6 +150,79 @@ int fermion_force32(Context &c, int nsys, const double *num, const float2 *x32, 
                      nsys, c.ipdot, accumulate);
   // stream-ordered for the caller (the staged coefficients live in hcoef)
   return hip_ok(c, hipGetLastError(), "fermion force") ? 0 : -1;
+}
+
+// ------------------------------------------------ allocation-free fermion force
+// For callers that cannot allocate device memory (graph capture, tight
+// serving loops): the force's scratch -- the coefficients and the hop
+// results h_j = D_oe s_j -- lives in one fixed file-scope device array.
+// Every GPU holds its own copy of it until the process ends.  In host code
+// the name `scratch` is only a stand-in for the symbol: kernels are given
+// the per-device address that hipGetSymbolAddress returned.
+constexpr size_t kNoAllocCoefBytes = 256 * sizeof(double);  // 2 KiB, keeps h 256-byte aligned
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxDeviceOrdinal = 64;
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[kMaxDeviceOrdinal];
+
+int noalloc_scratch_init(Context &c) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  c.noalloc_scratch = nullptr;
+  if (c.device < 0 || c.device >= kMaxDeviceOrdinal) return -1;
+  if (!g_scratch_ptr[c.device]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      (void)hipGetLastError();
+      return -1;
+    }
+    g_scratch_ptr[c.device] = p;
+  }
+  c.noalloc_scratch = g_scratch_ptr[c.device];
+  return 0;
+}
+
+// the coefficients travel by value in the kernel arguments, so nothing is
+// staged through host memory
+struct CoefTable {
+  double v[256];
+};
+__global__ void k_coef_store(CoefTable t, int n, double *dst) {
+  const int i = threadIdx.x;
+  if (i < n) dst[i] = t.v[i];
+}
+
+int fermion_force_noalloc(Context &c, int order, const double *num, int accumulate) {
+  const long Vh = c.Vh;
+  const int nsys = c.n_pf * order;
+  const size_t per_sys = sizeof(double2) * 3 * (size_t)Vh;
+  const size_t room = kNoAllocScratchBytes - kNoAllocCoefBytes;
+  if (per_sys > room) {
+    c.last_error = "gpu_fermion_force_noalloc: one system (48 V/2 bytes) exceeds the 64 MiB fixed scratch";
+    return -2;
+  }
+  if (!c.noalloc_scratch) {
+    c.last_error = "gpu_fermion_force_noalloc: device address of the fixed scratch unavailable";
+    return -1;
+  }
+  if (nsys < 1 || nsys > 256) return -2;
+  CoefTable t;
+  for (int j = 0; j < nsys; j++) t.v[j] = num[j % order];  // system j = ps * order + k
+  double *coef = (double *)c.noalloc_scratch;
+  double2 *hv = (double2 *)((unsigned char *)c.noalloc_scratch + kNoAllocCoefBytes);
+  hipLaunchKernelGGL(k_coef_store, dim3(1), dim3(256), 0, c.stream, t, nsys, coef);
+  // groups of systems whose hop results fit the scratch; the force of each
+  // group is added to the previous groups' (TA and U F are linear)
+  const int group = (int)std::min<size_t>((size_t)nsys, room / per_sys);
+  for (int j0 = 0; j0 < nsys; j0 += group) {
+    const int n = std::min(group, nsys - j0);
+    const double2 *x = c.smf + (long)j0 * 3 * Vh;
+    launch_hop_d_fixed(c, 1, x, hv, n, 3 * Vh, 3 * Vh);
+    hipLaunchKernelGGL(k_fermion_force<double2>, dim3(nblk(Vh), 2), dim3(kBlock), 0, c.stream, c.L, c.U64, x, hv,
+                       coef + j0, n, c.ipdot, (j0 > 0 || accumulate) ? 1 : 0);
+  }
+  return hip_ok(c, hipGetLastError(), "fermion force (fixed scratch)") ? 0 : -1;
 }
 
 // ----------------------------------------------------------------- gauge force

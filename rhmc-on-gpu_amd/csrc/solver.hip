@@ -1630,6 +1630,22 @@ void launch_hop_d(Context &c, int par_out, const double2 *in, double2 *out, int 
   dslash_dispatch<double, LINK_D18, DS_HOP, RED_NONE>(c, a);
 }
 
+// The same launch without the autotuner: the shape already tuned for this
+// (mode, nvec) when there is one, else the default shape.  Nothing is timed,
+// created or allocated (for the allocation-free entry points).
+void launch_hop_d_fixed(Context &c, int par_out, const double2 *in, double2 *out, int nvec, long vs_in, long vs_out) {
+  DsArgs a = {};
+  a.par_out = par_out;
+  a.U = c.U64;
+  a.in = in; a.in_vs = vs_in;
+  a.out = out; a.out_vs = vs_out;
+  a.nvec = nvec;
+  const auto &tab = c.ws->tuned_nr;
+  const auto it = tab.find(tune_key(sizeof(double), LINK_D18, DS_HOP, RED_NONE, nvec));
+  const int code = it != tab.end() ? it->second : make_shape(pick_nr(nvec), 1, 0, 0);
+  dslash_launch<double, LINK_D18, DS_HOP, RED_NONE>(c, a, code);
+}
+
 void launch_hop_d12(Context &c, int par_out, const double2 *in, double2 *out, int nvec, long vs_in, long vs_out) {
   DsArgs a = {};
   a.par_out = par_out;

@@ -107,6 +107,7 @@ def _declare(L):
         "gpu_momenta_sum_multiply": (I, [VP, D]),
         "gpu_gauge_deriv": (I, [VP, I]),
         "gpu_fermion_force": (I, [VP, I, VP, I]),
+        "gpu_fermion_force_noalloc": (I, [VP, I, VP, I]),
         "gpu_sol_sum": (I, [VP, I, D, VP]),
         "gpu_fermionforce": (I, [VP, D, I, I, VP, VP, I, I, I, PI]),
         "gpu_multi_trajectory": (I, [VP, I, VP, VP, VP]),
