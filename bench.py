@@ -19,7 +19,7 @@ Timed paths, each from the same start configuration and seed:
 Delta H of every mixed trajectory is recomputed on the same endpoints with
 FP64 solves to 1e-13 (untimed rerun); the difference is the Delta H error.
 roofline: the trajectory's dominant kernel, the MD solver's M^dag M launch
-with the CG's s and r updates, timed live inside the mixed trajectories (the
+with the CG's residual update, timed live inside the mixed trajectories (the
 hop before it is reported in dslash.live_*).
 
 Prints ONE JSON line.  --gpus above 1 is refused (one-GPU build).
@@ -50,9 +50,9 @@ def dslash_bytes(Vh, nrhs, link_b, spin_b):
 
 def mdagm_update_bytes(Vh, nrhs, link_b, spin_b):
     """Algorithmic bytes of the stored-basis solver's M^dag M launch: links,
-    the hop result h, r^(k-1) at the site, s read and written, r^(k) written:
-    N_out (8 L_b + 5 N_rhs S_b)."""
-    return Vh * (8 * link_b + 5 * nrhs * spin_b)
+    the hop result h, r^(k-1) and r^(k-2) at the site, r^(k) written (the
+    three-term residual update): N_out (8 L_b + 4 N_rhs S_b)."""
+    return Vh * (8 * link_b + 4 * nrhs * spin_b)
 
 
 def dslash_traffic(table, Vh, nrhs, mode=0, red=0):
@@ -528,7 +528,7 @@ def main():
     # by a 512 MB read pass before each (HBM-resident operands)
     ms32c = g.time_dslash_cold(1, n_pf, 50)
     ms64c = g.time_dslash_cold(0, 1, 50)
-    # live: the hop and the M^dag M launch (with the s and r updates) of the
+    # live: the hop and the M^dag M launch (with the residual update) of the
     # MD solver, timed inside the trajectories
     n_live = sum(h[1] for h, _ in live)
     live_ms = sum(h[0] * h[1] for h, _ in live) / n_live if n_live else ms32
@@ -588,9 +588,9 @@ def main():
                    "live_fp32_12real_8rhs_GBps": gbs_live, "live_frac_hbm": gbs_live / HBM_PEAK_GBS,
                    "live_traffic": traffic_hop},
         # the dominant kernel of the trajectory: the MD solver's M^dag M launch
-        # with the CG's s and r updates
+        # with the CG's residual update
         "roofline": {"bound": "hbm",
-                     "kernel": "k_dslash<float,S12,BMS> on 8 rhs (MD solver M^dag M with the CG s and r updates)",
+                     "kernel": "k_dslash<float,S12,BMS> on 8 rhs (MD solver M^dag M with the CG residual update)",
                      "achieved": gbs_mdm, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs_mdm / HBM_PEAK_GBS,
                      "traffic": traffic_mdm, "bytes_per_launch": b_mdm,
                      "measured": f"mean kernel duration of {n_mdm} launches inside the timed mixed trajectories "

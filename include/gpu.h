@@ -189,8 +189,8 @@ int gpu_time_dslash_cold(rhmc_gpu_t h, int precision, int nvec, int reps, double
 /* Live kernel timing inside real work: while enabled, the two launches of
  * each iteration of the MD solver (FP32 12-real links, the n_pf
  * pseudofermions) are timed on the device wall clock: [0] the half-lattice
- * hop D_oe, [1] the M^dag M launch after it (with the CG's s and r
- * updates).  Each interval runs from the end of the launch before to the end
+ * hop D_oe, [1] the M^dag M launch after it (with the CG's residual
+ * update).  Each interval runs from the end of the launch before to the end
  * of the timed launch's last workgroup, the per-dispatch interval a
  * kernel-trace profiler reports (dispatch, execution and end of kernel);
  * every workgroup stamps its own end.  The stamps sit in slots indexed by

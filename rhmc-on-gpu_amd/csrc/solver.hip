@@ -11,7 +11,7 @@
 // * stored residual basis (ms_cg_basis, the default): the base CG in its
 //   single-reduction form, two launches per iteration -- the hop
 //   h = D_oe r^(k-1) with |h|^2 partials (its first workgroups advance the
-//   scalar recurrences), then M^dag M with the s and r^(k) updates and
+//   scalar recurrences), then M^dag M with the three-term r^(k) update and
 //   |r^(k)|^2 partials; the shifted solutions are assembled from the basis
 //   at the end; a full basis is continued from its last residual
 //   (ms_basis_solve);
@@ -45,7 +45,7 @@ enum Slot {
   S_CR64, S_CX32, S_CR32, S_CP32, S_CS32, S_CH32,
   S_PW1, S_PW2, S_PW3,
   S_FH64,
-  S_PARTIAL2, S_BASIS, S_BSH, S_HIST, S_COEF, S_TUNE, S_HPART,
+  S_PARTIAL2, S_BASIS, S_BSH, S_HIST, S_COEF, S_HPART,
   S_FORCE_H, S_FORCE_COEF, S_SOLSUM_COEF,
   S_RS_SRC, S_RS_Y, S_RS_IDX, S_GROUP_TAB,
   S_COUNT
@@ -398,7 +398,7 @@ __device__ __forceinline__ double block_sum_partials(const double *p, int n) {
 }
 
 // ------------------------------------------------------------------ Dslash
-// DS_BMS: the stored-basis solver's M^dag M with the s and r updates
+// DS_BMS: the stored-basis solver's M^dag M with the residual update
 enum DsMode { DS_HOP = 0, DS_MDAGM = 1, DS_RESID = 2, DS_BMS = 3 };
 enum DsRed { RED_NONE = 0, RED_DOT = 1, RED_NORM = 2 };
 enum LinkFmt { LINK_D18 = 0, LINK_S12 = 1, LINK_D12 = 2 };
@@ -442,8 +442,9 @@ struct DsArgs {
   // set) has workgroup j < nvec advance system j's scalars from the rho
   // partials of the previous M^dag M launch (bms_nbr per system); the
   // DS_BMS launch forms alpha from the hop's |h|^2 partials (bms_hnbr per
-  // system), updates s (out) in place and writes r^(k) into the basis slot
-  // after pin's, below bms_cap, with |r^(k)|^2 partials in red
+  // system) and writes r^(k) into the basis slot after pin's, below
+  // bms_cap, from pin's slot and the one before it (out is unused), with
+  // |r^(k)|^2 partials in red
   BmsShared *bms; const double *bms_part; int bms_nbr;
   MsState *bms_ms; BmsHist *bms_hist; int *bms_active; int bms_cap, bms_nshift;
   const double *bms_hpart; int bms_hnbr;
@@ -530,8 +531,8 @@ __device__ __forceinline__ void bms_step(const DsArgs &a, int j) {
 // D is anti-hermitian, D_eo^+ = -D_oe), from the hop's |h|^2 partials summed
 // in a fixed order, so every workgroup forms the same value; <p, A p> = mu -
 // beta rho_{k-1} / alpha_prev (single-reduction CG), alpha = rho_{k-1} /
-// <p, A p>, into ab (workgroup memory: [0] alpha, [1] beta, read by the
-// epilogue).  The first site block's workgroup of each chunk stores alpha
+// <p, A p>, into ab (workgroup memory: [0] alpha, [1] the three-term
+// coefficient c1 = alpha beta / alpha_prev, read by the epilogue).  The first site block's workgroup of each chunk stores alpha
 // for iteration k+1.  A system that is stopped is not used (its output is
 // not stored).
 template <typename R, int NR>
@@ -567,9 +568,14 @@ __device__ __forceinline__ void bms_alpha(const DsArgs &a, int jbase, unsigned s
     const double mu = a.diag[j] * rho1 + h2;
     const double pap = k >= 2 ? mu - b * rho1 / sh.alpha[sl] : mu;
     const double alpha = rho1 / pap;
+    // three-term residual update r^(k) = r^(k-1) - alpha w - c1 (r^(k-2) -
+    // r^(k-1)): c1 = alpha beta / alpha_prev carries beta s_{k-2} = beta
+    // (r^(k-2) - r^(k-1)) / alpha_prev
+    const double ap = sh.alpha[sl];
+    const double c1 = k >= 2 && ap != 0.0 ? alpha * b / ap : 0.0;
     if (threadIdx.x == 0) {
       ab[0][r] = (R)alpha;
-      ab[1][r] = (R)b;
+      ab[1][r] = (R)c1;
       if (sb == 0 && jbase + r < a.nvec && !sh.stop[sl]) a.bms[j].alpha[sn] = alpha;
     }
   }
@@ -775,14 +781,11 @@ constexpr int ds_waves() {
 
 // Output of one site for the chunk's right-hand sides: the hop result (times
 // 1/2), or m^2 + shift times the input minus it (M^dag M), or the residual
-// b - M^dag M x, with the dot/norm partials of the reducing modes.  DS_BMS
-// (the stored-basis solver): w = m^2 r - D_eo h, s = w + beta s in place,
-// r^(k) = r - alpha s into the next basis slot (when rk_ok), |r^(k)|^2
-// partials.
+// b - M^dag M x, with the dot/norm partials of the reducing modes (the
+// stored-basis solver's DS_BMS output is ds_site_out_bms).
 template <typename R, int NR, int MODE, int RED>
 __device__ __forceinline__ void ds_site_out(const DsArgs &a, const Cx<R> (&acc)[NR][3], const bool (&act)[NR],
-                                            double (&red)[NR], int jbase, int h, const Cx<R> *pinb,
-                                            const R (&ab)[2][NR], bool rk_ok) {
+                                            double (&red)[NR], int jbase, int h, const Cx<R> *pinb) {
   using C = Cx<R>;
   const long Vh = a.L.Vh;
 #pragma unroll
@@ -797,7 +800,6 @@ __device__ __forceinline__ void ds_site_out(const DsArgs &a, const Cx<R> (&acc)[
     } else {
       const R d = (R)a.diag[j];
       const C *p = pinb + (long)j * a.pin_vs + h;
-      C *rk = (C *)(pinb + a.iter_stride) + (long)j * a.pin_vs + h;
 #pragma unroll
       for (int i = 0; i < 3; i++) {
         const C pv = p[(long)i * Vh];
@@ -808,24 +810,11 @@ __device__ __forceinline__ void ds_site_out(const DsArgs &a, const Cx<R> (&acc)[
           res[i] = csub(mk<R>((R)bv.x, (R)bv.y), res[i]);
         }
         if (RED == RED_DOT) red[r] += (double)pv.x * (double)res[i].x + (double)pv.y * (double)res[i].y;
-        if (MODE == DS_BMS) {
-          const R be = ab[1][r], al = ab[0][r];
-          if (be != (R)0) {
-            const C so = o[(long)i * Vh];
-            res[i].x = fmar(be, so.x, res[i].x);
-            res[i].y = fmar(be, so.y, res[i].y);
-          }
-          C rn;
-          rn.x = fmar(-al, res[i].x, pv.x);
-          rn.y = fmar(-al, res[i].y, pv.y);
-          if (rk_ok) rk[(long)i * Vh] = rn;
-          red[r] += (double)rn.x * (double)rn.x + (double)rn.y * (double)rn.y;
-        }
       }
     }
 #pragma unroll
     for (int i = 0; i < 3; i++) o[(long)i * Vh] = res[i];
-    if (RED == RED_NORM && MODE != DS_BMS) {
+    if (RED == RED_NORM) {
       // the site's |v|^2 in the working precision, accumulated in FP64
       R n2 = (R)0;
 #pragma unroll
@@ -835,48 +824,45 @@ __device__ __forceinline__ void ds_site_out(const DsArgs &a, const Cx<R> (&acc)[
   }
 }
 
-// DS_BMS output of one site: all of the site's loads (r^(k-1) and the old s
-// of every right-hand side of the chunk) are issued before the first store,
-// so they overlap instead of waiting on one another; a slot past nvec reads
-// the last vector, and beta = 0 (the first iteration) takes w without the
-// old s, whose contents are then undefined.
+// DS_BMS output of one site, the three-term form of the CG residual update:
+// w = m^2 r^(k-1) - D_eo h (= M^dag M r^(k-1)), r^(k) = r^(k-1) - alpha w +
+// c1 (r^(k-1) - r^(k-2)) into basis slot k (when rk_ok), |r^(k)|^2 partials.
+// It equals r^(k-1) - alpha s_{k-1} with s_{k-1} = w + beta s_{k-2} and
+// s_{k-2} = (r^(k-2) - r^(k-1)) / alpha_prev, so no search-direction vector
+// is stored.  All of the site's loads (r^(k-1) and, from the second
+// iteration, r^(k-2) of every right-hand side of the chunk) are issued
+// before the first store; a slot past nvec reads the last vector.
 template <typename R, int NR>
 __device__ __forceinline__ void ds_site_out_bms(const DsArgs &a, const Cx<R> (&acc)[NR][3], const bool (&act)[NR],
                                                 double (&red)[NR], int jbase, int h, const Cx<R> *pinb,
-                                                const R (&ab)[2][NR], bool rk_ok) {
+                                                const R (&ab)[2][NR], bool rk_ok, bool use_prev) {
   using C = Cx<R>;
   const long Vh = a.L.Vh;
-  C pv[NR][3], so[NR][3];
+  C pv[NR][3], qv[NR][3];
 #pragma unroll
   for (int r = 0; r < NR; r++) {
     const int j = jbase + r < a.nvec ? jbase + r : a.nvec - 1;
     const C *p = pinb + (long)j * a.pin_vs + h;
-    const C *o = (const C *)a.out + (long)j * a.out_vs + h;
 #pragma unroll
     for (int i = 0; i < 3; i++) {
       pv[r][i] = p[(long)i * Vh];
-      so[r][i] = o[(long)i * Vh];
+      qv[r][i] = use_prev ? p[(long)i * Vh - a.iter_stride] : pv[r][i];
     }
   }
 #pragma unroll
   for (int r = 0; r < NR; r++) {
     if (!act[r]) continue;
     const int j = jbase + r;
-    C *o = (C *)a.out + (long)j * a.out_vs + h;
     C *rk = (C *)(pinb + a.iter_stride) + (long)j * a.pin_vs + h;
     const R d = (R)a.diag[j];
-    const R be = ab[1][r], al = ab[0][r];
-    const bool use_s = be != (R)0;
+    const R al = ab[0][r], c1 = ab[1][r];
 #pragma unroll
     for (int i = 0; i < 3; i++) {
-      C w, sv, rn;
+      C w, rn;
       w.x = fmar(d, pv[r][i].x, (R)-0.5 * acc[r][i].x);
       w.y = fmar(d, pv[r][i].y, (R)-0.5 * acc[r][i].y);
-      sv.x = use_s ? fmar(be, so[r][i].x, w.x) : w.x;
-      sv.y = use_s ? fmar(be, so[r][i].y, w.y) : w.y;
-      rn.x = fmar(-al, sv.x, pv[r][i].x);
-      rn.y = fmar(-al, sv.y, pv[r][i].y);
-      o[(long)i * Vh] = sv;
+      rn.x = fmar(c1, pv[r][i].x - qv[r][i].x, fmar(-al, w.x, pv[r][i].x));
+      rn.y = fmar(c1, pv[r][i].y - qv[r][i].y, fmar(-al, w.y, pv[r][i].y));
       if (rk_ok) rk[(long)i * Vh] = rn;
       red[r] += (double)rn.x * (double)rn.x + (double)rn.y * (double)rn.y;
     }
@@ -899,6 +885,8 @@ __device__ __forceinline__ void ds_body(const DsArgs &a, unsigned bid, unsigned 
   __shared__ R bab[2][NR];
   if (MODE == DS_BMS) bms_alpha<R, NR>(a, jbase, sb, bab);
   const bool rk_ok = MODE == DS_BMS && *a.iterp + a.iter_off + 1 < a.bms_cap;
+  // iteration k >= 2 of the stored-basis solver reads basis slot k-2 too
+  const bool use_prev = MODE == DS_BMS && *a.iterp + a.iter_off + 1 >= 2;
   // this workgroup's run of tiles in the traversal order
   const int per = (a.ntiles + (int)a.nsb - 1) / (int)a.nsb;
   const int q0 = (int)sb * per;
@@ -954,8 +942,8 @@ __device__ __forceinline__ void ds_body(const DsArgs &a, unsigned bid, unsigned 
       if (k == 0 && xrow) xm_from_xp<R, LINK, NR>(buf[0], buf[1]);
       hop_fence<R, NR>(acc);
     }
-    if constexpr (MODE == DS_BMS) ds_site_out_bms<R, NR>(a, acc, act, red, jbase, h, pinb, bab, rk_ok);
-    else ds_site_out<R, NR, MODE, RED>(a, acc, act, red, jbase, h, pinb, bab, rk_ok);
+    if constexpr (MODE == DS_BMS) ds_site_out_bms<R, NR>(a, acc, act, red, jbase, h, pinb, bab, rk_ok, use_prev);
+    else ds_site_out<R, NR, MODE, RED>(a, acc, act, red, jbase, h, pinb);
   }
   if (RED != RED_NONE) block_partials<NR>(red, a.red.partial, jbase, a.nvec, a.nsb, sb);
   clk_end(a, bid);
@@ -1091,6 +1079,7 @@ __device__ __forceinline__ void ds_ls_body(const DsArgs &a, unsigned bid, unsign
   __shared__ R bab[2][NR];
   if (MODE == DS_BMS) bms_alpha<R, NR>(a, jbase, sb, bab);
   const bool rk_ok = MODE == DS_BMS && *a.iterp + a.iter_off + 1 < a.bms_cap;
+  const bool use_prev = MODE == DS_BMS && *a.iterp + a.iter_off + 1 >= 2;
   const int nunits = a.ntiles * (kBlock / kLsSites);
   const int per = (nunits + (int)a.nsb - 1) / (int)a.nsb;
   const int q0 = (int)sb * per;
@@ -1172,17 +1161,15 @@ __device__ __forceinline__ void ds_ls_body(const DsArgs &a, unsigned bid, unsign
         }
         if (RED == RED_DOT) red[r] += (double)pv.x * (double)res.x + (double)pv.y * (double)res.y;
         if (MODE == DS_BMS) {
-          const R be = bab[1][r], al = bab[0][r];
-          if (be != (R)0) {
-            const C so = ((const C *)a.out + (long)j * a.out_vs + h)[(long)col * Vh];
-            res.x = fmar(be, so.x, res.x);
-            res.y = fmar(be, so.y, res.y);
-          }
+          // three-term residual update (ds_site_out_bms); w = res is not stored
+          const R al = bab[0][r], c1 = bab[1][r];
+          const C qv = use_prev ? (pinb + (long)j * a.pin_vs + h)[(long)col * Vh - a.iter_stride] : pv;
           C rn;
-          rn.x = fmar(-al, res.x, pv.x);
-          rn.y = fmar(-al, res.y, pv.y);
+          rn.x = fmar(c1, pv.x - qv.x, fmar(-al, res.x, pv.x));
+          rn.y = fmar(c1, pv.y - qv.y, fmar(-al, res.y, pv.y));
           if (rk_ok) ((C *)(pinb + a.iter_stride) + (long)j * a.pin_vs + h)[(long)col * Vh] = rn;
           red[r] += (double)rn.x * (double)rn.x + (double)rn.y * (double)rn.y;
+          continue;
         }
       }
       ((C *)a.out + (long)j * a.out_vs + h)[(long)col * Vh] = res;
@@ -1447,11 +1434,10 @@ static void dslash_launch(Context &c, DsArgs &a, int code) {
 // for the lifetime of the context -- and, when a tuning cache file is set
 // (RHMC_TUNE_CACHE or gpu_tune_cache), stored there under (kernel, lattice
 // extents, precision, nvec, GPU architecture) and reused by later contexts
-// and processes without timing.  The launches are idempotent, except two of
-// the stored-basis solver: its hop advances the systems' scalar state (the
-// candidates skip that step) and its M^dag M updates s in place (the
-// candidates write s into a scratch vector; the r^(k) they write is
-// overwritten by the final launch).  RHMC_DSLASH_NR ("4" or "4x4") forces
+// and processes without timing.  The launches are idempotent, except the
+// stored-basis solver's hop, which advances the systems' scalar state (the
+// candidates skip that step); the r^(k) the candidates of its M^dag M
+// launch write is overwritten by the final launch.  RHMC_DSLASH_NR ("4" or "4x4") forces
 // the right-hand sides per lane and lanes per site.
 template <typename R, int LINK, int MODE, int RED>
 static void dslash_dispatch(Context &c, DsArgs &a) {
@@ -1480,9 +1466,8 @@ static void dslash_dispatch(Context &c, DsArgs &a) {
     code = make_shape(pick_nr(a.nvec), 1, 0, 0);
     DsArgs t = a;
     t.bms_ms = nullptr;  // the hop's scalar step runs once, in the final launch
-    if (MODE == DS_BMS) t.out = ws_get(c, S_TUNE, sizeof(Cx<R>) * (size_t)a.out_vs * a.nvec);
     const bool reducing = RED != RED_NONE && MODE != DS_HOP;
-    for (int lanes = 1; lanes <= 4 && t.out; lanes *= 4)
+    for (int lanes = 1; lanes <= 4; lanes *= 4)
       for (int v = 1; v <= 8 && v <= std::max(1, a.nvec); v *= 2)
         for (int ident = c.tiles ? 0 : 1; ident <= 1; ident++)
           for (int tpw = 0; tpw <= (reducing ? 2 : 0); tpw++) {
@@ -1501,11 +1486,6 @@ static void dslash_dispatch(Context &c, DsArgs &a) {
           }
     hipEventDestroy(e0);
     hipEventDestroy(e1);
-    if (c.ws->buf[S_TUNE]) {  // every candidate has finished (event sync above)
-      hipFree(c.ws->buf[S_TUNE]);
-      c.ws->buf[S_TUNE] = nullptr;
-      c.ws->cap[S_TUNE] = 0;
-    }
     tab[key] = code;
     c.ws->tuned_ms[key] = best / 3.0;
     tune_cache_store(c);
@@ -1901,7 +1881,10 @@ __global__ __launch_bounds__(kBlock) void k_copy_f2d(double2 *x64, const float2 
 //   <p_i, A p_i> = mu_i - beta_{i-1} rho_i / alpha_{i-1},  alpha_i = rho_i / <p_i, A p_i>,
 //   s_i = w_i + beta_{i-1} s_{i-1},  r_{i+1} = r_i - alpha_i s_i
 // (alpha, beta are those of the textbook CG; the shifted solutions come
-// from the basis).  With A = m^2 - D_eo D_oe and the staggered D
+// from the basis).  Since s_{i-1} = (r_{i-1} - r_i) / alpha_{i-1}, and both
+// residuals are basis slots, s is not stored either: the update is the
+// three-term recurrence
+//   r_{i+1} = r_i - alpha_i w_i + c_i (r_i - r_{i-1}),  c_i = alpha_i beta_{i-1} / alpha_{i-1}.  With A = m^2 - D_eo D_oe and the staggered D
 // anti-hermitian, mu = m^2 rho + |D_oe r|^2, which the hop can reduce, so
 // alpha is known when M^dag M runs and the residual update joins it.
 // Launches of iteration k:
@@ -1909,11 +1892,11 @@ __global__ __launch_bounds__(kBlock) void k_copy_f2d(double2 *x64, const float2 
 //   partials of rho_{k-1} in a fixed order, forms beta, advances the
 //   Jegerlehner recurrences of rhs j for iteration k-1 and the stopping
 //   test (bms_step);
-// B every workgroup forms alpha from the |h|^2 partials (bms_alpha), then
-//   w = m^2 r^(k-1) - D_eo h, s = w + beta s in place, r^(k) = r^(k-1) -
-//   alpha s into basis slot k, with partials of |r^(k)|^2 (w is never
-//   stored).
-// Vector traffic per iteration and rhs: 2 (A) + 5 (B) passes, independent
+// B every workgroup forms alpha and c from the |h|^2 partials (bms_alpha),
+//   then w = m^2 r^(k-1) - D_eo h and r^(k) = r^(k-1) - alpha w +
+//   c (r^(k-1) - r^(k-2)) into basis slot k, with partials of |r^(k)|^2
+//   (w is never stored).
+// Vector traffic per iteration and rhs: 2 (A) + 4 (B) passes, independent
 // of the number of shifts.  The stop decision lags by one iteration: one
 // extra application of the operator per solve, identical recurrences and
 // iteration counts.  Scalars read by every workgroup of a launch (rho_{k-2},
@@ -2294,7 +2277,7 @@ struct FirstPass {
 struct BasisRun {
   int nrhs = 0, nshift = 0, max_iter = 0;
   long cap = 0;
-  void *basis = nullptr, *sr = nullptr, *hh = nullptr;
+  void *basis = nullptr, *hh = nullptr;
   MsState *ms = nullptr;
   BmsShared *bsh = nullptr;
   BmsHist *hist = nullptr;
@@ -2330,7 +2313,6 @@ static int bms_setup(Context &c, BasisRun &br, const double2 *b, int nrhs, int n
   br.max_iter = max_iter;
   br.cap = cap;
   br.basis = ws_get(c, S_BASIS, slot_bytes * cap);
-  br.sr = ws_get(c, dbl ? S_P64 : S_P32, vb * nrhs);  // s = A p recurrence
   br.hh = ws_get(c, dbl ? S_H64 : S_H32, vb * nrhs);
   br.ms = (MsState *)ws_get(c, S_MS, sizeof(MsState) * nrhs);
   br.bsh = (BmsShared *)ws_get(c, S_BSH, sizeof(BmsShared) * nrhs);
@@ -2340,7 +2322,7 @@ static int bms_setup(Context &c, BasisRun &br, const double2 *b, int nrhs, int n
   // per tile, at most 4 nbx per rhs with four lanes per site)
   br.partC = (double *)ws_get(c, S_PARTIAL2, sizeof(double) * nbx_of(c) * std::max(nrhs, 1));
   br.partH = (double *)ws_get(c, S_HPART, sizeof(double) * 4 * nbx_of(c) * std::max(nrhs, 1));
-  if (!br.basis || !br.sr || !br.hh || !br.ms || !br.bsh || !br.hist || !br.diag || !br.partC || !br.partH)
+  if (!br.basis || !br.hh || !br.ms || !br.bsh || !br.hist || !br.diag || !br.partC || !br.partH)
     return -1;
 
   br.hs.assign(nrhs, MsState());
@@ -2384,7 +2366,7 @@ static int bms_setup(Context &c, BasisRun &br, const double2 *b, int nrhs, int n
   br.timed = c.time_kernels && c.kt_clk && sizeof(R) == 4 && nrhs == c.n_pf && iterc == iter_ptr(c);
   DsArgs &mdm = br.mdm;
   mdm = hop;
-  mdm.par_out = 0; mdm.in = br.hh; mdm.in_vs = 3 * Vh; mdm.out = br.sr; mdm.pin = br.basis; mdm.pin_vs = 3 * Vh;
+  mdm.par_out = 0; mdm.in = br.hh; mdm.in_vs = 3 * Vh; mdm.out = nullptr; mdm.pin = br.basis; mdm.pin_vs = 3 * Vh;
   mdm.diag = br.diag; mdm.iter_in = 0; mdm.iter_pin = 1;
   mdm.bms_ms = nullptr; mdm.bms_part = nullptr; mdm.bms_hpart = br.partH;
   mdm.red.partial = br.partC;

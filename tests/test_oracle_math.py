@@ -106,6 +106,86 @@ def test_mu_from_the_hop_norm():
         assert abs(mu_dot.real - mu_hop) < 1e-13 * mu_hop
 
 
+def _basis_cg(apply_A, hop_norm2, b, mass2, nit, dtype, three_term):
+    """The stored-basis solver's base CG in its single-reduction form:
+    mu = m^2 rho + |D_oe r|^2, <p, A p> = mu - beta rho / alpha_prev, and the
+    residual update either two-term through s = A p (s_i = w_i + beta
+    s_{i-1}, r_{i+1} = r_i - alpha s_i) or three-term (r_{i+1} = r_i -
+    alpha w_i + c (r_i - r_{i-1}), c = alpha beta / alpha_prev).  Vectors in
+    dtype, scalars in FP64.  Returns the residuals and alpha, beta."""
+    r = [b.astype(dtype)]
+    s = None
+    alphas, betas = [], []
+    rho_prev = None
+    for i in range(nit):
+        ri = r[-1]
+        rho = float(np.vdot(ri, ri).real)
+        beta = 0.0 if rho_prev is None else rho / rho_prev
+        mu = mass2 * rho + hop_norm2(ri)
+        pap = mu if not alphas else mu - beta * rho / alphas[-1]
+        alpha = rho / pap
+        w = apply_A(ri).astype(dtype)
+        if three_term:
+            c = alpha * beta / alphas[-1] if alphas else 0.0
+            rn = ri - dtype(alpha) * w
+            if alphas:
+                rn = rn + dtype(c) * (ri - r[-2])
+        else:
+            s = w if s is None else w + dtype(beta) * s
+            rn = ri - dtype(alpha) * s
+        r.append(rn.astype(dtype))
+        alphas.append(alpha)
+        betas.append(beta)
+        rho_prev = rho
+    return r, alphas, betas
+
+
+def test_three_term_residual_update():
+    """The stored-basis solver's M^dag M launch updates the residual with the
+    three-term recurrence r_{i+1} = r_i - alpha w_i + c (r_i - r_{i-1})
+    (c = alpha beta / alpha_prev) instead of carrying s = A p.  Finite-
+    precision CG amplifies any difference in rounding, so the two forms are
+    compared by what the solver relies on: the same convergence, and a true
+    residual of the solution built from the residuals (x = sum alpha_i p_i,
+    p_i = r_i + beta p_{i-1}) that follows the recursive one -- in FP64 down
+    to 1e-12 |b|, and in FP32 (complex64 vectors, FP64 scalars) a final
+    true residual within 3x of the two-term form's."""
+    dims = (4, 4, 2, 4)
+    lat = fields.Lattice(*dims)
+    o = Oracle(dims)
+    U = fields.random_links(lat, 17, 0.5)
+    mass = 0.1
+    b = fields.gauss_spinor(lat, 18)[0].ravel()
+    shp = (lat.Vh, 3)
+    A = lambda v: o.mdagm(U, mass, v.astype(np.complex128).reshape(shp)).ravel()  # noqa: E731
+    hn = lambda v: float(np.linalg.norm(o.doe(U, v.astype(np.complex128).reshape(shp))) ** 2)  # noqa: E731
+    bn = np.linalg.norm(b)
+    nit = 250
+    res = {}
+    for dtype in (np.complex128, np.complex64):
+        for three in (False, True):
+            r, al, be = _basis_cg(A, hn, b, mass * mass, nit, dtype, three)
+            x = np.zeros_like(b)
+            p = None
+            hist = {}
+            for i in range(nit):
+                ri = r[i].astype(np.complex128)
+                p = ri if p is None else ri + be[i] * p
+                x = x + al[i] * p
+                if i + 1 in (100, 225, nit):
+                    hist[i + 1] = (np.linalg.norm(b - A(x)) / bn,
+                                   np.linalg.norm(r[i + 1].astype(np.complex128)) / bn)
+            res[(dtype, three)] = hist
+    d2, d3 = res[(np.complex128, False)], res[(np.complex128, True)]
+    # same convergence
+    assert 0.7 < d3[100][1] / d2[100][1] < 1.4
+    # FP64: true residual follows the recursive one, final 1e-12 |b|
+    assert abs(d3[225][0] - d3[225][1]) < 1e-12 and d3[nit][0] < 1e-12
+    # FP32: attainable accuracy of the two-term form's order
+    s2, s3 = res[(np.complex64, False)], res[(np.complex64, True)]
+    assert s3[nit][0] < 3 * s2[nit][0] and s3[nit][0] < 1e-5
+
+
 def test_multishift_against_dense_solve():
     dims = (4, 4, 4, 4)
     mass = 0.2
