@@ -72,17 +72,22 @@ def dslash_traffic(table, Vh, nrhs, mode=0, red=0):
            and t["reduction"] == red and t["nvec"] == nrhs]
     if not ent:
         return None
-    nr, lanes = ent[0]["rhs_per_thread"], ent[0]["lanes_per_site"]
+    nr, lanes, tpw = ent[0]["rhs_per_thread"], ent[0]["lanes_per_site"], ent[0].get("tiles_per_workgroup", 1)
     name = f"void rhmc::k_dslash{'_ls' if lanes == 4 else ''}<float, 1, {nr}, {mode}, {red}>(rhmc::DsArgs)"
     nch = (nrhs + nr - 1) // nr
+    nbx = (Vh + 255) // 256
     if mode == 0:
-        grid = Vh * lanes * nch  # one workgroup per tile (kBlock threads each)
-    else:
+        grid = nbx * 256 * lanes * nch  # one workgroup per tile (kBlock threads each)
+    elif tpw == 1:
         # reducing modes: reduce_blocks(nch) workgroups per chunk (kBlock threads)
-        nbx = (Vh + 255) // 256
         want = max(64, (2048 + nch - 1) // nch)
         want = (want + 7) // 8 * 8
         grid = min(want, nbx) * 256 * nch
+    else:
+        # tpw tiles per workgroup: ceil(nbx / tpw) workgroups per chunk, a multiple of 8
+        n = (nbx + tpw - 1) // tpw
+        n = (n + 7) // 8 * 8
+        grid = min(n, nbx) * 256 * nch
     k = prof["kernels"].get(f"{name}|{grid}")
     return None if k is None else k["read_bytes"] + k["write_bytes"]
 
@@ -100,6 +105,44 @@ def kernel_source_hash():
         with open(os.path.join(ROOT, f), "rb") as fh:
             h.update(fh.read())
     return h.hexdigest()
+
+
+def counters_from(fetch_dir, write_dir, out_path):
+    """profiles/dslash_32c8_counters.json from two rocprofv3 --pmc runs of
+    `bench.py --dslash-only` (FETCH_SIZE, then WRITE_SIZE; csv output): per
+    (kernel, grid) the mean counter per dispatch in KiB, scaled by the factor
+    that makes the copy probe (1 GiB read + 1 GiB written) read its known
+    size, with the SHA-256 of the kernel sources it was taken on."""
+    import csv
+    import glob
+
+    def means(d, counter):
+        acc = {}
+        for f in glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True):
+            with open(f) as fh:
+                for row in csv.DictReader(fh):
+                    if row["Counter_Name"] != counter:
+                        continue
+                    k = (row["Kernel_Name"], int(row["Grid_Size"]))
+                    acc.setdefault(k, {})[row["Dispatch_Id"]] = acc.get(k, {}).get(row["Dispatch_Id"], 0.0) + \
+                        float(row["Counter_Value"])
+        return {k: (sum(v.values()) / len(v), len(v)) for k, v in acc.items()}
+
+    fe, wr = means(fetch_dir, "FETCH_SIZE"), means(write_dir, "WRITE_SIZE")
+    probe = [k for k in fe if "k_copy_probe" in k[0]]
+    if not probe:
+        raise SystemExit("no copy probe in the counter runs")
+    gib = float(1 << 30)
+    f_fetch = gib / (fe[probe[0]][0] * 1024.0)
+    f_write = gib / (wr[probe[0]][0] * 1024.0)
+    kern = {}
+    for k in sorted(set(fe) & set(wr)):
+        kern[f"{k[0]}|{k[1]}"] = {"read_bytes": fe[k][0] * 1024.0 * f_fetch, "write_bytes": wr[k][0] * 1024.0 * f_write,
+                                  "fetch_kib": fe[k][0], "write_kib": wr[k][0], "dispatches": fe[k][1]}
+    out = {"source_sha256": kernel_source_hash(), "factors": {"fetch": f_fetch, "write": f_write}, "kernels": kern}
+    with open(out_path, "w") as fh:
+        json.dump(out, fh, indent=1)
+    return out
 
 
 def cpu_baseline(dims, mass, beta, n_pf, shifts, md_a, chi, U, iters_total, n_force, n_q):
@@ -415,7 +458,15 @@ def main():
                     help="only the Dslash and copy-probe timings (for counter runs)")
     ap.add_argument("--workload", default="traj32c8", choices=sorted(WORKLOADS),
                     help="traj32c8 is the headline; the others are the per-configuration studies")
+    ap.add_argument("--counters-from", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
+                    help="write profiles/dslash_32c8_counters.json from rocprofv3 FETCH_SIZE / WRITE_SIZE csv runs "
+                         "of --dslash-only (no GPU needed)")
     args = ap.parse_args()
+    if args.counters_from:
+        out = counters_from(args.counters_from[0], args.counters_from[1],
+                            os.path.join(ROOT, "profiles", "dslash_32c8_counters.json"))
+        print(json.dumps({"kernels": len(out["kernels"]), "factors": out["factors"]}))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus above 1 is not supported"}))
         sys.exit(2)
