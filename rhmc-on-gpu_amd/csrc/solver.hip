@@ -79,6 +79,7 @@ struct Workspace {
   GraphSlot bms_graph[2];                      // FP32 / FP64 stored-basis chunk
   GraphSlot group_graph[2];                    // the same for a group of lattices led by this context
   std::vector<DsArgs> group_tab;               // argument tables of the group's chunk (host copy)
+  std::map<unsigned long long, int> group_tuned;  // lockstep launch shape per group composition
   long basis_kmax[2] = {0, 0};                 // most basis iterations a FP32 / FP64 solve needed
   void *buf[S_COUNT] = {};
   size_t cap[S_COUNT] = {};
@@ -2122,13 +2123,32 @@ __global__ __launch_bounds__(kBlock) void k_bms_assemble_mx(const float2 *basis,
   }
 }
 
-// single-value kernels: set / advance the device iteration base (kernels,
-// not memsets, so that they can sit in a captured graph)
+// set the device iteration base (a kernel, not a memset, so that it can sit
+// in a captured graph)
 __global__ void k_set_int(int *p, int v) {
   if (threadIdx.x == 0) *p = v;
 }
-__global__ void k_add_int(int *p, int v) {
-  if (threadIdx.x == 0) *p += v;
+
+// End of a chunk of solver steps: advances the device iteration base by
+// `step` and reports every run's running-system count (for the new
+// iteration's parity) into pinned host memory, slot (chunk number & 1) of
+// [2][n], with system-scope stores; the host reads them after an event
+// behind this launch, so polling needs no copy per run.
+constexpr int kMaxPoll = 16;
+struct ChunkPoll {
+  const int *active[kMaxPoll];  // per run: running-system counters, [iteration & 1]
+  int *host;                    // pinned, [2][n]
+  int n;
+};
+__global__ void k_chunk_end(int *iterc, int step, ChunkPoll p) {
+  const int k = *iterc + step;
+  __syncthreads();
+  const int slot = ((k - 1) / step - 1) & 1;
+  const int i = threadIdx.x;
+  if (i < p.n) __hip_atomic_store(p.host + slot * p.n + i, p.active[i][k & 1], __ATOMIC_RELAXED,
+                                  __HIP_MEMORY_SCOPE_SYSTEM);
+  if (i == 0) *iterc = k;
+  __threadfence_system();
 }
 
 // ------------------------------------------------------------- host drivers
@@ -2425,24 +2445,33 @@ static bool chunk_run(Context &c, GraphSlot &gs, bool eager, const void *key, si
 
 // Chunks until every system of every run has stopped or the iteration
 // count passes the largest capacity.  The host reads each chunk's
-// running-system counts (copied behind the chunk, stream-ordered before the
-// next chunk resets them) while the next chunk already runs, so the poll's
+// running-system counts (written by the chunk's last launch into pinned
+// memory, before the next chunk resets them) while the next chunk already
+// runs, so the poll's
 // round trip overlaps a chunk instead of idling the GPU.  A chunk launched
 // after a system has stopped does no work for it (stop flags are checked by
 // every launch; nothing is written at or past a basis capacity).
+// The chunk ends with k_chunk_end (chunk_poll below), which writes the
+// counts into the pinned slot (chunk number & 1).
+static ChunkPoll chunk_poll(Context &c, const std::vector<const int *> &active) {
+  ChunkPoll p = {};
+  p.n = (int)active.size();
+  for (int l = 0; l < p.n && l < kMaxPoll; l++) p.active[l] = active[l];
+  p.host = c.ws->h_flags;
+  return p;
+}
+
 template <typename Chunk>
 static int bms_iterate(Context &c, int *iterc, const std::vector<const int *> &active, long cap_max, Chunk &&chunk) {
   hipLaunchKernelGGL(k_set_int, dim3(1), dim3(64), 0, c.stream, iterc, 1);
   const int n = (int)active.size();
-  if (2 * n > 4096) return -1;
+  if (n > kMaxPoll) return -1;
   int *hf = c.ws->h_flags;
   hipEvent_t *pe = c.ws->poll_ev;
   int k = 1;
   auto launch_and_poll = [&](int slot) -> bool {
     if (!chunk(k == 1)) return false;
     k += kChunk;
-    for (int l = 0; l < n; l++)
-      hipMemcpyAsync(hf + slot * n + l, active[l] + (k & 1), sizeof(int), hipMemcpyDeviceToHost, c.stream);
     return hip_ok(c, hipEventRecord(pe[slot], c.stream), "poll event");
   };
   if (!launch_and_poll(0)) return -1;
@@ -2459,7 +2488,7 @@ static int bms_iterate(Context &c, int *iterc, const std::vector<const int *> &a
 
 // the chunk of one lattice: kChunk iterations of the two launches
 template <typename R, int LINK>
-static void bms_chunk(Context &c, BasisRun &br, int *iterc) {
+static void bms_chunk(Context &c, BasisRun &br, int *iterc, const ChunkPoll &poll) {
   DsArgs &hop = br.hop, &mdm = br.mdm;
   for (int q = 0; q < kChunk; q++) {
     hop.skip = &br.bsh[0].stop[(1 + q) & 1];
@@ -2473,7 +2502,7 @@ static void bms_chunk(Context &c, BasisRun &br, int *iterc) {
     hop.bms_nbr = mdm.red.nbr;
     hop.clk_nprev = (int)(mdm.nsb * mdm.nch);
   }
-  hipLaunchKernelGGL(k_add_int, dim3(1), dim3(64), 0, c.stream, iterc, kChunk);
+  hipLaunchKernelGGL(k_chunk_end, dim3(1), dim3(64), 0, c.stream, iterc, kChunk, poll);
 }
 
 // host state after the iterations, continuation bookkeeping, coefficients
@@ -2619,13 +2648,14 @@ static int ms_cg_basis(Context &c, const double2 *b, int nrhs, int nshift, const
   if (rc) return rc;
   GraphSlot &gs = c.ws->bms_graph[sizeof(R) == 8 ? 1 : 0];
   BmsGraphKey key;
+  const std::vector<const int *> active = {br.a.active};
+  const ChunkPoll poll = chunk_poll(c, active);
   auto chunk = [&](bool eager) -> bool {
     memset(&key, 0, sizeof(key));
     key.a = br.a; key.hop = br.hop; key.mdm = br.mdm; key.nbr = br.nbr;
     key.a.b = nullptr;  // the right-hand sides are read only by k_bms_init, outside the chunk
-    return chunk_run(c, gs, eager, &key, sizeof(key), [&]() { bms_chunk<R, LINK>(c, br, iterc); });
+    return chunk_run(c, gs, eager, &key, sizeof(key), [&]() { bms_chunk<R, LINK>(c, br, iterc, poll); });
   };
-  const std::vector<const int *> active = {br.a.active};
   if (bms_iterate(c, iterc, active, br.cap, chunk)) return -1;
   return bms_finish<R, LINK>(c, br, x, iters_out, x32, asm_fp32, rs);
 }
@@ -2638,10 +2668,8 @@ static int ms_cg_basis(Context &c, const double2 *b, int nrhs, int nshift, const
 // captured as one HIP graph, and the host polls every lattice's running
 // count per chunk.  Lattices whose systems have stopped cost only their
 // workgroups' early exit.  Every context must be on one stream (the
-// caller's group stream).  The launch shape is fixed (no timing): right-hand
-// sides per lane from the smallest number of systems, four lanes per site
-// while the lattices together have fewer than 2048 tiles (small lattices
-// need the extra waves), else one.
+// caller's group stream).  The launch shape is timed once per group
+// composition (group_shape).
 struct GroupJob {
   const double2 *b;
   int nrhs, nshift;
@@ -2692,6 +2720,77 @@ static void dslash_group_launch(hipStream_t st, const DsArgs *dtab, const DsArgs
 #undef DSG
 }
 
+// The lockstep launch shape of a group (right-hand sides per lane, lanes per
+// site, tile order), timed on its first solve: every candidate runs the first
+// step's hop (without its scalar step) and M^dag M over all lattices, once to
+// warm up and three times under HIP events; the r^(1), alpha and partials the
+// candidates write are rewritten by the real first step.  Kept per group
+// composition (precision, every lattice's extents and systems) in the lead
+// context's workspace.
+template <typename R, int LINK>
+static int group_shape(std::vector<Context *> &cs, std::vector<BasisRun> &runs, int nmin) {
+  const int n = (int)cs.size();
+  Context &c0 = *cs[0];
+  unsigned long long key = fnv1a(&n, sizeof n);
+  const int rb = (int)sizeof(R), lk = LINK;
+  key = fnv1a(&rb, sizeof rb, key);
+  key = fnv1a(&lk, sizeof lk, key);
+  for (int l = 0; l < n; l++) {
+    key = fnv1a(&cs[l]->L, sizeof(LatticeDims), key);
+    key = fnv1a(&runs[l].nrhs, sizeof(int), key);
+  }
+  auto &gt = c0.ws->group_tuned;
+  const auto it = gt.find(key);
+  if (it != gt.end()) return it->second;
+  std::vector<DsArgs> tab(2 * (size_t)n);
+  DsArgs *dt = (DsArgs *)ws_get(c0, S_GROUP_TAB, sizeof(DsArgs) * tab.size());
+  if (!dt) return -1;
+  hipEvent_t e0, e1;
+  hipEventCreate(&e0);
+  hipEventCreate(&e1);
+  float best = 1e30f;
+  int code = make_shape(pick_nr(nmin), 1, 0, 0);
+  for (int lanes = 1; lanes <= 4; lanes *= 4)
+    for (int nr = 1; nr <= 8 && nr <= std::max(1, nmin); nr *= 2)
+      for (int ident = 0; ident <= 1; ident++) {
+        const int cand = make_shape(nr, lanes, ident, 0);
+        for (int l = 0; l < n; l++) {
+          DsArgs hop = runs[l].hop, mdm = runs[l].mdm;
+          const int cl = cs[l]->tiles ? cand : make_shape(nr, lanes, 1, 0);
+          hop.bms_ms = nullptr;  // no scalar step while timing
+          hop.skip = &runs[l].bsh[0].stop[1];
+          mdm.skip = hop.skip;
+          hop.iter_off = -1;
+          mdm.iter_off = -1;
+          dslash_prep<DS_HOP, RED_NORM>(*cs[l], hop, cl);
+          dslash_prep<DS_BMS, RED_NORM>(*cs[l], mdm, cl);
+          mdm.bms_hnbr = hop.red.nbr;
+          hop.bms_nbr = mdm.red.nbr;
+          tab[l] = hop;
+          tab[n + l] = mdm;
+        }
+        hipMemcpyAsync(dt, tab.data(), sizeof(DsArgs) * tab.size(), hipMemcpyHostToDevice, c0.stream);
+        for (int rep = 0; rep < 4; rep++) {
+          if (rep == 1) hipEventRecord(e0, c0.stream);
+          dslash_group_launch<R, LINK, DS_HOP, RED_NORM>(c0.stream, dt, tab.data(), n, cand);
+          dslash_group_launch<R, LINK, DS_BMS, RED_NORM>(c0.stream, dt + n, tab.data() + n, n, cand);
+        }
+        hipEventRecord(e1, c0.stream);
+        hipEventSynchronize(e1);
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, e0, e1);
+        if (ms < best) {
+          best = ms;
+          code = cand;
+        }
+      }
+  hipEventDestroy(e0);
+  hipEventDestroy(e1);
+  if (!hip_ok(c0, hipGetLastError(), "group shape")) return -1;
+  gt[key] = code;
+  return code;
+}
+
 // Returns kBasisFull when some lattice's basis does not fit (nothing has run
 // then beyond set-up; the caller solves the lattices one by one), -1 on a
 // HIP error, else 0 with each job's first-pass status in jobs[l].out.
@@ -2703,7 +2802,7 @@ static int ms_cg_basis_group(std::vector<Context *> &cs, std::vector<GroupJob> &
   int *iterc = iter_ptr(c0);
   std::vector<BasisRun> runs(n);
   int nmin = 1 << 30;
-  long cap_max = 0, tiles = 0;
+  long cap_max = 0;
   for (int l = 0; l < n; l++) {
     GroupJob &j = jobs[l];
     const int rc = bms_setup<R, LINK>(*cs[l], runs[l], j.b, j.nrhs, j.nshift, j.shifts, j.res_abs, j.max_iter, iterc);
@@ -2717,10 +2816,9 @@ static int ms_cg_basis_group(std::vector<Context *> &cs, std::vector<GroupJob> &
     }
     nmin = std::min(nmin, j.nrhs);
     cap_max = std::max(cap_max, br.cap);
-    tiles += nbx_of(*cs[l]);
   }
-  const int lanes = tiles < 2048 ? 4 : 1;
-  const int code = make_shape(pick_nr(nmin), lanes, 0, 0);
+  const int code = group_shape<R, LINK>(cs, runs, nmin);
+  if (code < 0) return -1;
   // argument tables of the kChunk steps: [q][hop, M^dag M][lattice]
   std::vector<DsArgs> &htab = c0.ws->group_tab;
   htab.assign((size_t)kChunk * 2 * n, DsArgs{});
@@ -2728,7 +2826,7 @@ static int ms_cg_basis_group(std::vector<Context *> &cs, std::vector<GroupJob> &
     for (int l = 0; l < n; l++) {
       BasisRun &br = runs[l];
       DsArgs hop = br.hop, mdm = br.mdm;
-      const int cl = cs[l]->tiles ? code : make_shape(pick_nr(nmin), lanes, 1, 0);
+      const int cl = cs[l]->tiles ? code : make_shape(shape_nr(code), shape_lanes(code), 1, 0);
       hop.skip = &br.bsh[0].stop[(1 + q) & 1];
       mdm.skip = hop.skip;
       hop.iter_off = q - 1;
@@ -2748,17 +2846,18 @@ static int ms_cg_basis_group(std::vector<Context *> &cs, std::vector<GroupJob> &
   key[0] = fnv1a(htab.data(), sizeof(DsArgs) * htab.size());
   key[1] = (unsigned long long)(size_t)dtab;
   key[2] = ((unsigned long long)n << 32) | (unsigned)code;
+  std::vector<const int *> active(n);
+  for (int l = 0; l < n; l++) active[l] = runs[l].a.active;
+  const ChunkPoll poll = chunk_poll(c0, active);
   auto launch = [&]() {
     for (int q = 0; q < kChunk; q++) {
       const size_t h0 = ((size_t)q * 2 + 0) * n, m0 = ((size_t)q * 2 + 1) * n;
       dslash_group_launch<R, LINK, DS_HOP, RED_NORM>(c0.stream, dtab + h0, htab.data() + h0, n, code);
       dslash_group_launch<R, LINK, DS_BMS, RED_NORM>(c0.stream, dtab + m0, htab.data() + m0, n, code);
     }
-    hipLaunchKernelGGL(k_add_int, dim3(1), dim3(64), 0, c0.stream, iterc, kChunk);
+    hipLaunchKernelGGL(k_chunk_end, dim3(1), dim3(64), 0, c0.stream, iterc, kChunk, poll);
   };
   auto chunk = [&](bool) -> bool { return chunk_run(c0, gs, false, key, sizeof(key), launch); };
-  std::vector<const int *> active(n);
-  for (int l = 0; l < n; l++) active[l] = runs[l].a.active;
   if (bms_iterate(c0, iterc, active, cap_max, chunk)) return -1;
   for (int l = 0; l < n; l++) {
     GroupJob &j = jobs[l];
