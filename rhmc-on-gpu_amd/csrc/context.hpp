@@ -210,6 +210,11 @@ int kernel_timing_stamps(Context &c, size_t n);
 // rational sums' coefficients
 enum ScratchUse { SCR_FORCE_HOP = 0, SCR_FORCE_COEF, SCR_SOLSUM_COEF, SCR_COUNT };
 void *scratch_get(Context &c, int use, size_t bytes);
+// small host <-> device transfers through pinned staging (pageable copies
+// cost a runtime staging pass with the GPU idle): h2d is enqueued on
+// c.stream (the source may be reused at once), d2h_sync synchronises it
+bool h2d(Context &c, void *dst, const void *src, size_t bytes);
+bool d2h_sync(Context &c, void *dst, const void *src, size_t bytes);
 
 // host helpers
 bool hip_ok(Context &c, hipError_t e, const char *what);

@@ -120,7 +120,6 @@ __global__ __launch_bounds__(kBlock) void k_fermion_force(LatticeDims L, const d
 
 int fermion_force(Context &c, int nsys, const double *num, const double2 *x, int accumulate) {
   // nsys = n_pf * order systems, x = [n_pf][order][3][Vh]; num has nsys entries
-  static thread_local std::vector<double> hcoef;
   const long Vh = c.Vh;
   size_t need = sizeof(double2) * 3 * Vh * (size_t)nsys;
   // h_j = D_oe s_j for all systems (one multi-RHS hop sharing link loads),
@@ -128,27 +127,24 @@ int fermion_force(Context &c, int nsys, const double *num, const double2 *x, int
   double2 *hv = (double2 *)scratch_get(c, SCR_FORCE_HOP, need);
   double *coef = (double *)scratch_get(c, SCR_FORCE_COEF, sizeof(double) * 256);
   if (!hv || !coef || nsys > 256) return -1;
-  hcoef.assign(num, num + nsys);
-  hipMemcpyAsync(coef, hcoef.data(), sizeof(double) * nsys, hipMemcpyHostToDevice, c.stream);
+  if (!h2d(c, coef, num, sizeof(double) * nsys)) return -1;
   launch_hop_d(c, 1, x, hv, nsys, 3 * Vh, 3 * Vh);
   hipLaunchKernelGGL(k_fermion_force<double2>, dim3(nblk(Vh), 2), dim3(kBlock), 0, c.stream, c.L, c.U64, x, hv, coef,
                      nsys, c.ipdot, accumulate);
-  // stream-ordered for the caller (the staged coefficients live in hcoef)
+  // stream-ordered for the caller (the coefficients are staged in pinned memory)
   return hip_ok(c, hipGetLastError(), "fermion force") ? 0 : -1;
 }
 
 int fermion_force32(Context &c, int nsys, const double *num, const float2 *x32, int accumulate) {
-  static thread_local std::vector<double> hcoef;
   const long Vh = c.Vh;
   float2 *hv = (float2 *)scratch_get(c, SCR_FORCE_HOP, sizeof(float2) * 3 * Vh * (size_t)nsys);
   double *coef = (double *)scratch_get(c, SCR_FORCE_COEF, sizeof(double) * 256);
   if (!hv || !coef || nsys > 256) return -1;
-  hcoef.assign(num, num + nsys);
-  hipMemcpyAsync(coef, hcoef.data(), sizeof(double) * nsys, hipMemcpyHostToDevice, c.stream);
+  if (!h2d(c, coef, num, sizeof(double) * nsys)) return -1;
   launch_hop_s(c, 1, x32, hv, nsys, 3 * Vh, 3 * Vh);
   hipLaunchKernelGGL(k_fermion_force<float2>, dim3(nblk(Vh), 2), dim3(kBlock), 0, c.stream, c.L, c.U64, x32, hv, coef,
                      nsys, c.ipdot, accumulate);
-  // stream-ordered for the caller (the staged coefficients live in hcoef)
+  // stream-ordered for the caller (the coefficients are staged in pinned memory)
   return hip_ok(c, hipGetLastError(), "fermion force") ? 0 : -1;
 }
 
@@ -439,7 +435,7 @@ int sol_sum(Context &c, double2 *io, const double2 *x, int nshift, double a0, co
   double *d = (double *)scratch_get(c, SCR_SOLSUM_COEF, sizeof(double) * kMaxShifts);
   if (!d || nshift > kMaxShifts) return -1;
   std::vector<double> h(num, num + nshift);
-  hipMemcpyAsync(d, h.data(), sizeof(double) * nshift, hipMemcpyHostToDevice, c.stream);
+  if (!h2d(c, d, h.data(), sizeof(double) * nshift)) return -1;
   const long n3 = 3 * c.Vh;
   hipLaunchKernelGGL(k_sol_sum, dim3(nblk(n3), c.n_pf), dim3(kBlock), 0, c.stream, io, x, nshift, a0, d, n3);
   return hip_ok(c, hipStreamSynchronize(c.stream), "sol_sum") ? 0 : -1;
@@ -516,8 +512,7 @@ static void pairwise_sum(Context &c, double *v, long n) {
 
 int action_sum(Context &c, double *out) {
   pairwise_sum(c, c.action, c.V);
-  hipMemcpyAsync(out, c.action, sizeof(double), hipMemcpyDeviceToHost, c.stream);
-  return hip_ok(c, hipStreamSynchronize(c.stream), "action sum") ? 0 : -1;
+  return d2h_sync(c, out, c.action, sizeof(double)) ? 0 : -1;
 }
 
 // ------------------------------------------------------------ measurements

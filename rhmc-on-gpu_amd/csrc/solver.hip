@@ -72,7 +72,6 @@ struct Workspace {
   std::string tune_path;                       // tuning cache file ("" = none)
   std::string arch;                            // GPU architecture, e.g. gfx950
   bool tune_init = false;                      // cache path from RHMC_TUNE_CACHE looked up and loaded
-  std::vector<BmsHist> h_hist;                 // basis solver host arrays (reused)
   std::vector<double> h_coef;
   std::vector<float> h_coef32;
   std::vector<int> h_klen;
@@ -86,6 +85,17 @@ struct Workspace {
   unsigned *counter = nullptr;
   int *h_flags = nullptr;     // pinned
   double *h_sums = nullptr;   // pinned
+  // pinned staging of the solvers' small host <-> device transfers (a
+  // pageable copy costs the runtime a staging pass, tens of microseconds to
+  // a millisecond with the GPU idle): a ring for copies to the device (an
+  // event behind the latest one; the ring is reused only after it has
+  // executed) and a buffer for copies to the host (read after a sync)
+  char *h2d_ring = nullptr;
+  size_t h2d_cap = 0, h2d_off = 0;
+  hipEvent_t h2d_ev = nullptr;
+  bool h2d_used = false;
+  char *d2h_buf = nullptr;
+  size_t d2h_cap = 0;
   hipEvent_t poll_ev[2] = {}; // solver chunk polls, one chunk in flight ahead
 };
 
@@ -101,6 +111,7 @@ Workspace *workspace_create(Context &c) {
   if (!hip_ok(c, hipMemsetAsync(w->counter, 0, 64 * sizeof(unsigned), c.stream), "counter")) return nullptr;
   if (!hip_ok(c, hipHostMalloc(&w->h_flags, 4096 * sizeof(int), hipHostMallocDefault), "hipHostMalloc")) return nullptr;
   if (!hip_ok(c, hipHostMalloc(&w->h_sums, 4096 * sizeof(double), hipHostMallocDefault), "hipHostMalloc")) return nullptr;
+  if (!hip_ok(c, hipEventCreateWithFlags(&w->h2d_ev, hipEventDisableTiming), "staging event")) return nullptr;
   for (auto &e : w->poll_ev)
     if (!hip_ok(c, hipEventCreateWithFlags(&e, hipEventDisableTiming), "poll event")) return nullptr;
   // reduction partials for the largest launch (kMaxRed vectors), allocated once
@@ -123,6 +134,9 @@ void workspace_destroy(Workspace *w) {
   if (w->h_sums) hipHostFree(w->h_sums);
   for (auto e : w->poll_ev)
     if (e) hipEventDestroy(e);
+  if (w->h2d_ev) hipEventDestroy(w->h2d_ev);
+  if (w->h2d_ring) hipHostFree(w->h2d_ring);
+  if (w->d2h_buf) hipHostFree(w->d2h_buf);
   delete w;
 }
 
@@ -140,6 +154,60 @@ static void *ws_get(Context &c, int slot, size_t bytes) {
   w->buf[slot] = p;
   w->cap[slot] = bytes;
   return p;
+}
+
+// host -> device through the pinned ring, enqueued on c.stream
+bool h2d(Context &c, void *dst, const void *src, size_t bytes) {
+  Workspace *w = c.ws;
+  const size_t need = (bytes + 255) & ~(size_t)255;
+  if (w->h2d_off + need > w->h2d_cap) {
+    // wrap: every earlier copy from the ring has executed once the event
+    // behind the latest one has
+    if (w->h2d_used && !hip_ok(c, hipEventSynchronize(w->h2d_ev), "staging")) return false;
+    w->h2d_off = 0;
+    if (need > w->h2d_cap) {
+      if (w->h2d_ring) hipHostFree(w->h2d_ring);
+      w->h2d_ring = nullptr;
+      w->h2d_cap = 0;
+      const size_t cap = std::max<size_t>(need, 4u << 20);
+      if (!hip_ok(c, hipHostMalloc((void **)&w->h2d_ring, cap, hipHostMallocDefault), "staging ring")) return false;
+      w->h2d_cap = cap;
+    }
+  }
+  char *stage = w->h2d_ring + w->h2d_off;
+  w->h2d_off += need;
+  memcpy(stage, src, bytes);
+  if (!hip_ok(c, hipMemcpyAsync(dst, stage, bytes, hipMemcpyHostToDevice, c.stream), "staged copy")) return false;
+  w->h2d_used = true;
+  return hip_ok(c, hipEventRecord(w->h2d_ev, c.stream), "staging event");
+}
+
+// pinned host buffer of at least `bytes` for copies to the host; valid
+// until the next call (the caller syncs the stream before reading it)
+static void *d2h_stage(Context &c, size_t bytes) {
+  Workspace *w = c.ws;
+  if (w->d2h_cap < bytes) {
+    if (w->d2h_buf) {
+      hipStreamSynchronize(c.stream);
+      hipHostFree(w->d2h_buf);
+    }
+    w->d2h_buf = nullptr;
+    w->d2h_cap = 0;
+    const size_t cap = std::max<size_t>(bytes, 1u << 20);
+    if (!hip_ok(c, hipHostMalloc((void **)&w->d2h_buf, cap, hipHostMallocDefault), "staging buffer")) return nullptr;
+    w->d2h_cap = cap;
+  }
+  return w->d2h_buf;
+}
+
+// device -> host through the pinned buffer; synchronises the stream
+bool d2h_sync(Context &c, void *dst, const void *src, size_t bytes) {
+  void *stage = d2h_stage(c, bytes);
+  if (!stage) return false;
+  if (!hip_ok(c, hipMemcpyAsync(stage, src, bytes, hipMemcpyDeviceToHost, c.stream), "staged copy")) return false;
+  if (!hip_ok(c, hipStreamSynchronize(c.stream), "staged copy")) return false;
+  memcpy(dst, stage, bytes);
+  return true;
 }
 
 // workspace buffers of the gauge-field side (fields.hip), kept for the
@@ -1506,8 +1574,9 @@ __global__ void k_clk_clear(unsigned long long *t, int n) {
 void kernel_timing_harvest(Context &c, int n) {
   if (!c.kt_clk || n <= 0) return;
   n = std::min(n, c.kt_clk_cap);
-  std::vector<unsigned long long> t(2 * (size_t)n);
-  hipMemcpyAsync(t.data(), c.kt_clk, sizeof(unsigned long long) * t.size(), hipMemcpyDeviceToHost, c.stream);
+  const unsigned long long *t = (const unsigned long long *)d2h_stage(c, sizeof(unsigned long long) * 2 * (size_t)n);
+  if (!t) return;
+  hipMemcpyAsync((void *)t, c.kt_clk, sizeof(unsigned long long) * 2 * (size_t)n, hipMemcpyDeviceToHost, c.stream);
   hipLaunchKernelGGL(k_clk_clear, dim3((2 * n + 255) / 256), dim3(256), 0, c.stream, c.kt_clk, n);
   hipStreamSynchronize(c.stream);
   // Slot i holds the end of the M^dag M launch of iteration i (the launch
@@ -2197,9 +2266,8 @@ static int ms_cg(Context &c, const double2 *b, int nrhs, int nshift, const doubl
     st.max_iter = max_iter;
     for (int k = 0; k < nshift; k++) st.shift[k] = shifts[k];
   }
-  hipMemcpyAsync(ms, hs.data(), sizeof(MsState) * nrhs, hipMemcpyHostToDevice, c.stream);
   std::vector<double> hd(nrhs, c.mass * c.mass);
-  hipMemcpyAsync(diag, hd.data(), sizeof(double) * nrhs, hipMemcpyHostToDevice, c.stream);
+  if (!h2d(c, ms, hs.data(), sizeof(MsState) * nrhs) || !h2d(c, diag, hd.data(), sizeof(double) * nrhs)) return -1;
 
   const unsigned nbx = nbx_of(c);
   BlasArgs ba = {};
@@ -2235,8 +2303,7 @@ static int ms_cg(Context &c, const double2 *b, int nrhs, int nshift, const doubl
     it += chunk;
     if (it > max_iter + chunk) break;
   }
-  hipMemcpyAsync(hs.data(), ms, sizeof(MsState) * nrhs, hipMemcpyDeviceToHost, c.stream);
-  if (!hip_ok(c, hipStreamSynchronize(c.stream), "multishift")) return -1;
+  if (!d2h_sync(c, hs.data(), ms, sizeof(MsState) * nrhs)) return -1;
   int maxit = 0;
   for (int j = 0; j < nrhs; j++) maxit = std::max(maxit, hs[j].iter);
   *iters_out = maxit;
@@ -2354,9 +2421,9 @@ static int bms_setup(Context &c, BasisRun &br, const double2 *b, int nrhs, int n
     st.max_iter = max_iter;
     for (int k = 0; k < nshift; k++) st.shift[k] = shifts[k];
   }
-  hipMemcpyAsync(br.ms, br.hs.data(), sizeof(MsState) * nrhs, hipMemcpyHostToDevice, c.stream);
   std::vector<double> hd(nrhs, c.mass * c.mass);
-  hipMemcpyAsync(br.diag, hd.data(), sizeof(double) * nrhs, hipMemcpyHostToDevice, c.stream);
+  if (!h2d(c, br.ms, br.hs.data(), sizeof(MsState) * nrhs) || !h2d(c, br.diag, hd.data(), sizeof(double) * nrhs))
+    return -1;
 
   BmsArgs &a = br.a;
   a = BmsArgs{};
@@ -2515,8 +2582,7 @@ static int bms_finish(Context &c, BasisRun &br, double2 *x, int *iters_out, floa
   const int nrhs = br.nrhs, nshift = br.nshift;
   const long cap = br.cap;
   std::vector<MsState> &hs = br.hs;
-  hipMemcpyAsync(hs.data(), br.ms, sizeof(MsState) * nrhs, hipMemcpyDeviceToHost, c.stream);
-  if (!hip_ok(c, hipStreamSynchronize(c.stream), "basis multishift")) return -1;
+  if (!d2h_sync(c, hs.data(), br.ms, sizeof(MsState) * nrhs)) return -1;
   int kmax = 0;
   for (int j = 0; j < nrhs; j++) kmax = std::max(kmax, hs[j].iter);
   long &kseen = c.ws->basis_kmax[dbl ? 1 : 0];
@@ -2556,10 +2622,10 @@ static int bms_finish(Context &c, BasisRun &br, double2 *x, int *iters_out, floa
   // (host arrays sized by the iterations run, kept in the workspace)
   const size_t hst = (size_t)kmax + 2;  // history slots per rhs on the host
   const size_t cst = (size_t)kmax + 1;  // coefficients per (rhs, shift): K <= kmax
-  std::vector<BmsHist> &hh_host = c.ws->h_hist;
-  hh_host.resize((size_t)nrhs * hst * nshift);
+  const BmsHist *hh_host = (const BmsHist *)d2h_stage(c, sizeof(BmsHist) * (size_t)nrhs * hst * nshift);
+  if (!hh_host) return -1;
   for (int j = 0; j < nrhs; j++)
-    hipMemcpyAsync(hh_host.data() + (size_t)j * hst * nshift, br.hist + (size_t)j * cap * nshift,
+    hipMemcpyAsync((void *)(hh_host + (size_t)j * hst * nshift), br.hist + (size_t)j * cap * nshift,
                    sizeof(BmsHist) * (size_t)(kmax + 1) * nshift, hipMemcpyDeviceToHost, c.stream);
   if (!hip_ok(c, hipStreamSynchronize(c.stream), "basis history")) return -1;
   std::vector<double> &coef = c.ws->h_coef;
@@ -2597,8 +2663,7 @@ static int bms_finish(Context &c, BasisRun &br, double2 *x, int *iters_out, floa
     float *dcf = (float *)ws_get(c, S_COEF, sizeof(float) * cf.size() + sizeof(int) * 256);
     if (!dcf) return -1;
     int *dkl = (int *)(dcf + cf.size());
-    hipMemcpyAsync(dcf, cf.data(), sizeof(float) * cf.size(), hipMemcpyHostToDevice, c.stream);
-    hipMemcpyAsync(dkl, klen.data(), sizeof(int) * nrhs, hipMemcpyHostToDevice, c.stream);
+    if (!h2d(c, dcf, cf.data(), sizeof(float) * cf.size()) || !h2d(c, dkl, klen.data(), sizeof(int) * nrhs)) return -1;
     const dim3 grid((unsigned)((3 * Vh + 32 * (kBlock / 64) - 1) / (32 * (kBlock / 64))), nrhs, ng);
     if (x32)
       hipLaunchKernelGGL(k_bms_assemble_mx<float2>, grid, dim3(kBlock), 0, c.stream, (const float2 *)br.basis, dcf,
@@ -2610,8 +2675,8 @@ static int bms_finish(Context &c, BasisRun &br, double2 *x, int *iters_out, floa
     double *dcoef = (double *)ws_get(c, S_COEF, sizeof(double) * coef.size() + sizeof(int) * 256);
     if (!dcoef) return -1;
     int *dklen = (int *)(dcoef + coef.size());
-    hipMemcpyAsync(dcoef, coef.data(), sizeof(double) * coef.size(), hipMemcpyHostToDevice, c.stream);
-    hipMemcpyAsync(dklen, klen.data(), sizeof(int) * nrhs, hipMemcpyHostToDevice, c.stream);
+    if (!h2d(c, dcoef, coef.data(), sizeof(double) * coef.size()) || !h2d(c, dklen, klen.data(), sizeof(int) * nrhs))
+      return -1;
     const dim3 grid((unsigned)((3 * Vh + kBlock - 1) / kBlock), nrhs);
     for (int s0 = 0; s0 < nshift; s0 += 16)
       if (x32)
@@ -2769,7 +2834,7 @@ static int group_shape(std::vector<Context *> &cs, std::vector<BasisRun> &runs, 
           tab[l] = hop;
           tab[n + l] = mdm;
         }
-        hipMemcpyAsync(dt, tab.data(), sizeof(DsArgs) * tab.size(), hipMemcpyHostToDevice, c0.stream);
+        if (!h2d(c0, dt, tab.data(), sizeof(DsArgs) * tab.size())) return -1;
         for (int rep = 0; rep < 4; rep++) {
           if (rep == 1) hipEventRecord(e0, c0.stream);
           dslash_group_launch<R, LINK, DS_HOP, RED_NORM>(c0.stream, dt, tab.data(), n, cand);
@@ -2840,7 +2905,7 @@ static int ms_cg_basis_group(std::vector<Context *> &cs, std::vector<GroupJob> &
     }
   DsArgs *dtab = (DsArgs *)ws_get(c0, S_GROUP_TAB, sizeof(DsArgs) * htab.size());
   if (!dtab) return -1;
-  hipMemcpyAsync(dtab, htab.data(), sizeof(DsArgs) * htab.size(), hipMemcpyHostToDevice, c0.stream);
+  if (!h2d(c0, dtab, htab.data(), sizeof(DsArgs) * htab.size())) return -1;
   GraphSlot &gs = c0.ws->group_graph[sizeof(R) == 8 ? 1 : 0];
   unsigned long long key[3];
   key[0] = fnv1a(htab.data(), sizeof(DsArgs) * htab.size());
@@ -2943,8 +3008,8 @@ static int ms_basis_solve(Context &c, const double2 *b, int nrhs, int nshift, co
       hipLaunchKernelGGL(k_slot_to_src<R>, dim3((unsigned)((n3 + kBlock - 1) / kBlock)), dim3(kBlock), 0, c.stream,
                          v, src + (size_t)q * n3, n3);
     }
-    hipMemcpyAsync(dorig, orig2.data(), sizeof(int) * n2, hipMemcpyHostToDevice, c.stream);
-    hipMemcpyAsync(dscale, scale2.data(), sizeof(double) * scale2.size(), hipMemcpyHostToDevice, c.stream);
+    if (!h2d(c, dorig, orig2.data(), sizeof(int) * n2) || !h2d(c, dscale, scale2.data(), sizeof(double) * scale2.size()))
+      return -1;
     it = 0;
     rc = ms_cg_basis<R, LINK>(c, src, n2, nshift, shifts, res2.data(), std::max(1, max_iter - total),
                               x32 ? nullptr : (double2 *)y, &it, x32 ? (float2 *)y : nullptr, asm_fp32, &rs);
@@ -2988,8 +3053,7 @@ static int block_cg32(Context &c, const double2 *r64, int nsys, const double *sh
     hs[j].max_iter = max_iter;
     hd[j] = c.mass * c.mass + shifts[j];
   }
-  hipMemcpyAsync(cg, hs.data(), sizeof(CgState) * nsys, hipMemcpyHostToDevice, c.stream);
-  hipMemcpyAsync(diag, hd.data(), sizeof(double) * nsys, hipMemcpyHostToDevice, c.stream);
+  if (!h2d(c, cg, hs.data(), sizeof(CgState) * nsys) || !h2d(c, diag, hd.data(), sizeof(double) * nsys)) return -1;
   const unsigned nbx = nbx_of(c);
   BlasArgs ba = {};
   ba.Vh = Vh; ba.nvec = nsys; ba.nshift = 1;
@@ -3022,8 +3086,7 @@ static int block_cg32(Context &c, const double2 *r64, int nsys, const double *sh
     it += chunk;
     if (it > max_iter + chunk) break;
   }
-  hipMemcpyAsync(hs.data(), cg, sizeof(CgState) * nsys, hipMemcpyDeviceToHost, c.stream);
-  if (!hip_ok(c, hipStreamSynchronize(c.stream), "block cg")) return -1;
+  if (!d2h_sync(c, hs.data(), cg, sizeof(CgState) * nsys)) return -1;
   int maxit = 0;
   for (int j = 0; j < nsys; j++) maxit = std::max(maxit, hs[j].iter);
   *iters_out = maxit;
@@ -3045,7 +3108,7 @@ int residual_d(Context &c, const double2 *b, int b_group, const double2 *x, cons
   if (!hh || !diag || !sums || nvec > kMaxRed) return -1;
   std::vector<double> hd(nvec);
   for (int j = 0; j < nvec; j++) hd[j] = c.mass * c.mass + shifts[j];
-  hipMemcpyAsync(diag, hd.data(), sizeof(double) * nvec, hipMemcpyHostToDevice, c.stream);
+  if (!h2d(c, diag, hd.data(), sizeof(double) * nvec)) return -1;
   launch_hop_d(c, 1, x, hh, nvec, 3 * Vh, 3 * Vh);
   DsArgs a = {};
   a.par_out = 0; a.U = c.U64;
@@ -3057,8 +3120,7 @@ int residual_d(Context &c, const double2 *b, int b_group, const double2 *x, cons
   a.red = make_red(c, nvec, EPI_STORE, nullptr, sums);
   dslash_dispatch<double, LINK_D18, DS_RESID, RED_NORM>(c, a);
   finalize(c, a.red, nvec, false);
-  hipMemcpyAsync(norms2, sums, sizeof(double) * nvec, hipMemcpyDeviceToHost, c.stream);
-  return hip_ok(c, hipStreamSynchronize(c.stream), "residual") ? 0 : -1;
+  return d2h_sync(c, norms2, sums, sizeof(double) * nvec) ? 0 : -1;
 }
 
 // FP64 defect correction of the shifted solutions x ([nrhs][nshift], FP64):
@@ -3291,8 +3353,7 @@ int vec_dots(Context &c, const double2 *a, const double2 *b, int nvec, long vs, 
   red.nbr = (int)nbr;
   hipLaunchKernelGGL(k_dots, dim3(nbr, nvec), dim3(kBlock), 0, c.stream, a, b, vs, c.Vh, nvec, red);
   finalize(c, red, nvec, false);
-  hipMemcpyAsync(out, sums, sizeof(double) * nvec, hipMemcpyDeviceToHost, c.stream);
-  return hip_ok(c, hipStreamSynchronize(c.stream), "dots") ? 0 : -1;
+  return d2h_sync(c, out, sums, sizeof(double) * nvec) ? 0 : -1;
 }
 
 // Re<a1,b1> - Re<a2,b2> per vector, differenced per element before the
@@ -3324,8 +3385,7 @@ int vec_dot_diffs(Context &c, const double2 *a1, const double2 *b1, const double
   red.nbr = (int)nbr;
   hipLaunchKernelGGL(k_dot_diffs, dim3(nbr, nvec), dim3(kBlock), 0, c.stream, a1, b1, a2, b2, vs, c.Vh, nvec, red);
   finalize(c, red, nvec, false);
-  hipMemcpyAsync(out, sums, sizeof(double) * nvec, hipMemcpyDeviceToHost, c.stream);
-  return hip_ok(c, hipStreamSynchronize(c.stream), "dot diffs") ? 0 : -1;
+  return d2h_sync(c, out, sums, sizeof(double) * nvec) ? 0 : -1;
 }
 
 int vec_norms2(Context &c, const double2 *v, int nvec, long vs, double *out) {
@@ -3336,8 +3396,7 @@ int vec_norms2(Context &c, const double2 *v, int nvec, long vs, double *out) {
   red.nbr = (int)nbr;
   hipLaunchKernelGGL(k_norms, dim3(nbr, nvec), dim3(kBlock), 0, c.stream, v, vs, c.Vh, nvec, red);
   finalize(c, red, nvec, false);
-  hipMemcpyAsync(out, sums, sizeof(double) * nvec, hipMemcpyDeviceToHost, c.stream);
-  return hip_ok(c, hipStreamSynchronize(c.stream), "norms") ? 0 : -1;
+  return d2h_sync(c, out, sums, sizeof(double) * nvec) ? 0 : -1;
 }
 
 __global__ __launch_bounds__(kBlock) void k_scale_copy(double2 *dst, double2 *dst2, const double2 *src, double s,
@@ -3373,8 +3432,8 @@ static int power_iter(Context &c, const double2 *start, double tol, int is_min, 
   double *sums = (double *)ws_get(c, S_SUMS, sizeof(double) * 256);
   if (!p || !r || !hh || !diag || !sums) return -1;
   hipMemcpyAsync(p, start, sizeof(double2) * n, hipMemcpyDeviceToDevice, c.stream);
-  double m2 = c.mass * c.mass;
-  hipMemcpyAsync(diag, &m2, sizeof(double), hipMemcpyHostToDevice, c.stream);
+  const double m2 = c.mass * c.mass;
+  if (!h2d(c, diag, &m2, sizeof(double))) return -1;
   double nrm2;
   if (vec_norms2(c, p, 1, n, &nrm2)) return -1;
   double norm = sqrt(nrm2), rel;
@@ -3391,8 +3450,7 @@ static int power_iter(Context &c, const double2 *start, double tol, int is_min, 
       a.red = make_red(c, 1, EPI_STORE, nullptr, sums);
       dslash_dispatch<double, LINK_D18, DS_MDAGM, RED_NORM>(c, a);
       finalize(c, a.red, 1, false);
-      hipMemcpyAsync(&nrm2, sums, sizeof(double), hipMemcpyDeviceToHost, c.stream);
-      hipStreamSynchronize(c.stream);
+      if (!d2h_sync(c, &nrm2, sums, sizeof(double))) return -1;
     } else {
       dslash_dispatch<double, LINK_D18, DS_MDAGM, RED_NONE>(c, a);
       hipLaunchKernelGGL(k_axmy, dim3(nb), dim3(kBlock), 0, c.stream, p, r, mx, n);
