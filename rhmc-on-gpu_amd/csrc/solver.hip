@@ -898,16 +898,17 @@ __device__ __forceinline__ void ds_site_out(const DsArgs &a, const Cx<R> (&acc)[
 // c1 (r^(k-1) - r^(k-2)) into basis slot k (when rk_ok), |r^(k)|^2 partials.
 // It equals r^(k-1) - alpha s_{k-1} with s_{k-1} = w + beta s_{k-2} and
 // s_{k-2} = (r^(k-2) - r^(k-1)) / alpha_prev, so no search-direction vector
-// is stored.  All of the site's loads (r^(k-1) and, from the second
-// iteration, r^(k-2) of every right-hand side of the chunk) are issued
-// before the first store; a slot past nvec reads the last vector.
+// is stored.  bms_site_loads issues the site's loads (r^(k-1) and, from the
+// second iteration, r^(k-2) of every right-hand side of the chunk; a slot
+// past nvec reads the last vector): in FP32 before the site's hops, so that
+// they are in flight with the first hop's (38.3 against 39.4 us per launch
+// at 32^3x8, 8 right-hand sides), in FP64 after them (the registers they
+// would hold through the hops cost the FP64 launch a wave per SIMD).
 template <typename R, int NR>
-__device__ __forceinline__ void ds_site_out_bms(const DsArgs &a, const Cx<R> (&acc)[NR][3], const bool (&act)[NR],
-                                                double (&red)[NR], int jbase, int h, const Cx<R> *pinb,
-                                                const R (&ab)[2][NR], bool rk_ok, bool use_prev) {
+__device__ __forceinline__ void bms_site_loads(const DsArgs &a, int jbase, int h, const Cx<R> *pinb, bool use_prev,
+                                               Cx<R> (&pv)[NR][3], Cx<R> (&qv)[NR][3]) {
   using C = Cx<R>;
   const long Vh = a.L.Vh;
-  C pv[NR][3], qv[NR][3];
 #pragma unroll
   for (int r = 0; r < NR; r++) {
     const int j = jbase + r < a.nvec ? jbase + r : a.nvec - 1;
@@ -918,6 +919,15 @@ __device__ __forceinline__ void ds_site_out_bms(const DsArgs &a, const Cx<R> (&a
       qv[r][i] = use_prev ? p[(long)i * Vh - a.iter_stride] : pv[r][i];
     }
   }
+}
+
+template <typename R, int NR>
+__device__ __forceinline__ void ds_site_out_bms(const DsArgs &a, const Cx<R> (&acc)[NR][3], const bool (&act)[NR],
+                                                double (&red)[NR], int jbase, int h, const Cx<R> *pinb,
+                                                const R (&ab)[2][NR], bool rk_ok, const Cx<R> (&pv)[NR][3],
+                                                const Cx<R> (&qv)[NR][3]) {
+  using C = Cx<R>;
+  const long Vh = a.L.Vh;
 #pragma unroll
   for (int r = 0; r < NR; r++) {
     if (!act[r]) continue;
@@ -1002,6 +1012,9 @@ __device__ __forceinline__ void ds_body(const DsArgs &a, unsigned bid, unsigned 
       for (int k = 0; k < 3; k++) acc[r][k] = mk<R>(0, 0);
     // software pipeline over the eight hops: the operands of hop k+1 are in
     // flight while hop k is multiplied
+    constexpr bool kBmsEarly = MODE == DS_BMS && sizeof(R) == 4;
+    C pv[MODE == DS_BMS ? NR : 1][3], qv[MODE == DS_BMS ? NR : 1][3];
+    if constexpr (kBmsEarly) bms_site_loads<R, NR>(a, jbase, h, pinb, use_prev, pv, qv);
     HopIn<R, LINK, NR> buf[2];
     hop_load<R, LINK, NR>(a, inr, c, h, 0, buf[0]);
 #pragma unroll
@@ -1011,7 +1024,10 @@ __device__ __forceinline__ void ds_body(const DsArgs &a, unsigned bid, unsigned 
       if (k == 0 && xrow) xm_from_xp<R, LINK, NR>(buf[0], buf[1]);
       hop_fence<R, NR>(acc);
     }
-    if constexpr (MODE == DS_BMS) ds_site_out_bms<R, NR>(a, acc, act, red, jbase, h, pinb, bab, rk_ok, use_prev);
+    if constexpr (MODE == DS_BMS) {
+      if constexpr (!kBmsEarly) bms_site_loads<R, NR>(a, jbase, h, pinb, use_prev, pv, qv);
+      ds_site_out_bms<R, NR>(a, acc, act, red, jbase, h, pinb, bab, rk_ok, pv, qv);
+    }
     else ds_site_out<R, NR, MODE, RED>(a, acc, act, red, jbase, h, pinb);
   }
   if (RED != RED_NONE) block_partials<NR>(red, a.red.partial, jbase, a.nvec, a.nsb, sb);
