@@ -43,7 +43,7 @@ enum Slot {
   S_R64, S_P64, S_S64, S_H64, S_PS64,
   S_B32, S_R32, S_P32, S_S32, S_H32, S_PS32, S_X32,
   S_CR64, S_CX32, S_CR32, S_CP32, S_CS32, S_CH32,
-  S_PW1, S_PW2, S_PW3,
+  S_PW1, S_PW2, S_PW3, S_PWN,
   S_FH64,
   S_PARTIAL2, S_BASIS, S_BSH, S_HIST, S_COEF, S_HPART,
   S_FORCE_H, S_FORCE_COEF, S_SOLSUM_COEF,
@@ -3415,17 +3415,6 @@ int vec_norms2(Context &c, const double2 *v, int nvec, long vs, double *out) {
   return d2h_sync(c, out, sums, sizeof(double) * nvec) ? 0 : -1;
 }
 
-__global__ __launch_bounds__(kBlock) void k_scale_copy(double2 *dst, double2 *dst2, const double2 *src, double s,
-                                                       long n) {
-  const long i = (long)blockIdx.x * kBlock + threadIdx.x;
-  if (i < n) {
-    double2 v = src[i];
-    v.x *= s;
-    v.y *= s;
-    dst[i] = v;
-    if (dst2) dst2[i] = v;
-  }
-}
 // p = a*r - p
 __global__ __launch_bounds__(kBlock) void k_axmy(double2 *p, const double2 *r, double a, long n) {
   const long i = (long)blockIdx.x * kBlock + threadIdx.x;
@@ -3438,45 +3427,82 @@ __global__ __launch_bounds__(kBlock) void k_axmy(double2 *p, const double2 *r, d
   }
 }
 
-// power iterations of findminmax.cc:12-137 in FP64
+// r = p / sqrt(*n2) (the norm of the previous iteration, read on the device)
+__global__ __launch_bounds__(kBlock) void k_scale_by_norm(double2 *r, const double2 *p, const double *n2, long n) {
+  const long i = (long)blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) {
+    const double s = 1.0 / sqrt(*n2);
+    const double2 v = p[i];
+    r[i] = make_double2(s * v.x, s * v.y);
+  }
+}
+
+// |v|^2 of one vector into *dst on the device (no host round trip)
+static void norm2_to(Context &c, const double2 *v, long n, double *dst) {
+  RedArgs red = make_red(c, 1, EPI_STORE, nullptr, dst);
+  const unsigned nbr = reduce_blocks(c, 1);
+  red.nbr = (int)nbr;
+  hipLaunchKernelGGL(k_norms, dim3(nbr, 1), dim3(kBlock), 0, c.stream, v, n, n / 3, 1, red);
+  finalize(c, red, 1, false);
+}
+
+// Power iterations of findminmax.cc:12-137 in FP64: p <- A r with r =
+// p / |p|, A = M^dag M (largest eigenvalue |A r| at convergence) or
+// max - M^dag M (then min = max - |A r|), until the relative change of |p|
+// is at most tol.  Every iteration stores |p|^2 on the device and
+// normalises with it there; the host reads the norms after each chunk of
+// kPowChunk iterations and stops at the first iteration that met the test,
+// whose value it returns (the chunk's later iterations are discarded), so
+// the result is the one a test after every iteration gives, without a
+// host round trip per iteration.
+constexpr int kPowChunk = 8;
+constexpr int kPowMax = 20000;
 static int power_iter(Context &c, const double2 *start, double tol, int is_min, double mx, double *out, int *loops) {
   const long Vh = c.Vh, n = 3 * Vh;
   double2 *p = (double2 *)ws_get(c, S_PW1, sizeof(double2) * n);
   double2 *r = (double2 *)ws_get(c, S_PW2, sizeof(double2) * n);
   double2 *hh = (double2 *)ws_get(c, S_PW3, sizeof(double2) * n);
   double *diag = (double *)ws_get(c, S_DIAG, sizeof(double) * 256);
-  double *sums = (double *)ws_get(c, S_SUMS, sizeof(double) * 256);
-  if (!p || !r || !hh || !diag || !sums) return -1;
+  double *pn = (double *)ws_get(c, S_PWN, sizeof(double) * (kPowMax + kPowChunk + 1));
+  if (!p || !r || !hh || !diag || !pn) return -1;
   hipMemcpyAsync(p, start, sizeof(double2) * n, hipMemcpyDeviceToDevice, c.stream);
   const double m2 = c.mass * c.mass;
   if (!h2d(c, diag, &m2, sizeof(double))) return -1;
-  double nrm2;
-  if (vec_norms2(c, p, 1, n, &nrm2)) return -1;
-  double norm = sqrt(nrm2), rel;
+  norm2_to(c, p, n, pn);  // pn[0] = |start|^2
   const unsigned nb = (unsigned)((n + kBlock - 1) / kBlock);
-  int cnt = 0;
-  do {
-    hipLaunchKernelGGL(k_scale_copy, dim3(nb), dim3(kBlock), 0, c.stream, r, (double2 *)nullptr, p, 1.0 / norm, n);
-    const double old = norm;
-    launch_hop_d(c, 1, r, hh, 1, n, n);
-    DsArgs a = {};
-    a.par_out = 0; a.U = c.U64; a.in = hh; a.in_vs = n; a.out = p; a.out_vs = n; a.pin = r; a.pin_vs = n;
-    a.diag = diag; a.nvec = 1;
-    if (!is_min) {
-      a.red = make_red(c, 1, EPI_STORE, nullptr, sums);
-      dslash_dispatch<double, LINK_D18, DS_MDAGM, RED_NORM>(c, a);
-      finalize(c, a.red, 1, false);
-      if (!d2h_sync(c, &nrm2, sums, sizeof(double))) return -1;
-    } else {
-      dslash_dispatch<double, LINK_D18, DS_MDAGM, RED_NONE>(c, a);
-      hipLaunchKernelGGL(k_axmy, dim3(nb), dim3(kBlock), 0, c.stream, p, r, mx, n);
-      if (vec_norms2(c, p, 1, n, &nrm2)) return -1;
+  double prev = -1.0, norm = 0.0;
+  double h[kPowChunk + 1];
+  int done = 0;
+  for (int i0 = 1; !done && i0 <= kPowMax; i0 += kPowChunk) {
+    for (int i = i0; i < i0 + kPowChunk; i++) {
+      hipLaunchKernelGGL(k_scale_by_norm, dim3(nb), dim3(kBlock), 0, c.stream, r, p, pn + i - 1, n);
+      launch_hop_d(c, 1, r, hh, 1, n, n);
+      DsArgs a = {};
+      a.par_out = 0; a.U = c.U64; a.in = hh; a.in_vs = n; a.out = p; a.out_vs = n; a.pin = r; a.pin_vs = n;
+      a.diag = diag; a.nvec = 1;
+      if (!is_min) {
+        a.red = make_red(c, 1, EPI_STORE, nullptr, pn + i);
+        dslash_dispatch<double, LINK_D18, DS_MDAGM, RED_NORM>(c, a);
+        finalize(c, a.red, 1, false);
+      } else {
+        dslash_dispatch<double, LINK_D18, DS_MDAGM, RED_NONE>(c, a);
+        hipLaunchKernelGGL(k_axmy, dim3(nb), dim3(kBlock), 0, c.stream, p, r, mx, n);
+        norm2_to(c, p, n, pn + i);
+      }
     }
-    norm = sqrt(nrm2);
-    rel = fabs(old - norm) / norm;
-    cnt++;
-  } while (rel > tol && cnt < 20000);
-  *loops = cnt;
+    if (!d2h_sync(c, h, pn + i0 - 1, sizeof(double) * (kPowChunk + 1))) return -1;
+    if (prev < 0.0) prev = sqrt(h[0]);
+    for (int q = 1; q <= kPowChunk && i0 + q - 1 <= kPowMax; q++) {
+      norm = sqrt(h[q]);
+      const double rel = fabs(prev - norm) / norm;
+      prev = norm;
+      if (!(rel > tol) || i0 + q - 1 == kPowMax) {
+        *loops = i0 + q - 1;
+        done = 1;
+        break;
+      }
+    }
+  }
   *out = is_min ? mx - norm : norm;
   return hip_ok(c, hipStreamSynchronize(c.stream), "power") ? 0 : -1;
 }
