@@ -181,6 +181,56 @@ def cpu_baseline(dims, mass, beta, n_pf, shifts, md_a, chi, U, iters_total, n_fo
     return est[cores], cores, est[1], sample
 
 
+def cpu_reference_c1():
+    """The reference's own CPU path, compiled from its sources into
+    oracle/_ref/c1/ref_harness (8^3x4, am=0.05, beta=5.5, one pseudofermion,
+    12/8/12 poles, 10 Omelyan steps, residuals 1e-10: its update() without the
+    field draws), timed on one trajectory beside the oracle's restatement of
+    the same trajectory on one thread, same inputs: the ratio certifies that
+    the restatement the 32^3x8 baseline is scaled from is not slower than the
+    reference's own code.  None when the harness was not built."""
+    import subprocess
+    import tempfile
+    exe = os.path.join(ROOT, "oracle", "_ref", "c1", "ref_harness")
+    if not os.path.exists(exe):
+        return None
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    from oracle import Oracle, set_threads  # checker, timed here only as the CPU baseline
+    dims, mass, beta = (8, 8, 8, 4), 0.05, 5.5
+    lat = fields.Lattice(*dims)
+    U = fields.random_links(lat, 101, None)
+    P = fields.gauss_momenta(lat, 202)
+    phi = fields.gauss_spinor(lat, 303, 1)
+    first, md, last = rhmc_approximations(2, 1, 12, 8, 12, 1e-5)
+    eig_min, eig_max = mass * mass, 4.0 + mass * mass
+    with tempfile.TemporaryDirectory() as d:
+        U.tofile(os.path.join(d, "links.bin"))
+        P.tofile(os.path.join(d, "mom.bin"))
+        phi.tofile(os.path.join(d, "phi.bin"))
+        with open(os.path.join(d, "approx.txt"), "w") as fh:
+            fh.write(f"{eig_min!r} {eig_max!r}\n")
+            for ap in (first, md, last):
+                fh.write(f"{len(ap.a)} {ap.a0!r}\n")
+                fh.write(" ".join(repr(float(v)) for v in ap.a) + "\n")
+                fh.write(" ".join(repr(float(v)) for v in ap.b) + "\n")
+        t0 = time.perf_counter()
+        subprocess.run([exe, d, "traj"], check=True, stdout=subprocess.DEVNULL)
+        t_ref = time.perf_counter() - t0
+        dh_ref = float(np.fromfile(os.path.join(d, "out_dh.bin"), np.float64)[0])
+    o = Oracle(dims)
+    set_threads(1)
+    t0 = time.perf_counter()
+    dh = o.trajectory(U, P, phi, beta=beta, mass=mass, n_flavours=2, first=(first.a0, first.a, first.b),
+                      md=(md.a0, md.a, md.b), last=(last.a0, last.a, last.b), eig_min=eig_min, eig_max=eig_max,
+                      no_md=10, integrator="omelyan", gauge_scale=2, res_md=1e-10, res_metro=1e-10)[0]
+    t_port = time.perf_counter() - t0
+    return {"reference_s": t_ref, "port_s": t_port, "port_over_reference": t_port / t_ref, "cores": 1,
+            "delta_h_reference": dh_ref, "delta_h_port": float(dh),
+            "sample": "one 8^3x4 trajectory (am=0.05, beta=5.5, 1 pseudofermion, 12/8/12 poles, 10 Omelyan "
+                      "steps, residuals 1e-10), the reference's update() compiled from its sources vs the "
+                      "oracle's restatement, same inputs, one thread each"}
+
+
 def _bounds(g, mass):
     mx = g.find_max(1e-6, seed=11)
     mn = max(g.find_min(mx, 1e-6, seed=12), mass * mass)
@@ -640,7 +690,7 @@ def main():
         est, cores, est1, sample = cpu_baseline(dims, mass, beta, n_pf, np.array(b), np.array(a), chi, U0,
                                                 int(np.mean(iters_d)), n_force, n_q)
         cpu = {"value": est, "unit": "s/traj", "cores": cores, "kind": "port", "sample": sample,
-               "single_thread_value": est1}
+               "single_thread_value": est1, "reference_check_c1": cpu_reference_c1()}
 
     out = {
         "metric": "RHMC traj sec + Dslash GB/s (% HBM), mixed vs FP64 on GPU, 32^3x8",
