@@ -501,7 +501,8 @@ def main():
     ap.add_argument("--res-md", type=float, default=1e-6)
     ap.add_argument("--res-metro", type=float, default=1e-10)
     ap.add_argument("--metro-refined", action="store_true",
-                    help="also time mixed with FP32 + FP64 defect-correction heatbath/Metropolis solves")
+                    help="also time mixed with FP32 + FP64 defect-correction solves for the heatbath and "
+                         "Metropolis (mixed_metro_refined) or the Metropolis only (mixed_metro_refined_hb64)")
     ap.add_argument("--therm32", type=int, default=6,
                     help="headline workload: thermalisation trajectories before the mixed_thermalised leg (0: no leg)")
     ap.add_argument("--no-power-bounds", action="store_true",
@@ -571,6 +572,7 @@ def main():
     modes = {"mixed": dict(precision=1), "fp64": dict(precision=0),
              "fp64_plain": dict(precision=0, per_pf_solves=True),
              "mixed_metro_refined": dict(precision=1, metro_precision=1),
+             "mixed_metro_refined_hb64": dict(precision=1, metro_precision=2),
              "mixed_power_bounds": dict(precision=1, bounds=None, power_tol=1e-5)}
 
     def run(mode, seed, verify=False):
@@ -584,7 +586,7 @@ def main():
         return dt, r
 
     legs = ["mixed"] + ([] if args.no_fp64 else ["fp64", "fp64_plain"]) + \
-        (["mixed_metro_refined"] if args.metro_refined else []) + \
+        (["mixed_metro_refined", "mixed_metro_refined_hb64"] if args.metro_refined else []) + \
         ([] if args.no_power_bounds else ["mixed_power_bounds"])
     for w in range(args.warmup):
         for leg in legs:
@@ -593,6 +595,8 @@ def main():
     dh = {leg: [] for leg in legs}
     iters = {leg: [] for leg in legs}
     dh_err = {leg: [] for leg in legs if leg.startswith("mixed")}
+    # refined Metropolis solves: the error without the variational correction
+    dh_err_lin = {leg: [] for leg in legs if leg.startswith("mixed_metro_refined")}
     live = []
     pbounds = []
     for s in range(args.steps):
@@ -615,6 +619,8 @@ def main():
                 if leg == "mixed":
                     live.append(g.kernel_timing(0))
                 dh_err[leg].append(abs(rv.delta_h - rv.delta_h_fp64))
+                if leg in dh_err_lin:
+                    dh_err_lin[leg].append(abs(rv.delta_h - rv.metro_correction - rv.delta_h_fp64))
     # mixed_thermalised: the mixed leg on a configuration moved towards
     # equilibrium by --therm32 trajectories without the Metropolis test (the
     # reference's therm_updates), spectral bounds measured in every
@@ -716,6 +722,7 @@ def main():
         "s_per_traj": {leg: float(np.mean(v)) for leg, v in times.items()},
         "delta_h": dh,
         "delta_h_error_vs_fp64_1e-13": dh_err,
+        **({"delta_h_error_without_variational_correction": dh_err_lin} if dh_err_lin else {}),
         "cg_iterations": iters,
         "spectral_bounds": [mn, mx],
         "power_bounds_per_trajectory": pbounds,

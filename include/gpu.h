@@ -246,7 +246,10 @@ typedef struct {
                                      (heatbath, Metropolis): 0 FP64 as the
                                      reference dispatches them
                                      (cu_inverter.cc:45-52), 1 FP32 iterations
-                                     + FP64 defect correction               */
+                                     + FP64 defect correction for both, 2 for
+                                     the Metropolis solve only; a refined
+                                     Metropolis solve's pole terms are taken
+                                     variationally (metro_correction)       */
   int per_pf_solves;              /* 1: one pseudofermion per solve with
                                      per-shift search directions, the
                                      reference's loop (cu_inverter.cc:42-75);
@@ -266,6 +269,14 @@ typedef struct {
   double eig_min, eig_max;
   double seconds_total, seconds_solver, seconds_force, seconds_update;  /* without the verification */
   double seconds_verify;          /* the FP64 recomputation (verify_fp64=1)   */
+  double metro_correction;        /* included in delta_h when the Metropolis
+                                     solve is refined (metro_precision 1, 2):
+                                     sum over pseudofermions and poles of
+                                     a_s Re<x_s, r_s>, r_s = chi - (MdagM +
+                                     b_s) x_s the FP64 true residual, so that
+                                     each pole term is 2 Re<chi, x_s> -
+                                     <x_s, (MdagM + b_s) x_s>, with an error
+                                     quadratic in the solution's; 0 otherwise */
 } rhmc_traj_result;
 
 int gpu_trajectory(rhmc_gpu_t h, const rhmc_traj_params *p, rhmc_traj_result *r);

@@ -100,6 +100,11 @@ struct Context {
   int last_iters = 0, last_refine_cycles = 0, last_iters_fp32 = 0, last_iters_refine = 0;
   double last_true_res_max = 0.0;  // largest |b - (MdagM+s) x| / target of the last checked solve
   int last_restarts = 0;           // continuations past a full stored basis in the last solve
+  // With want_xr set, every FP64-checked solve appends Re<x_j, r_j> of each
+  // shifted system j (x its solution, r = b - (MdagM + s_j) x its FP64 true
+  // residual) to last_xr; the caller clears it.
+  bool want_xr = false;
+  std::vector<double> last_xr;
   int graph_captures = 0;          // solver chunk graphs captured (instantiated) so far
 };
 

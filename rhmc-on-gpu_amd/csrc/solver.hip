@@ -3193,6 +3193,12 @@ static int refine_fp64(Context &c, const double2 *b, int nrhs, int nshift, const
     c.last_error = "multishift: defect correction did not reach the target";
     return kStatusMaxCg;
   }
+  if (c.want_xr) {
+    // r64 holds the true residuals of the final x (the last check passed)
+    const size_t n0 = c.last_xr.size();
+    c.last_xr.resize(n0 + nsys);
+    if (vec_dots(c, x, r64, nsys, 3 * Vh, c.last_xr.data() + n0)) return -1;
+  }
   return 0;
 }
 

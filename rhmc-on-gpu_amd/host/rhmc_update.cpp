@@ -38,6 +38,9 @@ using clk = std::chrono::steady_clock;
 
 double secs(clk::time_point t0) { return std::chrono::duration<double>(clk::now() - t0).count(); }
 
+// Traj::solve's prec argument for the Metropolis solve's dispatch
+constexpr int kPrecMetro = -2;
+
 struct Coeffs {
   double a0 = 0.0;
   std::vector<double> a, b;
@@ -67,17 +70,19 @@ struct Lat {
   Coeffs md;              // rescaled MD approximation
   double mn = 0, mx = 0;  // spectral bounds of this trajectory
   int status = 0;         // first solver/HIP failure (solver codes, -1 HIP)
+  double metro_corr = 0;  // variational correction of the new fermion action
 
   Lat(Context &cc, const rhmc_traj_params &pp, rhmc_traj_result &rr) : c(cc), p(pp), r(rr) {}
 
   // mixed runs solves at or above fp32_threshold in FP32 only and the rest
   // in FP64, like the reference's inv_single_double_prec dispatch
-  // (cu_inverter.cc:45-60), or with metro_precision=1 the rest as FP32
-  // iterations with FP64 defect correction
-  int prec_for(double res) const {
+  // (cu_inverter.cc:45-60); the rest run as FP32 iterations with FP64
+  // defect correction instead with metro_precision=1 (heatbath and
+  // Metropolis) or 2 (the Metropolis solve only, metro = true)
+  int prec_for(double res, bool metro = false) const {
     if (p.precision != PREC_MIXED) return p.precision;
     if (p.fp32_threshold > 0.0 && res >= p.fp32_threshold) return PREC_FP32;
-    return p.metro_precision == 1 ? PREC_MIXED : PREC_FP64;
+    return p.metro_precision == 1 || (p.metro_precision == 2 && metro) ? PREC_MIXED : PREC_FP64;
   }
   // MD solve in FP32 with FP32 solutions and the FP32-input force
   bool md_fp32() const { return prec_for(p.res_md) == PREC_FP32 && !p.per_pf_solves; }
@@ -112,7 +117,8 @@ struct Traj {
   hipStream_t stream() const { return L[0].c.stream; }
 
   // One multi-shift solve per lattice of b[l] with the poles of co[l] to
-  // residual res: prec < 0 takes each lattice's dispatch for res; f32 writes
+  // residual res: prec < 0 takes each lattice's dispatch for res (kPrecMetro:
+  // the dispatch of the Metropolis solve); f32 writes
   // FP32 solutions into smf32 (the MD path), else FP64 into x[l] (default
   // smf).  Lattices in lockstep unless one pseudofermion at a time is asked
   // (the reference's per-pseudofermion loop, per lattice).
@@ -127,7 +133,7 @@ struct Traj {
       Context &c = la.c;
       cs[l] = &c;
       const int order = (int)co[l]->b.size();
-      const int pr = f32 ? PREC_FP32 : prec < 0 ? la.prec_for(res) : prec;
+      const int pr = f32 ? PREC_FP32 : prec < 0 ? la.prec_for(res, prec == kPrecMetro) : prec;
       if (f32 && !c.smf32) {
         c.smf32 = (float2 *)dev_alloc(c, sizeof(float2) * 3 * c.Vh * (size_t)c.n_pf * c.max_order);
         if (!c.smf32) la.fail(-1);
@@ -326,6 +332,10 @@ int check_params(Context &c, const rhmc_traj_params &p) {
     c.last_error = "trajectory: bad MD parameters";
     return RHMC_EINVAL;
   }
+  if (p.metro_precision < 0 || p.metro_precision > 2) {
+    c.last_error = "trajectory: metro_precision must be 0, 1 or 2";
+    return RHMC_EINVAL;
+  }
   if (!p.pure_gauge && (p.order_first < 1 || p.order_md < 1 || p.order_last < 1 || p.order_first > c.max_order ||
                         p.order_md > c.max_order || p.order_last > c.max_order)) {
     c.last_error = "trajectory: bad rational orders";
@@ -433,7 +443,9 @@ int verify_delta_h(Lat &la) {
   }
   double diff = 0.0;
   for (int ps = 0; ps < c.n_pf; ps++) diff += d[ps];
-  r.delta_h_fp64 = r.delta_h + diff;
+  // the FP64 recomputation's new action is Re<phi_new64, chi64>, without the
+  // variational correction of a refined Metropolis solve
+  r.delta_h_fp64 = r.delta_h - r.metro_correction + diff;
   r.seconds_verify = secs(tv);
   return RHMC_OK;
 }
@@ -499,19 +511,43 @@ int run_trajectory_group(std::vector<Context *> &cs, std::vector<const rhmc_traj
         b[l] = la.c.chi;
         co[l] = &cl[l];
       }
-      T.solve(b, co, T.p.res_metro, -1, false);
+      // A solve refined by FP64 defect correction is not Galerkin: Re<chi, x_s>
+      // then carries an error linear in x_s's error.  The variational estimate
+      // 2 Re<chi, x> - <x, (MdagM + s) x> = Re<chi, x> + Re<x, r> (r the true
+      // residual) differs from <chi, (MdagM + s)^-1 chi> by -<e, (MdagM + s) e>
+      // only, quadratic in the error e; its correction term sum_s a_s Re<x_s, r_s>
+      // is added to the new action.  (For the basis solver from zero r is
+      // orthogonal to the Krylov space that holds x, so the term vanishes.)
+      for (Lat &la : T.L) {
+        la.c.want_xr = la.prec_for(T.p.res_metro, true) == PREC_MIXED;
+        la.c.last_xr.clear();
+      }
+      T.solve(b, co, T.p.res_metro, kPrecMetro, false);
+      for (Lat &la : T.L) la.c.want_xr = false;
       if (!T.ok()) return T.abi_status();
       for (int l = 0; l < n; l++) {
-        Context &c = T.L[l].c;
+        Lat &la = T.L[l];
+        Context &c = la.c;
         hipMemcpyAsync(c.phinew, c.chi, sizeof(double2) * 3 * c.Vh * c.n_pf, hipMemcpyDeviceToDevice, c.stream);
-        if (sol_sum(c, c.phinew, c.smf, T.L[l].p.order_last, cl[l].a0, cl[l].a.data())) return RHMC_EHIP;
+        if (sol_sum(c, c.phinew, c.smf, la.p.order_last, cl[l].a0, cl[l].a.data())) return RHMC_EHIP;
+        la.metro_corr = 0.0;
+        const int order = la.p.order_last;
+        if (la.prec_for(T.p.res_metro, true) == PREC_MIXED) {
+          if (c.last_xr.size() != (size_t)c.n_pf * order) {
+            c.last_error = "trajectory: Metropolis residual dots missing";
+            return RHMC_EHIP;
+          }
+          for (int ps = 0; ps < c.n_pf; ps++)
+            for (int k = 0; k < order; k++) la.metro_corr += cl[l].a[k] * c.last_xr[(size_t)ps * order + k];
+        }
       }
     }
     for (Lat &la : T.L) {
       action_local(la.c, la.c.phinew, la.c.chi, !la.p.pure_gauge, -1);
       double s = 0.0;
       if (action_sum(la.c, &s)) return RHMC_EHIP;
-      la.r.delta_h = -s;
+      la.r.delta_h = -s + la.metro_corr;
+      la.r.metro_correction = la.metro_corr;
       la.r.delta_h_fp64 = la.r.delta_h;
       la.r.seconds_total = secs(t0);
       la.r.seconds_solver = T.t_solve;

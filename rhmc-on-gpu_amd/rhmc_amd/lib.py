@@ -82,7 +82,8 @@ class TrajResult(C.Structure):
                 ("cg_iterations_fp32", C.c_int),
                 ("n_solves", C.c_int), ("eig_min", C.c_double), ("eig_max", C.c_double),
                 ("seconds_total", C.c_double), ("seconds_solver", C.c_double),
-                ("seconds_force", C.c_double), ("seconds_update", C.c_double), ("seconds_verify", C.c_double)]
+                ("seconds_force", C.c_double), ("seconds_update", C.c_double), ("seconds_verify", C.c_double),
+                ("metro_correction", C.c_double)]
 
 
 VP = C.c_void_p
@@ -165,7 +166,9 @@ def traj_params(*, n_flavours, first, md, last, no_md, integrator="omelyan", gau
     for fixed spectral bounds, None to measure them with power iterations.
     precision=1 (mixed): solves with res >= fp32_threshold run in FP32, the
     others in FP64 (metro_precision=0) or as FP32 iterations with FP64
-    defect correction (metro_precision=1).  per_pf_solves: one
+    defect correction (metro_precision=1: heatbath and Metropolis, 2: the
+    Metropolis solve only; a refined Metropolis solve's pole terms are taken
+    variationally, see TrajResult.metro_correction).  per_pf_solves: one
     pseudofermion per solve with per-shift search directions (the
     reference's loop) instead of one multi-RHS solve."""
     keep = []

@@ -23,7 +23,7 @@ import pytest
 import rng
 from oracle import Oracle
 from rhmc_amd import Gpu, RhmcError, fields, multi_trajectory
-from rhmc_amd.lib import RHMC_EMAXCG, RHMC_ENONFINITE
+from rhmc_amd.lib import RHMC_EINVAL, RHMC_EMAXCG, RHMC_ENONFINITE
 from rhmc_amd.remez import rhmc_approximations
 
 pytestmark = pytest.mark.gpu
@@ -238,6 +238,42 @@ def test_c4_32c8_residuals_and_delta_h():
     err = abs(r.delta_h - r.delta_h_fp64)
     say("C4 trajectory", r.seconds_total, "s dH", r.delta_h, "dH(FP64 1e-13)", r.delta_h_fp64, "error", err)
     assert err <= 1e-9
+    g.close()
+
+
+def test_c4_refined_metropolis_is_variational():
+    """C4 trajectory with the Metropolis solve as FP32 iterations + FP64
+    defect correction (metro_precision=2, heatbath FP64).  The refined
+    solutions are not Galerkin, so each pole term is taken as
+    Re<chi, x> + Re<x, r> (r the FP64 true residual), whose error is
+    quadratic in the solution's: Delta H within 1e-9 of the FP64
+    recomputation at 1e-13.  The correction itself is nonzero, and zero
+    with the FP64 Metropolis solve (metro_precision=0)."""
+    dims, mass, beta, n_pf, order = (32, 32, 32, 8), 0.01, 5.6, 8, 16
+    lat = fields.Lattice(*dims)
+    U = fields.random_links(lat, 3208, 0.3)
+    first, md, last = rhmc_approximations(2, n_pf, order, order, order)
+    g = Gpu(dims, mass=mass, beta=beta, n_pf=n_pf, max_order=order)
+    g.init0(U)
+    mn, mx = measured_bounds(g, mass)
+    co = dict(first=(first.a0, first.a, first.b), md=(md.a0, md.a, md.b), last=(last.a0, last.a, last.b))
+    kw = dict(precision=1, fp32_threshold=1e-7, n_flavours=2, no_md=10, integrator="omelyan", res_md=1e-6,
+              res_metro=1e-10, relative=True, bounds=(mn, mx), draw_fields=True, seed=78, verify_fp64=True,
+              verify_res=1e-13, **co)
+    out = {}
+    for mp in (0, 2):
+        g.init0(U)
+        r = g.trajectory(metro_precision=mp, **kw)
+        err = abs(r.delta_h - r.delta_h_fp64)
+        lin = abs(r.delta_h - r.metro_correction - r.delta_h_fp64)
+        say("C4 metro_precision", mp, "dH", r.delta_h, "error", err, "without the correction", lin,
+            "correction", r.metro_correction, "iterations", r.cg_iterations, "s", r.seconds_total)
+        out[mp] = (r, err)
+    assert out[0][0].metro_correction == 0.0
+    assert out[2][0].metro_correction != 0.0
+    assert out[2][1] <= 1e-9
+    # same fields, same MD: the two Delta H agree to the Metropolis accuracy
+    assert abs(out[2][0].delta_h - out[0][0].delta_h) <= 2e-9
     g.close()
 
 
@@ -511,6 +547,24 @@ def test_nonfinite_links_and_max_cg(precision, monkeypatch):
         g.trajectory(precision=precision if precision < 2 else 1, n_flavours=2, no_md=2, res_md=1e-6,
                      res_metro=1e-10, relative=True, bounds=(0.01, 5.0), draw_fields=True, **co)
     assert e.value.rc == RHMC_ENONFINITE
+    g.close()
+
+
+def test_trajectory_rejects_bad_metro_precision():
+    """metro_precision outside 0-2 is RHMC_EINVAL with a message, before any
+    work is done."""
+    dims = (4, 4, 4, 4)
+    lat = fields.Lattice(*dims)
+    g = Gpu(dims, mass=0.1, n_pf=1, max_order=12)
+    g.init0(fields.random_links(lat, 5, 0.3))
+    gd = dict(np.load(os.path.join(GOLD, "golden_c1.npz")))
+    co = dict(first=(float(gd["first_a0"]), gd["first_a"], gd["first_b"]),
+              md=(float(gd["md_a0"]), gd["md_a"], gd["md_b"]),
+              last=(float(gd["last_a0"]), gd["last_a"], gd["last_b"]))
+    with pytest.raises(RhmcError) as e:
+        g.trajectory(precision=1, metro_precision=3, n_flavours=2, no_md=2, res_md=1e-6, res_metro=1e-10,
+                     relative=True, bounds=(0.01, 5.0), draw_fields=True, **co)
+    assert e.value.rc == RHMC_EINVAL and "metro_precision" in str(e.value)
     g.close()
 
 

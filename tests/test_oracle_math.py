@@ -205,6 +205,50 @@ def test_multishift_against_dense_solve():
     assert np.allclose(M, mass ** 2 * np.eye(M.shape[0]) - Deo @ Doe, atol=1e-13)
 
 
+def test_variational_pole_estimate():
+    """The Metropolis pole term of a defect-corrected solve is taken as
+    Re<b, x> + Re<x, r>, r = b - (MdagM + s) x its true residual (the
+    trajectory's metro_correction).  It equals <b, (MdagM + s)^-1 b> minus
+    <e, (MdagM + s) e> exactly (e = x - x*), so its error is quadratic in the
+    solution error where Re<b, x> alone is linear; a flipped sign or a
+    missing term of the correction fails the identity.  For the CG from zero
+    the correction vanishes: x lies in the Krylov space that r is orthogonal
+    to."""
+    dims = (4, 4, 4, 4)
+    mass = 0.2
+    lat = fields.Lattice(*dims)
+    o = Oracle(dims)
+    U = fields.random_links(lat, 5, 0.4)
+    M = dense_mdagm(o, U, mass)
+    b = fields.gauss_spinor(lat, 7)[0]
+    s = 0.05
+    A = M + s * np.eye(M.shape[0])
+    xs = np.linalg.solve(A, b.ravel())
+    exact = np.vdot(b.ravel(), xs).real
+    rng = np.random.default_rng(3)
+    d = rng.standard_normal(xs.shape) + 1j * rng.standard_normal(xs.shape)
+    d *= np.linalg.norm(xs) / np.linalg.norm(d)
+    errs_lin, errs_var = [], []
+    for t in (1e-3, 1e-4, 1e-5):
+        e = t * d
+        x = (xs + e).reshape(b.shape)
+        r = b - o.mdagm(U, mass, x) - s * x  # the oracle's operator, not the dense matrix
+        lin = np.vdot(b.ravel(), x.ravel()).real
+        var = lin + np.vdot(x.ravel(), r.ravel()).real
+        q = np.vdot(e, A @ e).real
+        assert abs((var - exact) + q) < 1e-12 * abs(exact)
+        errs_lin.append(abs(lin - exact))
+        errs_var.append(abs(var - exact))
+    for k in range(2):
+        assert 5 < errs_lin[k] / errs_lin[k + 1] < 20  # linear
+        assert 50 < errs_var[k] / errs_var[k + 1] < 200  # quadratic
+    # a CG solution from zero (stopped early): Re<x, r> vanishes to rounding
+    x, _ = o.multishift(U, mass, b, np.array([s]), 1e-3)
+    r = b - o.mdagm(U, mass, x[0]) - s * x[0]
+    assert np.linalg.norm(r) < 1e-2 * np.linalg.norm(b)
+    assert abs(np.vdot(x[0].ravel(), r.ravel())) < 1e-10 * np.linalg.norm(x[0]) * np.linalg.norm(r)  # ~1e-3 when not Galerkin
+
+
 @pytest.mark.parametrize("seed,eps", [(4, None), (8, 0.5)])
 def test_power_method_against_dense_spectrum(seed, eps):
     """orc_findmaxeig / orc_findmineig (findminmax.cc:12-137) converge to the
