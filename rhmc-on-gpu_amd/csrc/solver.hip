@@ -72,9 +72,6 @@ struct Workspace {
   std::string tune_path;                       // tuning cache file ("" = none)
   std::string arch;                            // GPU architecture, e.g. gfx950
   bool tune_init = false;                      // cache path from RHMC_TUNE_CACHE looked up and loaded
-  std::vector<double> h_coef;
-  std::vector<float> h_coef32;
-  std::vector<int> h_klen;
   GraphSlot bms_graph[2];                      // FP32 / FP64 stored-basis chunk
   GraphSlot group_graph[2];                    // the same for a group of lattices led by this context
   std::vector<DsArgs> group_tab;               // argument tables of the group's chunk (host copy)
@@ -2588,8 +2585,64 @@ static void bms_chunk(Context &c, BasisRun &br, int *iterc, const ChunkPoll &pol
   hipLaunchKernelGGL(k_chunk_end, dim3(1), dim3(64), 0, c.stream, iterc, kChunk, poll);
 }
 
-// host state after the iterations, continuation bookkeeping, coefficients
-// and the assembly of the shifted solutions
+// Coefficients of the assembly, on the device:  d_{s,i} = -z_s^(i) S_i with
+// S_{K-1} = om^(K), S_i = om^(i+1) + ga^(i+1) S_{i+1}, K the iteration at
+// which shift s froze.  S_i = f_i(f_{i+1}(... f_{K-1}(0))) for the affine maps
+// f_i(S) = om^(i+1) + ga^(i+1) S.  One wave per (rhs, shift slot): lane l
+// composes the maps of its contiguous segment of 0..K-1, a suffix scan over
+// the wave's lanes composes the segments after it (the value entering the
+// segment), and the lane runs its segment's recurrence from there.  Rows of
+// slots K..klen[rhs]-1 (rounded up to 16 for the matrix-core layout) and
+// absent shifts are written as zero; klen[rhs] = the largest K of its
+// shifts.  Layouts: FP64 [rhs][shift][cst], or with c32 the matrix-core
+// assembly's FP32 [rhs][shift group][kp][16].
+__global__ __launch_bounds__(64) void k_bms_coef(const MsState *ms, const BmsHist *hist, long cap, int nshift,
+                                                 double *c64, long cst, float *c32, long kp, int *klen) {
+  const int sft = blockIdx.x, j = blockIdx.y, lane = threadIdx.x;
+  const MsState &st = ms[j];
+  int kmaxj = 0;
+  for (int q = 0; q < nshift; q++) kmaxj = max(kmaxj, st.flag[q] ? st.iter : st.its[q]);
+  const int K = sft < nshift ? (st.flag[sft] ? st.iter : st.its[sft]) : 0;
+  if (sft == 0 && lane == 0) klen[j] = kmaxj;
+  const BmsHist *hj = hist + (long)j * cap * nshift + sft;  // entry i at hj[i * nshift]
+  const int seg = (K + 63) / 64;
+  const int i0 = min(lane * seg, K), i1 = min(i0 + seg, K);
+  // this segment's map f_{i0} o ... o f_{i1-1}: S -> A + B S
+  double A = 0.0, B = 1.0;
+  for (int i = i1 - 1; i >= i0; i--) {
+    const BmsHist n1 = hj[(long)(i + 1) * nshift];
+    A = n1.om + n1.ga * A;
+    B = n1.ga * B;
+  }
+  // inclusive suffix composition P_l = M_l o M_{l+1} o ... o M_63
+  for (int off = 1; off < 64; off *= 2) {
+    const double a2 = __shfl_down(A, off), b2 = __shfl_down(B, off);
+    if (lane + off < 64) {
+      A = A + B * a2;
+      B = B * b2;
+    }
+  }
+  // S entering the segment (S_{i1}) = P_{l+1}(0); S_K = 0
+  double S = __shfl_down(A, 1);
+  if (lane == 63) S = 0.0;
+  for (int i = i1 - 1; i >= i0; i--) {
+    const BmsHist n1 = hj[(long)(i + 1) * nshift];
+    S = n1.om + n1.ga * S;
+    const double d = -(i == 0 ? 1.0 : hj[(long)i * nshift].z) * S;
+    if (c32) c32[((long)(j * (int)gridDim.x / 16 + sft / 16) * kp + i) * 16 + sft % 16] = (float)d;
+    else c64[((long)j * nshift + sft) * cst + i] = d;
+  }
+  const long k16 = (long)(kmaxj + 15) / 16 * 16;
+  const int kz = c32 ? (int)(k16 < kp ? k16 : kp) : kmaxj;
+  for (int i = K + lane; i < kz; i += 64) {
+    if (c32) c32[((long)(j * (int)gridDim.x / 16 + sft / 16) * kp + i) * 16 + sft % 16] = 0.f;
+    else c64[((long)j * nshift + sft) * cst + i] = 0.0;
+  }
+}
+
+// Assembly of the shifted solutions (coefficients on the device, queued
+// behind the last chunk), then the host state after the iterations:
+// statuses and continuation bookkeeping.
 template <typename R, int LINK>
 static int bms_finish(Context &c, BasisRun &br, double2 *x, int *iters_out, float2 *x32, bool asm_fp32,
                       BasisRestart *rs) {
@@ -2597,6 +2650,43 @@ static int bms_finish(Context &c, BasisRun &br, double2 *x, int *iters_out, floa
   const bool dbl = sizeof(R) == 8;
   const int nrhs = br.nrhs, nshift = br.nshift;
   const long cap = br.cap;
+  // every coefficient index i < K <= iterations run < cap
+  const long cst = cap + 1;
+  if (asm_fp32 && sizeof(R) == 4) {
+    // matrix-core assembly: FP32 coefficients [rhs][shift group][kp][16]
+    const int ng = (nshift + 15) / 16;
+    const long kp = (cst + 15) / 16 * 16;
+    const size_t ncf = (size_t)nrhs * ng * kp * 16;
+    float *dcf = (float *)ws_get(c, S_COEF, sizeof(float) * ncf + sizeof(int) * 256);
+    if (!dcf) return -1;
+    int *dkl = (int *)(dcf + ncf);
+    hipLaunchKernelGGL(k_bms_coef, dim3(16 * ng, nrhs), dim3(64), 0, c.stream, br.ms, br.hist, cap, nshift,
+                       (double *)nullptr, cst, dcf, kp, dkl);
+    const dim3 grid((unsigned)((3 * Vh + 32 * (kBlock / 64) - 1) / (32 * (kBlock / 64))), nrhs, ng);
+    if (x32)
+      hipLaunchKernelGGL(k_bms_assemble_mx<float2>, grid, dim3(kBlock), 0, c.stream, (const float2 *)br.basis, dcf,
+                         dkl, x32, Vh, (int)kp, nshift, nrhs);
+    else
+      hipLaunchKernelGGL(k_bms_assemble_mx<double2>, grid, dim3(kBlock), 0, c.stream, (const float2 *)br.basis, dcf,
+                         dkl, x, Vh, (int)kp, nshift, nrhs);
+  } else {
+    const size_t ncf = (size_t)nrhs * nshift * cst;
+    double *dcoef = (double *)ws_get(c, S_COEF, sizeof(double) * ncf + sizeof(int) * 256);
+    if (!dcoef) return -1;
+    int *dklen = (int *)(dcoef + ncf);
+    hipLaunchKernelGGL(k_bms_coef, dim3(nshift, nrhs), dim3(64), 0, c.stream, br.ms, br.hist, cap, nshift, dcoef,
+                       cst, (float *)nullptr, 0L, dklen);
+    const dim3 grid((unsigned)((3 * Vh + kBlock - 1) / kBlock), nrhs);
+    for (int s0 = 0; s0 < nshift; s0 += 16)
+      if (x32)
+        hipLaunchKernelGGL((k_bms_assemble<R, 16, float2>), grid, dim3(kBlock), 0, c.stream, br.basis, dcoef, dklen,
+                           x32, Vh, (int)cst, nshift, s0, nrhs);
+      else
+        hipLaunchKernelGGL((k_bms_assemble<R, 16, double2>), grid, dim3(kBlock), 0, c.stream, br.basis, dcoef, dklen,
+                           x, Vh, (int)cst, nshift, s0, nrhs);
+  }
+  if (!hip_ok(c, hipGetLastError(), "basis assemble")) return -1;
+  // the systems' final state (after the assembly: one wait for both)
   std::vector<MsState> &hs = br.hs;
   if (!d2h_sync(c, hs.data(), br.ms, sizeof(MsState) * nrhs)) return -1;
   int kmax = 0;
@@ -2612,7 +2702,7 @@ static int bms_finish(Context &c, BasisRun &br, double2 *x, int *iters_out, floa
       return kStatusNonFinite;
     }
   // basis filled before convergence: the solutions so far are assembled
-  // below and the caller continues from r^(K)
+  // above and the caller continues from r^(K)
   if (rs) {
     rs->rhs.clear();
     rs->slot.clear();
@@ -2634,76 +2724,6 @@ static int bms_finish(Context &c, BasisRun &br, double2 *x, int *iters_out, floa
     c.last_error = "multishift: stored basis filled";
     return kStatusMaxCg;
   }
-  // coefficients d_{s,i} = -z_s^(i) S_i, S_{K-1} = om^(K), S_i = om^(i+1) + ga^(i+1) S_{i+1}
-  // (host arrays sized by the iterations run, kept in the workspace)
-  const size_t hst = (size_t)kmax + 2;  // history slots per rhs on the host
-  const size_t cst = (size_t)kmax + 1;  // coefficients per (rhs, shift): K <= kmax
-  const BmsHist *hh_host = (const BmsHist *)d2h_stage(c, sizeof(BmsHist) * (size_t)nrhs * hst * nshift);
-  if (!hh_host) return -1;
-  for (int j = 0; j < nrhs; j++)
-    hipMemcpyAsync((void *)(hh_host + (size_t)j * hst * nshift), br.hist + (size_t)j * cap * nshift,
-                   sizeof(BmsHist) * (size_t)(kmax + 1) * nshift, hipMemcpyDeviceToHost, c.stream);
-  if (!hip_ok(c, hipStreamSynchronize(c.stream), "basis history")) return -1;
-  std::vector<double> &coef = c.ws->h_coef;
-  coef.assign((size_t)nrhs * nshift * cst, 0.0);
-  for (int j = 0; j < nrhs; j++)
-    for (int sft = 0; sft < nshift; sft++) {
-      const int K = hs[j].flag[sft] ? hs[j].iter : hs[j].its[sft];
-      double S = 0.0;
-      double *cf = coef.data() + ((size_t)j * nshift + sft) * cst;
-      for (int i = K - 1; i >= 0; i--) {
-        const BmsHist &n1 = hh_host[((size_t)j * hst + i + 1) * nshift + sft];
-        S = n1.om + n1.ga * S;
-        const double z = i == 0 ? 1.0 : hh_host[((size_t)j * hst + i) * nshift + sft].z;
-        cf[i] = -z * S;
-      }
-    }
-  // per-rhs basis length = largest K over its shifts
-  std::vector<int> &klen = c.ws->h_klen;
-  klen.assign(nrhs, 0);
-  for (int j = 0; j < nrhs; j++)
-    for (int sft = 0; sft < nshift; sft++)
-      klen[j] = std::max(klen[j], hs[j].flag[sft] ? hs[j].iter : hs[j].its[sft]);
-  if (asm_fp32 && sizeof(R) == 4) {
-    // matrix-core assembly: FP32 coefficients [rhs][shift group][kp][16]
-    const int ng = (nshift + 15) / 16;
-    const size_t kp = (cst + 15) / 16 * 16;
-    std::vector<float> &cf = c.ws->h_coef32;
-    cf.assign((size_t)nrhs * ng * kp * 16, 0.f);
-    for (int j = 0; j < nrhs; j++)
-      for (int sft = 0; sft < nshift; sft++) {
-        const double *src = coef.data() + ((size_t)j * nshift + sft) * cst;
-        float *dst = cf.data() + ((size_t)j * ng + sft / 16) * kp * 16 + sft % 16;
-        for (size_t i = 0; i < cst; i++) dst[i * 16] = (float)src[i];
-      }
-    float *dcf = (float *)ws_get(c, S_COEF, sizeof(float) * cf.size() + sizeof(int) * 256);
-    if (!dcf) return -1;
-    int *dkl = (int *)(dcf + cf.size());
-    if (!h2d(c, dcf, cf.data(), sizeof(float) * cf.size()) || !h2d(c, dkl, klen.data(), sizeof(int) * nrhs)) return -1;
-    const dim3 grid((unsigned)((3 * Vh + 32 * (kBlock / 64) - 1) / (32 * (kBlock / 64))), nrhs, ng);
-    if (x32)
-      hipLaunchKernelGGL(k_bms_assemble_mx<float2>, grid, dim3(kBlock), 0, c.stream, (const float2 *)br.basis, dcf,
-                         dkl, x32, Vh, (int)kp, nshift, nrhs);
-    else
-      hipLaunchKernelGGL(k_bms_assemble_mx<double2>, grid, dim3(kBlock), 0, c.stream, (const float2 *)br.basis, dcf,
-                         dkl, x, Vh, (int)kp, nshift, nrhs);
-  } else {
-    double *dcoef = (double *)ws_get(c, S_COEF, sizeof(double) * coef.size() + sizeof(int) * 256);
-    if (!dcoef) return -1;
-    int *dklen = (int *)(dcoef + coef.size());
-    if (!h2d(c, dcoef, coef.data(), sizeof(double) * coef.size()) || !h2d(c, dklen, klen.data(), sizeof(int) * nrhs))
-      return -1;
-    const dim3 grid((unsigned)((3 * Vh + kBlock - 1) / kBlock), nrhs);
-    for (int s0 = 0; s0 < nshift; s0 += 16)
-      if (x32)
-        hipLaunchKernelGGL((k_bms_assemble<R, 16, float2>), grid, dim3(kBlock), 0, c.stream, br.basis, dcoef, dklen,
-                           x32, Vh, (int)cst, nshift, s0, nrhs);
-      else
-        hipLaunchKernelGGL((k_bms_assemble<R, 16, double2>), grid, dim3(kBlock), 0, c.stream, br.basis, dcoef, dklen,
-                           x, Vh, (int)cst, nshift, s0, nrhs);
-  }
-  // stream-ordered for the caller; the host arrays stay alive in the workspace
-  if (!hip_ok(c, hipGetLastError(), "basis assemble")) return -1;
   *iters_out = kmax;
   if (restart) return kBasisRestart;
   for (int j = 0; j < nrhs; j++)
