@@ -19,6 +19,10 @@ enum Prec { PREC_FP64 = 0, PREC_MIXED = 1, PREC_FP32 = 2 };
 // per right-hand side state of the multi-shift CG (FP64 scalars on device)
 struct MsState {
   double delta, omega, omega_prev, gammag, res;
+  // res < 0 on entry: the solver's first launch sets res = max(tol_abs,
+  // tol_rel |b|) from the |b|^2 it forms (bnorm2), so the host needs no
+  // norm before the solve
+  double tol_abs, tol_rel, bnorm2;
   int iter, max_iter, done, stop, nshift, nact;
   int nonfinite;  // a reduction or recurrence produced NaN/Inf (the solve stops)
   double shift[kMaxShifts], z1[kMaxShifts], z2[kMaxShifts], z3[kMaxShifts];

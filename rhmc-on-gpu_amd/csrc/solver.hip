@@ -2037,6 +2037,8 @@ __global__ __launch_bounds__(kBlock) void k_bms_init_fin(BmsArgs a) {
     s.gammag = 0.0;
     s.iter = 0;
     s.delta = sums[j];
+    s.bnorm2 = sums[j];
+    if (s.res < 0.0) s.res = fmax(s.tol_abs, s.tol_rel * sqrt(sums[j]));
     s.nact = s.nshift;
     for (int k = 0; k < s.nshift; k++) {
       s.z1[k] = 1.0; s.z2[k] = 1.0; s.z3[k] = 1.0;
@@ -2356,6 +2358,8 @@ struct BasisRestart {
   std::vector<int> rhs;         // systems to continue (indices into that call's right-hand sides)
   std::vector<int> slot;        // their K
   std::vector<double> zeta;     // [rhs.size()][nshift]: zeta_s^(K) of the running shifts, 0 for frozen ones
+  std::vector<double> res;      // per right-hand side of that call: the stopping threshold on |r|
+  std::vector<double> bnorm2;   // and |b|^2 as the solver formed it
 };
 
 // The first pass of a stored-basis solve when it was already run (a
@@ -2393,17 +2397,21 @@ struct BasisRun {
 // iterc (device) is read by every launch of the iterations
 template <typename R, int LINK>
 static int bms_setup(Context &c, BasisRun &br, const double2 *b, int nrhs, int nshift, const double *shifts,
-                     const double *res_abs, int max_iter, int *iterc) {
+                     const double *res_abs, int max_iter, int *iterc, double tol_abs = 0.0, double tol_rel = 0.0) {
   using C = Cx<R>;
   const long Vh = c.Vh;
   const size_t vb = sizeof(C) * 3 * Vh;
   const bool dbl = sizeof(R) == 8;
-  size_t fr = 0, tot = 0;
-  hipMemGetInfo(&fr, &tot);
   const size_t slot_bytes = vb * nrhs;
   const size_t have = c.ws->cap[S_BASIS];
-  const long budget = (long)((fr + have) / 2 / slot_bytes);
   long want = std::max<long>(kMinSlots, 2L * c.ws->basis_kmax[dbl ? 1 : 0]);
+  long budget = (long)(have / slot_bytes);
+  if (budget < want) {
+    // growing: up to half of the free device memory (with the current basis)
+    size_t fr = 0, tot = 0;
+    hipMemGetInfo(&fr, &tot);
+    budget = (long)((fr + have) / 2 / slot_bytes);
+  }
   want = std::max<long>(want, (long)(have / slot_bytes));
   long cap = std::min<long>(std::min<long>((long)max_iter + 2, kMaxSlots), std::min<long>(budget, want));
   if (const char *e = getenv("RHMC_BASIS_CAP")) cap = std::min<long>(cap, std::max(atol(e), kMinBasis));
@@ -2430,7 +2438,10 @@ static int bms_setup(Context &c, BasisRun &br, const double2 *b, int nrhs, int n
     MsState &st = br.hs[j];
     memset(&st, 0, sizeof(st));
     st.nshift = nshift;
-    st.res = res_abs[j];
+    // without res_abs: max(tol_abs, tol_rel |b|), set on the device (k_bms_init_fin)
+    st.res = res_abs ? res_abs[j] : -1.0;
+    st.tol_abs = tol_abs;
+    st.tol_rel = tol_rel;
     st.max_iter = max_iter;
     for (int k = 0; k < nshift; k++) st.shift[k] = shifts[k];
   }
@@ -2689,6 +2700,14 @@ static int bms_finish(Context &c, BasisRun &br, double2 *x, int *iters_out, floa
   // the systems' final state (after the assembly: one wait for both)
   std::vector<MsState> &hs = br.hs;
   if (!d2h_sync(c, hs.data(), br.ms, sizeof(MsState) * nrhs)) return -1;
+  if (rs) {
+    rs->res.resize(nrhs);
+    rs->bnorm2.resize(nrhs);
+    for (int j = 0; j < nrhs; j++) {
+      rs->res[j] = hs[j].res;
+      rs->bnorm2[j] = hs[j].bnorm2;
+    }
+  }
   int kmax = 0;
   for (int j = 0; j < nrhs; j++) kmax = std::max(kmax, hs[j].iter);
   long &kseen = c.ws->basis_kmax[dbl ? 1 : 0];
@@ -2742,10 +2761,10 @@ static int bms_finish(Context &c, BasisRun &br, double2 *x, int *iters_out, floa
 template <typename R, int LINK>
 static int ms_cg_basis(Context &c, const double2 *b, int nrhs, int nshift, const double *shifts,
                        const double *res_abs, int max_iter, double2 *x, int *iters_out, float2 *x32,
-                       bool asm_fp32, BasisRestart *rs) {
+                       bool asm_fp32, BasisRestart *rs, double tol_abs = 0.0, double tol_rel = 0.0) {
   BasisRun br;
   int *iterc = iter_ptr(c);
-  const int rc = bms_setup<R, LINK>(c, br, b, nrhs, nshift, shifts, res_abs, max_iter, iterc);
+  const int rc = bms_setup<R, LINK>(c, br, b, nrhs, nshift, shifts, res_abs, max_iter, iterc, tol_abs, tol_rel);
   if (rc) return rc;
   GraphSlot &gs = c.ws->bms_graph[sizeof(R) == 8 ? 1 : 0];
   BmsGraphKey key;
@@ -2998,9 +3017,12 @@ __global__ __launch_bounds__(kBlock) void k_add_scaled(XO *x, const XO *y, const
 // first call did not fit) or -1.
 // first: when not null, the first pass was already run (multi-lattice solve).
 template <typename R, int LINK>
+// res_abs null: the first pass's thresholds are max(tol_abs, tol_rel |b|),
+// formed on the device; bn2_out (if given) receives the |b|^2 it used.
 static int ms_basis_solve(Context &c, const double2 *b, int nrhs, int nshift, const double *shifts,
                           const double *res_abs, int max_iter, double2 *x, int *iters_out, float2 *x32 = nullptr,
-                          bool asm_fp32 = false, FirstPass *first = nullptr) {
+                          bool asm_fp32 = false, FirstPass *first = nullptr, double tol_abs = 0.0,
+                          double tol_rel = 0.0, std::vector<double> *bn2_out = nullptr) {
   using C = Cx<R>;
   BasisRestart rs;
   int it = 0;
@@ -3010,7 +3032,16 @@ static int ms_basis_solve(Context &c, const double2 *b, int nrhs, int nshift, co
     it = first->iters;
     rs = first->rs;
   } else {
-    rc = ms_cg_basis<R, LINK>(c, b, nrhs, nshift, shifts, res_abs, max_iter, x, &it, x32, asm_fp32, &rs);
+    rc = ms_cg_basis<R, LINK>(c, b, nrhs, nshift, shifts, res_abs, max_iter, x, &it, x32, asm_fp32, &rs, tol_abs,
+                              tol_rel);
+  }
+  // the first pass's thresholds (continuations divide them by the largest |zeta|)
+  std::vector<double> res_first;
+  if (!res_abs) {
+    if (rc < 0 || rc == kBasisFull) return rc;
+    res_first = rs.res;
+    res_abs = res_first.data();
+    if (bn2_out) *bn2_out = rs.bnorm2;
   }
   int total = it;
   std::vector<int> orig(nrhs);
@@ -3235,21 +3266,45 @@ static int ms_solve(Context &c, const double2 *b, int nrhs, int nshift, const do
     return 1;
   }
   const long Vh = c.Vh;
-  // |b| per rhs for relative residuals and the FP32 stage target
-  std::vector<double> bn2(nrhs), res_abs(nrhs);
-  if (vec_norms2(c, b, nrhs, 3 * Vh, bn2.data())) return -1;
-  for (int j = 0; j < nrhs; j++) res_abs[j] = rel ? res * sqrt(bn2[j]) : res;
   c.last_refine_cycles = 0;
   c.last_iters_refine = 0;
   c.last_restarts = 0;
   c.last_true_res_max = 0.0;
   const bool explicit_shifts = c.ms_algorithm == 1 || getenv("RHMC_MS_EXPLICIT") != nullptr;
+  // |b| per rhs (relative residuals, the FP32 stage target, the FP64 checks).
+  // The stored-basis pass forms it on the device and sets its own
+  // thresholds there, max(tol_abs, tol_rel |b|), so the host waits for no
+  // norm before the solve; the explicit-shift solver and a first pass run
+  // by a multi-lattice solve take host norms.
+  std::vector<double> bn2(nrhs), res_abs(nrhs);
+  bool have_norms = false;
+  auto set_norms = [&](const std::vector<double> &n2) {
+    bn2 = n2;
+    for (int j = 0; j < nrhs; j++) res_abs[j] = rel ? res * sqrt(bn2[j]) : res;
+    have_norms = true;
+  };
+  auto host_norms = [&]() -> bool {
+    if (have_norms) return true;
+    std::vector<double> n2(nrhs);
+    if (vec_norms2(c, b, nrhs, 3 * Vh, n2.data())) return false;
+    set_norms(n2);
+    return true;
+  };
+  if ((explicit_shifts || first) && !host_norms()) return -1;
+  std::vector<double> dev_bn2;
   if (prec == PREC_FP64) {
     int rc = explicit_shifts ? kBasisFull
-                             : ms_basis_solve<double, LINK_D12>(c, b, nrhs, nshift, shifts, res_abs.data(), max_iter,
-                                                                x, iters, nullptr, false, first);
+                             : ms_basis_solve<double, LINK_D12>(c, b, nrhs, nshift, shifts,
+                                                                have_norms ? res_abs.data() : nullptr, max_iter, x,
+                                                                iters, nullptr, false, first, rel ? 0.0 : res,
+                                                                rel ? res : 0.0, &dev_bn2);
+    if (rc < 0) return rc;
+    if (!have_norms && rc != kBasisFull) set_norms(dev_bn2);
     const bool basis = rc != kBasisFull;
-    if (rc == kBasisFull) rc = ms_cg<double, LINK_D12>(c, b, nrhs, nshift, shifts, res_abs.data(), max_iter, x, iters);
+    if (rc == kBasisFull) {
+      if (!host_norms()) return -1;
+      rc = ms_cg<double, LINK_D12>(c, b, nrhs, nshift, shifts, res_abs.data(), max_iter, x, iters);
+    }
     c.last_iters = *iters;
     c.last_iters_fp32 = 0;
     if (rc != 0 || !basis) return rc;
@@ -3266,15 +3321,23 @@ static int ms_solve(Context &c, const double2 *b, int nrhs, int nshift, const do
   }
   // mixed: FP32 multi-shift to ~1e-6 relative, then FP64 defect correction;
   // FP32 only: the FP32 recursion to the requested residual
+  // the FP32 stage's threshold max(res_abs, rel32 |b|)
   const double rel32 = prec == PREC_FP32 ? 0.0 : 2e-6;
   std::vector<double> res32(nrhs);
-  for (int j = 0; j < nrhs; j++) res32[j] = std::max(res_abs[j], rel32 * sqrt(bn2[j]));
+  if (have_norms)
+    for (int j = 0; j < nrhs; j++) res32[j] = std::max(res_abs[j], rel32 * sqrt(bn2[j]));
   int it32 = 0;
   const long n = 3 * Vh * (long)nrhs * nshift;
   int rc = explicit_shifts ? kBasisFull
-                           : ms_basis_solve<float, LINK_S12>(c, b, nrhs, nshift, shifts, res32.data(), max_iter, x,
-                                                             &it32, x32, prec == PREC_FP32, first);
+                           : ms_basis_solve<float, LINK_S12>(c, b, nrhs, nshift, shifts,
+                                                             have_norms ? res32.data() : nullptr, max_iter, x, &it32,
+                                                             x32, prec == PREC_FP32, first, rel ? 0.0 : res,
+                                                             rel ? std::max(res, rel32) : rel32, &dev_bn2);
+  if (rc < 0) return rc;
+  if (!have_norms && rc != kBasisFull) set_norms(dev_bn2);
   if (rc == kBasisFull) {
+    if (!host_norms()) return -1;
+    for (int j = 0; j < nrhs; j++) res32[j] = std::max(res_abs[j], rel32 * sqrt(bn2[j]));
     float2 *xs = x32 ? x32 : (float2 *)ws_get(c, S_X32, sizeof(float2) * 3 * Vh * nrhs * nshift);
     if (!xs) return -1;
     rc = ms_cg<float, LINK_S12>(c, b, nrhs, nshift, shifts, res32.data(), max_iter, xs, &it32);

@@ -100,9 +100,37 @@ struct Lat {
 struct Traj {
   std::vector<Lat> L;
   const rhmc_traj_params &p;  // integrator parameters (equal for every lattice)
-  double t_solve = 0, t_force = 0, t_update = 0;
+  double t_solve = 0;
+  // GPU time of the force and update sections: event pairs recorded around
+  // them, read once at the end, so the MD has no host wait besides the solves
+  std::vector<hipEvent_t> ev_force, ev_update;
 
   explicit Traj(const rhmc_traj_params &pp) : p(pp) {}
+  ~Traj() {
+    for (hipEvent_t e : ev_force)
+      if (e) hipEventDestroy(e);
+    for (hipEvent_t e : ev_update)
+      if (e) hipEventDestroy(e);
+  }
+  void mark(std::vector<hipEvent_t> &v) {
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) e = nullptr;
+    else hipEventRecord(e, stream());
+    v.push_back(e);
+  }
+  // seconds between the events of each pair (all recorded work complete)
+  static double gpu_seconds(std::vector<hipEvent_t> &v) {
+    double t = 0.0;
+    if (!v.empty() && v.back()) hipEventSynchronize(v.back());
+    for (size_t i = 0; i + 1 < v.size(); i += 2) {
+      float ms = 0.f;
+      if (v[i] && v[i + 1] && hipEventElapsedTime(&ms, v[i], v[i + 1]) == hipSuccess) t += 1e-3 * ms;
+    }
+    for (hipEvent_t e : v)
+      if (e) hipEventDestroy(e);
+    v.clear();
+    return t;
+  }
 
   bool ok() const {
     for (const Lat &l : L)
@@ -190,7 +218,7 @@ struct Traj {
       }
       solve(b, co, p.res_md, -1, f32);
       if (!ok()) return;
-      const auto t0 = clk::now();
+      mark(ev_force);
       for (Lat &la : L) {
         Context &c = la.c;
         const int order = (int)la.md.a.size();
@@ -202,25 +230,23 @@ struct Traj {
         if (rcf) la.fail(-1);
         if (gauge) gauge_force(c, 1);
       }
-      hipStreamSynchronize(stream());
-      t_force += secs(t0);
+      mark(ev_force);
     } else {
-      const auto t0 = clk::now();
+      mark(ev_force);
       for (Lat &la : L) gauge_force(la.c, 0);
-      hipStreamSynchronize(stream());
-      t_force += secs(t0);
+      mark(ev_force);
     }
   }
   void P(double step, bool fermion, bool gauge) {
     force(fermion, gauge);
-    const auto t0 = clk::now();
+    mark(ev_update);
     for (Lat &la : L) momenta_update(la.c, step);
-    t_update += secs(t0);
+    mark(ev_update);
   }
   void Q(double step) {
-    const auto t0 = clk::now();
+    mark(ev_update);
     for (Lat &la : L) exp_update(la.c, step);
-    t_update += secs(t0);
+    mark(ev_update);
   }
 
   static constexpr double kLambda = 0.1931833275037836;  // Omelyan et al.
@@ -542,6 +568,7 @@ int run_trajectory_group(std::vector<Context *> &cs, std::vector<const rhmc_traj
         }
       }
     }
+    const double t_force = Traj::gpu_seconds(T.ev_force), t_update = Traj::gpu_seconds(T.ev_update);
     for (Lat &la : T.L) {
       action_local(la.c, la.c.phinew, la.c.chi, !la.p.pure_gauge, -1);
       double s = 0.0;
@@ -551,8 +578,8 @@ int run_trajectory_group(std::vector<Context *> &cs, std::vector<const rhmc_traj
       la.r.delta_h_fp64 = la.r.delta_h;
       la.r.seconds_total = secs(t0);
       la.r.seconds_solver = T.t_solve;
-      la.r.seconds_force = T.t_force;
-      la.r.seconds_update = T.t_update;
+      la.r.seconds_force = t_force;
+      la.r.seconds_update = t_update;
     }
     for (Lat &la : T.L) {
       if (la.p.verify_fp64 && !la.p.pure_gauge)
@@ -619,8 +646,8 @@ int run_rev_update(Context &c, const rhmc_traj_params &p, rhmc_traj_result &r, d
   *d_mom = std::sqrt(s2) * norm;
   r.seconds_total = secs(t0);
   r.seconds_solver = T.t_solve;
-  r.seconds_force = T.t_force;
-  r.seconds_update = T.t_update;
+  r.seconds_force = Traj::gpu_seconds(T.ev_force);
+  r.seconds_update = Traj::gpu_seconds(T.ev_update);
   return RHMC_OK;
 }
 
