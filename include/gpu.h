@@ -50,7 +50,13 @@ int gpu_destroy(rhmc_gpu_t h);
 const char *gpu_last_error(rhmc_gpu_t h);
 
 /* include/gpu.h:6 gpu_init0 -> cuda_init0 (cuda_init.cu:5-57): gauge field
- * to the device (FP64 18-real and the FP32 12-real copy). */
+ * to the device (FP64 18-real and the FP32 and FP64 12-real copies).  The
+ * links must be unitary, with the staggered phase in row 2 as the reference
+ * stores them: RHMC_EINVAL (with the largest deviation in gpu_last_error)
+ * when row 2 differs from eta (row 0 x row 1)^* by more than 1e-10, since
+ * the 12-real copies rebuild row 2.  The reference computes with such links
+ * as given (conf.cc:86-92 does not re-project them); every configuration it
+ * writes or evolves is unitary to rounding. */
 int gpu_init0(rhmc_gpu_t h, const double *links);
 
 /* include/gpu.h:7 gpu_init1 -> cuda_init1 (cuda_init.cu:61-108): pseudofermion
@@ -69,8 +75,9 @@ int gpu_write_conf(rhmc_gpu_t h, const char *path, double beta, double mass, dou
                    int update_iteration);
 
 /* lib/Main/Conf/conf.cc:55-96 (Conf(init=2)): links from such a file to the
- * device, as stored; the extents must match the handle's lattice.  Header
- * values are returned through the pointers (each may be NULL). */
+ * device, as stored; the extents must match the handle's lattice, and the
+ * links must be unitary as for gpu_init0.  Header values are returned
+ * through the pointers (each may be NULL). */
 int gpu_read_conf(rhmc_gpu_t h, const char *path, int *update_iteration, double *beta, double *mass,
                   double *n_flavours);
 

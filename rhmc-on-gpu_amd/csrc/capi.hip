@@ -17,7 +17,9 @@
 #include "gpu.h"
 
 namespace rhmc {
-int upload_links(Context &c, const double *host, double2 *dst);
+int upload_links(Context &c, const double *host, double2 *dst, double tol, double *max_dev);
+// links must be unitary to this (row 2 is rebuilt from rows 0 and 1 by the 12-real copies)
+constexpr double kUnitarityTol = 1e-10;
 int download_links(Context &c, const double2 *src, double *host);
 int upload_alg(Context &c, const double *host, double2 *dst, int herm);
 int download_alg(Context &c, const double2 *src, double *host, int herm);
@@ -158,13 +160,34 @@ int gpu_destroy(rhmc_gpu_t h) {
 
 const char *gpu_last_error(rhmc_gpu_t h) { return h ? h->c.last_error.c_str() : "null handle"; }
 
+// Links to the device (FP64 18-real), checked for unitarity, then the
+// 12-real copies.  Links with a row 2 that is not eta (row 0 x row 1)^* (to
+// kUnitarityTol) are refused and the device keeps its links: the 12-real
+// copies rebuild row 2 and would differ from it.  Every configuration the
+// reference writes or evolves is unitary to rounding (its text format keeps
+// 16 digits).
+static int set_links(Context &c, const double *links, const char *who) {
+  double dev = 0.0;
+  const int up = upload_links(c, links, c.U64, kUnitarityTol, &dev);
+  if (up < 0) return RHMC_EHIP;
+  if (up > 0) {
+    char msg[256];
+    snprintf(msg, sizeof msg,
+             "%s: links not unitary: row 2 differs from eta (row 0 x row 1)^* by up to %.3g (limit %.0e); "
+             "the 12-real link copies would change the operator",
+             who, dev, kUnitarityTol);
+    c.last_error = msg;
+    return RHMC_EINVAL;
+  }
+  if (refresh_links32(c)) return RHMC_EHIP;
+  return hip_ok(c, hipStreamSynchronize(c.stream), who) ? RHMC_OK : RHMC_EHIP;
+}
+
 int gpu_init0(rhmc_gpu_t h, const double *links) {
   if (!h || !links) return RHMC_EINVAL;
   Context &c = h->c;
   hipSetDevice(c.device);
-  if (upload_links(c, links, c.U64)) return RHMC_EHIP;
-  if (refresh_links32(c)) return RHMC_EHIP;
-  return hip_ok(c, hipStreamSynchronize(c.stream), "init0") ? RHMC_OK : RHMC_EHIP;
+  return set_links(c, links, "gpu_init0");
 }
 
 int gpu_init1(rhmc_gpu_t h, const double *phi, const double *ipdot, const double *momenta) {
@@ -277,8 +300,7 @@ int gpu_read_conf(rhmc_gpu_t h, const char *path, int *update_iteration, double 
     if (*e != ')') return fail(h, "gpu_read_conf: bad complex number");
     p = e + 1;
   }
-  if (upload_links(c, u.data(), c.U64) || refresh_links32(c)) return RHMC_EHIP;
-  if (!hip_ok(c, hipStreamSynchronize(c.stream), "read conf")) return RHMC_EHIP;
+  if (const int rc = set_links(c, u.data(), "gpu_read_conf")) return rc;
   if (update_iteration) *update_iteration = (int)it;
   if (beta) *beta = hv[0];
   if (mass) *mass = hv[1];

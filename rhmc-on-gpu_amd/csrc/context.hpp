@@ -109,6 +109,9 @@ struct Context {
   // residual) to last_xr; the caller clears it.
   bool want_xr = false;
   std::vector<double> last_xr;
+  // the fermion force's coefficients held by the device buffer force_coef_dev
+  std::vector<double> force_coef_held;
+  const double *force_coef_dev = nullptr;
   int graph_captures = 0;          // solver chunk graphs captured (instantiated) so far
 };
 

@@ -118,6 +118,20 @@ __global__ __launch_bounds__(kBlock) void k_fermion_force(LatticeDims L, const d
   }
 }
 
+// The force's coefficients in device memory: uploaded only when they differ
+// from the ones the buffer already holds (every MD force of a trajectory
+// uses the same ones), so the force launches follow the solve without a copy.
+static bool force_coef_upload(Context &c, double *dcoef, const double *num, int nsys) {
+  std::vector<double> &held = c.force_coef_held;
+  if (c.force_coef_dev == dcoef && held.size() == (size_t)nsys && std::equal(held.begin(), held.end(), num))
+    return true;
+  c.force_coef_dev = nullptr;
+  if (!h2d(c, dcoef, num, sizeof(double) * nsys)) return false;
+  held.assign(num, num + nsys);
+  c.force_coef_dev = dcoef;
+  return true;
+}
+
 int fermion_force(Context &c, int nsys, const double *num, const double2 *x, int accumulate) {
   // nsys = n_pf * order systems, x = [n_pf][order][3][Vh]; num has nsys entries
   const long Vh = c.Vh;
@@ -127,7 +141,7 @@ int fermion_force(Context &c, int nsys, const double *num, const double2 *x, int
   double2 *hv = (double2 *)scratch_get(c, SCR_FORCE_HOP, need);
   double *coef = (double *)scratch_get(c, SCR_FORCE_COEF, sizeof(double) * 256);
   if (!hv || !coef || nsys > 256) return -1;
-  if (!h2d(c, coef, num, sizeof(double) * nsys)) return -1;
+  if (!force_coef_upload(c, coef, num, nsys)) return -1;
   launch_hop_d(c, 1, x, hv, nsys, 3 * Vh, 3 * Vh);
   hipLaunchKernelGGL(k_fermion_force<double2>, dim3(nblk(Vh), 2), dim3(kBlock), 0, c.stream, c.L, c.U64, x, hv, coef,
                      nsys, c.ipdot, accumulate);
@@ -140,7 +154,7 @@ int fermion_force32(Context &c, int nsys, const double *num, const float2 *x32, 
   float2 *hv = (float2 *)scratch_get(c, SCR_FORCE_HOP, sizeof(float2) * 3 * Vh * (size_t)nsys);
   double *coef = (double *)scratch_get(c, SCR_FORCE_COEF, sizeof(double) * 256);
   if (!hv || !coef || nsys > 256) return -1;
-  if (!h2d(c, coef, num, sizeof(double) * nsys)) return -1;
+  if (!force_coef_upload(c, coef, num, nsys)) return -1;
   launch_hop_s(c, 1, x32, hv, nsys, 3 * Vh, 3 * Vh);
   hipLaunchKernelGGL(k_fermion_force<float2>, dim3(nblk(Vh), 2), dim3(kBlock), 0, c.stream, c.L, c.U64, x32, hv, coef,
                      nsys, c.ipdot, accumulate);
@@ -956,11 +970,65 @@ __global__ __launch_bounds__(kBlock) void k_spin_out(const double2 *v, double2 *
   for (int k = 0; k < 3; k++) aos[((long)j * Vh + h) * 3 + k] = v[((long)j * 3 + k) * Vh + h];
 }
 
-int upload_links(Context &c, const double *host, double2 *dst) {
+// Largest |row 2 - eta (row 0 x row 1)^*| over the links, per workgroup, on
+// the host layout (link i = (2 mu + parity) Vh + h).  The 12-real copies
+// (the FP32 MD operator, the FP64 solves, the gauge force) rebuild row 2
+// this way, which reproduces the stored link only when it is unitary.  NaN
+// entries are skipped (the solvers report them).
+__global__ __launch_bounds__(kBlock) void k_unitarity(LatticeDims L, const double2 *aos, double *part) {
+  __shared__ double red[kBlock];
+  const long i = (long)blockIdx.x * kBlock + threadIdx.x;
+  double d = 0.0;
+  if (i < 8 * L.Vh) {
+    const long blk = i / L.Vh, h = i - blk * L.Vh;
+    const int mu = (int)(blk / 2), par = (int)(blk % 2);
+    Mat<double> m;
+#pragma unroll
+    for (int k = 0; k < 9; k++) m.e[k / 3][k % 3] = aos[i * 9 + k];
+    const SiteCoord c = site_coords_i(L, par, (int)h);
+    Mat<double> r = m;
+    rebuild_row2(r, (double)eta_phase(L, c.x, c.y, c.z, c.t, mu));
+#pragma unroll
+    for (int k = 0; k < 3; k++) d = fmax(d, hypot(r.e[2][k].x - m.e[2][k].x, r.e[2][k].y - m.e[2][k].y));
+  }
+  red[threadIdx.x] = d;
+  __syncthreads();
+  for (int w = kBlock / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + w]);
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+// Links from the host layout to dst (FP64 18-real).  *max_dev receives the
+// largest deviation from unitarity (k_unitarity); above tol dst is left
+// unchanged and 1 is returned.
+int upload_links(Context &c, const double *host, double2 *dst, double tol, double *max_dev) {
   const size_t bytes = sizeof(double2) * 9 * 8 * c.Vh;
+  const unsigned nb = nblk(8 * c.Vh);
   double2 *st = nullptr;
+  double *part = nullptr;
   if (!hip_ok(c, hipMallocAsync((void **)&st, bytes, c.stream), "stage")) return -1;
+  if (!hip_ok(c, hipMallocAsync((void **)&part, sizeof(double) * nb, c.stream), "stage")) {
+    hipFreeAsync(st, c.stream);
+    return -1;
+  }
   hipMemcpyAsync(st, host, bytes, hipMemcpyHostToDevice, c.stream);
+  hipLaunchKernelGGL(k_unitarity, dim3(nb), dim3(kBlock), 0, c.stream, c.L, st, part);
+  std::vector<double> hp(nb);
+  hipMemcpyAsync(hp.data(), part, sizeof(double) * nb, hipMemcpyDeviceToHost, c.stream);
+  hipFreeAsync(part, c.stream);
+  if (!hip_ok(c, hipStreamSynchronize(c.stream), "upload links")) {
+    hipFreeAsync(st, c.stream);
+    return -1;
+  }
+  double m = 0.0;
+  for (double v : hp) m = std::max(m, v);
+  *max_dev = m;
+  if (m > tol) {
+    hipFreeAsync(st, c.stream);
+    return hip_ok(c, hipStreamSynchronize(c.stream), "upload links") ? 1 : -1;
+  }
   hipLaunchKernelGGL(k_links_in, dim3(nblk(8 * c.Vh)), dim3(kBlock), 0, c.stream, st, dst, c.Vh);
   hipFreeAsync(st, c.stream);
   return hip_ok(c, hipStreamSynchronize(c.stream), "upload links") ? 0 : -1;

@@ -142,6 +142,7 @@ static void *ws_get(Context &c, int slot, size_t bytes) {
   if (w->cap[slot] >= bytes) return w->buf[slot];
   if (w->buf[slot]) {
     hipStreamSynchronize(c.stream);
+    if (c.force_coef_dev == w->buf[slot]) c.force_coef_dev = nullptr;  // its held coefficients go with it
     hipFree(w->buf[slot]);
     w->buf[slot] = nullptr;
     w->cap[slot] = 0;
@@ -3347,7 +3348,7 @@ static int ms_solve(Context &c, const double2 *b, int nrhs, int nshift, const do
   }
   if (rc < 0 || rc == kStatusNonFinite) return rc;
   if (prec == PREC_FP32) {
-    if (!hip_ok(c, hipStreamSynchronize(c.stream), "fp32 solve")) return -1;
+    if (!hip_ok(c, hipGetLastError(), "fp32 solve")) return -1;
     c.last_iters_fp32 = it32;
     c.last_iters = it32;
     *iters = it32;

@@ -568,6 +568,32 @@ def test_trajectory_rejects_bad_metro_precision():
     g.close()
 
 
+def test_links_must_be_unitary():
+    """The 12-real link copies rebuild row 2 as eta (row 0 x row 1)^*, which
+    reproduces only unitary links, so gpu_init0 refuses links that differ
+    from that by more than 1e-10: RHMC_EINVAL naming the deviation, and the
+    device keeps its previous links.  A rounding-level deviation passes."""
+    dims = (8, 8, 8, 4)
+    lat = fields.Lattice(*dims)
+    U = fields.random_links(lat, 5, 0.3)
+    g = Gpu(dims, mass=0.1)
+    g.init0(U)
+    bad = U.copy()
+    bad[77, 2, 1] += 3e-7
+    with pytest.raises(RhmcError) as e:
+        g.init0(bad)
+    say("refused:", str(e.value))
+    assert e.value.rc == RHMC_EINVAL and "not unitary" in str(e.value) and "3e-07" in str(e.value)
+    U1, _, _ = g.end(links=True, ipdot=False, momenta=False)
+    assert np.array_equal(U1, U)
+    ok = U.copy()
+    ok[77, 2, 1] += 1e-13
+    g.init0(ok)
+    U2, _, _ = g.end(links=True, ipdot=False, momenta=False)
+    assert np.array_equal(U2, ok)
+    g.close()
+
+
 # --------------------------------------------------- basis continuation
 @pytest.mark.parametrize("precision", [0, 1, 2])
 def test_basis_continuation_with_small_capacity(precision, monkeypatch):
