@@ -2339,11 +2339,14 @@ static int ms_cg(Context &c, const double2 *b, int nrhs, int nshift, const doubl
 // Basis capacity policy.  The slots a solve needs are not known in advance,
 // so the capacity starts at max(kMinSlots, twice the most iterations a solve
 // of this precision has needed, the slots already allocated) and is bounded
-// by half of the device memory not held elsewhere (a 32^3x8 FP64 slot of 8
-// right-hand sides is 25 MB).  A solve that fills it is continued from its
-// last residual (ms_basis_solve) instead of being restarted from zero.
+// by half of the device memory not held elsewhere and by kMaxBasisBytes (a
+// 32^3x8 FP64 slot of 8 right-hand sides is 50 MB).  A solve that fills it
+// is continued from its last residual (ms_basis_solve) instead of being
+// restarted from zero, so a bounded basis loses no iterations; growing a
+// basis of ~100 GB stalled the host for seconds at 48^3x12.
 constexpr long kMinSlots = 128;
 constexpr long kMaxSlots = 16384;
+constexpr size_t kMaxBasisBytes = 32ull << 30;
 constexpr int kBasisFull = 100;     // not even kMinBasis slots fit: the explicit-shift solver runs
 constexpr int kBasisRestart = 101;  // the basis filled: x holds the solutions so far (see BasisRestart)
 constexpr long kMinBasis = 16;
@@ -2408,10 +2411,11 @@ static int bms_setup(Context &c, BasisRun &br, const double2 *b, int nrhs, int n
   long want = std::max<long>(kMinSlots, 2L * c.ws->basis_kmax[dbl ? 1 : 0]);
   long budget = (long)(have / slot_bytes);
   if (budget < want) {
-    // growing: up to half of the free device memory (with the current basis)
+    // growing: up to half of the free device memory (with the current basis),
+    // at most kMaxBasisBytes
     size_t fr = 0, tot = 0;
     hipMemGetInfo(&fr, &tot);
-    budget = (long)((fr + have) / 2 / slot_bytes);
+    budget = (long)(std::min((fr + have) / 2, std::max(have, kMaxBasisBytes)) / slot_bytes);
   }
   want = std::max<long>(want, (long)(have / slot_bytes));
   long cap = std::min<long>(std::min<long>((long)max_iter + 2, kMaxSlots), std::min<long>(budget, want));
