@@ -1522,6 +1522,8 @@ static void dslash_launch(Context &c, DsArgs &a, int code) {
 // candidates skip that step); the r^(k) the candidates of its M^dag M
 // launch write is overwritten by the final launch.  RHMC_DSLASH_NR ("4" or "4x4") forces
 // the right-hand sides per lane and lanes per site.
+constexpr int kTuneRounds = 3;
+
 template <typename R, int LINK, int MODE, int RED>
 static void dslash_dispatch(Context &c, DsArgs &a) {
   const unsigned long long key = tune_key(sizeof(R), LINK, MODE, RED, a.nvec);
@@ -1550,23 +1552,31 @@ static void dslash_dispatch(Context &c, DsArgs &a) {
     DsArgs t = a;
     t.bms_ms = nullptr;  // the hop's scalar step runs once, in the final launch
     const bool reducing = RED != RED_NONE && MODE != DS_HOP;
+    std::vector<int> cands;
     for (int lanes = 1; lanes <= 4; lanes *= 4)
       for (int v = 1; v <= 8 && v <= std::max(1, a.nvec); v *= 2)
         for (int ident = c.tiles ? 0 : 1; ident <= 1; ident++)
-          for (int tpw = 0; tpw <= (reducing ? 2 : 0); tpw++) {
-            const int cand = make_shape(v, lanes, ident, tpw);
-            dslash_launch<R, LINK, MODE, RED>(c, t, cand);
-            hipEventRecord(e0, c.stream);
-            for (int k = 0; k < 3; k++) dslash_launch<R, LINK, MODE, RED>(c, t, cand);
-            hipEventRecord(e1, c.stream);
-            hipEventSynchronize(e1);
-            float ms = 0.f;
-            hipEventElapsedTime(&ms, e0, e1);
-            if (ms < best) {
-              best = ms;
-              code = cand;
-            }
-          }
+          for (int tpw = 0; tpw <= (reducing ? 2 : 0); tpw++) cands.push_back(make_shape(v, lanes, ident, tpw));
+    // every candidate launched once, then the best of kTuneRounds interleaved
+    // rounds of 3 launches each (one round alone picked the FP32 8-RHS hop's
+    // shape differently from run to run: 26.2 or 28.6 us per launch)
+    for (const int cand : cands) dslash_launch<R, LINK, MODE, RED>(c, t, cand);
+    std::vector<float> cand_ms(cands.size(), 1e30f);
+    for (int round = 0; round < kTuneRounds; round++)
+      for (size_t i = 0; i < cands.size(); i++) {
+        hipEventRecord(e0, c.stream);
+        for (int k = 0; k < 3; k++) dslash_launch<R, LINK, MODE, RED>(c, t, cands[i]);
+        hipEventRecord(e1, c.stream);
+        hipEventSynchronize(e1);
+        float ms = 0.f;
+        hipEventElapsedTime(&ms, e0, e1);
+        cand_ms[i] = std::min(cand_ms[i], ms);
+      }
+    for (size_t i = 0; i < cands.size(); i++)
+      if (cand_ms[i] < best) {
+        best = cand_ms[i];
+        code = cands[i];
+      }
     hipEventDestroy(e0);
     hipEventDestroy(e1);
     tab[key] = code;
