@@ -259,9 +259,14 @@ def test_forces_golden_reference(name):
     g.init0(gd["links"])
     g.init1(phi=gd["chi"])
     g.multishift(b, float(gd["res_md"]), relative=False, precision=0)
+    # other coefficients of the same length first: the device copy of the
+    # coefficients (uploaded only when they change) must follow the call's
+    g.fermion_force(2.0 * np.asarray(a))
+    _, ip2, _ = g.end(links=False, ipdot=True, momenta=False)
     g.fermion_force(a)
     _, ip, _ = g.end(links=False, ipdot=True, momenta=False)
     assert rel(ip, gd["force_fermion"]) < 1e-10
+    assert rel(ip2, 2.0 * ip) < 1e-13
     g.gauge_deriv(accumulate=True)
     _, ip, _ = g.end(links=False, ipdot=True, momenta=False)
     assert rel(ip, gd["force_full"]) < 1e-10
