@@ -382,6 +382,20 @@ def wl_traj48c12(args):
                       "accepted": bool(r.accepted), "iters": r.cg_iterations})
         print(f"[bench] 48^3x12 trajectory {k} {r.seconds_total:.3f}s dH={r.delta_h:.4e} "
               f"err={abs(r.delta_h - r.delta_h_fp64):.2e} acc={r.accepted}", file=sys.stderr, flush=True)
+    # the MD solver's two launches per iteration (FP32 12-real, 2 RHS) timed
+    # on the device wall clock in one more trajectory from the chain's last
+    # configuration (untimed; the chain itself runs uninstrumented)
+    g.kernel_timing(1)
+    g.trajectory(precision=1, seed=6000, metropolis=False, **kw)
+    (hop_ms, n_hop), (mdm_ms, n_mdm) = g.kernel_timing(0)
+    b_hop = dslash_bytes(lat.Vh, n_pf, 48, 24)
+    b_mdm = mdagm_update_bytes(lat.Vh, n_pf, 48, 24)
+    live = {"hop_ms": hop_ms, "hop_launches": n_hop, "hop_bytes": b_hop,
+            "hop_GBps": b_hop / (hop_ms * 1e-3) / 1e9 if hop_ms > 0 else None,
+            "mdagm_ms": mdm_ms, "mdagm_launches": n_mdm, "mdagm_bytes": b_mdm,
+            "mdagm_GBps": b_mdm / (mdm_ms * 1e-3) / 1e9 if mdm_ms > 0 else None}
+    for k_ in ("hop", "mdagm"):
+        live[k_ + "_frac_hbm"] = live[k_ + "_GBps"] / HBM_PEAK_GBS if live[k_ + "_GBps"] else None
     t64 = []
     for k in range(2):
         g.init0(U0)
@@ -407,6 +421,7 @@ def wl_traj48c12(args):
     g.close()
     return {"metric": "RHMC trajectory seconds, mixed vs FP64, 48^3x12", "unit": "s/traj",
             "value": float(np.mean([c["s"] for c in chain])),
+            "live_md_solver": live,
             "config": {"workload": "48^3x12, Nf=2, am=0.01, beta=5.6, 2 pseudofermions, Omelyan 20 steps, "
                                    "16 MD / 20 heatbath+Metropolis poles, residuals 1e-6 (MD) / 1e-12",
                        "thermalisation_trajectories": args.therm},
