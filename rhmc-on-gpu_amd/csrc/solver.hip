@@ -1520,8 +1520,8 @@ static void dslash_launch(Context &c, DsArgs &a, int code) {
 // and processes without timing.  The launches are idempotent, except the
 // stored-basis solver's hop, which advances the systems' scalar state (the
 // candidates skip that step); the r^(k) the candidates of its M^dag M
-// launch write is overwritten by the final launch.  RHMC_DSLASH_NR ("4" or "4x4") forces
-// the right-hand sides per lane and lanes per site.
+// launch write is overwritten by the final launch.  RHMC_DSLASH_NR ("4", "4x4" or "2t4")
+// forces the right-hand sides per lane, lanes per site and tiles per workgroup.
 constexpr int kTuneRounds = 3;
 
 template <typename R, int LINK, int MODE, int RED>
@@ -1538,10 +1538,13 @@ static void dslash_dispatch(Context &c, DsArgs &a) {
   if (it != tab.end()) {
     code = it->second;
   } else if (const char *env = getenv("RHMC_DSLASH_NR")) {
-    // "<rhs per thread>" or "<rhs per thread>x4" for four lanes per site
+    // "<rhs per thread>", "<rhs per thread>x4" for four lanes per site, and
+    // "...t<2|4>" for tiles per workgroup in the reducing non-hop launches
     int v = std::max(1, std::min(8, atoi(env)));
     v = v >= 8 ? 8 : v >= 4 ? 4 : v >= 2 ? 2 : 1;
-    code = make_shape(v, strchr(env, 'x') ? 4 : 1, 0, 0);
+    const char *tp = strchr(env, 't');
+    const int tiles = tp && RED != RED_NONE && MODE != DS_HOP ? atoi(tp + 1) : 1;
+    code = make_shape(v, strchr(env, 'x') ? 4 : 1, 0, tiles >= 4 ? 2 : tiles >= 2 ? 1 : 0);
     tab[key] = code;
   } else {
     hipEvent_t e0, e1;

@@ -126,6 +126,46 @@ def test_multishift_on_16_site_rows(precision):
     g.close()
 
 
+@pytest.mark.parametrize("shape", ["2t2", "4t4"])
+def test_tiles_per_workgroup_shapes(shape):
+    """Reducing M^dag M launches with several tiles per workgroup (the shape
+    the autotuner picks for the 32^3x8 MD solver), forced through
+    RHMC_DSLASH_NR on 32 x 8 x 8 x 4 (16 tiles per parity, so each workgroup
+    runs two or four tiles and sums their partials): the Dslash on 5
+    right-hand sides against the oracle, and the stored-basis multi-shift
+    solve against the oracle's solutions in FP64 and to its residual in FP32."""
+    dims = (32, 8, 8, 4)
+    mass = 0.05
+    say("tiles per workgroup", shape, dims)
+    lat, U, _, phi = make_fields(dims, 29, 0.3, 2)
+    o = Oracle(dims)
+    v = fields.gauss_spinor(lat, 31, 5)
+    ref_doe = np.stack([o.doe(U, v[k]) for k in range(5)])
+    os.environ["RHMC_DSLASH_NR"] = shape
+    try:
+        g = Gpu(dims, mass=mass, n_pf=2, max_order=12)
+        g.init0(U)
+        assert rel(g.dslash(1, v, 0), ref_doe) < 1e-14
+        assert rel(g.dslash(1, v, 1), ref_doe) < 2e-6
+        g.init1(phi=phi)
+        _, b = _shifts(12, mass)
+        bn = np.linalg.norm(phi.reshape(2, -1), axis=1)
+        g.multishift(b, 1e-10, relative=True, precision=0)
+        assert any(t["tiles_per_workgroup"] > 1 for t in g.tuning_table())
+        x = g.get_fermions("smf").reshape(2, 12, lat.Vh, 3)
+        for ps in range(2):
+            xo, _ = o.multishift(U, mass, phi[ps], b, 1e-12 * bn[ps])
+            for k in (0, 5, 11):
+                assert rel(x[ps, k], xo[k]) < 1e-9, (ps, k)
+        # the FP32 solver (FP32 launch shapes of the same forms) to its own level
+        g.multishift(b, 1e-6, relative=True, precision=2)
+        res = g.shifted_residuals(b) / bn[:, None]
+        assert np.all(res < 2e-5), res.max()
+        g.close()
+    finally:
+        del os.environ["RHMC_DSLASH_NR"]
+
+
 def test_dslash_golden_reference():
     """GPU Dslash against the reference CPU output (tests/golden)."""
     gd = dict(np.load(os.path.join(GOLD, "golden_t4.npz")))
