@@ -112,7 +112,9 @@ def counters_from(fetch_dir, write_dir, out_path):
     `bench.py --dslash-only` (FETCH_SIZE, then WRITE_SIZE; csv output): per
     (kernel, grid) the mean counter per dispatch in KiB, scaled by the factor
     that makes the copy probe (1 GiB read + 1 GiB written) read its known
-    size, with the SHA-256 of the kernel sources it was taken on."""
+    size, with the SHA-256 of the kernel sources it was taken on.  Dispatches
+    that read and wrote under 1% of the kernel's largest dispatch (no-op
+    launches after a solve has stopped) are left out of the mean."""
     import csv
     import glob
 
@@ -126,7 +128,14 @@ def counters_from(fetch_dir, write_dir, out_path):
                     k = (row["Kernel_Name"], int(row["Grid_Size"]))
                     acc.setdefault(k, {})[row["Dispatch_Id"]] = acc.get(k, {}).get(row["Dispatch_Id"], 0.0) + \
                         float(row["Counter_Value"])
-        return {k: (sum(v.values()) / len(v), len(v)) for k, v in acc.items()}
+        out = {}
+        for k, v in acc.items():
+            # a solve's last chunk launches after every system has stopped exit
+            # at once (a few KiB each): the mean is over the launches that ran
+            top = max(v.values())
+            run = [x for x in v.values() if x >= 0.01 * top] or list(v.values())
+            out[k] = (sum(run) / len(run), len(run))
+        return out
 
     fe, wr = means(fetch_dir, "FETCH_SIZE"), means(write_dir, "WRITE_SIZE")
     probe = [k for k in fe if "k_copy_probe" in k[0]]
