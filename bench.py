@@ -771,6 +771,9 @@ def main():
                                  f"last workgroup, the per-dispatch interval of a kernel trace"},
         "cpu_baseline": cpu,
         "dslash_autotune": g.tuning_table(),
+        # the device as the runtime reports it, beside the published peak the
+        # roofline uses
+        "device": g.device_info(),
     }
     print(json.dumps(out))
     g.close()

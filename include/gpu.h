@@ -218,6 +218,13 @@ int gpu_time_copy(rhmc_gpu_t h, double bytes, int reps, double *ms);
  * JSON array; returns the length written (truncated to buflen-1). */
 int gpu_tuning_table(rhmc_gpu_t h, char *buf, int buflen);
 
+/* The context's device as reported by hipGetDeviceProperties, as a JSON
+ * object (name, architecture, compute units, wavefront size, HBM bytes,
+ * memory clock and bus width, L2 bytes, LDS bytes per CU and per
+ * workgroup, shader clock), truncated to buflen-1.  No counterpart in the
+ * reference; bench records it beside the roofline's assumed peaks. */
+int gpu_device_info(rhmc_gpu_t h, char *buf, int buflen);
+
 /* --------------------------------------------------------------------- */
 /* One RHMC trajectory on the device: update.cc:58-152 with the integrators
  * of lib/Update/MD_integrators. */

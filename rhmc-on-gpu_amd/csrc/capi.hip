@@ -753,6 +753,28 @@ int gpu_tuning_table(rhmc_gpu_t h, char *buf, int buflen) {
   return n;
 }
 
+int gpu_device_info(rhmc_gpu_t h, char *buf, int buflen) {
+  if (!h || !buf || buflen < 1) return RHMC_EINVAL;
+  hipDeviceProp_t pr;
+  if (hipGetDeviceProperties(&pr, h->c.device) != hipSuccess) return RHMC_EHIP;
+  int lds_cu = 0;
+  hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, h->c.device);
+  char t[1024];
+  const int n = snprintf(t, sizeof t,
+                         "{\"name\": \"%s\", \"arch\": \"%s\", \"compute_units\": %d, \"wavefront\": %d, "
+                         "\"hbm_bytes\": %zu, \"memory_clock_khz\": %d, \"memory_bus_bits\": %d, "
+                         "\"l2_bytes\": %d, \"lds_bytes_per_cu\": %d, \"lds_bytes_per_workgroup\": %zu, "
+                         "\"clock_khz\": %d}",
+                         pr.name, pr.gcnArchName, pr.multiProcessorCount, pr.warpSize, pr.totalGlobalMem,
+                         pr.memoryClockRate, pr.memoryBusWidth, pr.l2CacheSize, lds_cu, pr.sharedMemPerBlock,
+                         pr.clockRate);
+  if (n < 0 || n >= (int)sizeof t) return RHMC_EINVAL;
+  const int m = std::min(n, buflen - 1);
+  memcpy(buf, t, m);
+  buf[m] = 0;
+  return RHMC_OK;
+}
+
 int gpu_trajectory(rhmc_gpu_t h, const rhmc_traj_params *p, rhmc_traj_result *r) {
   if (!h || !p || !r) return RHMC_EINVAL;
   hipSetDevice(h->c.device);

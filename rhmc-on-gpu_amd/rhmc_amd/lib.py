@@ -130,6 +130,7 @@ def _declare(L):
         "gpu_trajectory": (I, [VP, C.POINTER(TrajParams), C.POINTER(TrajResult)]),
         "gpu_rev_update": (I, [VP, C.POINTER(TrajParams), PD, PD, C.POINTER(TrajResult)]),
         "gpu_tuning_table": (I, [VP, C.c_char_p, I]),
+        "gpu_device_info": (I, [VP, C.c_char_p, I]),
         "gpu_time_copy": (I, [VP, D, I, PD]),
         "cu_multips_shifted_invert": (I, [VP, D, I, I, VP, I, I, PI]),
         "gpu_shifted_inverter": (I, [VP, D, VP, I, I, I, PI]),
@@ -339,6 +340,12 @@ class Gpu:
         import json
         buf = C.create_string_buffer(1 << 16)
         self.L.gpu_tuning_table(self.h, buf, len(buf))
+        return json.loads(buf.value.decode())
+
+    def device_info(self):
+        import json
+        buf = C.create_string_buffer(4096)
+        self._check(self.L.gpu_device_info(self.h, buf, len(buf)), "gpu_device_info")
         return json.loads(buf.value.decode())
 
     def time_copy(self, nbytes, reps):

@@ -166,6 +166,17 @@ def test_tiles_per_workgroup_shapes(shape):
         del os.environ["RHMC_DSLASH_NR"]
 
 
+def test_device_info():
+    """gpu_device_info reports the context's device: a gfx950 part with
+    wave64, its compute units, HBM and LDS sizes as the runtime gives them."""
+    g = Gpu((4, 4, 4, 4), mass=0.1)
+    d = g.device_info()
+    g.close()
+    assert d["arch"].startswith("gfx950"), d
+    assert d["wavefront"] == 64 and d["compute_units"] > 0, d
+    assert d["hbm_bytes"] > 1 << 30 and d["lds_bytes_per_workgroup"] >= 64 << 10, d
+
+
 def test_dslash_golden_reference():
     """GPU Dslash against the reference CPU output (tests/golden)."""
     gd = dict(np.load(os.path.join(GOLD, "golden_t4.npz")))
