@@ -78,14 +78,20 @@ __device__ __forceinline__ void ff_systems(const C *own_v, const C *nb_v, long V
   }
 }
 
+// Grid: both parities of each kBlock-site tile, consecutive on one XCD, the
+// tiles in the XCD-compact order of the Dslash (tiles, or lattice order when
+// null): the even sites read the odd spinors that the odd sites of the same
+// tile read as their own, and the +mu neighbours stay in the XCD's share, so
+// each spinor is fetched into that XCD's L2 once for its five uses.
 template <typename C>
-__global__ __launch_bounds__(kBlock) void k_fermion_force(LatticeDims L, const double2 *U, const C *s,
-                                                          const C *hv, const double *coef, int nsys,
+__global__ __launch_bounds__(kBlock) void k_fermion_force(LatticeDims L, const int *tiles, const double2 *U,
+                                                          const C *s, const C *hv, const double *coef, int nsys,
                                                           double2 *ipdot, int accumulate) {
   const long Vh = L.Vh;
-  const int bx = xcd_block(blockIdx.x, gridDim.x);
-  const int h = bx * kBlock + (int)threadIdx.x;
-  const int par = blockIdx.y;
+  const unsigned b = xcd_block(blockIdx.x, gridDim.x);
+  const int par = (int)(b & 1u);
+  const int q = (int)(b >> 1);
+  const int h = (tiles ? tiles[q] : q) * kBlock + (int)threadIdx.x;
   if (h >= (int)Vh) return;
   const SiteCoord c = site_coords_i(L, par, h);
   unsigned nbo[4];
@@ -143,7 +149,7 @@ int fermion_force(Context &c, int nsys, const double *num, const double2 *x, int
   if (!hv || !coef || nsys > 256) return -1;
   if (!force_coef_upload(c, coef, num, nsys)) return -1;
   launch_hop_d(c, 1, x, hv, nsys, 3 * Vh, 3 * Vh);
-  hipLaunchKernelGGL(k_fermion_force<double2>, dim3(nblk(Vh), 2), dim3(kBlock), 0, c.stream, c.L, c.U64, x, hv, coef,
+  hipLaunchKernelGGL(k_fermion_force<double2>, dim3(2 * nblk(Vh)), dim3(kBlock), 0, c.stream, c.L, c.tiles, c.U64, x, hv, coef,
                      nsys, c.ipdot, accumulate);
   // stream-ordered for the caller (the coefficients are staged in pinned memory)
   return hip_ok(c, hipGetLastError(), "fermion force") ? 0 : -1;
@@ -156,7 +162,7 @@ int fermion_force32(Context &c, int nsys, const double *num, const float2 *x32, 
   if (!hv || !coef || nsys > 256) return -1;
   if (!force_coef_upload(c, coef, num, nsys)) return -1;
   launch_hop_s(c, 1, x32, hv, nsys, 3 * Vh, 3 * Vh);
-  hipLaunchKernelGGL(k_fermion_force<float2>, dim3(nblk(Vh), 2), dim3(kBlock), 0, c.stream, c.L, c.U64, x32, hv, coef,
+  hipLaunchKernelGGL(k_fermion_force<float2>, dim3(2 * nblk(Vh)), dim3(kBlock), 0, c.stream, c.L, c.tiles, c.U64, x32, hv, coef,
                      nsys, c.ipdot, accumulate);
   // stream-ordered for the caller (the coefficients are staged in pinned memory)
   return hip_ok(c, hipGetLastError(), "fermion force") ? 0 : -1;
@@ -229,7 +235,7 @@ int fermion_force_noalloc(Context &c, int order, const double *num, int accumula
     const int n = std::min(group, nsys - j0);
     const double2 *x = c.smf + (long)j0 * 3 * Vh;
     launch_hop_d_fixed(c, 1, x, hv, n, 3 * Vh, 3 * Vh);
-    hipLaunchKernelGGL(k_fermion_force<double2>, dim3(nblk(Vh), 2), dim3(kBlock), 0, c.stream, c.L, c.U64, x, hv,
+    hipLaunchKernelGGL(k_fermion_force<double2>, dim3(2 * nblk(Vh)), dim3(kBlock), 0, c.stream, c.L, c.tiles, c.U64, x, hv,
                        coef + j0, n, c.ipdot, (j0 > 0 || accumulate) ? 1 : 0);
   }
   return hip_ok(c, hipGetLastError(), "fermion force (fixed scratch)") ? 0 : -1;
