@@ -264,13 +264,18 @@ __device__ __forceinline__ Mat<double> link12_at(const double2 *U12, const Latti
 // [U_nu(x+mu) U_mu(x+nu)^+ U_nu(x)^+ + U_nu(x+mu-nu)^+ U_mu(x-nu)^+ U_nu(x-nu)]).
 // Links from the FP64 12-real copy (96 instead of 144 bytes per load; each
 // thread loads 19), neighbours by compare-and-wrap in 32-bit arithmetic.
-__global__ __launch_bounds__(kBlock) void k_gauge_force(LatticeDims L, const double2 *U12, double b3, double2 *ipdot,
-                                                        int accumulate) {
+// Grid: the eight (parity, mu) workgroups of each kBlock-site tile
+// consecutive on one XCD, the tiles in the XCD-compact order of the Dslash
+// (tiles, or lattice order when null), so the links a tile's staples share
+// are fetched into that XCD's L2 together.
+__global__ __launch_bounds__(kBlock) void k_gauge_force(LatticeDims L, const int *tiles, const double2 *U12,
+                                                        double b3, double2 *ipdot, int accumulate) {
   const long Vh = L.Vh;
-  const int bx = xcd_block(blockIdx.x, gridDim.x);
-  const int h = bx * kBlock + (int)threadIdx.x;
-  const int par = blockIdx.y;
-  const int mu = blockIdx.z;
+  const unsigned b = xcd_block(blockIdx.x, gridDim.x);
+  const int mu = (int)(b & 3u);
+  const int par = (int)((b >> 2) & 1u);
+  const int q = (int)(b >> 3);
+  const int h = (tiles ? tiles[q] : q) * kBlock + (int)threadIdx.x;
   if (h >= (int)Vh) return;
   const SiteCoord c = site_coords_i(L, par, h);
   Mat<double> st;
@@ -319,8 +324,8 @@ __global__ __launch_bounds__(kBlock) void k_gauge_force(LatticeDims L, const dou
 }
 
 int gauge_force(Context &c, int accumulate) {
-  hipLaunchKernelGGL(k_gauge_force, dim3(nblk(c.Vh), 2, 4), dim3(kBlock), 0, c.stream, c.L, c.U64c, c.beta / 3.0,
-                     c.ipdot, accumulate);
+  hipLaunchKernelGGL(k_gauge_force, dim3(8 * nblk(c.Vh)), dim3(kBlock), 0, c.stream, c.L, c.tiles, c.U64c,
+                     c.beta / 3.0, c.ipdot, accumulate);
   return hip_ok(c, hipGetLastError(), "gauge force") ? 0 : -1;
 }
 
