@@ -533,6 +533,9 @@ def main():
                     help="skip the leg that measures the spectral bounds in every trajectory")
     ap.add_argument("--dslash-only", action="store_true",
                     help="only the Dslash and copy-probe timings (for counter runs)")
+    ap.add_argument("--force-only", action="store_true",
+                    help="only the force kernels (FP64 fermion force on n_pf x poles systems, gauge force) and "
+                         "the copy probe (for counter runs)")
     ap.add_argument("--workload", default="traj32c8", choices=sorted(WORKLOADS),
                     help="traj32c8 is the headline; the others are the per-configuration studies")
     ap.add_argument("--counters-from", nargs=2, metavar=("FETCH_DIR", "WRITE_DIR"),
@@ -560,6 +563,25 @@ def main():
     g = Gpu(dims, mass=mass, beta=beta, n_pf=n_pf, max_order=order)
     g.init0(U0)
     # spectral bounds of MdagM on the start configuration (power iterations)
+    if args.force_only:
+        # the MD force as the FP64 trajectory runs it: the solve, h_j = D_oe s_j
+        # for every system, then the force over all of them; and the gauge force
+        g.init1(phi=fields.gauss_spinor(lat, 7, n_pf))
+        shifts = np.geomspace(1e-4, 1.0, order) * 4.0
+        num = np.linspace(0.1, 1.0, order)
+        res = {"copy_1GiB_ms": g.time_copy(1 << 30, 5)}
+        for rep in range(3):
+            g.fermionforce(shifts, num, 1e-10, relative=True, precision=0)
+            g.gauge_deriv()
+        t0 = time.perf_counter()
+        for rep in range(20):
+            g.gauge_deriv()
+        g.gauge_meas()
+        res["gauge_force_wall_ms"] = (time.perf_counter() - t0) / 20 * 1e3
+        res["systems"] = n_pf * order
+        print(json.dumps(res))
+        g.close()
+        return
     if args.dslash_only:
         reps = 50
         res = {}

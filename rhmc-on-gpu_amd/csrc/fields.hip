@@ -43,9 +43,10 @@ __device__ __forceinline__ void shift_coord(const LatticeDims &L, int &x, int &y
 // Taking two systems per step keeps more loads in flight but needs ~280
 // VGPRs (one wave per SIMD) and measured slower.  So did two lanes per site
 // (two F_mu each, loads of system j+1 issued before system j's multiply-
-// adds, 166 VGPRs): 522 against 415 us for 128 FP32 systems at 32^3x8.  The
-// kernel is bound by load-instruction issue, and that split issues 18 loads
-// per site and system instead of 15.
+// adds, 166 VGPRs): 522 against 415 us for 128 FP32 systems at 32^3x8 (both
+// with the earlier parity-major grid).  The kernel is bound by load-
+// instruction issue, and that split issues 18 loads per site and system
+// instead of 15.
 template <typename C, int NQ>
 __device__ __forceinline__ void ff_systems(const C *own_v, const C *nb_v, long Vh, int j0, unsigned ho,
                                            const unsigned (&nbo)[4], const double *coef, double sgn,
