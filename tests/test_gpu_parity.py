@@ -327,6 +327,30 @@ def test_forces_golden_reference(name):
     g.close()
 
 
+@pytest.mark.parametrize("dims", [(6, 4, 4, 6), (16, 16, 16, 8)])
+def test_force_grids_ragged_and_tiled(dims):
+    """Fermion force (solve + force) and gauge force against the oracle where
+    the last 256-site tile is partial (6 x 4 x 4 x 6: 288 sites per parity)
+    and where the XCD-compact tile order permutes the tiles (16^3 x 8: 64
+    tiles): the force grids take both parities (and the gauge force every
+    mu) of a tile back to back in that order."""
+    say("force grids", dims)
+    mass, beta = 0.05, 5.5
+    lat, U, _, phi = make_fields(dims, 41, 0.3, 2)
+    a = np.linspace(0.2, 1.0, 6)
+    _, b = _shifts(6, mass)
+    res = 1e-13 * float(np.linalg.norm(phi))
+    ip_ref, _ = Oracle(dims).full_force(U, beta, mass, phi, 0.0, a, b, res, with_gauge=True)
+    g = Gpu(dims, mass=mass, beta=beta, n_pf=2, max_order=6)
+    g.init0(U)
+    g.init1(phi=phi)
+    g.fermionforce(b, a, res, relative=False, precision=0)
+    g.gauge_deriv(accumulate=True)
+    _, ip, _ = g.end(links=False, ipdot=True, momenta=False)
+    g.close()
+    assert rel(ip, ip_ref) < 1e-9
+
+
 @pytest.mark.parametrize("name", ["t4", "t4o"])
 def test_update_and_action_golden_reference(name):
     gd = dict(np.load(os.path.join(GOLD, f"golden_{name}.npz")))
