@@ -601,6 +601,12 @@ def main():
         t0 = time.perf_counter()
         it = g.multishift(shifts, 1e-6, relative=True, precision=2)
         res["solve_fp32_8rhs_16shifts"] = {"iterations": it, "s": time.perf_counter() - t0}
+        # and the FP64 solve of the heatbath and Metropolis steps (FP64
+        # 12-real links, the same two launches per iteration)
+        g.init1(phi=fields.gauss_spinor(lat, 7, n_pf))
+        t0 = time.perf_counter()
+        it = g.multishift(shifts, 1e-10, relative=True, precision=0)
+        res["solve_fp64_8rhs_16shifts"] = {"iterations": it, "s": time.perf_counter() - t0}
         res["autotune"] = g.tuning_table()
         print(json.dumps(res))
         g.close()
