@@ -154,6 +154,18 @@ def counters_from(fetch_dir, write_dir, out_path):
     return out
 
 
+def host_cpu_model():
+    """The host CPU's model name (/proc/cpuinfo), reported beside the CPU baseline."""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
 def cpu_baseline(dims, mass, beta, n_pf, shifts, md_a, chi, U, iters_total, n_force, n_q):
     """FP64 oracle on a bounded sample of the same workload, scaled to one
     trajectory with the FP64 GPU trajectory's own operation counts; timed on
@@ -750,7 +762,7 @@ def main():
         est, cores, est1, sample = cpu_baseline(dims, mass, beta, n_pf, np.array(b), np.array(a), chi, U0,
                                                 int(np.mean(iters_d)), n_force, n_q)
         cpu = {"value": est, "unit": "s/traj", "cores": cores, "kind": "port", "sample": sample,
-               "single_thread_value": est1, "reference_check_c1": cpu_reference_c1()}
+               "single_thread_value": est1, "cpu_model": host_cpu_model(), "reference_check_c1": cpu_reference_c1()}
 
     out = {
         "metric": "RHMC traj sec + Dslash GB/s (% HBM), mixed vs FP64 on GPU, 32^3x8",
