@@ -83,7 +83,8 @@ __device__ __forceinline__ void ff_systems(const C *own_v, const C *nb_v, long V
 // tiles in the XCD-compact order of the Dslash (tiles, or lattice order when
 // null): the even sites read the odd spinors that the odd sites of the same
 // tile read as their own, and the +mu neighbours stay in the XCD's share, so
-// each spinor is fetched into that XCD's L2 once for its five uses.
+// more of a spinor's five uses find it in that XCD's L2 (FP64, 128 systems
+// at 32^3x8: 5.84 -> 4.36 GB fetched per call, 816 -> 648 us).
 template <typename C>
 __global__ __launch_bounds__(kBlock) void k_fermion_force(LatticeDims L, const int *tiles, const double2 *U,
                                                           const C *s, const C *hv, const double *coef, int nsys,
