@@ -812,6 +812,16 @@ def main():
                                  f"on the device wall clock from the end of the hop before it to the end of its "
                                  f"last workgroup, the per-dispatch interval of a kernel trace"},
         "cpu_baseline": cpu,
+        # the stated targets, each with the figure of this run it is read from
+        "targets": {
+            "mixed_vs_fp64_plain_at_least_3x": (fp64_plain / val) if fp64_plain else None,
+            "mixed_vs_fp64_same_solver": (fp64 / val) if fp64 else None,
+            "max_delta_h_error_at_most_1e-9": max((e for leg in ("mixed", "mixed_power_bounds")
+                                                   for e in dh_err.get(leg, [])), default=None),
+            "dslash_frac_hbm_at_least_0.7": {"fp64_18real_1rhs": gbs64 / HBM_PEAK_GBS,
+                                             "fp32_12real_8rhs": gbs32 / HBM_PEAK_GBS,
+                                             "fp32_12real_8rhs_live_in_solver": gbs_live / HBM_PEAK_GBS},
+        },
         "dslash_autotune": g.tuning_table(),
         # the device as the runtime reports it, beside the published peak the
         # roofline uses
