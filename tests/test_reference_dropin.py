@@ -61,7 +61,7 @@ def test_reference_update_on_gpu(name):
         V = int(np.prod(g["dims"]))
         links = np.fromfile(os.path.join(d, "out_links.bin"), np.complex128).reshape(4 * V, 3, 3)
     # The reference's own boundary rounds P and ipdot to FP32 and the links to
-    # hi/lo float pairs (packer.cc:8-64,443-569; SURVEY App. B #7), so the GPU
+    # hi/lo float pairs (packer.cc:8-64,443-569), so the GPU
     # build of the reference agrees with its CPU build to that precision.
     ref = float(g["traj_dh"])
     assert abs(dh - ref) < 1e-4 * max(1.0, abs(ref)), (dh, ref)
