@@ -79,3 +79,12 @@ def test_traffic_lookup_of_the_autotuned_hop():
     assert t == k["read_bytes"] + k["write_bytes"]
     table[0]["rhs_per_thread"] = 1  # a 1-per-lane hop was never profiled at 8 RHS
     assert bench.dslash_traffic(table, 32 * 32 * 32 * 8 // 2, 8, mode=0, red=2) is None
+
+
+def test_bench_refuses_more_than_one_gpu():
+    """One GPU only: --gpus above 1 exits with a message, before any GPU call."""
+    import subprocess
+    p = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True,
+                       text=True, timeout=120)
+    assert p.returncode == 2
+    assert "one GPU" in json.loads(p.stdout.strip().splitlines()[-1])["error"]
